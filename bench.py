@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Throughput of the AdaptaHOP hot path (kNN + SPH density + hop/group ids) on one MI355X.
+
+One step = one pass of the hot path over the resident 16,777,216-particle mixed-family
+box (k=64, nhop=16): Morton sort, device tree, kNN + density, hop targets, group ids.
+Inputs are in HBM before the timed region; outputs (density, group ids) stay in HBM.
+Prints one JSON line.  Also reports, in the same line, the plain FP64 device path, the
+saddle-edge stage, the roofline of the dominant kernel (kNN + density) and the
+reference's own OpenMP Fortran path timed on this host's cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "halomaker-python_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "Mparticles/s kNN+density+hop at 16.8M; mixed vs FP64 GPU at ≤1e-6 rel ρ err"
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per particle of the kNN+density kernel: read position 24 B and
+# mass 8 B, write density 8 B (the whole step adds the hop stage's density re-read 8 B and
+# group-id write 8 B: 56 B per particle)
+KNN_BYTES_PER_PARTICLE = 40
+STEP_BYTES_PER_PARTICLE = 56
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sample_n, nthreads):
+    """Reference OpenMP Fortran (oracle/_ref) on a bounded sample of the same workload:
+    the same generator and parameters at sample_n particles (clump count scaled)."""
+    import oracle as O
+    from amdahop import generate as G
+    ps = G.config("c3", n=sample_n)
+    if O.ref_available():
+        R = O.Reference(ps.pos, ps.mass, ps.nvoisins, nthreads=nthreads)
+        t = min(R.time_knn_density_hop() for _ in range(2))
+        kind = "reference"
+    else:
+        o = O.Oracle(ps.pos, ps.mass, ps.nvoisins, nthreads=nthreads)
+        t0 = time.perf_counter()
+        rho, _ = o.density()
+        o.maxima(rho)
+        t = time.perf_counter() - t0
+        o.close()
+        kind = "port"
+    return {"value": round(sample_n / t / 1e6, 4), "unit": "Mparticles/s", "cores": nthreads,
+            "kind": kind,
+            "sample": f"c3 generator at N={sample_n} (k=64, nhop=16, clumps scaled), "
+                      f"tree+density+local-maxima stages, best of 2, {t:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--bucket", type=int, default=32)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--cpu-sample-n", type=int, default=2_097_152)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(sharding design only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from amdahop import generate as G, run_device
+
+    dev = torch.device("cuda:0")
+    ps = G.config(args.workload)
+    n, k = ps.n, ps.nvoisins
+    log(f"workload {args.workload}: n={n} k={k}")
+    pos = torch.from_numpy(ps.pos).to(dev)
+    mass = torch.from_numpy(ps.mass).to(dev)
+    dens = torch.empty(n, dtype=torch.float64, device=dev)
+    gid = torch.empty(n, dtype=torch.int32, device=dev)
+
+    def step(mixed, saddle=False):
+        return run_device(pos, mass, k, nhop=16, mixed=mixed, saddle=saddle, bucket=args.bucket,
+                          density_out=dens, gid_out=gid)[2]
+
+    for _ in range(args.warmup):
+        step(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = [step(True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / args.steps
+    value = n / (ms * 1e-3) / 1e6
+    knn_ms = sum(i.t_knn for i in infos) / len(infos)
+    last = infos[-1]
+    log(f"mixed: {ms:.2f} ms/step  sort {last.t_sort:.2f} tree {last.t_tree:.2f} knn {last.t_knn:.2f} "
+        f"fallback {last.t_fallback:.2f} ({last.nfallback} q) hop {last.t_hop:.2f}")
+
+    extra = {}
+    if not args.no_fp64:
+        step(False)
+        torch.cuda.synchronize()
+        f0 = time.perf_counter()
+        nf = max(2, args.steps // 2)
+        finfo = [step(False) for _ in range(nf)]
+        torch.cuda.synchronize()
+        fms = (time.perf_counter() - f0) * 1e3 / nf
+        extra["fp64_path"] = {"value": round(n / (fms * 1e-3) / 1e6, 3), "ms_per_step": round(fms, 3),
+                              "knn_ms": round(sum(i.t_knn for i in finfo) / nf, 3)}
+        extra["speedup_mixed_vs_fp64"] = round(fms / ms, 3)
+        log(f"fp64: {fms:.2f} ms/step knn {extra['fp64_path']['knn_ms']:.2f}")
+    sinfo = step(True, saddle=True)
+    extra["saddle_stage"] = {"ms": round(sinfo.t_saddle, 3), "directed_edges_kept": int(sinfo.nsaddle)}
+    extra["stages_ms"] = {"sort": round(last.t_sort, 3), "tree": round(last.t_tree, 3),
+                          "knn_density": round(last.t_knn, 3), "fp64_rerun": round(last.t_fallback, 3),
+                          "hop_groups": round(last.t_hop, 3)}
+    extra["nactive"], extra["ngroups"], extra["nfallback"] = int(last.nactive), int(last.ngroups), int(last.nfallback)
+
+    achieved = KNN_BYTES_PER_PARTICLE * n / (knn_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "kernel": "k_knn<64,mixed> (kNN + SPH density)", "kernel_ms": round(knn_ms, 3),
+                "bytes_per_particle": KNN_BYTES_PER_PARTICLE,
+                "step_achieved_GBs": round(STEP_BYTES_PER_PARTICLE * n / (ms * 1e-3) / 1e9, 2)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+        nthreads = min(nthreads, 16)
+        log(f"cpu baseline: {args.cpu_sample_n} particles on {nthreads} threads")
+        cpu = cpu_baseline(args.cpu_sample_n, nthreads)
+
+    out = {"metric": METRIC, "value": round(value, 3), "unit": "Mparticles/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32+f64", "data": "synthetic",
+           "config": {"workload": f"{args.workload}: {n} particles, periodic unit box, k={k}, nhop=16, "
+                                  "Plummer clumps + uniform background, mixed precision",
+                      "n": n, "nvoisins": k, "nhop": 16, "precision": "mixed", "bucket": args.bucket},
+           "roofline": roofline, "cpu_baseline": cpu}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

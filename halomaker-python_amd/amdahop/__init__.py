@@ -1,0 +1,59 @@
+"""amdahop -- MI355X-native AdaptaHOP hot path (kNN, SPH density, hop, saddle edges).
+
+Host-side entry points over libamdahop.so:
+
+* :func:`run_device` -- the hot path on device-resident arrays (torch tensors are used
+  only as device allocations; the library sees raw pointers);
+* the drop-in ``compute_adaptahop.neikdtree`` module (package ``compute_adaptahop``);
+* :mod:`amdahop.generate` -- the synthetic workloads.
+"""
+from __future__ import annotations
+
+import ctypes
+
+from ._lib import HotParams, RunInfo, check, lib
+
+__all__ = ["run_device", "device_saddle", "HotParams", "RunInfo"]
+
+
+def run_device(pos, mass, nvoisins, nhop=16, rho_threshold=80.0, box=1.0, mixed=True,
+               bucket=32, saddle=True, density_out=None, gid_out=None):
+    """Run kNN + density + hop + group ids (+ saddle edges) on device tensors.
+
+    pos: (N,3) float64 contiguous on the GPU; mass: (N,) float64.  Returns
+    (density, gid, info) with density/gid in original particle order (torch tensors on
+    the same device) and ``info`` the :class:`RunInfo` of the run."""
+    import torch
+
+    if pos.dtype != torch.float64 or mass.dtype != torch.float64:
+        raise TypeError("pos and mass must be float64")
+    if not (pos.is_cuda and mass.is_cuda):
+        raise ValueError("pos and mass must be device tensors")
+    pos = pos.contiguous()
+    mass = mass.contiguous()
+    n = int(mass.shape[0])
+    if tuple(pos.shape) != (n, 3):
+        raise ValueError("pos must have shape (N,3)")
+    Lx, Ly, Lz = (box, box, box) if not isinstance(box, (tuple, list)) else box
+    p = HotParams(int(nvoisins), int(nhop), float(rho_threshold), float(Lx), float(Ly), float(Lz),
+                  1 if mixed else 0, int(bucket), 1 if saddle else 0)
+    dens = density_out if density_out is not None else torch.empty(n, dtype=torch.float64, device=pos.device)
+    gid = gid_out if gid_out is not None else torch.empty(n, dtype=torch.int32, device=pos.device)
+    info = RunInfo()
+    torch.cuda.synchronize(pos.device)
+    check(lib().ahop_run_device(ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(mass.data_ptr()), n,
+                                ctypes.byref(p), ctypes.c_void_p(dens.data_ptr()),
+                                ctypes.c_void_p(gid.data_ptr()), ctypes.byref(info)))
+    return dens, gid, info
+
+
+def device_saddle(info):
+    """Host copy (numpy) of the saddle CSR of the last :func:`run_device` call."""
+    import numpy as np
+
+    from ._lib import ptr
+    off = np.empty(info.ngroups + 1, dtype=np.int32)
+    isad = np.empty(max(info.nsaddle, 1), dtype=np.int32)
+    rho = np.empty(max(info.nsaddle, 1), dtype=np.float64)
+    check(lib().ahop_device_saddle(ptr(off), ptr(isad), ptr(rho)))
+    return off, isad[:info.nsaddle], rho[:info.nsaddle]

@@ -1,0 +1,86 @@
+"""ctypes binding of libamdahop.so (C ABI declared in include/amdahop.h).
+
+The library is built in-tree (``make -C halomaker-python_amd/csrc``) and loaded from
+this directory.  There is no fallback: if the library is missing the import fails
+loudly, so a run can never silently compute on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libamdahop.so")
+
+_i64, _i32, _dbl, _vp, _f32 = ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p, ctypes.c_float
+
+
+class HotParams(ctypes.Structure):
+    _fields_ = [("nvoisins", _i32), ("nhop", _i32), ("rho_threshold", _dbl),
+                ("xlong", _dbl), ("ylong", _dbl), ("zlong", _dbl),
+                ("mixed", _i32), ("bucket", _i32), ("saddle", _i32)]
+
+
+class RunInfo(ctypes.Structure):
+    _fields_ = [("nactive", _i64), ("ngroups", _i32), ("nfallback", _i64), ("nsaddle", _i64),
+                ("t_sort", _f32), ("t_tree", _f32), ("t_knn", _f32), ("t_fallback", _f32),
+                ("t_hop", _f32), ("t_saddle", _f32), ("t_total", _f32)]
+
+
+# name -> (restype, argtypes); must list every function declared in include/amdahop.h
+SIGNATURES = {
+    "ahop_sync_from_change_pos": (ctypes.c_int, [_i64, _i32, _dbl, _dbl, _dbl, _dbl]),
+    "ahop_sync_from_init_adaptahop": (ctypes.c_int, [_i64, _i32, _i32] + [_dbl] * 19 + [_vp]),
+    "ahop_sync_others": (ctypes.c_int, [_i32, _i32, _i32, _i32, _dbl, _dbl, _dbl, _i32, _i32,
+                                        _dbl, _dbl, ctypes.c_char_p, _i32, _i32, _i32]),
+    "ahop_compute_adaptahop": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64]),
+    "ahop_close": (ctypes.c_int, []),
+    "ahop_liste_parts": (_vp, [ctypes.POINTER(_i64)]),
+    "ahop_density": (_vp, [ctypes.POINTER(_i64)]),
+    "ahop_nnodes": (_i32, []),
+    "ahop_real_table": (_vp, [ctypes.POINTER(_i64)]),
+    "ahop_integer_table": (_vp, [ctypes.POINTER(_i64)]),
+    "ahop_last_error": (ctypes.c_char_p, []),
+    "ahop_set_precision": (ctypes.c_int, [_i32]),
+    "ahop_set_bucket": (ctypes.c_int, [_i32]),
+    "ahop_ngroups": (_i32, []),
+    "ahop_get_groups": (ctypes.c_int, [_vp, _vp, _i64]),
+    "ahop_nsaddle": (_i64, []),
+    "ahop_get_saddle": (ctypes.c_int, [_vp, _vp, _vp]),
+    "ahop_run_device": (ctypes.c_int, [_vp, _vp, _i64, ctypes.POINTER(HotParams), _vp, _vp,
+                                       ctypes.POINTER(RunInfo)]),
+    "ahop_device_saddle": (ctypes.c_int, [_vp, _vp, _vp]),
+    "ahop_create_group_tree": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp,
+                                              _dbl, _dbl, _dbl, _dbl, _i32, _dbl, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libamdahop.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libamdahop.so not built at {LIB_PATH}: run "
+                              "`make -C halomaker-python_amd/csrc` (or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status):
+    """Turn a nonzero ABI status into RuntimeError carrying ahop_last_error()."""
+    if status != 0:
+        msg = lib().ahop_last_error()
+        raise RuntimeError((msg or b"amdahop error").decode())
+
+
+def ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)

@@ -1,0 +1,489 @@
+// ahop_capi.cpp -- C ABI (include/amdahop.h) over the device engine and the host
+// structure-tree stage.  Holds the process-wide module state that the reference keeps
+// in Fortran module variables (fhalo_defs / neikdtree, compute_adaptahop.f90:12-407).
+#include "../../include/amdahop.h"
+#include "ahop_common.h"
+#include "create_nodes.h"
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+using namespace ahop;
+
+namespace {
+
+struct ModuleState {
+    // sync_from_init_adaptahop (f90:591-641)
+    int64_t npart = 0;
+    int32_t nmembthresh = 0, nmembers = 0;
+    double omegal = 0, omega_lambda_f = 0, omega0 = 0, omega_f = 0, aexp_max = 0, af = 0;
+    double hubble = 0, h_f = 0, boxsize2 = 1, lf = 1;
+    double xlong = 1, ylong = 1, zlong = 1, xlongs2 = 0.5, ylongs2 = 0.5, zlongs2 = 0.5;
+    double hub_pt = 0, aexp = 0, pos_shift[3] = {0, 0, 0};
+    bool fsub = false;
+    // sync_from_change_pos (f90:576-589)
+    int32_t nusedpart = 0;
+    double epsilon = 0, fudgepsilon = 0;
+    // sync_others (f90:643-678)
+    bool verbose = false, megaverbose = false;
+    int32_t nbpes = 1;
+    double rho_threshold = 80, massp = 0, boxsize = 0;
+    int32_t nhop = 16, nvoisins = 32;
+    double fudge = 4, alphap = 1;
+    std::string method = "MSM";
+    int32_t nlevelmax = 30;
+    bool optimize_nodes = false;
+    // extensions
+    int32_t mixed = 1, bucket = 32;
+    // outputs (allocated by compute_adaptahop, freed by close)
+    bool allocated = false;
+    std::vector<int32_t> liste_parts;
+    std::vector<double> density;
+    int32_t nnodes = 0;
+    std::vector<double> real_table;
+    std::vector<int32_t> integer_table;
+    std::vector<int32_t> gid, hop, sad_offset, sad_isad;
+    std::vector<double> sad_rho, densityg;
+    int32_t ngroups = 0;
+    // last device run (ahop_run_device)
+    std::vector<int32_t> dev_sad_offset, dev_sad_isad;
+    std::vector<double> dev_sad_rho;
+};
+
+ModuleState &S()
+{
+    static ModuleState s;
+    return s;
+}
+
+Engine &engine()
+{
+    static std::unique_ptr<Engine> e;
+    if (!e) e.reset(new Engine());
+    return *e;
+}
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        set_error("");
+        return 0;
+    } catch (const std::exception &ex) {
+        set_error(ex.what());
+        return 1;
+    } catch (...) {
+        set_error("unknown error");
+        return 1;
+    }
+}
+
+struct DeviceArray {
+    void *p = nullptr;
+    explicit DeviceArray(size_t bytes) { AHOP_HIP(hipMalloc(&p, bytes < 8 ? 8 : bytes)); }
+    ~DeviceArray() { if (p) (void)hipFree(p); }
+};
+
+void free_outputs(ModuleState &s)
+{
+    s.allocated = false;
+    s.liste_parts.clear();
+    s.liste_parts.shrink_to_fit();
+    s.density.clear();
+    s.density.shrink_to_fit();
+    s.real_table.clear();
+    s.integer_table.clear();
+    s.nnodes = 0;
+    s.gid.clear();
+    s.hop.clear();
+    s.sad_offset.clear();
+    s.sad_isad.clear();
+    s.sad_rho.clear();
+    s.densityg.clear();
+    s.ngroups = 0;
+}
+
+Params hot_params(const ModuleState &s)
+{
+    Params p;
+    p.nvoisins = s.nvoisins;
+    p.nhop = s.nhop;
+    p.rho_threshold = s.rho_threshold;
+    p.xlong = s.xlong;
+    p.ylong = s.ylong;
+    p.zlong = s.zlong;
+    p.mixed = s.mixed;
+    p.bucket = s.bucket;
+    return p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ahop_sync_from_change_pos(int64_t npart, int32_t nusedpart, double epsilon, double fudgepsilon,
+                              double xlong, double boxsize2)
+{
+    return guarded([&] {
+        auto &s = S();
+        s.npart = npart;
+        s.nusedpart = nusedpart;
+        s.epsilon = epsilon;
+        s.fudgepsilon = fudgepsilon;
+        s.xlong = xlong;
+        s.boxsize2 = boxsize2;
+    });
+}
+
+int ahop_sync_from_init_adaptahop(int64_t npart, int32_t nmembthresh, int32_t nmembers,
+                                  double omegal, double omega_lambda_f, double omega0,
+                                  double omega_f, double aexp_max, double af, double hubble,
+                                  double h_f, double boxsize2, double lf, double xlong,
+                                  double ylong, double zlong, double xlongs2, double ylongs2,
+                                  double zlongs2, double hub_pt, double aexp,
+                                  const double *pos_shift)
+{
+    return guarded([&] {
+        auto &s = S();
+        s.npart = npart;
+        s.nmembthresh = nmembthresh;
+        s.nmembers = nmembers;
+        s.omegal = omegal;
+        s.omega_lambda_f = omega_lambda_f;
+        s.omega0 = omega0;
+        s.omega_f = omega_f;
+        s.aexp_max = aexp_max;
+        s.af = af;
+        s.hubble = hubble;
+        s.h_f = h_f;
+        s.boxsize2 = boxsize2;
+        s.lf = lf;
+        s.xlong = xlong;
+        s.ylong = ylong;
+        s.zlong = zlong;
+        s.xlongs2 = xlongs2;
+        s.ylongs2 = ylongs2;
+        s.zlongs2 = zlongs2;
+        s.hub_pt = hub_pt;
+        s.aexp = aexp;
+        if (pos_shift)
+            for (int d = 0; d < 3; ++d) s.pos_shift[d] = pos_shift[d];
+        s.fsub = true;
+    });
+}
+
+int ahop_sync_others(int32_t verbose, int32_t megaverbose, int32_t npart, int32_t nbpes,
+                     double rho_threshold, double massp, double boxsize, int32_t nhop,
+                     int32_t nvoisins, double fudge, double alphap, const char *method,
+                     int32_t method_len, int32_t nlevelmax, int32_t optimize_nodes)
+{
+    return guarded([&] {
+        auto &s = S();
+        s.verbose = verbose != 0;
+        s.megaverbose = megaverbose != 0;
+        s.npart = npart;
+        s.nbpes = nbpes;
+        s.rho_threshold = rho_threshold;
+        s.massp = massp;
+        s.boxsize = boxsize;
+        s.nhop = nhop;
+        s.nvoisins = nvoisins;
+        s.fudge = fudge;
+        s.alphap = alphap;
+        // method is character(len=3) in fhalo_defs: keep at most 3 characters (f90:669)
+        s.method = method ? std::string(method, method_len < 3 ? (method_len < 0 ? 0 : method_len) : 3) : "";
+        s.nlevelmax = nlevelmax;
+        s.optimize_nodes = optimize_nodes != 0;
+    });
+}
+
+int ahop_close(void)
+{
+    return guarded([&] { free_outputs(S()); });
+}
+
+int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_cols,
+                           const double *mass_in, int64_t mass_len)
+{
+    return guarded([&] {
+        auto &s = S();
+        free_outputs(s);                           // f90:501, close() first
+        if (pos_rows != 3) throw std::runtime_error("pos_in should have shape (3,npart)");
+        const int64_t n = s.npart;
+        if (n <= 0) throw std::runtime_error("npart must be set (sync_* calls) before compute_adaptahop");
+        if (pos_cols != n || mass_len != n)
+            throw std::runtime_error("pos_in/mass_in sizes do not match npart=" + std::to_string(n));
+        if (s.nusedpart < n) throw std::runtime_error("nusedpart must be >= npart");
+        if (!pos_in || !mass_in) throw std::runtime_error("null input array");
+        auto t0 = std::chrono::steady_clock::now();
+        Engine &e = engine();
+        Params p = hot_params(s);
+        DeviceArray dpos(sizeof(double) * 3 * n), dmass(sizeof(double) * n);
+        AHOP_HIP(hipMemcpy(dpos.p, pos_in, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+        AHOP_HIP(hipMemcpy(dmass.p, mass_in, sizeof(double) * n, hipMemcpyHostToDevice));
+        e.run((const double *)dpos.p, (const double *)dmass.p, n, p, true);
+        const DeviceResults &r = e.results();
+        s.density.resize(n);
+        s.gid.resize(n);
+        s.hop.resize(n);
+        s.ngroups = r.ngroups;
+        s.densityg.resize(r.ngroups);
+        s.sad_offset.resize(r.ngroups + 1);
+        s.sad_isad.resize(r.nsaddle);
+        s.sad_rho.resize(r.nsaddle);
+        AHOP_HIP(hipMemcpy(s.density.data(), r.density, sizeof(double) * n, hipMemcpyDeviceToHost));
+        AHOP_HIP(hipMemcpy(s.gid.data(), r.gid, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        AHOP_HIP(hipMemcpy(s.hop.data(), r.hop, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        if (r.ngroups > 0)
+            AHOP_HIP(hipMemcpy(s.densityg.data(), r.densityg, sizeof(double) * r.ngroups,
+                               hipMemcpyDeviceToHost));
+        AHOP_HIP(hipMemcpy(s.sad_offset.data(), r.sad_offset, sizeof(int32_t) * (r.ngroups + 1),
+                           hipMemcpyDeviceToHost));
+        if (r.nsaddle > 0) {
+            AHOP_HIP(hipMemcpy(s.sad_isad.data(), r.sad_isad, sizeof(int32_t) * r.nsaddle,
+                               hipMemcpyDeviceToHost));
+            AHOP_HIP(hipMemcpy(s.sad_rho.data(), r.sad_rho, sizeof(double) * r.nsaddle,
+                               hipMemcpyDeviceToHost));
+        }
+        auto t1 = std::chrono::steady_clock::now();
+        TreeInputs ti;
+        ti.npart = n;
+        ti.pos = pos_in;
+        ti.mass = mass_in;
+        ti.density = s.density.data();
+        ti.gid = s.gid.data();
+        ti.ngroups = s.ngroups;
+        ti.densityg = s.densityg.data();
+        ti.sad_offset = s.sad_offset.data();
+        ti.sad_isad = s.sad_isad.data();
+        ti.sad_rho = s.sad_rho.data();
+        TreeParams tp;
+        tp.rho_threshold = s.rho_threshold;
+        tp.fudge = s.fudge;
+        tp.alphap = s.alphap;
+        tp.epsilon = s.epsilon;
+        tp.nmembthresh = s.nmembthresh;
+        tp.xlong = s.xlong;
+        tp.ylong = s.ylong;
+        tp.zlong = s.zlong;
+        tp.xlongs2 = s.xlongs2;
+        tp.ylongs2 = s.ylongs2;
+        tp.zlongs2 = s.zlongs2;
+        TreeOutputs to;
+        create_group_tree(ti, tp, to);
+        auto t2 = std::chrono::steady_clock::now();
+        s.liste_parts.assign(s.nusedpart, 0);
+        std::copy(to.liste_parts.begin(), to.liste_parts.end(), s.liste_parts.begin());
+        s.nnodes = to.nnodes;
+        s.real_table = std::move(to.real_table);
+        s.integer_table = std::move(to.integer_table);
+        s.allocated = true;
+        if (s.verbose) {
+            const StageTimes &t = e.times();
+            double host_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+            double all_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
+            std::fprintf(stderr,
+                         "    [amdahop] sort %.3f ms, tree %.3f ms, kNN+density %.3f ms "
+                         "(FP64 re-run of %lld queries %.3f ms), hop+groups %.3f ms, saddle %.3f ms\n"
+                         "    [amdahop] active %lld, groups %d, saddle entries %lld, nodes %d; "
+                         "create nodes (host) %.3f ms, call total %.3f ms\n",
+                         t.sort, t.tree, t.knn, (long long)r.nfallback, t.fallback, t.hop,
+                         t.saddle, (long long)r.nactive, r.ngroups, (long long)r.nsaddle, s.nnodes,
+                         host_ms, all_ms);
+        }
+    });
+}
+
+const int32_t *ahop_liste_parts(int64_t *len)
+{
+    auto &s = S();
+    if (len) *len = s.allocated ? (int64_t)s.liste_parts.size() : 0;
+    return s.allocated ? s.liste_parts.data() : nullptr;
+}
+
+const double *ahop_density(int64_t *len)
+{
+    auto &s = S();
+    if (len) *len = s.allocated ? (int64_t)s.density.size() : 0;
+    return s.allocated ? s.density.data() : nullptr;
+}
+
+int32_t ahop_nnodes(void) { return S().nnodes; }
+
+const double *ahop_real_table(int64_t *ncols)
+{
+    auto &s = S();
+    if (ncols) *ncols = s.allocated ? s.nnodes : 0;
+    return s.allocated ? s.real_table.data() : nullptr;
+}
+
+const int32_t *ahop_integer_table(int64_t *ncols)
+{
+    auto &s = S();
+    if (ncols) *ncols = s.allocated ? s.nnodes : 0;
+    return s.allocated ? s.integer_table.data() : nullptr;
+}
+
+const char *ahop_last_error(void) { return last_error(); }
+
+int ahop_set_precision(int32_t mixed)
+{
+    return guarded([&] {
+        if (mixed != 0 && mixed != 1) throw std::runtime_error("precision must be 0 (fp64) or 1 (mixed)");
+        S().mixed = mixed;
+    });
+}
+
+int ahop_set_bucket(int32_t bucket)
+{
+    return guarded([&] {
+        if (bucket < 1 || bucket > 64) throw std::runtime_error("bucket must be in [1, 64]");
+        S().bucket = bucket;
+    });
+}
+
+int32_t ahop_ngroups(void) { return S().ngroups; }
+
+int ahop_get_groups(int32_t *gid, int32_t *hop, int64_t n)
+{
+    return guarded([&] {
+        auto &s = S();
+        if (!s.allocated) throw std::runtime_error("no results: call compute_adaptahop first");
+        if (n != (int64_t)s.gid.size()) throw std::runtime_error("size mismatch");
+        if (gid) std::memcpy(gid, s.gid.data(), sizeof(int32_t) * n);
+        if (hop) std::memcpy(hop, s.hop.data(), sizeof(int32_t) * n);
+    });
+}
+
+int64_t ahop_nsaddle(void) { return (int64_t)S().sad_isad.size(); }
+
+int ahop_get_saddle(int32_t *offset, int32_t *isad, double *rho)
+{
+    return guarded([&] {
+        auto &s = S();
+        if (!s.allocated) throw std::runtime_error("no results: call compute_adaptahop first");
+        if (offset) std::memcpy(offset, s.sad_offset.data(), sizeof(int32_t) * s.sad_offset.size());
+        if (isad && !s.sad_isad.empty()) std::memcpy(isad, s.sad_isad.data(), sizeof(int32_t) * s.sad_isad.size());
+        if (rho && !s.sad_rho.empty()) std::memcpy(rho, s.sad_rho.data(), sizeof(double) * s.sad_rho.size());
+    });
+}
+
+int ahop_run_device(const double *d_pos, const double *d_mass, int64_t n,
+                    const ahop_hot_params *params, double *d_density, int32_t *d_gid,
+                    ahop_run_info *info)
+{
+    return guarded([&] {
+        if (!params) throw std::runtime_error("null params");
+        if (!d_pos || !d_mass) throw std::runtime_error("null input array");
+        Params p;
+        p.nvoisins = params->nvoisins;
+        p.nhop = params->nhop;
+        p.rho_threshold = params->rho_threshold;
+        p.xlong = params->xlong;
+        p.ylong = params->ylong;
+        p.zlong = params->zlong;
+        p.mixed = params->mixed;
+        p.bucket = params->bucket;
+        Engine &e = engine();
+        e.run(d_pos, d_mass, n, p, params->saddle != 0);
+        const DeviceResults &r = e.results();
+        hipStream_t st = e.stream();
+        if (d_density)
+            AHOP_HIP(hipMemcpyAsync(d_density, r.density, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        if (d_gid)
+            AHOP_HIP(hipMemcpyAsync(d_gid, r.gid, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, st));
+        auto &s = S();
+        if (params->saddle) {
+            s.dev_sad_offset.resize(r.ngroups + 1);
+            s.dev_sad_isad.resize(r.nsaddle);
+            s.dev_sad_rho.resize(r.nsaddle);
+            AHOP_HIP(hipMemcpyAsync(s.dev_sad_offset.data(), r.sad_offset,
+                                    sizeof(int32_t) * (r.ngroups + 1), hipMemcpyDeviceToHost, st));
+            if (r.nsaddle > 0) {
+                AHOP_HIP(hipMemcpyAsync(s.dev_sad_isad.data(), r.sad_isad,
+                                        sizeof(int32_t) * r.nsaddle, hipMemcpyDeviceToHost, st));
+                AHOP_HIP(hipMemcpyAsync(s.dev_sad_rho.data(), r.sad_rho, sizeof(double) * r.nsaddle,
+                                        hipMemcpyDeviceToHost, st));
+            }
+        } else {
+            s.dev_sad_offset.clear();
+            s.dev_sad_isad.clear();
+            s.dev_sad_rho.clear();
+        }
+        AHOP_HIP(hipStreamSynchronize(st));
+        if (info) {
+            const StageTimes &t = e.times();
+            info->nactive = r.nactive;
+            info->ngroups = r.ngroups;
+            info->nfallback = r.nfallback;
+            info->nsaddle = params->saddle ? r.nsaddle : 0;
+            info->t_sort = t.sort;
+            info->t_tree = t.tree;
+            info->t_knn = t.knn;
+            info->t_fallback = t.fallback;
+            info->t_hop = t.hop;
+            info->t_saddle = t.saddle;
+            info->t_total = t.total;
+        }
+    });
+}
+
+int ahop_device_saddle(int32_t *offset, int32_t *isad, double *rho)
+{
+    return guarded([&] {
+        auto &s = S();
+        if (offset && !s.dev_sad_offset.empty())
+            std::memcpy(offset, s.dev_sad_offset.data(), sizeof(int32_t) * s.dev_sad_offset.size());
+        if (isad && !s.dev_sad_isad.empty())
+            std::memcpy(isad, s.dev_sad_isad.data(), sizeof(int32_t) * s.dev_sad_isad.size());
+        if (rho && !s.dev_sad_rho.empty())
+            std::memcpy(rho, s.dev_sad_rho.data(), sizeof(double) * s.dev_sad_rho.size());
+    });
+}
+
+int ahop_create_group_tree(int64_t npart, const double *pos, const double *mass,
+                           const double *density, const int32_t *gid, int32_t ngroups,
+                           const double *densityg, const int32_t *sad_offset,
+                           const int32_t *sad_isad, const double *sad_rho, double rho_threshold,
+                           double fudge, double alphap, double epsilon, int32_t nmembthresh,
+                           double xlong, int32_t *liste_parts, int32_t *nnodes,
+                           double *real_table, int32_t *integer_table)
+{
+    return guarded([&] {
+        if (npart <= 0 || ngroups < 0) throw std::runtime_error("bad sizes");
+        TreeInputs ti;
+        ti.npart = npart;
+        ti.pos = pos;
+        ti.mass = mass;
+        ti.density = density;
+        ti.gid = gid;
+        ti.ngroups = ngroups;
+        ti.densityg = densityg;
+        ti.sad_offset = sad_offset;
+        ti.sad_isad = sad_isad;
+        ti.sad_rho = sad_rho;
+        TreeParams tp;
+        tp.rho_threshold = rho_threshold;
+        tp.fudge = fudge;
+        tp.alphap = alphap;
+        tp.epsilon = epsilon;
+        tp.nmembthresh = nmembthresh;
+        tp.xlong = tp.ylong = tp.zlong = xlong;
+        tp.xlongs2 = tp.ylongs2 = tp.zlongs2 = xlong * 0.5;
+        TreeOutputs to;
+        create_group_tree(ti, tp, to);
+        if (liste_parts) std::memcpy(liste_parts, to.liste_parts.data(), sizeof(int32_t) * npart);
+        if (nnodes) *nnodes = to.nnodes;
+        if (real_table) std::memcpy(real_table, to.real_table.data(), sizeof(double) * to.real_table.size());
+        if (integer_table)
+            std::memcpy(integer_table, to.integer_table.data(), sizeof(int32_t) * to.integer_table.size());
+    });
+}
+
+}  // extern "C"

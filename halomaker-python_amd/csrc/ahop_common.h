@@ -1,0 +1,111 @@
+// ahop_common.h -- shared definitions of the MI355X AdaptaHOP hot path.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+
+namespace ahop {
+
+// Hot-path parameters (the subset of the reference's module state that the
+// kNN/density/hop/saddle stages read: compute_adaptahop.f90:682-881, :2433-2962).
+struct Params {
+    int nvoisins = 32;          // k of the kNN (self excluded)
+    int nhop = 16;              // neighbours scanned for the hop target / saddles
+    double rho_threshold = 80.0;
+    double xlong = 1.0, ylong = 1.0, zlong = 1.0;
+    int mixed = 1;              // 1: FP32 box-relative filter + FP64 re-rank, 0: FP64 throughout
+    int bucket = 32;            // particles per leaf bucket of the device tree (<= 64)
+};
+
+// Device tree built over Morton-sorted particles (binary radix tree, leaves = particles).
+// Internal node i in [0, n-2] covers sorted range [first[i], last[i]].
+// A child reference c >= 0 is internal; c < 0 encodes particle ~c.
+struct DeviceTree {
+    int64_t n = 0;
+    uint64_t *key = nullptr, *key_tmp = nullptr;
+    int32_t *perm = nullptr, *iota = nullptr;   // sorted slot -> original index
+    double4 *spos = nullptr;                    // sorted (x, y, z, mass)
+    int32_t *first = nullptr, *last = nullptr;
+    int32_t *left = nullptr, *right = nullptr;
+    int32_t *parent = nullptr, *leaf_parent = nullptr;
+    int32_t *flag = nullptr;
+    double *box = nullptr;                      // 6 doubles per internal node: lo xyz, hi xyz
+};
+
+// Per-stage device timings of the last run (milliseconds, HIP events on the run's stream).
+struct StageTimes {
+    float sort = 0, tree = 0, knn = 0, fallback = 0, hop = 0, saddle = 0, total = 0;
+};
+
+// Results of the hot path in original particle order, resident on the device.
+struct DeviceResults {
+    double *density = nullptr;        // [n]
+    int32_t *gid = nullptr;           // [n] 1-based group id, 0 = inactive
+    int32_t *hop = nullptr;           // [n] 0-based original index of hop target, -1 inactive
+    int32_t *nbr = nullptr;           // [n*nhop] sorted-slot neighbour ids (internal)
+    int64_t nactive = 0;
+    int32_t ngroups = 0;
+    int64_t nfallback = 0;
+    // saddle CSR (0-based offsets, 1-based group ids)
+    int32_t *sad_offset = nullptr;    // [ngroups+1]
+    int32_t *sad_isad = nullptr;
+    double *sad_rho = nullptr;
+    double *densityg = nullptr;       // [ngroups]
+    int64_t nsaddle = 0;
+};
+
+// Growable device allocation owned by an Engine.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    void *ensure(size_t nbytes);
+    void release();
+    ~DevBuf() { release(); }
+};
+
+class Engine {
+  public:
+    Engine();
+    ~Engine();
+    // Full hot path on device-resident inputs: pos (n,3) row-major doubles, mass (n).
+    // Fills results (device pointers valid until the next run or close).
+    void run(const double *d_pos, const double *d_mass, int64_t n, const Params &p, bool saddle);
+    const DeviceResults &results() const { return res_; }
+    const StageTimes &times() const { return times_; }
+    hipStream_t stream() const { return stream_; }
+    void release_all();
+
+  private:
+    void build_tree(const double *d_pos, const double *d_mass, int64_t n, const Params &p);
+    void knn_density(const Params &p);
+    void hop_groups(const Params &p);
+    void saddles(const Params &p);
+
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev_[8];
+    DeviceTree t_;
+    DeviceResults res_;
+    StageTimes times_;
+    // buffers
+    DevBuf b_key, b_key_tmp, b_perm, b_iota, b_spos, b_first, b_last, b_left, b_right, b_parent,
+        b_leaf_parent, b_flag, b_box, b_temp, b_rho_s, b_nbr, b_hop_s, b_gid_s, b_flag_o,
+        b_rank_o, b_density, b_gid, b_hop, b_counter, b_fallback, b_emit_cnt, b_emit_off,
+        b_ekey, b_eval, b_ekey2, b_eval2, b_ukey, b_uval, b_keep, b_keep_off, b_skey, b_sval,
+        b_sad_off, b_sad_isad, b_sad_rho, b_sad_cnt, b_densityg, b_misc, b_roots;
+    int64_t nmax_ = 0;
+};
+
+const char *last_error();
+void set_error(const std::string &msg);
+
+}  // namespace ahop
+
+#define AHOP_HIP(call)                                                                        \
+    do {                                                                                      \
+        hipError_t _e = (call);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(_e) +     \
+                                     " at " + __FILE__ + ":" + std::to_string(__LINE__));     \
+    } while (0)

@@ -1,0 +1,1044 @@
+// ahop_kernels.hip -- MI355X (gfx950) kernels of the AdaptaHOP hot path.
+//
+// Data layout in HBM (N particles):
+//   key[N] u64 Morton codes -> radix sort -> perm[N] (sorted slot -> original index)
+//   spos[N] double4 (x, y, z, mass) in sorted order (32 B, one aligned load per particle)
+//   binary radix tree over the sorted codes: first/last/left/right[N-1] int32,
+//   box[6*(N-1)] double (lo xyz, hi xyz) per internal node
+//   rho_s[N] f64, nbr[N*nhop] i32 (sorted slots of the first nhop neighbours)
+//
+// Reference semantics reproduced (fortran/compute_adaptahop.f90):
+//   kNN (walk_tree2 :2867-2962): self excluded by index, d2 = (dx*dx + dy*dy) + dz*dz with
+//     dx = min(|dx|, L-|dx|), k smallest kept (strict <)
+//   density (omp_compute_density :2816-2838, spline :3270-3287): sequential sum over the
+//     first k-1 neighbours in ascending distance, single-rounded pi
+//   hop (find_local_maxima :774-795): densest of {i} U first nhop neighbours, ties -> min index
+//   group ids (:797-861): root of the hop forest, roots numbered by ascending particle index
+//   saddle list (dev_compute_saddle_list :2433-2813): per (g1,g2) max over member pairs of
+//     min(rho_i, rho_j), kept when both directions exist, value = min of the two directions
+//
+// All FP64 arithmetic that decides a reference-visible value is compiled without
+// contraction (-ffp-contract=off) so it rounds exactly like the reference on x86-64.
+#include "ahop_common.h"
+
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_reduce_by_key.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
+
+namespace ahop {
+
+static thread_local std::string g_err;
+const char *last_error() { return g_err.c_str(); }
+void set_error(const std::string &m) { g_err = m; }
+
+void *DevBuf::ensure(size_t nbytes)
+{
+    if (nbytes <= bytes && p) return p;
+    release();
+    size_t nb = nbytes < 256 ? 256 : nbytes;
+    AHOP_HIP(hipMalloc(&p, nb));
+    bytes = nb;
+    return p;
+}
+void DevBuf::release()
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+}
+
+// single-rounded pi of the reference (f90:58: default-real literal in a real(8) parameter)
+__device__ __constant__ double PI_REF = 3.1415927410125732421875;
+
+constexpr int WAVE = 64;
+constexpr int STACK = 128;
+constexpr float U32 = 5.9604644775390625e-8f;   // 2^-24, unit roundoff of binary32
+
+// ------------------------------------------------------------------------------------------
+// small device helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t spread21(uint64_t x)
+{
+    x &= 0x1fffffULL;
+    x = (x | (x << 32)) & 0x1f00000000ffffULL;
+    x = (x | (x << 16)) & 0x1f0000ff0000ffULL;
+    x = (x | (x << 8)) & 0x100f00f00f00f00fULL;
+    x = (x | (x << 4)) & 0x10c30c30c30c30c3ULL;
+    x = (x | (x << 2)) & 0x1249249249249249ULL;
+    return x;
+}
+
+__device__ __forceinline__ double wrapd(double d, double L)
+{
+    double h = 0.5 * L;
+    if (d >= h) return d - L;
+    if (d < -h) return d + L;
+    return d;
+}
+
+// spline f90:3270-3287
+__device__ __forceinline__ double spline_ref(double x)
+{
+    if (x <= 1.0) {
+        double x2 = x * x;
+        double x3 = x2 * x;
+        return 1.0 - 1.5 * x2 + 0.75 * x3;
+    } else if (x <= 2.0) {
+        double x2 = 2.0 - x;
+        double x3 = x2 * x2 * x2;
+        return 0.25 * x3;
+    }
+    return 0.0;
+}
+
+// reference periodic distance of two absolute positions (walk_tree2 :2931-2941)
+__device__ __forceinline__ double d2_ref(double ax, double ay, double az, double bx, double by,
+                                         double bz, double Lx, double Ly, double Lz)
+{
+    double dx = fabs(bx - ax);
+    dx = fmin(dx, Lx - dx);
+    double d = dx * dx;
+    double dy = fabs(by - ay);
+    dy = fmin(dy, Ly - dy);
+    d = d + dy * dy;
+    double dz = fabs(bz - az);
+    dz = fmin(dz, Lz - dz);
+    d = d + dz * dz;
+    return d;
+}
+
+__device__ __forceinline__ int logical_block(int nblocks)
+{
+    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
+    // logical waves so that neighbouring query waves share their L2.
+    int b = blockIdx.x;
+    int g8 = nblocks / 8, rem = nblocks % 8;
+    int x = b % 8;
+    return x * g8 + min(x, rem) + b / 8;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_min(T v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o));
+    return v;
+}
+
+// ------------------------------------------------------------------------------------------
+// tree build
+// ------------------------------------------------------------------------------------------
+__global__ void k_morton(const double *__restrict__ pos, int64_t n, double Lx, double Ly, double Lz,
+                         uint64_t *__restrict__ key, int32_t *__restrict__ iota, int *__restrict__ bad)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = pos[3 * i], y = pos[3 * i + 1], z = pos[3 * i + 2];
+    bool ok = (x >= -0.5 * Lx) && (x < 0.5 * Lx) && (y >= -0.5 * Ly) && (y < 0.5 * Ly) &&
+              (z >= -0.5 * Lz) && (z < 0.5 * Lz);
+    if (!ok) atomicAdd(bad, 1);
+    const double S = 2097152.0;
+    double fx = (x / Lx + 0.5) * S, fy = (y / Ly + 0.5) * S, fz = (z / Lz + 0.5) * S;
+    uint64_t ix = (uint64_t)fmin(fmax(fx, 0.0), S - 1.0);
+    uint64_t iy = (uint64_t)fmin(fmax(fy, 0.0), S - 1.0);
+    uint64_t iz = (uint64_t)fmin(fmax(fz, 0.0), S - 1.0);
+    key[i] = (spread21(ix) << 2) | (spread21(iy) << 1) | spread21(iz);
+    iota[i] = (int32_t)i;
+}
+
+__global__ void k_gather(const double *__restrict__ pos, const double *__restrict__ mass,
+                         const int32_t *__restrict__ perm, int64_t n, double4 *__restrict__ spos)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    int64_t o = perm[s];
+    spos[s] = make_double4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], mass[o]);
+}
+
+__device__ __forceinline__ int delta(const uint64_t *__restrict__ k, int n, int i, int j)
+{
+    if (j < 0 || j >= n) return -1;
+    uint64_t a = k[i], b = k[j];
+    if (a != b) return __clzll((long long)(a ^ b));
+    return 64 + __clz((int)((uint32_t)i ^ (uint32_t)j));
+}
+
+// Binary radix tree (Karras 2012) over sorted keys; internal node i in [0, n-2].
+__global__ void k_radix_tree(const uint64_t *__restrict__ key, int n, int32_t *__restrict__ first,
+                             int32_t *__restrict__ last, int32_t *__restrict__ left,
+                             int32_t *__restrict__ right, int32_t *__restrict__ parent,
+                             int32_t *__restrict__ leaf_parent)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int d = (delta(key, n, i, i + 1) - delta(key, n, i, i - 1)) > 0 ? 1 : -1;
+    int dmin = delta(key, n, i, i - d);
+    int lmax = 2;
+    while (delta(key, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(key, n, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(key, n, i, j);
+    int s = 0, t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(key, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    int gamma = i + s * d + min(d, 0);
+    int f = min(i, j), la = max(i, j);
+    int lc = (f == gamma) ? ~gamma : gamma;
+    int rc = (la == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    first[i] = f;
+    last[i] = la;
+    left[i] = lc;
+    right[i] = rc;
+    if (lc >= 0) parent[lc] = i; else leaf_parent[~lc] = i;
+    if (rc >= 0) parent[rc] = i; else leaf_parent[~rc] = i;
+    if (i == 0) parent[0] = -1;
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ void child_box(int c, const double4 *__restrict__ spos,
+                                          const double *box, double b[6])
+{
+    if (c < 0) {
+        double4 p = spos[~c];
+        b[0] = b[3] = p.x;
+        b[1] = b[4] = p.y;
+        b[2] = b[5] = p.z;
+    } else {
+        for (int a = 0; a < 6; ++a) b[a] = box[6 * (int64_t)c + a];
+    }
+}
+
+// Bottom-up bounding boxes: the second thread to reach a node merges its children.
+__global__ void k_refit(int n, const double4 *__restrict__ spos, const int32_t *__restrict__ left,
+                        const int32_t *__restrict__ right, const int32_t *__restrict__ parent,
+                        const int32_t *__restrict__ leaf_parent, int32_t *flag, double *box)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    int node = leaf_parent[p];
+    while (node >= 0) {
+        __threadfence();
+        int old = atomicAdd(&flag[node], 1);
+        if (old == 0) return;
+        __threadfence();
+        double a[6], b[6];
+        child_box(left[node], spos, box, a);
+        child_box(right[node], spos, box, b);
+        for (int q = 0; q < 3; ++q) {
+            box[6 * (int64_t)node + q] = fmin(a[q], b[q]);
+            box[6 * (int64_t)node + q + 3] = fmax(a[q + 3], b[q + 3]);
+        }
+        node = parent[node];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// kNN + SPH density
+// ------------------------------------------------------------------------------------------
+struct KnnArgs {
+    const double4 *spos;
+    const int32_t *first, *last, *left, *right;
+    const double *box;
+    int n, K, nhop, bucket;
+    double Lx, Ly, Lz, xyzl, rho_thr;
+    const int32_t *qlist;   // optional explicit queries (sorted slots); null -> all slots
+    int nq;
+    double *rho_s;
+    int32_t *nbr;
+    int32_t *fb_list;       // mixed path: queries that need the FP64 path
+    int32_t *fb_count;
+    unsigned long long *nact;
+    int *overflow;
+    int nblocks;
+};
+
+// Heap of the KH smallest keys per lane, lane-interleaved in LDS (entry e of lane l at e*64+l,
+// so any per-lane entry index is bank-conflict free).
+template <typename KeyT>
+__device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int size, KeyT d,
+                                          int32_t id)
+{
+    int e = 0;
+    while (true) {
+        int c1 = 2 * e + 1;
+        if (c1 >= size) break;
+        KeyT kc = hk[c1 * WAVE + lane];
+        int ic = c1;
+        if (c1 + 1 < size) {
+            KeyT k2 = hk[(c1 + 1) * WAVE + lane];
+            if (k2 > kc) {
+                kc = k2;
+                ic = c1 + 1;
+            }
+        }
+        if (!(kc > d)) break;
+        hk[e * WAVE + lane] = kc;
+        hi[e * WAVE + lane] = hi[ic * WAVE + lane];
+        e = ic;
+    }
+    hk[e * WAVE + lane] = d;
+    hi[e * WAVE + lane] = id;
+}
+
+template <int KCAP, bool MIXED>
+__global__ __launch_bounds__(64) void k_knn(KnnArgs A)
+{
+    using KeyT = typename std::conditional<MIXED, float, double>::type;
+    constexpr int KHCAP = MIXED ? KCAP + 1 : KCAP;
+    __shared__ KeyT hk[KHCAP * WAVE];
+    __shared__ int32_t hi[KHCAP * WAVE];
+    __shared__ typename std::conditional<MIXED, float4, double4>::type stage[WAVE];
+    __shared__ int32_t stack[STACK];
+
+    const int lane = threadIdx.x;
+    const int wave = logical_block(A.nblocks);
+    const int slot = wave * WAVE + lane;
+    const bool active = slot < A.nq;
+    const int q = active ? (A.qlist ? A.qlist[slot] : slot) : 0;
+    const int K = A.K;
+    const int KH = MIXED ? K + 1 : K;
+    const double4 P = A.spos[q];
+    const double Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
+
+    // ---- query frame (mixed): centre of the wave's queries, box-relative FP32 coordinates
+    double cx = 0, cy = 0, cz = 0;
+    float qx = 0, qy = 0, qz = 0, hQ = 0;
+    bool wave_unsafe = false;
+    if constexpr (MIXED) {
+        const double BIG = 1e300;
+        double lx = wave_min(active ? P.x : BIG), hx = wave_max(active ? P.x : -BIG);
+        double ly = wave_min(active ? P.y : BIG), hy = wave_max(active ? P.y : -BIG);
+        double lz = wave_min(active ? P.z : BIG), hz = wave_max(active ? P.z : -BIG);
+        cx = 0.5 * (lx + hx);
+        cy = 0.5 * (ly + hy);
+        cz = 0.5 * (lz + hz);
+        double rx = wrapd(P.x - cx, Lx), ry = wrapd(P.y - cy, Ly), rz = wrapd(P.z - cz, Lz);
+        qx = (float)rx;
+        qy = (float)ry;
+        qz = (float)rz;
+        double hq = active ? fmax(fabs(rx), fmax(fabs(ry), fabs(rz))) : 0.0;
+        hq = wave_max(hq);
+        hQ = (float)hq * (1.0f + 4 * U32);
+        double Lmin = fmin(Lx, fmin(Ly, Lz));
+        wave_unsafe = hq >= 0.25 * Lmin;
+    }
+
+    for (int e = 0; e < KH; ++e) {
+        hk[e * WAVE + lane] = (KeyT)INFINITY;
+        hi[e * WAVE + lane] = -1;
+    }
+    KeyT tau = active ? (KeyT)INFINITY : (KeyT)-1;
+    if (wave_unsafe) tau = (KeyT)-1;
+
+    // per-lane visit test of a node / particle box
+    auto need_box = [&](int c) -> bool {
+        double b[6];
+        child_box(c, A.spos, A.box, b);
+        if constexpr (MIXED) {
+            float dd[3];
+            const double cq[3] = {cx, cy, cz};
+            const float qq[3] = {qx, qy, qz};
+            const double L[3] = {Lx, Ly, Lz};
+            for (int a = 0; a < 3; ++a) {
+                double cn = wrapd(0.5 * (b[a] + b[a + 3]) - cq[a], L[a]);
+                float cnf = (float)cn;
+                float hn = (float)(0.5 * (b[a + 3] - b[a])) * (1.0f + 2 * U32);
+                float Lf = (float)L[a];
+                float t = fabsf(qq[a] - cnf);
+                bool img = t > 0.5f * Lf;
+                float tw = img ? Lf - t : t;
+                float m = 4 * U32 * (fabsf(cnf) + t + hn + hQ + (img ? Lf : 0.0f));
+                float d = tw - hn - m;
+                dd[a] = d > 0.0f ? d : 0.0f;
+            }
+            float dist2 = (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]) * (1.0f - 4 * U32);
+            float st = sqrtf(tau) + hQ;
+            float E = 8 * U32 * st * st;
+            return dist2 < tau + 2 * E;
+        } else {
+            const double pq[3] = {P.x, P.y, P.z};
+            const double L[3] = {Lx, Ly, Lz};
+            double dist2 = 0;
+            for (int a = 0; a < 3; ++a) {
+                double cn = 0.5 * (b[a] + b[a + 3]);
+                double hn = 0.5 * (b[a + 3] - b[a]);
+                double t = fabs(pq[a] - cn);
+                double tw = fmin(t, L[a] - t);
+                double d = tw - hn - 1e-15 * L[a];
+                d = d > 0.0 ? d : 0.0;
+                dist2 = dist2 + d * d;
+            }
+            return dist2 < tau;
+        }
+    };
+
+    int sp = 0;
+    stack[0] = 0;
+    sp = 1;
+    const bool root_is_bucket = (A.n <= A.bucket);
+    while (sp > 0) {
+        int node = stack[--sp];
+        int f, l;
+        if (node < 0) f = l = ~node;
+        else {
+            f = A.first[node];
+            l = A.last[node];
+        }
+        bool need = active && need_box(node);
+        if (!__any(need)) continue;
+        if (l - f + 1 <= A.bucket || (root_is_bucket && node == 0)) {
+            // ---- bucket: stage candidates in LDS, test all, then insert accepted ones
+            int cnt = l - f + 1;
+            __syncthreads();
+            if (lane < cnt) {
+                double4 c = A.spos[f + lane];
+                if constexpr (MIXED) {
+                    float rx = (float)wrapd(c.x - cx, Lx);
+                    float ry = (float)wrapd(c.y - cy, Ly);
+                    float rz = (float)wrapd(c.z - cz, Lz);
+                    stage[lane] = make_float4(rx, ry, rz, 0.0f);
+                } else {
+                    stage[lane] = c;
+                }
+            }
+            __syncthreads();
+            uint64_t mask = 0;
+            for (int j = 0; j < cnt; ++j) {
+                auto c = stage[j];
+                KeyT d;
+                if constexpr (MIXED) {
+                    float dx = c.x - qx, dy = c.y - qy, dz = c.z - qz;
+                    d = dx * dx + dy * dy + dz * dz;
+                } else {
+                    d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+                }
+                if (d < tau && (f + j) != q) mask |= (1ull << j);
+            }
+            while (__any(mask != 0)) {
+                if (mask) {
+                    int j = __ffsll((long long)mask) - 1;
+                    mask &= mask - 1;
+                    auto c = stage[j];
+                    KeyT d;
+                    if constexpr (MIXED) {
+                        float dx = c.x - qx, dy = c.y - qy, dz = c.z - qz;
+                        d = dx * dx + dy * dy + dz * dz;
+                    } else {
+                        d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+                    }
+                    if (d < tau) {
+                        heap_sift<KeyT>(hk, hi, lane, KH, d, f + j);
+                        tau = hk[lane];
+                    }
+                }
+            }
+            continue;
+        }
+        int c1 = A.left[node], c2 = A.right[node];
+        bool n1 = active && need_box(c1);
+        bool n2 = active && need_box(c2);
+        int p1 = __popcll(__ballot(n1)), p2 = __popcll(__ballot(n2));
+        if (sp + 2 > STACK) {
+            if (lane == 0) atomicAdd(A.overflow, 1);
+            break;
+        }
+        if (p1 && p2) {
+            if (p1 >= p2) {
+                stack[sp++] = c2;
+                stack[sp++] = c1;
+            } else {
+                stack[sp++] = c1;
+                stack[sp++] = c2;
+            }
+        } else if (p1) {
+            stack[sp++] = c1;
+        } else if (p2) {
+            stack[sp++] = c2;
+        }
+    }
+
+    // ---- heap -> ascending order (in place heapsort, same trip count on every lane)
+    for (int end = KH - 1; end >= 1; --end) {
+        KeyT top = hk[lane];
+        int32_t tid = hi[lane];
+        KeyT mv = hk[end * WAVE + lane];
+        int32_t mid = hi[end * WAVE + lane];
+        hk[end * WAVE + lane] = top;
+        hi[end * WAVE + lane] = tid;
+        heap_sift<KeyT>(hk, hi, lane, end, mv, mid);
+    }
+    if (!active) return;
+
+    bool ok = !wave_unsafe;
+    if constexpr (MIXED) {
+        // certify that FP32 ranking cannot have changed the k-nearest set
+        float lk = hk[K * WAVE + lane], lk1 = hk[(K - 1) * WAVE + lane];
+        float st = sqrtf(lk) + hQ;
+        float E = 8 * U32 * st * st;
+        if (!(lk == INFINITY) && !(lk - lk1 > 2 * E && E < 0.01f * lk)) ok = false;
+    }
+    if (!ok) {
+        int at = atomicAdd(A.fb_count, 1);
+        A.fb_list[at] = q;
+        return;
+    }
+    // ---- density in FP64, reference order (omp_compute_density f90:2816-2838)
+    auto exact_d2 = [&](int i) -> double {
+        if constexpr (MIXED) {
+            double4 c = A.spos[hi[i * WAVE + lane]];
+            return d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+        } else {
+            return hk[i * WAVE + lane];
+        }
+    };
+    double d2k = exact_d2(K - 1);
+    double r = sqrt(d2k) * 0.5;
+    double r3 = r * r * r;
+    double unsr = 1.0 / r;
+    double contrib = 0.0;
+    double prev = -1.0;
+    for (int i = 0; i < K - 1; ++i) {
+        double d = exact_d2(i);
+        ok = ok && (d >= prev);
+        prev = d;
+        double m = A.spos[hi[i * WAVE + lane]].w;
+        contrib = contrib + m * spline_ref(sqrt(d) * unsr);
+    }
+    ok = ok && (d2k >= prev);
+    if (MIXED && !ok) {
+        int at = atomicAdd(A.fb_count, 1);
+        A.fb_list[at] = q;
+        return;
+    }
+    double rho = A.xyzl * (contrib + P.w) / (PI_REF * r3);
+    A.rho_s[q] = rho;
+    for (int t = 0; t < A.nhop; ++t) A.nbr[(int64_t)q * A.nhop + t] = hi[t * WAVE + lane];
+}
+
+// active-particle count (density > rho_threshold, f90:712-715)
+__global__ void k_count_active(const double *__restrict__ rho, int64_t n, double thr,
+                               unsigned long long *cnt)
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    bool a = i < n && rho[i] > thr;
+    unsigned long long b = __ballot(a);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, (unsigned long long)__popcll(b));
+}
+
+// ------------------------------------------------------------------------------------------
+// hop, group ids
+// ------------------------------------------------------------------------------------------
+__global__ void k_hop(const double *__restrict__ rho, const int32_t *__restrict__ nbr,
+                      const int32_t *__restrict__ perm, int64_t n, int nhop, double thr,
+                      int32_t *__restrict__ hop)
+{
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double densest = rho[q];
+    if (!(densest > thr)) {
+        hop[q] = -1;
+        return;
+    }
+    int32_t sel = (int32_t)q;
+    int32_t osel = perm[q];
+    for (int t = 0; t < nhop; ++t) {
+        int32_t j = nbr[q * nhop + t];
+        double rj = rho[j];
+        if (rj > densest) {
+            sel = j;
+            densest = rj;
+            osel = perm[j];
+        } else if (rj == densest) {
+            int32_t oj = perm[j];
+            if (oj < osel) {
+                sel = j;
+                osel = oj;
+            }
+        }
+    }
+    hop[q] = sel;
+}
+
+// one pointer-doubling round: out = in[in]; flags a change
+__global__ void k_jump(const int32_t *__restrict__ in, int32_t *__restrict__ out, int64_t n,
+                       int *changed)
+{
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int32_t p = in[q];
+    int32_t r = p;
+    if (p >= 0) {
+        r = in[p];
+        if (r != p) *changed = 1;
+    }
+    out[q] = r;
+}
+
+__global__ void k_root_flags(const int32_t *__restrict__ hop, const int32_t *__restrict__ perm,
+                             int64_t n, int32_t *__restrict__ flag_o)
+{
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    flag_o[perm[q]] = (hop[q] == q) ? 1 : 0;
+}
+
+__global__ void k_groups(const int32_t *__restrict__ root, const int32_t *__restrict__ hop,
+                         const int32_t *__restrict__ perm, const int32_t *__restrict__ rank_o,
+                         const double *__restrict__ rho_s, int64_t n, int32_t *__restrict__ gid_s,
+                         int32_t *__restrict__ gid_o, int32_t *__restrict__ hop_o,
+                         double *__restrict__ rho_o, double *__restrict__ densityg)
+{
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int32_t o = perm[q];
+    int32_t r = root[q];
+    int32_t g = 0;
+    if (r >= 0) {
+        g = rank_o[perm[r]] + 1;
+        if (r == q) densityg[g - 1] = rho_s[q];
+    }
+    gid_s[q] = g;
+    gid_o[o] = g;
+    hop_o[o] = (hop[q] >= 0) ? perm[hop[q]] : -1;
+    rho_o[o] = rho_s[q];
+}
+
+// ------------------------------------------------------------------------------------------
+// saddle edges
+// ------------------------------------------------------------------------------------------
+__global__ void k_emit_count(const int32_t *__restrict__ gid_s, const int32_t *__restrict__ nbr,
+                             int64_t n, int nhop, int32_t *__restrict__ cnt)
+{
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int32_t g1 = gid_s[q];
+    int c = 0;
+    if (g1 > 0)
+        for (int t = 0; t < nhop; ++t) {
+            int32_t g2 = gid_s[nbr[q * nhop + t]];
+            c += (g2 > 0 && g2 != g1);
+        }
+    cnt[q] = c;
+}
+
+__global__ void k_emit_fill(const int32_t *__restrict__ gid_s, const int32_t *__restrict__ nbr,
+                            const double *__restrict__ rho_s, const int32_t *__restrict__ off,
+                            int64_t n, int nhop, uint64_t *__restrict__ ekey,
+                            double *__restrict__ eval)
+{
+    int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    int32_t g1 = gid_s[q];
+    if (g1 <= 0) return;
+    int32_t o = off[q];
+    double r1 = rho_s[q];
+    for (int t = 0; t < nhop; ++t) {
+        int32_t j = nbr[q * nhop + t];
+        int32_t g2 = gid_s[j];
+        if (g2 > 0 && g2 != g1) {
+            double r2 = rho_s[j];
+            ekey[o] = ((uint64_t)(uint32_t)g1 << 32) | (uint32_t)g2;
+            eval[o] = r1 < r2 ? r1 : r2;
+            ++o;
+        }
+    }
+}
+
+__global__ void k_symmetrize(const uint64_t *__restrict__ ukey, const double *__restrict__ uval,
+                             const int32_t *__restrict__ nuniq_p, int32_t *__restrict__ keep,
+                             double *__restrict__ sval)
+{
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    int nu = *nuniq_p;
+    if (u >= nu) return;
+    uint64_t k = ukey[u];
+    uint64_t rk = (k << 32) | (k >> 32);
+    int lo = 0, hi = nu;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (ukey[mid] < rk) lo = mid + 1; else hi = mid;
+    }
+    bool found = lo < nu && ukey[lo] == rk;
+    keep[u] = found ? 1 : 0;
+    double a = uval[u];
+    sval[u] = found ? (uval[lo] < a ? uval[lo] : a) : a;
+}
+
+__global__ void k_compact(const uint64_t *__restrict__ ukey, const double *__restrict__ sval,
+                          const int32_t *__restrict__ keep, const int32_t *__restrict__ koff,
+                          const int32_t *__restrict__ nuniq_p, uint32_t *__restrict__ g1_out,
+                          int32_t *__restrict__ g2_out, double *__restrict__ s_out)
+{
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= *nuniq_p || !keep[u]) return;
+    int o = koff[u];
+    g1_out[o] = (uint32_t)(ukey[u] >> 32);
+    g2_out[o] = (int32_t)(uint32_t)(ukey[u] & 0xffffffffu);
+    s_out[o] = sval[u];
+}
+
+__global__ void k_seg_offsets(const uint32_t *__restrict__ g1, int64_t ne, int ngroups,
+                              int32_t *__restrict__ off)
+{
+    int g = blockIdx.x * blockDim.x + threadIdx.x;   // offset for group g+1 (1-based), g <= ngroups
+    if (g > ngroups) return;
+    uint32_t target = (uint32_t)(g + 1);
+    int64_t lo = 0, hi = ne;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (g1[mid] < target) lo = mid + 1; else hi = mid;
+    }
+    off[g] = (int32_t)lo;
+}
+
+// ------------------------------------------------------------------------------------------
+// host orchestration
+// ------------------------------------------------------------------------------------------
+struct MaxOp {
+    __host__ __device__ double operator()(double a, double b) const { return a > b ? a : b; }
+};
+
+static inline unsigned nblk(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+Engine::Engine()
+{
+    AHOP_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    for (auto &e : ev_) AHOP_HIP(hipEventCreate(&e));
+}
+
+Engine::~Engine()
+{
+    release_all();
+    for (auto &e : ev_) (void)hipEventDestroy(e);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::release_all()
+{
+    DevBuf *all[] = {&b_key, &b_key_tmp, &b_perm, &b_iota, &b_spos, &b_first, &b_last, &b_left,
+                     &b_right, &b_parent, &b_leaf_parent, &b_flag, &b_box, &b_temp, &b_rho_s,
+                     &b_nbr, &b_hop_s, &b_gid_s, &b_flag_o, &b_rank_o, &b_density, &b_gid, &b_hop,
+                     &b_counter, &b_fallback, &b_emit_cnt, &b_emit_off, &b_ekey, &b_eval,
+                     &b_ekey2, &b_eval2, &b_ukey, &b_uval, &b_keep, &b_keep_off, &b_skey,
+                     &b_sval, &b_sad_off, &b_sad_isad, &b_sad_rho, &b_sad_cnt, &b_densityg,
+                     &b_misc, &b_roots};
+    for (auto *b : all) b->release();
+    res_ = DeviceResults{};
+}
+
+template <typename T>
+static T *as(DevBuf &b, size_t count)
+{
+    return (T *)b.ensure(sizeof(T) * (count ? count : 1));
+}
+
+void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, const Params &p)
+{
+    hipStream_t s = stream_;
+    t_.n = n;
+    t_.key = as<uint64_t>(b_key, n);
+    t_.key_tmp = as<uint64_t>(b_key_tmp, n);
+    t_.iota = as<int32_t>(b_iota, n);
+    t_.perm = as<int32_t>(b_perm, n);
+    t_.spos = as<double4>(b_spos, n);
+    int *misc = as<int>(b_misc, 16);
+    AHOP_HIP(hipMemsetAsync(misc, 0, 16 * sizeof(int), s));
+    hipLaunchKernelGGL(k_morton, dim3(nblk(n)), dim3(256), 0, s, d_pos, n, p.xlong, p.ylong,
+                       p.zlong, t_.key_tmp, t_.iota, misc);
+    int bad = 0;
+    AHOP_HIP(hipMemcpyAsync(&bad, misc, sizeof(int), hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+    if (bad != 0)
+        throw std::runtime_error("positions must be finite and inside [-L/2, L/2) on every axis (" +
+                                 std::to_string(bad) + " particles are not)");
+    size_t tb = 0;
+    AHOP_HIP(rocprim::radix_sort_pairs(nullptr, tb, t_.key_tmp, t_.key, t_.iota, t_.perm,
+                                       (size_t)n, 0, 63, s));
+    void *temp = b_temp.ensure(tb);
+    AHOP_HIP(rocprim::radix_sort_pairs(temp, tb, t_.key_tmp, t_.key, t_.iota, t_.perm, (size_t)n,
+                                       0, 63, s));
+    hipLaunchKernelGGL(k_gather, dim3(nblk(n)), dim3(256), 0, s, d_pos, d_mass, t_.perm, n,
+                       t_.spos);
+    AHOP_HIP(hipEventRecord(ev_[1], s));
+    int64_t ni = n - 1;
+    t_.first = as<int32_t>(b_first, ni);
+    t_.last = as<int32_t>(b_last, ni);
+    t_.left = as<int32_t>(b_left, ni);
+    t_.right = as<int32_t>(b_right, ni);
+    t_.parent = as<int32_t>(b_parent, ni);
+    t_.leaf_parent = as<int32_t>(b_leaf_parent, n);
+    t_.flag = as<int32_t>(b_flag, ni);
+    t_.box = as<double>(b_box, 6 * ni);
+    hipLaunchKernelGGL(k_radix_tree, dim3(nblk(ni)), dim3(256), 0, s, t_.key, (int)n, t_.first,
+                       t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent);
+    hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ni)), dim3(256), 0, s, t_.flag, ni, 0);
+    hipLaunchKernelGGL(k_refit, dim3(nblk(n)), dim3(256), 0, s, (int)n, t_.spos, t_.left,
+                       t_.right, t_.parent, t_.leaf_parent, t_.flag, t_.box);
+    AHOP_HIP(hipEventRecord(ev_[2], s));
+}
+
+template <int KCAP>
+static void launch_knn(bool mixed, const KnnArgs &a, unsigned nb, hipStream_t s)
+{
+    if (mixed) hipLaunchKernelGGL((k_knn<KCAP, true>), dim3(nb), dim3(64), 0, s, a);
+    else hipLaunchKernelGGL((k_knn<KCAP, false>), dim3(nb), dim3(64), 0, s, a);
+}
+
+static void dispatch_knn(int K, bool mixed, const KnnArgs &a, unsigned nb, hipStream_t s)
+{
+    if (K <= 16) launch_knn<16>(mixed, a, nb, s);
+    else if (K <= 32) launch_knn<32>(mixed, a, nb, s);
+    else if (K <= 64) launch_knn<64>(mixed, a, nb, s);
+    else launch_knn<128>(mixed, a, nb, s);
+}
+
+void Engine::knn_density(const Params &p)
+{
+    hipStream_t s = stream_;
+    const int64_t n = t_.n;
+    double *rho_s = as<double>(b_rho_s, n);
+    int32_t *nbr = as<int32_t>(b_nbr, n * p.nhop);
+    int32_t *fb = as<int32_t>(b_fallback, n);
+    int *misc = (int *)b_misc.p;  // [0] bad, [1] overflow, [2] fb count, [3] changed
+    unsigned long long *cnt = as<unsigned long long>(b_counter, 4);
+    AHOP_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), s));
+    KnnArgs a{};
+    a.spos = t_.spos;
+    a.first = t_.first;
+    a.last = t_.last;
+    a.left = t_.left;
+    a.right = t_.right;
+    a.box = t_.box;
+    a.n = (int)n;
+    a.K = p.nvoisins;
+    a.nhop = p.nhop;
+    a.bucket = p.bucket;
+    a.Lx = p.xlong;
+    a.Ly = p.ylong;
+    a.Lz = p.zlong;
+    a.xyzl = p.xlong * p.ylong * p.zlong;
+    a.rho_thr = p.rho_threshold;
+    a.qlist = nullptr;
+    a.nq = (int)n;
+    a.rho_s = rho_s;
+    a.nbr = nbr;
+    a.fb_list = fb;
+    a.fb_count = misc + 2;
+    a.nact = cnt;
+    a.overflow = misc + 1;
+    unsigned nb = (unsigned)((n + 63) / 64);
+    a.nblocks = (int)nb;
+    dispatch_knn(p.nvoisins, p.mixed != 0, a, nb, s);
+    AHOP_HIP(hipEventRecord(ev_[3], s));
+    int host_misc[4];
+    AHOP_HIP(hipMemcpyAsync(host_misc, misc, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+    if (host_misc[1] != 0) throw std::runtime_error("device tree deeper than the traversal stack");
+    res_.nfallback = host_misc[2];
+    if (host_misc[2] > 0) {
+        KnnArgs b = a;
+        b.qlist = fb;
+        b.nq = host_misc[2];
+        unsigned nb2 = (unsigned)((b.nq + 63) / 64);
+        b.nblocks = (int)nb2;
+        dispatch_knn(p.nvoisins, false, b, nb2, s);
+    }
+    AHOP_HIP(hipEventRecord(ev_[4], s));
+    hipLaunchKernelGGL(k_count_active, dim3(nblk(n)), dim3(256), 0, s, rho_s, n, p.rho_threshold,
+                       cnt);
+}
+
+void Engine::hop_groups(const Params &p)
+{
+    hipStream_t s = stream_;
+    const int64_t n = t_.n;
+    double *rho_s = (double *)b_rho_s.p;
+    int32_t *nbr = (int32_t *)b_nbr.p;
+    int32_t *hop_s = as<int32_t>(b_hop_s, n);
+    int32_t *root = as<int32_t>(b_roots, n);
+    int32_t *tmp = as<int32_t>(b_gid_s, n);
+    int *misc = (int *)b_misc.p;
+    hipLaunchKernelGGL(k_hop, dim3(nblk(n)), dim3(256), 0, s, rho_s, nbr, t_.perm, n, p.nhop,
+                       p.rho_threshold, hop_s);
+    AHOP_HIP(hipMemcpyAsync(root, hop_s, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    // pointer doubling until every active particle points at its root
+    for (int round = 0; round < 64; ++round) {
+        AHOP_HIP(hipMemsetAsync(misc + 3, 0, sizeof(int), s));
+        hipLaunchKernelGGL(k_jump, dim3(nblk(n)), dim3(256), 0, s, root, tmp, n, misc + 3);
+        std::swap(root, tmp);
+        int changed = 0;
+        AHOP_HIP(hipMemcpyAsync(&changed, misc + 3, sizeof(int), hipMemcpyDeviceToHost, s));
+        AHOP_HIP(hipStreamSynchronize(s));
+        if (!changed) break;
+    }
+    int32_t *flag_o = as<int32_t>(b_flag_o, n);
+    int32_t *rank_o = as<int32_t>(b_rank_o, n);
+    hipLaunchKernelGGL(k_root_flags, dim3(nblk(n)), dim3(256), 0, s, hop_s, t_.perm, n, flag_o);
+    size_t tb = 0;
+    AHOP_HIP(rocprim::exclusive_scan(nullptr, tb, flag_o, rank_o, 0, (size_t)n,
+                                     rocprim::plus<int32_t>(), s));
+    void *temp = b_temp.ensure(tb);
+    AHOP_HIP(rocprim::exclusive_scan(temp, tb, flag_o, rank_o, 0, (size_t)n,
+                                     rocprim::plus<int32_t>(), s));
+    int32_t last2[2];
+    AHOP_HIP(hipMemcpyAsync(&last2[0], rank_o + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipMemcpyAsync(&last2[1], flag_o + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    unsigned long long nact = 0;
+    AHOP_HIP(hipMemcpyAsync(&nact, b_counter.p, sizeof(nact), hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+    res_.ngroups = last2[0] + last2[1];
+    res_.nactive = (int64_t)nact;
+    int32_t *gid_s = as<int32_t>(b_gid_s, n);  // tmp no longer needed (root holds the result)
+    if (gid_s == root) {
+        // root lives in b_gid_s after an odd number of swaps: move it out first
+        AHOP_HIP(hipMemcpyAsync(as<int32_t>(b_keep, n), root, n * sizeof(int32_t),
+                                hipMemcpyDeviceToDevice, s));
+        root = (int32_t *)b_keep.p;
+    }
+    res_.density = as<double>(b_density, n);
+    res_.gid = as<int32_t>(b_gid, n);
+    res_.hop = as<int32_t>(b_hop, n);
+    res_.densityg = as<double>(b_densityg, res_.ngroups);
+    res_.nbr = nbr;
+    hipLaunchKernelGGL(k_groups, dim3(nblk(n)), dim3(256), 0, s, root, hop_s, t_.perm, rank_o,
+                       rho_s, n, gid_s, res_.gid, res_.hop, res_.density, res_.densityg);
+    AHOP_HIP(hipEventRecord(ev_[5], s));
+}
+
+void Engine::saddles(const Params &p)
+{
+    hipStream_t s = stream_;
+    const int64_t n = t_.n;
+    const int G = res_.ngroups;
+    int32_t *gid_s = (int32_t *)b_gid_s.p;
+    double *rho_s = (double *)b_rho_s.p;
+    int32_t *nbr = (int32_t *)b_nbr.p;
+    int32_t *cnt = as<int32_t>(b_emit_cnt, n);
+    int32_t *off = as<int32_t>(b_emit_off, n);
+    hipLaunchKernelGGL(k_emit_count, dim3(nblk(n)), dim3(256), 0, s, gid_s, nbr, n, p.nhop, cnt);
+    size_t tb = 0;
+    AHOP_HIP(rocprim::exclusive_scan(nullptr, tb, cnt, off, 0, (size_t)n, rocprim::plus<int32_t>(), s));
+    AHOP_HIP(rocprim::exclusive_scan(b_temp.ensure(tb), tb, cnt, off, 0, (size_t)n,
+                                     rocprim::plus<int32_t>(), s));
+    int32_t tail[2];
+    AHOP_HIP(hipMemcpyAsync(&tail[0], off + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipMemcpyAsync(&tail[1], cnt + n - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+    const int64_t ne = (int64_t)tail[0] + tail[1];
+    res_.sad_offset = as<int32_t>(b_sad_off, G + 1);
+    if (ne == 0 || G == 0) {
+        AHOP_HIP(hipMemsetAsync(res_.sad_offset, 0, (G + 1) * sizeof(int32_t), s));
+        res_.sad_isad = as<int32_t>(b_sad_isad, 1);
+        res_.sad_rho = as<double>(b_sad_rho, 1);
+        res_.nsaddle = 0;
+        AHOP_HIP(hipEventRecord(ev_[6], s));
+        return;
+    }
+    uint64_t *ekey = as<uint64_t>(b_ekey, ne);
+    double *eval = as<double>(b_eval, ne);
+    uint64_t *ekey2 = as<uint64_t>(b_ekey2, ne);
+    double *eval2 = as<double>(b_eval2, ne);
+    hipLaunchKernelGGL(k_emit_fill, dim3(nblk(n)), dim3(256), 0, s, gid_s, nbr, rho_s, off, n,
+                       p.nhop, ekey, eval);
+    unsigned gbits = 1;
+    while ((1ull << gbits) <= (unsigned long long)G) ++gbits;
+    unsigned end_bit = 32 + gbits;
+    tb = 0;
+    AHOP_HIP(rocprim::radix_sort_pairs(nullptr, tb, ekey, ekey2, eval, eval2, (size_t)ne, 0,
+                                       end_bit, s));
+    AHOP_HIP(rocprim::radix_sort_pairs(b_temp.ensure(tb), tb, ekey, ekey2, eval, eval2,
+                                       (size_t)ne, 0, end_bit, s));
+    uint64_t *ukey = as<uint64_t>(b_ukey, ne);
+    double *uval = as<double>(b_uval, ne);
+    int32_t *nuniq = (int32_t *)b_misc.p + 8;
+    tb = 0;
+    MaxOp maxop;
+    AHOP_HIP(rocprim::reduce_by_key(nullptr, tb, ekey2, eval2, (size_t)ne, ukey, uval, nuniq,
+                                    maxop, rocprim::equal_to<uint64_t>(), s));
+    AHOP_HIP(rocprim::reduce_by_key(b_temp.ensure(tb), tb, ekey2, eval2, (size_t)ne, ukey, uval,
+                                    nuniq, maxop, rocprim::equal_to<uint64_t>(), s));
+    int32_t *keep = as<int32_t>(b_keep, ne);
+    int32_t *koff = as<int32_t>(b_keep_off, ne);
+    double *sval = as<double>(b_sval, ne);
+    hipLaunchKernelGGL(k_symmetrize, dim3(nblk(ne)), dim3(256), 0, s, ukey, uval, nuniq, keep,
+                       sval);
+    int32_t hnu = 0;
+    AHOP_HIP(hipMemcpyAsync(&hnu, nuniq, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+    tb = 0;
+    AHOP_HIP(rocprim::exclusive_scan(nullptr, tb, keep, koff, 0, (size_t)hnu, rocprim::plus<int32_t>(), s));
+    AHOP_HIP(rocprim::exclusive_scan(b_temp.ensure(tb), tb, keep, koff, 0, (size_t)hnu,
+                                     rocprim::plus<int32_t>(), s));
+    int32_t kt[2];
+    AHOP_HIP(hipMemcpyAsync(&kt[0], koff + hnu - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipMemcpyAsync(&kt[1], keep + hnu - 1, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+    const int64_t nk = (int64_t)kt[0] + kt[1];
+    uint32_t *g1 = (uint32_t *)as<int32_t>(b_sad_cnt, nk);
+    int32_t *g2 = as<int32_t>(b_emit_cnt, nk);       // emission counts are no longer needed
+    double *sv = as<double>(b_eval, nk);
+    hipLaunchKernelGGL(k_compact, dim3(nblk(hnu)), dim3(256), 0, s, ukey, sval, keep, koff, nuniq,
+                       g1, g2, sv);
+    hipLaunchKernelGGL(k_seg_offsets, dim3(nblk(G + 1)), dim3(256), 0, s, g1, nk, G,
+                       res_.sad_offset);
+    res_.sad_isad = as<int32_t>(b_sad_isad, nk);
+    res_.sad_rho = as<double>(b_sad_rho, nk);
+    tb = 0;
+    AHOP_HIP(rocprim::segmented_radix_sort_pairs_desc(nullptr, tb, sv, res_.sad_rho, g2,
+                                                      res_.sad_isad, (unsigned)nk, (unsigned)G,
+                                                      res_.sad_offset, res_.sad_offset + 1, 0,
+                                                      64, s));
+    AHOP_HIP(rocprim::segmented_radix_sort_pairs_desc(b_temp.ensure(tb), tb, sv, res_.sad_rho, g2,
+                                                      res_.sad_isad, (unsigned)nk, (unsigned)G,
+                                                      res_.sad_offset, res_.sad_offset + 1, 0,
+                                                      64, s));
+    res_.nsaddle = nk;
+    AHOP_HIP(hipEventRecord(ev_[6], s));
+}
+
+void Engine::run(const double *d_pos, const double *d_mass, int64_t n, const Params &p, bool saddle)
+{
+    if (n < 2) throw std::runtime_error("need at least 2 particles");
+    if (n >= ((int64_t)1 << 30)) throw std::runtime_error("particle count must be < 2^30");
+    if (p.nvoisins < 1 || p.nvoisins >= n) throw std::runtime_error("need 1 <= nvoisins < npart");
+    if (p.nvoisins > 128) throw std::runtime_error("nvoisins > 128 is not supported");
+    if (p.nhop < 0 || p.nhop > p.nvoisins) throw std::runtime_error("need 0 <= nhop <= nvoisins");
+    if (p.bucket < 1 || p.bucket > 64) throw std::runtime_error("bucket must be in [1, 64]");
+    if (!(p.xlong > 0 && p.ylong > 0 && p.zlong > 0)) throw std::runtime_error("box sides must be > 0");
+    times_ = StageTimes{};
+    AHOP_HIP(hipEventRecord(ev_[0], stream_));
+    build_tree(d_pos, d_mass, n, p);
+    knn_density(p);
+    hop_groups(p);
+    if (saddle) saddles(p);
+    else AHOP_HIP(hipEventRecord(ev_[6], stream_));
+    AHOP_HIP(hipStreamSynchronize(stream_));
+    AHOP_HIP(hipEventElapsedTime(&times_.sort, ev_[0], ev_[1]));
+    AHOP_HIP(hipEventElapsedTime(&times_.tree, ev_[1], ev_[2]));
+    AHOP_HIP(hipEventElapsedTime(&times_.knn, ev_[2], ev_[3]));
+    AHOP_HIP(hipEventElapsedTime(&times_.fallback, ev_[3], ev_[4]));
+    AHOP_HIP(hipEventElapsedTime(&times_.hop, ev_[4], ev_[5]));
+    AHOP_HIP(hipEventElapsedTime(&times_.saddle, ev_[5], ev_[6]));
+    AHOP_HIP(hipEventElapsedTime(&times_.total, ev_[0], ev_[6]));
+}
+
+}  // namespace ahop

@@ -1,0 +1,459 @@
+// create_nodes.cpp -- host stage of AdaptaHOP: the structure tree built from the
+// device results (group ids, densities, saddle CSR).  It stays on the host, as in
+// the reference, and reproduces:
+//   create_group_tree             compute_adaptahop.f90:885-983
+//   treat_particles(_linkonly)    :2317-2429
+//   dev_do_colorize_local         :1188-1242
+//   omp_dev_create_nodes_task     :1245-1628
+//   reorder_nodes_deterministic   :987-1131
+//   node2table                    :550-573
+// Child subtrees are processed depth first on one thread.  The reference runs them as
+// OpenMP tasks whose only scheduling-dependent effect is the pre-reorder node numbering,
+// which reorder_nodes_deterministic normalises, so the result is the same.
+// All sums run in the reference's linked-list order, so the tables match bit for bit.
+#include "create_nodes.h"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace ahop {
+
+namespace {
+
+struct Node {
+    int32_t mass = 0, level = 0, mother = 0, firstchild = 0, nsisters = 0, sister = 0;
+    double rho_saddle = 0, density = 0, densmax = 0, radius = 0, truemass = 0;
+    double position[3] = {0, 0, 0};
+};
+
+struct Builder {
+    const TreeInputs &in;
+    const TreeParams &p;
+    int64_t npart;
+    int32_t ngroups;
+    // 1-based arrays mirroring the reference module state
+    std::vector<int32_t> firstpart, idpart_adapt, liste_parts;
+    std::vector<int32_t> nhnei, offset1;   // dev_group_nhnei (truncated while colouring), offsets
+    std::vector<int32_t> idgroup, igroupid, idgroup_tmp, color;
+    std::vector<Node> node;
+    int32_t nnodes = 0, nnodesmax = 0;
+
+    Builder(const TreeInputs &i, const TreeParams &pp) : in(i), p(pp)
+    {
+        npart = in.npart;
+        ngroups = in.ngroups;
+    }
+
+    double pos(int32_t ipar, int d) const { return in.pos[3 * (int64_t)(ipar - 1) + d]; }
+    double dens(int32_t ipar) const { return in.density[ipar - 1]; }
+    double densityg(int32_t g) const { return in.densityg[g - 1]; }
+    int32_t isad(int32_t idx) const { return in.sad_isad[idx - 1]; }         // 1-based idx
+    double rho_sad(int32_t idx) const { return in.sad_rho[idx - 1]; }
+
+    // member linked lists: find_local_maxima f90:847-861 (prepend in index order)
+    void link_members()
+    {
+        firstpart.assign(ngroups + 1, 0);
+        idpart_adapt.assign(npart + 1, 0);
+        for (int64_t ipar = 1; ipar <= npart; ++ipar) {
+            if (in.density[ipar - 1] <= p.rho_threshold) continue;
+            int32_t g = in.gid[ipar - 1];
+            if (g > 0) {
+                idpart_adapt[ipar] = firstpart[g];
+                firstpart[g] = (int32_t)ipar;
+            }
+        }
+    }
+
+    // treat_particles_linkonly f90:2406-2429
+    void treat_linkonly(int32_t igroup, double rhot)
+    {
+        int32_t ipar = firstpart[igroup], iparold = 0;
+        bool first_good = false;
+        while (ipar > 0) {
+            if (dens(ipar) > rhot) {
+                if (!first_good) {
+                    first_good = true;
+                    firstpart[igroup] = ipar;
+                } else {
+                    idpart_adapt[iparold] = ipar;
+                }
+                iparold = ipar;
+            }
+            ipar = idpart_adapt[ipar];
+        }
+        if (!first_good) firstpart[igroup] = 0;
+    }
+
+    // treat_particles f90:2317-2401
+    void treat(int32_t igroup, double rhot, double posg[3], int32_t &imass, int32_t &igroupref,
+               double posref[3], double &rsquare, double &densmoy, double &truemass)
+    {
+        imass = 0;
+        truemass = 0.0;
+        rsquare = 0.0;
+        densmoy = 0.0;
+        posg[0] = posg[1] = posg[2] = 0.0;
+        int32_t ipar = firstpart[igroup], iparold = 0;
+        bool first_good = false;
+        double densmax = 0, densmin = 0;
+        const double L[3] = {p.xlong, p.ylong, p.zlong};
+        const double Ls2[3] = {p.xlongs2, p.ylongs2, p.zlongs2};
+        while (ipar > 0) {
+            double density_ipar = dens(ipar);
+            if (density_ipar > rhot) {
+                if (!first_good) {
+                    if (igroupref == 0) {
+                        for (int d = 0; d < 3; ++d) posref[d] = pos(ipar, d);
+                        igroupref = igroup;
+                    }
+                    first_good = true;
+                    firstpart[igroup] = ipar;
+                    densmin = density_ipar;
+                    densmax = densmin;
+                } else {
+                    idpart_adapt[iparold] = ipar;
+                }
+                iparold = ipar;
+                imass = imass + 1;
+                double xmasspart = in.mass[ipar - 1];
+                truemass = truemass + xmasspart;
+                double pd[3];
+                for (int d = 0; d < 3; ++d) {
+                    pd[d] = pos(ipar, d) - posref[d];
+                    if (pd[d] >= Ls2[d]) pd[d] = pd[d] - L[d];
+                    else if (pd[d] < -Ls2[d]) pd[d] = pd[d] + L[d];
+                }
+                for (int d = 0; d < 3; ++d) pd[d] = pd[d] + posref[d];
+                posg[0] = posg[0] + pd[0] * xmasspart;
+                posg[1] = posg[1] + pd[1] * xmasspart;
+                posg[2] = posg[2] + pd[2] * xmasspart;
+                rsquare = rsquare + xmasspart * (pd[0] * pd[0] + pd[1] * pd[1] + pd[2] * pd[2]);
+                densmoy = densmoy + density_ipar;
+                densmax = std::max(densmax, density_ipar);
+                densmin = std::min(densmin, density_ipar);
+            }
+            ipar = idpart_adapt[ipar];
+        }
+        if (first_good) idpart_adapt[iparold] = 0;
+        else firstpart[igroup] = 0;
+        if (!first_good || densmin <= rhot || densmax != densityg(igroup))
+            throw std::runtime_error("treat_particles: group " + std::to_string(igroup) +
+                                     " is inconsistent with its density maximum");
+    }
+
+    // dev_do_colorize_local f90:1188-1242
+    void colorize(int32_t igroup, int32_t igr, double rhot, std::vector<int32_t> &queue,
+                  int32_t color_id)
+    {
+        queue[0] = igroup;
+        size_t front = 0, back = 1;
+        color[igr] = color_id;
+        while (front < back) {
+            int32_t g = queue[front++];
+            int32_t neig = nhnei[g];
+            int32_t base = offset1[g];
+            for (int32_t ineig = 1; ineig <= neig; ++ineig) {
+                int32_t idx = base + ineig - 1;
+                if (rho_sad(idx) > rhot) {
+                    int32_t g2 = isad(idx);
+                    int32_t igr2 = igroupid[g2];
+                    if (color[igr2] == 0) {
+                        color[igr2] = color_id;
+                        if (back >= queue.size())
+                            throw std::runtime_error("colorize: queue overflow");
+                        queue[back++] = g2;
+                    } else if (color[igr2] != color_id) {
+                        throw std::runtime_error(
+                            "colorize: saddle connections are not symmetric");
+                    }
+                } else {
+                    nhnei[g] = ineig - 1;
+                    break;
+                }
+            }
+        }
+    }
+
+    int32_t reserve(int32_t nnew)
+    {
+        int32_t first = nnodes + 1;
+        nnodes += nnew;
+        if (nnodes > nnodesmax) throw std::runtime_error("create nodes: nnodes > nnodesmax");
+        return first;
+    }
+
+    // omp_dev_create_nodes_task f90:1245-1628, run depth first
+    void create_nodes(double rhot0, int32_t inode0, int32_t igr1_0, int32_t igr2_0, bool do_paint)
+    {
+        double rhot = rhot0;
+        int32_t inode = inode0, igr1 = igr1_0, igr2 = igr2_0;
+        if (do_paint) {
+            for (int32_t igr = igr1; igr <= igr2; ++igr) {
+                int32_t ipar = firstpart[idgroup[igr]];
+                while (ipar > 0) {
+                    liste_parts[ipar] = inode;
+                    ipar = idpart_adapt[ipar];
+                }
+            }
+        }
+        const double L[3] = {p.xlong, p.ylong, p.zlong};
+        const double Ls2[3] = {p.xlongs2, p.ylongs2, p.zlongs2};
+        for (;;) {
+            std::vector<int32_t> queue(std::max(igr2 - igr1 + 1, 1));
+            for (int32_t igr = igr1; igr <= igr2; ++igr) color[igr] = 0;
+            int32_t ncolors = 0;
+            for (int32_t igr = igr1; igr <= igr2; ++igr) {
+                if (color[igr] == 0) {
+                    ++ncolors;
+                    colorize(idgroup[igr], igr, rhot, queue, ncolors);
+                }
+            }
+            std::vector<int32_t> igrpos(ncolors + 1, 0), igrinc(ncolors + 1, 0);
+            igrpos[0] = igr1 - 1;
+            for (int32_t igr = igr1; igr <= igr2; ++igr)
+                if (densityg(idgroup[igr]) > rhot) igrpos[color[igr]]++;
+            for (int32_t c = 1; c <= ncolors; ++c) igrpos[c] = igrpos[c - 1] + igrpos[c];
+            for (int32_t igr = igr1; igr <= igr2; ++igr) {
+                int32_t c = color[igr];
+                int32_t g = idgroup[igr];
+                if (densityg(g) > rhot) {
+                    igrinc[c]++;
+                    int32_t igr_eff = igrinc[c] + igrpos[c - 1];
+                    idgroup_tmp[igr_eff] = g;
+                    igroupid[g] = igr_eff;
+                }
+            }
+            igr2 = igrpos[ncolors];
+            for (int32_t igr = igr1; igr <= igr2; ++igr) idgroup[igr] = idgroup_tmp[igr];
+            int32_t inc_color_tot = 0;
+            for (int32_t c = 1; c <= ncolors; ++c)
+                if (igrinc[c] > 0) ++inc_color_tot;
+            std::vector<int32_t> igrposnew(inc_color_tot + 1);
+            igrposnew[0] = igrpos[0];
+            inc_color_tot = 0;
+            for (int32_t c = 1; c <= ncolors; ++c)
+                if (igrinc[c] > 0) igrposnew[++inc_color_tot] = igrpos[c];
+
+            std::vector<double> posgg(3 * (inc_color_tot + 1)), truemassg(inc_color_tot + 1),
+                densmaxg(inc_color_tot + 1), rsquare(inc_color_tot + 1),
+                densmoy(inc_color_tot + 1);
+            std::vector<int32_t> massg(inc_color_tot + 1), mass_compg(inc_color_tot + 1),
+                ok_color(inc_color_tot + 1);
+            std::vector<char> ifok(inc_color_tot + 1, 0);
+            double posref[3] = {0, 0, 0};
+            for (int32_t ic = 1; ic <= inc_color_tot; ++ic) {
+                double posg[3] = {0, 0, 0};
+                int32_t mass_loc = 0;
+                double truemass = 0.0, rsquareg = 0.0, densmoyg = 0.0;
+                int32_t igr1out = igrposnew[ic - 1] + 1, igr2out = igrposnew[ic];
+                double densmaxgroup = -1.0;
+                int32_t mass_comp = 0;
+                double densmoy_comp_max = -1.0;
+                int32_t igroupref = 0;
+                for (int32_t igr = igr1out; igr <= igr2out; ++igr) {
+                    int32_t g = idgroup[igr];
+                    densmaxgroup = std::max(densmaxgroup, densityg(g));
+                    double posgtmp[3], rsquaretmp, densmoytmp, truemasstmp;
+                    int32_t masstmp;
+                    treat(g, rhot, posgtmp, masstmp, igroupref, posref, rsquaretmp, densmoytmp,
+                          truemasstmp);
+                    posg[0] = posg[0] + posgtmp[0];
+                    posg[1] = posg[1] + posgtmp[1];
+                    posg[2] = posg[2] + posgtmp[2];
+                    rsquareg = rsquareg + rsquaretmp;
+                    mass_loc = mass_loc + masstmp;
+                    truemass = truemass + truemasstmp;
+                    densmoyg = densmoyg + densmoytmp;
+                    mass_comp = std::max(mass_comp, masstmp);
+                    if (masstmp > 0) {
+                        densmoytmp = densmoytmp / (double)masstmp;
+                        densmoy_comp_max =
+                            std::max(densmoy_comp_max,
+                                     densmoytmp / (1.0 + p.fudge / std::sqrt((double)masstmp)));
+                    }
+                }
+                massg[ic] = mass_loc;
+                truemassg[ic] = truemass;
+                for (int d = 0; d < 3; ++d) posgg[3 * ic + d] = posg[d];
+                densmaxg[ic] = densmaxgroup;
+                mass_compg[ic] = mass_comp;
+                rsquare[ic] = std::sqrt(std::fabs(
+                    (truemass * rsquareg - (posg[0] * posg[0] + posg[1] * posg[1] + posg[2] * posg[2])) /
+                    (truemass * truemass)));
+                densmoy[ic] = densmoyg / (double)mass_loc;
+                ifok[ic] = mass_loc >= p.nmembthresh &&
+                           (densmoy[ic] > rhot * (1.0 + p.fudge / std::sqrt((double)mass_loc)) ||
+                            densmoy_comp_max > rhot) &&
+                           densmaxg[ic] >= p.alphap * densmoy[ic] && rsquare[ic] >= p.epsilon;
+            }
+            int32_t nsisters = 0, icolor_ref = 0;
+            for (int32_t ic = 1; ic <= inc_color_tot; ++ic)
+                if (ifok[ic]) {
+                    ok_color[++nsisters] = ic;
+                    icolor_ref = ic;
+                }
+            if (nsisters > 1) {
+                int32_t inode1 = reserve(nsisters);
+                for (int32_t is = 1; is <= nsisters; ++is) {
+                    int32_t ic = ok_color[is];
+                    int32_t io = inode1 + is - 1;
+                    Node &nd = node[io];
+                    nd.mother = inode;
+                    nd.densmax = densmaxg[ic];
+                    nd.sister = is > 1 ? io - 1 : 0;
+                    nd.nsisters = nsisters;
+                    nd.mass = massg[ic];
+                    nd.truemass = truemassg[ic];
+                    if (massg[ic] == 0) throw std::runtime_error("create nodes: null mass");
+                    double posfin[3];
+                    for (int d = 0; d < 3; ++d) posfin[d] = posgg[3 * ic + d] / truemassg[ic];
+                    nd.radius = rsquare[ic];
+                    nd.density = densmoy[ic];
+                    for (int d = 0; d < 3; ++d) {
+                        if (posfin[d] >= Ls2[d]) posfin[d] = posfin[d] - L[d];
+                        else if (posfin[d] < -Ls2[d]) posfin[d] = posfin[d] + L[d];
+                        nd.position[d] = posfin[d];
+                    }
+                    nd.rho_saddle = rhot;
+                    nd.level = node[inode].level + 1;
+                }
+                node[inode].firstchild = inode1 + nsisters - 1;
+                for (int32_t is = 1; is <= nsisters; ++is) {
+                    int32_t ic = ok_color[is];
+                    double trho = rhot * (1.0 + p.fudge / std::sqrt((double)mass_compg[ic]));
+                    create_nodes(trho, inode1 + is - 1, igrposnew[ic - 1] + 1, igrposnew[ic], true);
+                }
+                return;
+            } else if (nsisters == 1) {
+                int32_t ic = icolor_ref;
+                double rhotout = rhot * (1.0 + p.fudge / std::sqrt((double)mass_compg[ic]));
+                int32_t igr1out = igrposnew[0] + 1, igr2out = igrposnew[inc_color_tot];
+                if (igr2out != igr1out) {
+                    rhot = rhotout;
+                    igr1 = igr1out;
+                    igr2 = igr2out;
+                    continue;
+                }
+                node[inode].firstchild = 0;
+                return;
+            } else {
+                node[inode].firstchild = 0;
+                return;
+            }
+        }
+    }
+
+    // node_precedes f90:1083-1131
+    bool precedes(int32_t a, int32_t b) const
+    {
+        const Node &x = node[a], &y = node[b];
+        if (x.level != y.level) return x.level < y.level;
+        if (x.truemass != y.truemass) return x.truemass > y.truemass;
+        if (x.mass != y.mass) return x.mass > y.mass;
+        if (x.densmax != y.densmax) return x.densmax > y.densmax;
+        if (x.density != y.density) return x.density > y.density;
+        if (x.rho_saddle != y.rho_saddle) return x.rho_saddle > y.rho_saddle;
+        for (int k = 0; k < 3; ++k)
+            if (x.position[k] != y.position[k]) return x.position[k] < y.position[k];
+        return a < b;
+    }
+
+    void reorder()
+    {
+        if (nnodes <= 1) return;
+        std::vector<int32_t> new_to_old(nnodes);
+        for (int32_t i = 0; i < nnodes; ++i) new_to_old[i] = i + 1;
+        std::sort(new_to_old.begin(), new_to_old.end(),
+                  [&](int32_t a, int32_t b) { return precedes(a, b); });
+        std::vector<int32_t> old_to_new(nnodes + 1, 0);
+        for (int32_t i = 0; i < nnodes; ++i) old_to_new[new_to_old[i]] = i + 1;
+        std::vector<Node> sorted(nnodes + 1);
+        sorted[0] = node[0];
+        for (int32_t i = 1; i <= nnodes; ++i) {
+            Node nd = node[new_to_old[i - 1]];
+            if (nd.mother > 0) nd.mother = old_to_new[nd.mother];
+            if (nd.firstchild > 0) nd.firstchild = old_to_new[nd.firstchild];
+            if (nd.sister > 0) nd.sister = old_to_new[nd.sister];
+            sorted[i] = nd;
+        }
+        for (int64_t i = 1; i <= npart; ++i)
+            if (liste_parts[i] > 0) liste_parts[i] = old_to_new[liste_parts[i]];
+        for (int32_t i = 0; i <= nnodes; ++i) node[i] = sorted[i];
+    }
+
+    void run(TreeOutputs &out)
+    {
+        link_members();
+        nhnei.assign(ngroups + 1, 0);
+        offset1.assign(ngroups + 2, 1);
+        for (int32_t g = 1; g <= ngroups; ++g) {
+            offset1[g] = in.sad_offset[g - 1] + 1;
+            nhnei[g] = in.sad_offset[g] - in.sad_offset[g - 1];
+        }
+        nnodesmax = 2 * ngroups;
+        node.assign(nnodesmax + 1, Node{});
+        idgroup.assign(ngroups + 1, 0);
+        igroupid.assign(ngroups + 1, 0);
+        idgroup_tmp.assign(ngroups + 1, 0);
+        color.assign(ngroups + 1, 0);
+        liste_parts.assign(npart + 1, 0);
+        double rhot = p.rho_threshold;
+        nnodes = 0;
+        node[0].mother = 0;
+        for (int32_t g = 1; g <= ngroups; ++g) treat_linkonly(g, rhot);
+        node[0].mass = 0;
+        node[0].truemass = 0.0;
+        node[0].radius = 0.0;
+        node[0].density = 0.0;
+        double dmx = -HUGE_VAL;
+        for (int32_t g = 1; g <= ngroups; ++g) dmx = std::max(dmx, densityg(g));
+        node[0].densmax = dmx;
+        node[0].rho_saddle = 0.0;
+        node[0].level = 0;
+        node[0].nsisters = 0;
+        node[0].sister = 0;
+        for (int32_t g = 1; g <= ngroups; ++g) {
+            idgroup[g] = g;
+            igroupid[g] = g;
+        }
+        if (ngroups > 0) create_nodes(rhot, 0, 1, ngroups, false);
+        reorder();
+        out.nnodes = nnodes;
+        out.liste_parts.assign(liste_parts.begin() + 1, liste_parts.end());
+        out.real_table.assign(8 * (size_t)nnodes, 0.0);
+        out.integer_table.assign(6 * (size_t)nnodes, 0);
+        for (int32_t i = 1; i <= nnodes; ++i) {   // node2table f90:550-573, column-major (8|6, nnodes)
+            const Node &nd = node[i];
+            double *r = &out.real_table[8 * (size_t)(i - 1)];
+            r[0] = nd.rho_saddle;
+            r[1] = nd.density;
+            r[2] = nd.densmax;
+            r[3] = nd.radius;
+            r[4] = nd.truemass;
+            r[5] = nd.position[0];
+            r[6] = nd.position[1];
+            r[7] = nd.position[2];
+            int32_t *t = &out.integer_table[6 * (size_t)(i - 1)];
+            t[0] = nd.mass;
+            t[1] = nd.level;
+            t[2] = nd.mother;
+            t[3] = nd.firstchild;
+            t[4] = nd.nsisters;
+            t[5] = nd.sister;
+        }
+    }
+};
+
+}  // namespace
+
+void create_group_tree(const TreeInputs &in, const TreeParams &p, TreeOutputs &out)
+{
+    Builder b(in, p);
+    b.run(out);
+}
+
+}  // namespace ahop

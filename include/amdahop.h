@@ -1,0 +1,121 @@
+/*
+ * amdahop.h -- C ABI of the MI355X-native AdaptaHOP hot path (libamdahop.so).
+ *
+ * The first group of entry points is the drop-in for the reference's f2py module
+ * compute_adaptahop.neikdtree (fortran/compute_adaptahop.pyf:1-70): the same five
+ * calls and five readable arrays, with the same argument meaning, called the same
+ * way by src/compute_neiKDtree_mod.py.  The differences are the error behaviour --
+ * every call returns 0 on success and a nonzero status (message in
+ * ahop_last_error()) where the Fortran executes STOP -- and that array outputs are
+ * read through accessor functions instead of module variables.
+ *
+ * Plain C types only: pointers, sizes, doubles.  Not re-entrant (one module state
+ * per process, like the Fortran module), synchronous (outputs are host-visible when a
+ * call returns).
+ */
+#ifndef AMDAHOP_H
+#define AMDAHOP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- drop-in surface of compute_adaptahop.neikdtree --------------------------------- */
+
+/* replaces neikdtree.sync_from_change_pos (compute_adaptahop.pyf:15-22, f90:576-589) */
+int ahop_sync_from_change_pos(int64_t npart, int32_t nusedpart, double epsilon,
+                              double fudgepsilon, double xlong, double boxsize2);
+
+/* replaces neikdtree.sync_from_init_adaptahop (.pyf:24-47, f90:591-641) */
+int ahop_sync_from_init_adaptahop(int64_t npart, int32_t nmembthresh, int32_t nmembers,
+                                  double omegal, double omega_lambda_f, double omega0,
+                                  double omega_f, double aexp_max, double af, double hubble,
+                                  double h_f, double boxsize2, double lf, double xlong,
+                                  double ylong, double zlong, double xlongs2, double ylongs2,
+                                  double zlongs2, double hub_pt, double aexp,
+                                  const double *pos_shift /* [3] */);
+
+/* replaces neikdtree.sync_others (.pyf:49-64, f90:643-678); method is not NUL-terminated */
+int ahop_sync_others(int32_t verbose, int32_t megaverbose, int32_t npart, int32_t nbpes,
+                     double rho_threshold, double massp, double boxsize, int32_t nhop,
+                     int32_t nvoisins, double fudge, double alphap, const char *method,
+                     int32_t method_len, int32_t nlevelmax, int32_t optimize_nodes);
+
+/* replaces neikdtree.compute_adaptahop(pos_in, mass_in) (.pyf:10-13, f90:495-543).
+ * pos_in is the (3, npart) Fortran-ordered array (i.e. (npart,3) row-major), host memory. */
+int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_cols,
+                           const double *mass_in, int64_t mass_len);
+
+/* replaces neikdtree.close (.pyf:66-67, f90:3227-3264) */
+int ahop_close(void);
+
+/* readable module arrays of neikdtree (.pyf:4-8); NULL / 0 when not allocated */
+const int32_t *ahop_liste_parts(int64_t *len);      /* liste_parts(nusedpart) */
+const double *ahop_density(int64_t *len);           /* density(npart) */
+int32_t ahop_nnodes(void);                          /* nnodes */
+const double *ahop_real_table(int64_t *ncols);      /* real_table(8, nnodes), column-major */
+const int32_t *ahop_integer_table(int64_t *ncols);  /* integer_table(6, nnodes), column-major */
+
+const char *ahop_last_error(void);
+
+/* ---- extensions behind the same surface (no counterpart in the f2py module) ---------- */
+
+/* 1: FP32 box-relative search with certified FP64 re-rank (default); 0: FP64 throughout. */
+int ahop_set_precision(int32_t mixed);
+/* particles per device-tree leaf bucket, 1..64 (default 32) */
+int ahop_set_bucket(int32_t bucket);
+
+/* Stage outputs of the last ahop_compute_adaptahop, original particle order:
+ * group ids of find_local_maxima (f90:797-861) and 0-based hop targets (-1 inactive). */
+int32_t ahop_ngroups(void);
+int ahop_get_groups(int32_t *gid, int32_t *hop, int64_t n);
+/* saddle CSR of dev_compute_saddle_list (f90:2433-2813): offset[ngroups+1] 0-based,
+ * isad 1-based group ids, rho saddle densities, decreasing within each group. */
+int64_t ahop_nsaddle(void);
+int ahop_get_saddle(int32_t *offset, int32_t *isad, double *rho);
+
+/* Hot-path parameters for the device-resident entry point. */
+typedef struct {
+    int32_t nvoisins, nhop;
+    double rho_threshold;
+    double xlong, ylong, zlong;
+    int32_t mixed;       /* as ahop_set_precision */
+    int32_t bucket;      /* as ahop_set_bucket */
+    int32_t saddle;      /* also extract saddle edges */
+} ahop_hot_params;
+
+typedef struct {
+    int64_t nactive;
+    int32_t ngroups;
+    int64_t nfallback;   /* mixed path: queries re-run in FP64 */
+    int64_t nsaddle;
+    float t_sort, t_tree, t_knn, t_fallback, t_hop, t_saddle, t_total;  /* ms, HIP events */
+} ahop_run_info;
+
+/* kNN + SPH density + hop + group ids (+ saddle edges) on device-resident inputs:
+ * d_pos (n,3) row-major doubles and d_mass (n) in device memory.  Writes density and
+ * 1-based group ids in original particle order to d_density / d_gid (device pointers,
+ * either may be NULL).  Synchronous.  Replaces the OpenMP stages compute_mean_density_and_np
+ * (f90:682), find_local_maxima (f90:748) and dev_compute_saddle_list (f90:2433). */
+int ahop_run_device(const double *d_pos, const double *d_mass, int64_t n,
+                    const ahop_hot_params *params, double *d_density, int32_t *d_gid,
+                    ahop_run_info *info);
+/* Host copies of the saddle CSR of the last ahop_run_device (sizes from its info). */
+int ahop_device_saddle(int32_t *offset, int32_t *isad, double *rho);
+
+/* Host structure-tree stage alone (create_group_tree f90:885-983) on host inputs, for
+ * callers that computed the hot-path stages elsewhere.  Outputs: liste_parts (npart),
+ * *nnodes, and the tables, which must have room for 2*ngroups nodes (nnodes <= 2*ngroups,
+ * f90:906): real_table 8*2*ngroups doubles, integer_table 6*2*ngroups int32. */
+int ahop_create_group_tree(int64_t npart, const double *pos, const double *mass,
+                           const double *density, const int32_t *gid, int32_t ngroups,
+                           const double *densityg, const int32_t *sad_offset,
+                           const int32_t *sad_isad, const double *sad_rho, double rho_threshold,
+                           double fudge, double alphap, double epsilon, int32_t nmembthresh,
+                           double xlong, int32_t *liste_parts, int32_t *nnodes,
+                           double *real_table, int32_t *integer_table);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

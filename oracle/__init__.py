@@ -60,6 +60,7 @@ def lib():
         L.orc_knn.argtypes = [_vp, _i64, _vp, _vp]
         L.orc_density.restype = _i64
         L.orc_density.argtypes = [_vp, _vp, _vp]
+        L.orc_density_at.argtypes = [_vp, _i64, _vp, _vp]
         L.orc_maxima.restype = _i32
         L.orc_maxima.argtypes = [_vp, _vp, _vp, _vp, _vp]
         L.orc_saddle.restype = _i64
@@ -103,6 +104,12 @@ class Oracle:
         nb = np.empty((self.n, self.nhop), dtype=np.int32) if with_nbrs else None
         nact = lib().orc_density(self._c, _ptr(rho), _ptr(nb) if with_nbrs else None)
         return (rho, int(nact), nb) if with_nbrs else (rho, int(nact))
+
+    def density_at(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        out = np.empty(idx.shape[0])
+        lib().orc_density_at(self._c, idx.shape[0], _ptr(idx), _ptr(out))
+        return out
 
     def maxima(self, rho):
         hop = np.empty(self.n, dtype=np.int32)
@@ -216,6 +223,21 @@ class Reference:
             out.update(saddle_offset=off - 1, saddle_isad=isad[:tot].copy(), saddle_rho=rs[:tot].copy())
         L.ref_close()
         return out
+
+    def time_knn_density_hop(self):
+        """Wall seconds of the reference's tree + density + local-maxima stages
+        (f90:2967, :682, :748) -- its kNN+density+hop path as shipped."""
+        import time
+        L = ref_lib()
+        L.ref_configure(*self.args)
+        L.ref_stage_begin(self.n, _ptr(self.pos_f), _ptr(self.mass))
+        t0 = time.perf_counter()
+        L.ref_stage_tree()
+        L.ref_stage_density()
+        L.ref_stage_maxima()
+        t1 = time.perf_counter()
+        L.ref_close()
+        return t1 - t0
 
     def full(self):
         """The public compute_adaptahop entry (f90:495-543): final node ids and tables."""

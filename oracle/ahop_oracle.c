@@ -304,6 +304,25 @@ int64_t orc_density(const orc_ctx *c, double *density, int32_t *nbr_nhop)
     return nactive;
 }
 
+/* density of selected particles only (same arithmetic as orc_density) */
+void orc_density_at(const orc_ctx *c, int64_t m, const int64_t *idx, double *out)
+{
+    const int k = c->k;
+#pragma omp parallel num_threads(c->nthreads)
+    {
+        double *dist2 = malloc(sizeof(double) * (k + 1));
+        int32_t *nei = malloc(sizeof(int32_t) * (k + 1));
+#pragma omp for schedule(dynamic, 16)
+        for (int64_t q = 0; q < m; ++q) {
+            int32_t ipar = (int32_t)(idx[q] + 1);
+            find_nearest(c, ipar, dist2, nei);
+            out[q] = sph_density(c, ipar, dist2, nei);
+        }
+        free(dist2);
+        free(nei);
+    }
+}
+
 /* find_local_maxima f90:748-881 */
 int32_t orc_maxima(const orc_ctx *c, const double *density, int32_t *hop, int32_t *gid,
                    double *densityg)

@@ -34,6 +34,9 @@ void orc_knn(const orc_ctx *c, int64_t i, double *d2, int32_t *ids);
  * Optionally also stores the first nhop neighbour ids (n x nhop, 0-based). Returns nactive. */
 int64_t orc_density(const orc_ctx *c, double *density, int32_t *nbr_nhop);
 
+/* Density of the m particles idx[0..m-1] (0-based), for sampled checks on large inputs. */
+void orc_density_at(const orc_ctx *c, int64_t m, const int64_t *idx, double *out);
+
 /* Hop targets and group ids (find_local_maxima :748-881).  hop: 0-based target of
  * each active particle (-1 inactive); gid: 1-based group id, 0 inactive;
  * densityg: density of each group's root (capacity n).  Returns ngroups. */

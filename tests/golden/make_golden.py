@@ -1,0 +1,60 @@
+"""Writes the golden fixtures of the AdaptaHOP hot path from the compiled reference.
+
+The reference Fortran (fortran/compute_adaptahop.f90) is built from the reference
+source by ``make -f oracle/Makefile ref`` into oracle/_ref/libref_ahop.so and driven
+through oracle/ref_driver.f90: stage by stage for the kNN lists, densities, group ids
+and saddle CSR, and through its public compute_adaptahop entry for the final node ids
+and node tables.  Inputs are regenerated from amdahop.generate; their SHA-256 is stored
+so a generator change is detected.  Run from the repository root:
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "halomaker-python_amd"))
+
+import oracle as O  # noqa: E402
+from amdahop import generate as G  # noqa: E402
+
+CASES = {
+    # name: (config, n, nclumps)
+    "clumpy_16k": ("c1", 16384, 10),
+    "mixed_32k": ("c3", 32768, 18),
+}
+PARAMS = dict(nhop=16, rho_threshold=80.0, fudge=4.0, alphap=1.0, fudgepsilon=0.001,
+              nmembers=20, nlevelmax=30)
+
+
+def main():
+    out_dir = os.path.dirname(os.path.abspath(__file__))
+    for name, (cfg, n, ncl) in CASES.items():
+        ps = G.config(cfg, n=n, nclumps=ncl)
+        R = O.Reference(ps.pos, ps.mass, ps.nvoisins, **PARAMS, nthreads=4)
+        rng = np.random.default_rng(123)
+        sample = np.sort(rng.choice(n, size=32, replace=False))
+        st = R.staged(knn_ids=sample)
+        full = R.full()
+        if not np.array_equal(st["density"], full["density"]):
+            raise SystemExit("staged and full reference runs disagree")
+        np.savez_compressed(
+            os.path.join(out_dir, f"{name}.npz"),
+            config=cfg, n=n, nclumps=ncl, nvoisins=ps.nvoisins,
+            input_digest=G.digest(ps.pos, ps.mass),
+            density=st["density"], gid=st["gid"], ngroups=st["ngroups"], densityg=st["densityg"],
+            saddle_offset=st["saddle_offset"], saddle_isad=st["saddle_isad"],
+            saddle_rho=st["saddle_rho"],
+            knn_sample=sample, knn_d2=np.array([d for d, _ in st["knn"]]),
+            knn_ids=np.array([i for _, i in st["knn"]]),
+            liste_parts=full["liste_parts"], nnodes=full["nnodes"],
+            real_table=full["real_table"], integer_table=full["integer_table"])
+        print(name, "ngroups", st["ngroups"], "nnodes", full["nnodes"],
+              "saddle entries", len(st["saddle_isad"]))
+
+
+if __name__ == "__main__":
+    main()

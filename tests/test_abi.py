@@ -1,0 +1,115 @@
+"""C ABI: the library loads without a GPU, exports every declared symbol, and its host
+logic (argument checks, the structure-tree stage) is correct.  No device call here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, load_golden, golden_input
+from amdahop._lib import LIB_PATH, SIGNATURES, check, lib, ptr
+
+HEADER = os.path.join(ROOT, "include", "amdahop.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ahop_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+
+
+def test_binding_covers_header():
+    assert sorted(SIGNATURES) == declared_functions()
+
+
+def test_dropin_module_surface():
+    from compute_adaptahop import neikdtree
+    for f in ("compute_adaptahop", "sync_from_change_pos", "sync_from_init_adaptahop",
+              "sync_others", "close"):
+        assert callable(getattr(neikdtree, f))
+    assert neikdtree.liste_parts is None and neikdtree.density is None
+    assert int(neikdtree.nnodes) == 0 and neikdtree.nnodes.dtype == np.int32
+
+
+def test_bad_shape_is_an_error_not_a_stop():
+    """The reference STOPs on pos_in not shaped (3,npart) (f90:509-512)."""
+    from compute_adaptahop import neikdtree
+    with pytest.raises(RuntimeError, match=r"shape \(3,npart\)"):
+        neikdtree.compute_adaptahop(np.zeros((4, 10)), np.zeros(10))
+    pos = np.zeros((3, 10), order="F")
+    status = lib().ahop_compute_adaptahop(ptr(pos), 2, 10, ptr(np.zeros(10)), 10)
+    assert status != 0 and b"shape" in lib().ahop_last_error()
+
+
+def test_size_mismatch_is_an_error():
+    from compute_adaptahop import neikdtree
+    neikdtree.sync_from_change_pos(100, 100, 0.1, 0.001, 1.0, 1.0)
+    with pytest.raises(RuntimeError, match="do not match npart"):
+        neikdtree.compute_adaptahop(np.zeros((3, 50), order="F"), np.zeros(50))
+
+
+def test_parameter_validation():
+    assert lib().ahop_set_precision(2) != 0
+    assert lib().ahop_set_bucket(0) != 0 and lib().ahop_set_bucket(65) != 0
+    check(lib().ahop_set_bucket(32))
+    check(lib().ahop_set_precision(1))
+
+
+def _run_tree(ps, g, eps):
+    n = ps.n
+    ng = int(g["ngroups"])
+    lp = np.empty(n, dtype=np.int32)
+    nn = ctypes.c_int32(0)
+    rt = np.zeros(8 * 2 * max(ng, 1))
+    it = np.zeros(6 * 2 * max(ng, 1), dtype=np.int32)
+    check(lib().ahop_create_group_tree(
+        n, ptr(ps.pos), ptr(ps.mass), ptr(g["density"]), ptr(g["gid"]), ng, ptr(g["densityg"]),
+        ptr(np.ascontiguousarray(g["saddle_offset"], dtype=np.int32)),
+        ptr(np.ascontiguousarray(g["saddle_isad"], dtype=np.int32)),
+        ptr(np.ascontiguousarray(g["saddle_rho"])), 80.0, 4.0, 1.0, eps, 20, 1.0, ptr(lp),
+        ctypes.byref(nn), ptr(rt), ptr(it)))
+    k = nn.value
+    return lp, k, rt[:8 * k].reshape(8, k, order="F"), it[:6 * k].reshape(6, k, order="F")
+
+
+@pytest.mark.parametrize("name", ["clumpy_16k", "mixed_32k"])
+def test_host_structure_tree_matches_reference(name):
+    """create_group_tree (f90:885-983) on the reference's own stage outputs reproduces
+    the reference's node ids and node tables bit for bit."""
+    g = load_golden(name)
+    ps = golden_input(g)
+    eps = 0.001 * 1.0 / ps.n ** (1.0 / 3.0)
+    lp, k, rt, it = _run_tree(ps, g, eps)
+    assert k == int(g["nnodes"])
+    assert np.array_equal(lp, g["liste_parts"])
+    assert np.array_equal(rt, g["real_table"])
+    assert np.array_equal(it, g["integer_table"])
+
+
+def test_host_structure_tree_sensitive_to_threshold():
+    """Planted change: a higher membership threshold must change the tree."""
+    g = load_golden("clumpy_16k")
+    ps = golden_input(g)
+    eps = 0.001 / ps.n ** (1.0 / 3.0)
+    n = ps.n
+    ng = int(g["ngroups"])
+    lp = np.empty(n, dtype=np.int32)
+    nn = ctypes.c_int32(0)
+    rt = np.zeros(16 * ng)
+    it = np.zeros(12 * ng, dtype=np.int32)
+    check(lib().ahop_create_group_tree(
+        n, ptr(ps.pos), ptr(ps.mass), ptr(g["density"]), ptr(g["gid"]), ng, ptr(g["densityg"]),
+        ptr(np.ascontiguousarray(g["saddle_offset"], dtype=np.int32)),
+        ptr(np.ascontiguousarray(g["saddle_isad"], dtype=np.int32)),
+        ptr(np.ascontiguousarray(g["saddle_rho"])), 80.0, 4.0, 1.0, eps, 2000, 1.0, ptr(lp),
+        ctypes.byref(nn), ptr(rt), ptr(it)))
+    assert nn.value < int(g["nnodes"])

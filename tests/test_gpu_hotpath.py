@@ -1,0 +1,210 @@
+"""HIP path vs the oracle / the compiled reference's golden fixtures, through the C ABI.
+
+Bit-exact bars: densities (FP64 values computed in the reference's arithmetic order),
+hop targets, group ids, node ids and node tables.  The saddle list is compared as a
+multiset per (group, saddle density) because the reference sorts it with an unstable
+quicksort.  Both precision modes must meet the same bars: the mixed path certifies
+its FP32 neighbour ranking and re-ranks in FP64, so its outputs are exact too.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import golden_input, load_golden, saddle_as_sets
+from amdahop import generate as G
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def dropin_run(ps, mode="mixed", bucket=32, nhop=16, rho_threshold=80.0, xlong=1.0,
+               nvoisins=None):
+    """Drive the drop-in module exactly like compute_neiKDtree_mod.py does."""
+    from compute_adaptahop import neikdtree
+    n = ps.n
+    k = nvoisins or ps.nvoisins
+    neikdtree.set_precision(mode)
+    neikdtree.set_bucket(bucket)
+    neikdtree.sync_from_init_adaptahop(n, 20, 20, 0.7, 0.7, 0.3, 0.3, 1.0, 1.0, 0.7, 70.0, xlong,
+                                       xlong, xlong, xlong, xlong, xlong / 2, xlong / 2, xlong / 2,
+                                       70.0, 1.0, [0.0, 0.0, 0.0])
+    neikdtree.sync_from_change_pos(n, n, 0.001 * xlong / n ** (1 / 3), 0.001, xlong, xlong)
+    neikdtree.sync_others(False, False, n, 16, rho_threshold, 1.0 / n, 0.0, nhop, k, 4.0, 1.0,
+                          "MSM", 30, False)
+    neikdtree.compute_adaptahop(np.asfortranarray(ps.pos.T), ps.mass)
+    out = dict(density=neikdtree.density.copy(), liste_parts=neikdtree.liste_parts.copy(),
+               nnodes=int(neikdtree.nnodes), real_table=neikdtree.real_table.copy(order="F"),
+               integer_table=neikdtree.integer_table.copy(order="F"))
+    out["gid"], out["hop"] = neikdtree.groups()
+    out["saddle"] = neikdtree.saddle()
+    neikdtree.close()
+    neikdtree.set_precision("mixed")
+    neikdtree.set_bucket(32)
+    return out
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["clumpy_16k", "mixed_32k"])
+def test_dropin_matches_reference_golden(gpu, name, mode):
+    g = load_golden(name)
+    ps = golden_input(g)
+    log(f"golden {name} mode={mode} n={ps.n}")
+    r = dropin_run(ps, mode)
+    assert np.array_equal(r["density"], g["density"])
+    assert np.array_equal(r["gid"], g["gid"])
+    assert r["nnodes"] == int(g["nnodes"])
+    assert np.array_equal(r["liste_parts"], g["liste_parts"])
+    assert np.array_equal(r["real_table"], g["real_table"])
+    assert np.array_equal(r["integer_table"], g["integer_table"])
+    assert saddle_as_sets(*r["saddle"]) == saddle_as_sets(g["saddle_offset"], g["saddle_isad"],
+                                                          g["saddle_rho"])
+
+
+@pytest.fixture(scope="module")
+def c1_oracle():
+    ps = G.config("c1")
+    log(f"oracle on c1 n={ps.n}")
+    o = O.Oracle(ps.pos, ps.mass, ps.nvoisins)
+    rho, nact = o.density()
+    hop, gid, dg, ng = o.maxima(rho)
+    sad = o.saddle(rho, gid, ng)
+    o.close()
+    return ps, rho, hop, gid, ng, sad
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+def test_c1_full_size_matches_oracle(gpu, c1_oracle, mode):
+    ps, rho, hop, gid, ng, sad = c1_oracle
+    log(f"c1 mode={mode}")
+    r = dropin_run(ps, mode)
+    assert np.array_equal(r["density"], rho)
+    assert np.array_equal(r["hop"], hop)
+    assert np.array_equal(r["gid"], gid)
+    assert saddle_as_sets(*r["saddle"]) == saddle_as_sets(*sad)
+
+
+@pytest.mark.parametrize("bucket", [1, 7, 64])
+def test_tree_shape_does_not_change_results(gpu, c1_oracle, bucket):
+    ps, rho, hop, gid, ng, sad = c1_oracle
+    log(f"c1 bucket={bucket}")
+    r = dropin_run(ps, "mixed", bucket=bucket)
+    assert np.array_equal(r["density"], rho)
+    assert np.array_equal(r["gid"], gid)
+
+
+def _device_run(ps, mixed, saddle=True, **kw):
+    import torch
+    from amdahop import run_device
+    pos = torch.from_numpy(ps.pos).cuda()
+    mass = torch.from_numpy(ps.mass).cuda()
+    dens, gid, info = run_device(pos, mass, ps.nvoisins, mixed=mixed, saddle=saddle, **kw)
+    return dens.cpu().numpy(), gid.cpu().numpy(), info
+
+
+def test_c2_mixed_equals_fp64_and_oracle_sample(gpu):
+    ps = G.config("c2")
+    log(f"c2 n={ps.n}")
+    d64, g64, i64 = _device_run(ps, False)
+    dmx, gmx, imx = _device_run(ps, True)
+    log(f"c2 fallback queries {imx.nfallback}, groups {imx.ngroups}")
+    assert np.array_equal(d64, dmx)
+    assert np.array_equal(g64, gmx)
+    assert i64.ngroups == imx.ngroups and i64.nactive == imx.nactive
+    assert imx.nfallback < ps.n // 100
+    o = O.Oracle(ps.pos, ps.mass, ps.nvoisins)
+    idx = np.random.default_rng(9).choice(ps.n, size=20000, replace=False)
+    assert np.array_equal(o.density_at(idx), dmx[idx])
+
+
+def test_c3_full_size_sample_and_invariants(gpu):
+    """Headline workload: 16.8M particles, k=64, mixed families.  Densities of a random
+    sample are bit-exact vs the oracle; hop/group invariants hold on every particle."""
+    ps = G.config("c3")
+    log(f"c3 n={ps.n}")
+    dmx, gmx, imx = _device_run(ps, True, saddle=False)
+    log(f"c3 fallback {imx.nfallback} groups {imx.ngroups} knn {imx.t_knn:.1f} ms")
+    o = O.Oracle(ps.pos, ps.mass, ps.nvoisins)
+    idx = np.random.default_rng(10).choice(ps.n, size=4000, replace=False)
+    assert np.array_equal(o.density_at(idx), dmx[idx])
+    o.close()
+    act = dmx > 80.0
+    assert int(act.sum()) == imx.nactive
+    assert np.all(gmx[~act] == 0) and np.all(gmx[act] >= 1) and gmx.max() == imx.ngroups
+    # group ids number the roots by ascending particle index: first occurrence order
+    first = np.unique(gmx[act], return_index=True)[1]
+    assert np.all(np.diff(np.nonzero(act)[0][first]) > 0)
+
+
+def test_saddle_symmetric_and_bounded_c2(gpu):
+    import torch
+    from amdahop import run_device, device_saddle
+    ps = G.config("c2")
+    pos = torch.from_numpy(ps.pos).cuda()
+    mass = torch.from_numpy(ps.mass).cuda()
+    dens, gid, info = run_device(pos, mass, ps.nvoisins, saddle=True)
+    off, isad, rs = device_saddle(info)
+    rho = dens.cpu().numpy()
+    g = gid.cpu().numpy()
+    dg = np.zeros(info.ngroups + 1)
+    np.maximum.at(dg, g, rho)
+    g1 = np.repeat(np.arange(1, info.ngroups + 1), np.diff(off))
+    fwd = dict(zip(zip(g1.tolist(), isad.tolist()), rs.tolist()))
+    assert len(fwd) == len(rs)
+    for (a, b), s in fwd.items():
+        assert fwd[(b, a)] == s
+        assert 80.0 < s <= min(dg[a], dg[b])
+
+
+def test_small_and_edge_inputs(gpu):
+    # tiny uniform box, k close to N
+    rng = np.random.default_rng(4)
+    pos = rng.uniform(-0.5, 0.5, size=(150, 3))
+    ps = G.ParticleSet(pos, np.full(150, 1 / 150), 100, 16, "tiny", 150)
+    o = O.Oracle(ps.pos, ps.mass, 100)
+    rho, _ = o.density()
+    for mode in ("fp64", "mixed"):
+        r = dropin_run(ps, mode, rho_threshold=0.5)
+        assert np.array_equal(r["density"], rho)
+    # particles exactly on the lower box face and duplicated positions
+    ps = G.config("c1", n=6000, nclumps=4, seed=3)
+    ps.pos[:20, 0] = -0.5
+    ps.pos[100:110] = ps.pos[200:210]
+    o = O.Oracle(ps.pos, ps.mass, ps.nvoisins)
+    rho, _ = o.density()
+    hop, gid, _, ng = o.maxima(rho)
+    r = dropin_run(ps, "mixed")
+    assert np.array_equal(r["density"], rho)
+    assert np.array_equal(r["gid"], gid)
+    # nothing above threshold: no groups, no nodes
+    r = dropin_run(ps, "mixed", rho_threshold=1e12)
+    assert r["nnodes"] == 0 and not r["gid"].any() and not r["liste_parts"].any()
+    # nhop == nvoisins
+    r = dropin_run(ps, "fp64", nhop=32)
+    hop2, gid2, _, _ = O.Oracle(ps.pos, ps.mass, 32, nhop=32).maxima(rho)
+    assert np.array_equal(r["gid"], gid2)
+
+
+def test_non_unit_box(gpu):
+    ps = G.config("c1", n=8000, nclumps=5, seed=8)
+    ps.pos *= 2.0
+    ps.mass /= 8.0
+    o = O.Oracle(ps.pos, ps.mass, ps.nvoisins, xlong=2.0)
+    rho, _ = o.density()
+    r = dropin_run(ps, "mixed", xlong=2.0)
+    assert np.array_equal(r["density"], rho)
+
+
+def test_out_of_box_position_is_an_error(gpu):
+    ps = G.config("c1", n=4000, nclumps=3, seed=2)
+    ps.pos[7, 1] = 0.5
+    with pytest.raises(RuntimeError, match="inside"):
+        dropin_run(ps, "mixed")
+    ps.pos[7, 1] = np.nan
+    with pytest.raises(RuntimeError, match="inside"):
+        dropin_run(ps, "mixed")
