@@ -57,3 +57,36 @@ def device_saddle(info):
     rho = np.empty(max(info.nsaddle, 1), dtype=np.float64)
     check(lib().ahop_device_saddle(ptr(off), ptr(isad), ptr(rho)))
     return off, isad[:info.nsaddle], rho[:info.nsaddle]
+
+
+HALO_FIELDS = ["m", "px", "py", "pz", "vx", "vy", "vz", "sigma", "Lx", "Ly", "Lz", "r",
+               "Ixx", "Ixy", "Ixz", "Iyy", "Iyz", "Izz", "sha", "shb", "shc", "ek", "ep", "et",
+               "rvir", "mvir", "cvel", "tvir", "spin"]
+
+
+def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst=430.1,
+               vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000,
+               tile=0, workgroup=0):
+    """Per-halo properties of a CSR member layout (device tensors), one row per halo in
+    the order of HALO_FIELDS.  Returns (out tensor (H, 29) float64, HaloInfo)."""
+    import numpy as np
+    import torch
+
+    from ._lib import HaloInfo, HaloParams, ptr
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nh = off.shape[0] - 1
+    for t in (pos, vel, mass):
+        if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+            raise TypeError("pos, vel, mass must be contiguous float64 device tensors")
+    out = torch.empty((nh, len(HALO_FIELDS)), dtype=torch.float64, device=pos.device)
+    p = HaloParams(float(lbox), float(hub_pt), float(gravconst), float(vir_overdens),
+                   float(rho_mean), 1 if cdm else 0, int(nshell), int(direct_pe_nmax), int(tile),
+                   int(workgroup))
+    info = HaloInfo()
+    torch.cuda.synchronize(pos.device)
+    dptr = ctypes.c_void_p(density.data_ptr()) if density is not None else None
+    check(lib().ahop_halo_props_device(nh, ptr(off), ctypes.c_void_p(pos.data_ptr()),
+                                       ctypes.c_void_p(vel.data_ptr()),
+                                       ctypes.c_void_p(mass.data_ptr()), dptr, ctypes.byref(p),
+                                       ctypes.c_void_p(out.data_ptr()), ctypes.byref(info)))
+    return out, info

@@ -23,6 +23,16 @@ class HotParams(ctypes.Structure):
                 ("mixed", _i32), ("bucket", _i32), ("saddle", _i32)]
 
 
+class HaloParams(ctypes.Structure):
+    _fields_ = [("lbox", _dbl), ("hub_pt", _dbl), ("gravconst", _dbl), ("vir_overdens", _dbl),
+                ("rho_mean", _dbl), ("cdm", _i32), ("nshell", _i32), ("direct_pe_nmax", _i32),
+                ("tile", _i32), ("workgroup", _i32)]
+
+
+class HaloInfo(ctypes.Structure):
+    _fields_ = [("tile", _i32), ("workgroup", _i32), ("trials", _i32), ("ms", _f32)]
+
+
 class RunInfo(ctypes.Structure):
     _fields_ = [("nactive", _i64), ("ngroups", _i32), ("nfallback", _i64), ("nsaddle", _i64),
                 ("t_sort", _f32), ("t_tree", _f32), ("t_knn", _f32), ("t_fallback", _f32),
@@ -32,7 +42,7 @@ class RunInfo(ctypes.Structure):
 # name -> (restype, argtypes); must list every function declared in include/amdahop.h
 SIGNATURES = {
     "ahop_sync_from_change_pos": (ctypes.c_int, [_i64, _i32, _dbl, _dbl, _dbl, _dbl]),
-    "ahop_sync_from_init_adaptahop": (ctypes.c_int, [_i64, _i32, _i32] + [_dbl] * 19 + [_vp]),
+    "ahop_sync_from_init_adaptahop": (ctypes.c_int, [_i64, _i32, _i32] + [_dbl] * 18 + [_vp]),
     "ahop_sync_others": (ctypes.c_int, [_i32, _i32, _i32, _i32, _dbl, _dbl, _dbl, _i32, _i32,
                                         _dbl, _dbl, ctypes.c_char_p, _i32, _i32, _i32]),
     "ahop_compute_adaptahop": (ctypes.c_int, [_vp, _i64, _i64, _vp, _i64]),
@@ -52,6 +62,9 @@ SIGNATURES = {
     "ahop_run_device": (ctypes.c_int, [_vp, _vp, _i64, ctypes.POINTER(HotParams), _vp, _vp,
                                        ctypes.POINTER(RunInfo)]),
     "ahop_device_saddle": (ctypes.c_int, [_vp, _vp, _vp]),
+    "ahop_halo_props_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp,
+                                              ctypes.POINTER(HaloParams), _vp,
+                                              ctypes.POINTER(HaloInfo)]),
     "ahop_create_group_tree": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp,
                                               _dbl, _dbl, _dbl, _dbl, _i32, _dbl, _vp, _vp, _vp, _vp]),
 }

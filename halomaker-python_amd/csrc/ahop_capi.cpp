@@ -4,6 +4,7 @@
 #include "../../include/amdahop.h"
 #include "ahop_common.h"
 #include "create_nodes.h"
+#include "halo_props.h"
 
 #include <chrono>
 #include <cstdio>
@@ -304,14 +305,16 @@ const int32_t *ahop_liste_parts(int64_t *len)
 {
     auto &s = S();
     if (len) *len = s.allocated ? (int64_t)s.liste_parts.size() : 0;
-    return s.allocated ? s.liste_parts.data() : nullptr;
+    static const int32_t empty_i = 0;
+    return s.allocated ? (s.liste_parts.empty() ? &empty_i : s.liste_parts.data()) : nullptr;
 }
 
 const double *ahop_density(int64_t *len)
 {
     auto &s = S();
     if (len) *len = s.allocated ? (int64_t)s.density.size() : 0;
-    return s.allocated ? s.density.data() : nullptr;
+    static const double empty_d = 0;
+    return s.allocated ? (s.density.empty() ? &empty_d : s.density.data()) : nullptr;
 }
 
 int32_t ahop_nnodes(void) { return S().nnodes; }
@@ -320,14 +323,16 @@ const double *ahop_real_table(int64_t *ncols)
 {
     auto &s = S();
     if (ncols) *ncols = s.allocated ? s.nnodes : 0;
-    return s.allocated ? s.real_table.data() : nullptr;
+    static const double empty_d = 0;
+    return s.allocated ? (s.real_table.empty() ? &empty_d : s.real_table.data()) : nullptr;
 }
 
 const int32_t *ahop_integer_table(int64_t *ncols)
 {
     auto &s = S();
     if (ncols) *ncols = s.allocated ? s.nnodes : 0;
-    return s.allocated ? s.integer_table.data() : nullptr;
+    static const int32_t empty_i = 0;
+    return s.allocated ? (s.integer_table.empty() ? &empty_i : s.integer_table.data()) : nullptr;
 }
 
 const char *ahop_last_error(void) { return last_error(); }
@@ -444,6 +449,36 @@ int ahop_device_saddle(int32_t *offset, int32_t *isad, double *rho)
             std::memcpy(isad, s.dev_sad_isad.data(), sizeof(int32_t) * s.dev_sad_isad.size());
         if (rho && !s.dev_sad_rho.empty())
             std::memcpy(rho, s.dev_sad_rho.data(), sizeof(double) * s.dev_sad_rho.size());
+    });
+}
+
+int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
+                           const double *d_vel, const double *d_mass, const double *d_density,
+                           const ahop_halo_params *params, double *d_out, ahop_halo_info *info)
+{
+    return guarded([&] {
+        if (!params || !offsets || !d_pos || !d_vel || !d_mass || !d_out)
+            throw std::runtime_error("null argument");
+        static_assert(AHOP_HALO_NFIELDS == HP_NFIELDS, "field count mismatch");
+        HaloParamsDev P;
+        P.lbox = params->lbox;
+        P.hub_pt = params->hub_pt;
+        P.gravconst = params->gravconst;
+        P.vir_overdens = params->vir_overdens;
+        P.rho_mean = params->rho_mean;
+        P.cdm = params->cdm;
+        P.nshell = params->nshell;
+        P.direct_pe_nmax = params->direct_pe_nmax;
+        if (!(P.lbox > 0)) throw std::runtime_error("lbox must be > 0");
+        Engine &e = engine();
+        HaloTuning t = halo_props_device(offsets, nhalos, d_pos, d_vel, d_mass, d_density, P,
+                                         d_out, params->tile, params->workgroup, e.stream());
+        if (info) {
+            info->tile = t.tile;
+            info->workgroup = t.bs;
+            info->trials = t.trials;
+            info->ms = t.ms;
+        }
     });
 }
 

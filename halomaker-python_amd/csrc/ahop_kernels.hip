@@ -319,11 +319,12 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     const double4 P = A.spos[q];
     const double Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
 
-    // ---- query frame (mixed): centre of the wave's queries, box-relative FP32 coordinates
-    double cx = 0, cy = 0, cz = 0;
+    // ---- query frame: centre of the wave's queries (orders the traversal; in the mixed
+    // path also the origin of the box-relative FP32 coordinates)
+    double cx, cy, cz;
     float qx = 0, qy = 0, qz = 0, hQ = 0;
     bool wave_unsafe = false;
-    if constexpr (MIXED) {
+    {
         const double BIG = 1e300;
         double lx = wave_min(active ? P.x : BIG), hx = wave_max(active ? P.x : -BIG);
         double ly = wave_min(active ? P.y : BIG), hy = wave_max(active ? P.y : -BIG);
@@ -331,6 +332,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         cx = 0.5 * (lx + hx);
         cy = 0.5 * (ly + hy);
         cz = 0.5 * (lz + hz);
+    }
+    if constexpr (MIXED) {
         double rx = wrapd(P.x - cx, Lx), ry = wrapd(P.y - cy, Ly), rz = wrapd(P.z - cz, Lz);
         qx = (float)rx;
         qy = (float)ry;
@@ -348,6 +351,36 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     }
     KeyT tau = active ? (KeyT)INFINITY : (KeyT)-1;
     if (wave_unsafe) tau = (KeyT)-1;
+
+    // exact distance key of candidate slot j for this lane
+    auto cand_key = [&](const double4 &c) -> KeyT {
+        if constexpr (MIXED) {
+            float dx = (float)wrapd(c.x - cx, Lx) - qx;
+            float dy = (float)wrapd(c.y - cy, Ly) - qy;
+            float dz = (float)wrapd(c.z - cz, Lz) - qz;
+            return dx * dx + dy * dy + dz * dz;
+        } else {
+            return d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+        }
+    };
+
+    // ---- seed each lane with its Morton-order neighbours [q-W, q+W] so that the search
+    // radius is already close to the k-th neighbour distance when the traversal starts;
+    // the traversal then skips these slots
+    const int W = K / 2 + 16;
+    int seed_lo = 1, seed_hi = 0;
+    if (active && !wave_unsafe) {
+        seed_lo = max(0, q - W);
+        seed_hi = min(A.n - 1, q + W);
+    }
+    for (int j = seed_lo; j <= seed_hi; ++j) {
+        if (j == q) continue;
+        KeyT d = cand_key(A.spos[j]);
+        if (d < tau) {
+            heap_sift<KeyT>(hk, hi, lane, KH, d, j);
+            tau = hk[lane];
+        }
+    }
 
     // per-lane visit test of a node / particle box
     auto need_box = [&](int c) -> bool {
@@ -391,6 +424,22 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         }
     };
 
+    // squared periodic distance from the wave centre to a node box (traversal order only)
+    auto centre_dist2 = [&](int c) -> double {
+        double b[6];
+        child_box(c, A.spos, A.box, b);
+        const double cq[3] = {cx, cy, cz};
+        const double L[3] = {Lx, Ly, Lz};
+        double s = 0;
+        for (int a = 0; a < 3; ++a) {
+            double t = fabs(cq[a] - 0.5 * (b[a] + b[a + 3]));
+            double d = fmin(t, L[a] - t) - 0.5 * (b[a + 3] - b[a]);
+            d = d > 0 ? d : 0;
+            s += d * d;
+        }
+        return s;
+    };
+
     int sp = 0;
     stack[0] = 0;
     sp = 1;
@@ -431,7 +480,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
                 } else {
                     d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
                 }
-                if (d < tau && (f + j) != q) mask |= (1ull << j);
+                const int sj = f + j;
+                if (d < tau && (sj < seed_lo || sj > seed_hi) && sj != q) mask |= (1ull << j);
             }
             while (__any(mask != 0)) {
                 if (mask) {
@@ -462,7 +512,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
             break;
         }
         if (p1 && p2) {
-            if (p1 >= p2) {
+            // nearer child (to the wave centre) first
+            if (centre_dist2(c1) <= centre_dist2(c2)) {
                 stack[sp++] = c2;
                 stack[sp++] = c1;
             } else {

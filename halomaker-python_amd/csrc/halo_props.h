@@ -1,0 +1,39 @@
+// halo_props.h -- batched per-halo property reductions (see halo_props.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ahop {
+
+// Output row layout (one row of HP_NFIELDS doubles per halo); names follow halo_dtype
+// (src/halo_defs.py:302-334) and the inertia tensor of det_inertial_tensor_1b20.
+enum HaloField {
+    HP_M = 0, HP_PX, HP_PY, HP_PZ, HP_VX, HP_VY, HP_VZ, HP_SIGMA, HP_LX, HP_LY, HP_LZ, HP_R,
+    HP_IXX, HP_IXY, HP_IXZ, HP_IYY, HP_IYZ, HP_IZZ, HP_SHA, HP_SHB, HP_SHC, HP_EK, HP_EP, HP_ET,
+    HP_RVIR, HP_MVIR, HP_CVEL, HP_TVIR, HP_SPIN, HP_NFIELDS
+};
+
+struct HaloParamsDev {
+    double lbox = 1.0;          // Lbox_pt: periodic box side in the members' units
+    double hub_pt = 70.3;       // Hub_pt (km/s/Mpc)
+    double gravconst = 430.1;   // gravconst (halo_defs.py:482)
+    double vir_overdens = 340.0;
+    double rho_mean = 1.0;
+    int cdm = 1;                // 1: member nearest the COM (det_center_18 cdm branch); 0: densest
+    int nshell = 1000;          // ellipsoidal shells of tab_props_inside_1b40 (ttab)
+    int direct_pe_nmax = 1000;  // pair-sum potential energy below this member count
+};
+
+struct HaloTuning {
+    int tile = 0, bs = 0, trials = 0;
+    float ms = 0;   // device time of the final run, all passes
+};
+
+// pos/vel (M,3), mass (M), dens (M, may be null when cdm) are device pointers in CSR order;
+// h_off (nhalos+1) host offsets; d_out device (nhalos x HP_NFIELDS).  tile/bs <= 0 autotune.
+HaloTuning halo_props_device(const int64_t *h_off, int64_t nhalos, const double *pos,
+                             const double *vel, const double *mass, const double *dens,
+                             const HaloParamsDev &P, double *d_out, int tile, int bs,
+                             hipStream_t s);
+
+}  // namespace ahop

@@ -1,0 +1,730 @@
+// halo_props.hip -- batched per-halo property reductions on MI355X.
+//
+// Restates, for every halo of a CSR member layout at once, reference
+// src/compute_halo_props.py:
+//   det_mass_17 :754-780 (m), det_center_18 :901-966 (centre, mean velocity, sigma),
+//   compute_ang_mom_19 :783-824 (L), r_halos_1a :828-851 (r),
+//   det_inertial_tensor_1b20 :1932-1951 + det_main_axis_1b2 :1954-1967 (I, axes),
+//   det_halo_energies_1b3 :1400-1444 (ek, direct-sum ep below 1000 members),
+//   tab_props_inside_1b40 :969-1032 + potential_profile_1b40 :1069-1123 (ellipsoidal
+//   shell tables), det_vir_props_1b4 :1127-1229 (rvir, mvir, cvel, tvir),
+//   compute_spin_parameter_1c :1599-1602 (spin).
+//
+// Work decomposition: halos are cut into tiles of at most T members (T picked by the
+// autotuner in this file); every member pass is one launch with one workgroup per tile
+// writing per-tile partials, combined per halo in tile order, so results do not depend
+// on scheduling.  Accumulation is FP64.  Memory per member: pos 24 B, vel 24 B, mass 8 B
+// read per pass.
+#include "ahop_common.h"
+#include "halo_props.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <stdexcept>
+#include <vector>
+
+namespace ahop {
+
+namespace {
+
+constexpr int NSUM1 = 7;    // m, m*dr0 (3), m*v (3)
+constexpr int NSUM3 = 12;   // m dv^2, L (3), I (6), m v2 (Hubble), r2max
+constexpr double PI = 3.141592653589793;
+
+struct HaloDev {
+    const int64_t *off;
+    const double *pos, *vel, *mass, *dens;
+    const int32_t *tile_halo;
+    const int64_t *tile_start, *tile_end;
+    double L, hub, G;
+};
+
+__device__ __forceinline__ double wrap_strict(double d, double L)
+{
+    // correct_for_periodicity_1d :888-897
+    if (d > 0.5 * L) d -= L;
+    if (d < -0.5 * L) d += L;
+    return d;
+}
+
+template <int BS, int NQ>
+__device__ __forceinline__ void block_sum(double (&v)[NQ], double *sh)
+{
+    const int t = threadIdx.x;
+    for (int q = 0; q < NQ; ++q) sh[q * BS + t] = v[q];
+    __syncthreads();
+    for (int s = BS / 2; s > 0; s >>= 1) {
+        if (t < s)
+            for (int q = 0; q < NQ; ++q) sh[q * BS + t] += sh[q * BS + t + s];
+        __syncthreads();
+    }
+    for (int q = 0; q < NQ; ++q) v[q] = sh[q * BS];
+    __syncthreads();
+}
+
+// ---- pass 1: mass, COM about the halo's first member, momentum
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pass1(HaloDev D, double *part)
+{
+    __shared__ double sh[NSUM1 * BS];
+    const int tile = blockIdx.x;
+    const int h = D.tile_halo[tile];
+    const int64_t s = D.tile_start[tile], e = D.tile_end[tile];
+    const int64_t f = D.off[h];
+    const double p0x = D.pos[3 * f], p0y = D.pos[3 * f + 1], p0z = D.pos[3 * f + 2];
+    double v[NSUM1] = {0, 0, 0, 0, 0, 0, 0};
+    for (int64_t i = s + threadIdx.x; i < e; i += BS) {
+        double m = D.mass[i];
+        v[0] += m;
+        v[1] += m * wrap_strict(D.pos[3 * i] - p0x, D.L);
+        v[2] += m * wrap_strict(D.pos[3 * i + 1] - p0y, D.L);
+        v[3] += m * wrap_strict(D.pos[3 * i + 2] - p0z, D.L);
+        v[4] += m * D.vel[3 * i];
+        v[5] += m * D.vel[3 * i + 1];
+        v[6] += m * D.vel[3 * i + 2];
+    }
+    block_sum<BS, NSUM1>(v, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < NSUM1; ++q) part[(int64_t)tile * NSUM1 + q] = v[q];
+}
+
+// per-halo state written by the combine kernels
+struct HaloState {
+    double m, pcx, pcy, pcz, vx, vy, vz;   // after pass 1
+    double px, py, pz;                     // centre (after pass 2)
+    double ax[3];                          // sha, shb, shc
+    double ev[9];                          // eigenvectors, column j = axis j (row-major v[i][j])
+    double amax, bmax, cmax;
+    int64_t icenter;
+};
+
+__global__ void k_combine1(int nh, const int32_t *tile_first, const double *part,
+                           const int64_t *off, const double *pos, double L, HaloState *st)
+{
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    double a[NSUM1] = {0, 0, 0, 0, 0, 0, 0};
+    for (int t = tile_first[h]; t < tile_first[h + 1]; ++t)
+        for (int q = 0; q < NSUM1; ++q) a[q] += part[(int64_t)t * NSUM1 + q];
+    HaloState &S = st[h];
+    int64_t f = off[h];
+    double m = a[0];
+    S.m = m;
+    double c[3];
+    for (int d = 0; d < 3; ++d) {
+        double x = a[1 + d] / m + pos[3 * f + d];
+        if (x > 0.5 * L) x -= L;             // correct_for_periodicity :869-876
+        if (x < -0.5 * L) x += L;
+        c[d] = x;
+    }
+    S.pcx = c[0];
+    S.pcy = c[1];
+    S.pcz = c[2];
+    S.vx = a[4] / m;
+    S.vy = a[5] / m;
+    S.vz = a[6] / m;
+}
+
+// ---- pass 2: member nearest to the COM (cdm) or densest member; first index on ties
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pass2(HaloDev D, const HaloState *st, int cdm,
+                                              double *pkey, int64_t *pidx)
+{
+    __shared__ double sk[BS];
+    __shared__ int64_t si[BS];
+    const int tile = blockIdx.x;
+    const int h = D.tile_halo[tile];
+    const int64_t s = D.tile_start[tile], e = D.tile_end[tile];
+    const HaloState &S = st[h];
+    double best = INFINITY;
+    int64_t bi = INT64_MAX;
+    for (int64_t i = s + threadIdx.x; i < e; i += BS) {
+        double key;
+        if (cdm) {
+            double dx = wrap_strict(D.pos[3 * i] - S.pcx, D.L);
+            double dy = wrap_strict(D.pos[3 * i + 1] - S.pcy, D.L);
+            double dz = wrap_strict(D.pos[3 * i + 2] - S.pcz, D.L);
+            key = dx * dx + dy * dy + dz * dz;
+        } else {
+            key = -D.dens[i];
+        }
+        if (key < best) {
+            best = key;
+            bi = i;
+        }
+    }
+    const int t = threadIdx.x;
+    sk[t] = best;
+    si[t] = bi;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            double k2 = sk[t + w];
+            int64_t i2 = si[t + w];
+            if (k2 < sk[t] || (k2 == sk[t] && i2 < si[t])) {
+                sk[t] = k2;
+                si[t] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        pkey[tile] = sk[0];
+        pidx[tile] = si[0];
+    }
+}
+
+__global__ void k_combine2(int nh, const int32_t *tile_first, const double *pkey,
+                           const int64_t *pidx, const double *pos, HaloState *st)
+{
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    double best = INFINITY;
+    int64_t bi = -1;
+    for (int t = tile_first[h]; t < tile_first[h + 1]; ++t)
+        if (pkey[t] < best || (pkey[t] == best && pidx[t] < bi)) {
+            best = pkey[t];
+            bi = pidx[t];
+        }
+    HaloState &S = st[h];
+    S.icenter = bi;
+    S.px = pos[3 * bi];
+    S.py = pos[3 * bi + 1];
+    S.pz = pos[3 * bi + 2];
+}
+
+// ---- pass 3: second moments about the centre
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pass3(HaloDev D, const HaloState *st, double *part)
+{
+    __shared__ double sh[NSUM3 * BS];
+    const int tile = blockIdx.x;
+    const int h = D.tile_halo[tile];
+    const int64_t s = D.tile_start[tile], e = D.tile_end[tile];
+    const HaloState &S = st[h];
+    double v[NSUM3] = {0};
+    double r2max = 0.0;
+    for (int64_t i = s + threadIdx.x; i < e; i += BS) {
+        double m = D.mass[i];
+        double dx = wrap_strict(D.pos[3 * i] - S.px, D.L);
+        double dy = wrap_strict(D.pos[3 * i + 1] - S.py, D.L);
+        double dz = wrap_strict(D.pos[3 * i + 2] - S.pz, D.L);
+        double ux = D.vel[3 * i] - S.vx, uy = D.vel[3 * i + 1] - S.vy, uz = D.vel[3 * i + 2] - S.vz;
+        v[0] += m * (ux * ux + uy * uy + uz * uz);
+        double pvx = m * ux, pvy = m * uy, pvz = m * uz;
+        v[1] += dy * pvz - dz * pvy;
+        v[2] += dz * pvx - dx * pvz;
+        v[3] += dx * pvy - dy * pvx;
+        v[4] += m * dx * dx;
+        v[5] += m * dx * dy;
+        v[6] += m * dx * dz;
+        v[7] += m * dy * dy;
+        v[8] += m * dy * dz;
+        v[9] += m * dz * dz;
+        double tx = ux + dx * D.hub, ty = uy + dy * D.hub, tz = uz + dz * D.hub;
+        v[10] += m * (tx * tx + ty * ty + tz * tz);
+        r2max = fmax(r2max, dx * dx + dy * dy + dz * dz);
+    }
+    v[11] = r2max;
+    // sums for 0..10, max for 11
+    const int t = threadIdx.x;
+    for (int q = 0; q < NSUM3; ++q) sh[q * BS + t] = v[q];
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            for (int q = 0; q < NSUM3 - 1; ++q) sh[q * BS + t] += sh[q * BS + t + w];
+            sh[11 * BS + t] = fmax(sh[11 * BS + t], sh[11 * BS + t + w]);
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int q = 0; q < NSUM3; ++q) part[(int64_t)tile * NSUM3 + q] = sh[q * BS];
+}
+
+// Symmetric 3x3 eigen-decomposition by cyclic Jacobi rotations (FP64), eigenvalues
+// ascending with matching eigenvector columns, as scipy.linalg.eigh returns them.
+__device__ void eigh3(const double A[6], double d[3], double V[9])
+{
+    double a[3][3] = {{A[0], A[1], A[2]}, {A[1], A[3], A[4]}, {A[2], A[4], A[5]}};
+    double v[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 60; ++sweep) {
+        double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+        double diag = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+        if (off <= 1e-34 * diag || off == 0.0) break;
+        for (int p = 0; p < 2; ++p)
+            for (int q = p + 1; q < 3; ++q) {
+                double apq = a[p][q];
+                if (apq == 0.0) continue;
+                double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+                double t = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < 3; ++k) {   // A <- A J
+                    double akp = a[k][p], akq = a[k][q];
+                    a[k][p] = c * akp - s * akq;
+                    a[k][q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < 3; ++k) {   // A <- J^T A
+                    double apk = a[p][k], aqk = a[q][k];
+                    a[p][k] = c * apk - s * aqk;
+                    a[q][k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    double vkp = v[k][p], vkq = v[k][q];
+                    v[k][p] = c * vkp - s * vkq;
+                    v[k][q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    int idx[3] = {0, 1, 2};
+    double w[3] = {a[0][0], a[1][1], a[2][2]};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2 - i; ++j)
+            if (w[idx[j]] > w[idx[j + 1]]) {
+                int tmp = idx[j];
+                idx[j] = idx[j + 1];
+                idx[j + 1] = tmp;
+            }
+    for (int j = 0; j < 3; ++j) {
+        d[j] = w[idx[j]];
+        for (int i = 0; i < 3; ++i) V[3 * i + j] = v[i][idx[j]];
+    }
+}
+
+__global__ void k_combine3(int nh, const int32_t *tile_first, const double *part, double G,
+                           HaloState *st, double *out)
+{
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    double a[NSUM3] = {0};
+    for (int t = tile_first[h]; t < tile_first[h + 1]; ++t) {
+        for (int q = 0; q < NSUM3 - 1; ++q) a[q] += part[(int64_t)t * NSUM3 + q];
+        a[11] = fmax(a[11], part[(int64_t)t * NSUM3 + 11]);
+    }
+    HaloState &S = st[h];
+    double *o = out + (int64_t)h * HP_NFIELDS;
+    o[HP_M] = S.m;
+    o[HP_PX] = S.px;
+    o[HP_PY] = S.py;
+    o[HP_PZ] = S.pz;
+    o[HP_VX] = S.vx;
+    o[HP_VY] = S.vy;
+    o[HP_VZ] = S.vz;
+    o[HP_SIGMA] = sqrt(a[0] / S.m);
+    o[HP_LX] = a[1];
+    o[HP_LY] = a[2];
+    o[HP_LZ] = a[3];
+    o[HP_R] = sqrt(a[11]);
+    for (int q = 0; q < 6; ++q) o[HP_IXX + q] = a[4 + q];
+    o[HP_EK] = 0.5 * a[10];
+    double d[3];
+    eigh3(&a[4], d, S.ev);
+    for (int j = 0; j < 3; ++j) {
+        S.ax[j] = sqrt(d[j] / S.m);
+        o[HP_SHA + j] = S.ax[j];
+    }
+}
+
+__device__ __forceinline__ void ell_proj(const HaloDev &D, const HaloState &S, int64_t i,
+                                         double &ra, double &rb, double &rc)
+{
+    double dx = wrap_strict(D.pos[3 * i] - S.px, D.L);
+    double dy = wrap_strict(D.pos[3 * i + 1] - S.py, D.L);
+    double dz = wrap_strict(D.pos[3 * i + 2] - S.pz, D.L);
+    // dras = drx*v[0,0] + dry*v[1,0] + drz*v[2,0] (tab_props_inside_1b40 :984-986)
+    ra = dx * S.ev[0] + dy * S.ev[3] + dz * S.ev[6];
+    rb = dx * S.ev[1] + dy * S.ev[4] + dz * S.ev[7];
+    rc = dx * S.ev[2] + dy * S.ev[5] + dz * S.ev[8];
+}
+
+// ---- pass 4a: largest ellipsoidal radius in units of the principal axes
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pass4a(HaloDev D, const HaloState *st, double *pmax)
+{
+    __shared__ double sh[BS];
+    const int tile = blockIdx.x;
+    const int h = D.tile_halo[tile];
+    const int64_t s = D.tile_start[tile], e = D.tile_end[tile];
+    const HaloState &S = st[h];
+    double mx = 0.0;
+    for (int64_t i = s + threadIdx.x; i < e; i += BS) {
+        double ra, rb, rc;
+        ell_proj(D, S, i, ra, rb, rc);
+        double qa = ra / S.ax[0], qb = rb / S.ax[1], qc = rc / S.ax[2];
+        mx = fmax(mx, sqrt(qa * qa + qb * qb + qc * qc));
+    }
+    const int t = threadIdx.x;
+    sh[t] = mx;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] = fmax(sh[t], sh[t + w]);
+        __syncthreads();
+    }
+    if (t == 0) pmax[tile] = sh[0];
+}
+
+__global__ void k_combine4a(int nh, const int32_t *tile_first, const double *pmax, HaloState *st)
+{
+    int h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    double rmax = 0.0;
+    for (int t = tile_first[h]; t < tile_first[h + 1]; ++t) rmax = fmax(rmax, pmax[t]);
+    HaloState &S = st[h];
+    const double eps = 1e-2;
+    S.amax = rmax * S.ax[0] * (1.0 + eps);
+    S.bmax = rmax * S.ax[1] * (1.0 + eps);
+    S.cmax = rmax * S.ax[2] * (1.0 + eps);
+}
+
+// ---- pass 4b: per-tile shell histograms (mass, kinetic energy, count) in LDS
+template <int BS>
+__global__ __launch_bounds__(BS) void k_pass4b(HaloDev D, const HaloState *st, int nr, double *hist,
+                                               int *missed)
+{
+    extern __shared__ double hs[];   // 3*nr doubles
+    const int tile = blockIdx.x;
+    const int h = D.tile_halo[tile];
+    const int64_t s = D.tile_start[tile], e = D.tile_end[tile];
+    const HaloState &S = st[h];
+    for (int b = threadIdx.x; b < 3 * nr; b += BS) hs[b] = 0.0;
+    __syncthreads();
+    for (int64_t i = s + threadIdx.x; i < e; i += BS) {
+        double ra, rb, rc;
+        ell_proj(D, S, i, ra, rb, rc);
+        double qa = ra / S.amax, qb = rb / S.bmax, qc = rc / S.cmax;
+        double r = sqrt(qa * qa + qb * qb + qc * qc);
+        double fi = r * nr;
+        int ib = (int)fi;
+        if (!(fi < (double)nr) || ib >= nr) {
+            atomicAdd(missed, 1);
+            continue;
+        }
+        double m = D.mass[i];
+        double dx = wrap_strict(D.pos[3 * i] - S.px, D.L);
+        double dy = wrap_strict(D.pos[3 * i + 1] - S.py, D.L);
+        double dz = wrap_strict(D.pos[3 * i + 2] - S.pz, D.L);
+        double tx = D.vel[3 * i] - S.vx + dx * D.hub;
+        double ty = D.vel[3 * i + 1] - S.vy + dy * D.hub;
+        double tz = D.vel[3 * i + 2] - S.vz + dz * D.hub;
+        double v2 = tx * tx + ty * ty + tz * tz;
+        atomicAdd(&hs[ib], m);
+        atomicAdd(&hs[nr + ib], 0.5 * m * v2);
+        atomicAdd(&hs[2 * nr + ib], 1.0);
+    }
+    __syncthreads();
+    double *out = hist + (int64_t)tile * 3 * nr;
+    for (int b = threadIdx.x; b < 3 * nr; b += BS) out[b] = hs[b];
+}
+
+// ---- direct potential energy of halos below the pair-sum limit (det_halo_energies_1b3 :1417-1427)
+template <int BS>
+__global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *small_halos,
+                                                  double *ep_out)
+{
+    __shared__ double sh[BS];
+    const int h = small_halos[blockIdx.x];
+    const int64_t f = D.off[h];
+    const int64_t n = D.off[h + 1] - f;
+    double acc = 0.0;
+    for (int64_t i = 0; i < n - 1; ++i) {     // pairs (i, j > i), each counted once
+        const int64_t a = f + i;
+        const double ax = D.pos[3 * a], ay = D.pos[3 * a + 1], az = D.pos[3 * a + 2];
+        const double ma = D.mass[a];
+        for (int64_t j = i + 1 + threadIdx.x; j < n; j += BS) {
+            const int64_t b = f + j;
+            double dx = wrap_strict(D.pos[3 * b] - ax, D.L);
+            double dy = wrap_strict(D.pos[3 * b + 1] - ay, D.L);
+            double dz = wrap_strict(D.pos[3 * b + 2] - az, D.L);
+            double dist = sqrt(dx * dx + dy * dy + dz * dz) / D.L;
+            acc += -(ma * D.mass[b]) / dist;
+        }
+    }
+    const int t = threadIdx.x;
+    sh[t] = acc;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) ep_out[h] = sh[0] * D.G / D.L;
+}
+
+// ---- shell tables, potential profile and the virial scan, one workgroup per halo
+template <int BS>
+__global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first,
+                                               const int64_t *off, const double *hist, int nr,
+                                               const HaloState *st, const double *ep_direct,
+                                               HaloParamsDev P, double *out)
+{
+    extern __shared__ double work[];          // 4*nr doubles
+    const int h = blockIdx.x;
+    if (h >= nh) return;
+    double *tm = work;                         // cumulative mass
+    double *tk = tm + nr;                      // cumulative kinetic energy
+    double *cn = tk + nr;                      // shell counts
+    double *tp = cn + nr;                      // potential profile
+    const HaloState &S = st[h];
+    const int t0 = tile_first[h], t1 = tile_first[h + 1];
+    for (int b = threadIdx.x; b < nr; b += BS) {
+        double m = 0, k = 0, c = 0;
+        for (int t = t0; t < t1; ++t) {
+            const double *hh = hist + (int64_t)t * 3 * nr;
+            m += hh[b];
+            k += hh[nr + b];
+            c += hh[2 * nr + b];
+        }
+        tm[b] = m;
+        tk[b] = k;
+        cn[b] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    // cumulative tables, potential profile (shell / count self pairs / profile outer for large)
+    const int64_t count = off[h + 1] - off[h];
+    double reff0 = cbrt(S.amax * S.bmax * S.cmax);
+    double accm = 0, acck = 0, accp = 0;
+    for (int b = 0; b < nr; ++b) {
+        double dm = tm[b];
+        double before = accm;
+        accm += dm;
+        acck += tk[b];
+        double sf = cn[b] > 0 ? 0.5 * (cn[b] - 1.0) / cn[b] : 0.0;
+        double scale = (b == 0) ? 0.5 / (double)(nr - 1) : (double)b / (double)(nr - 1);
+        accp += (before + sf * dm) * dm / (scale * reff0);
+        tm[b] = accm;
+        tk[b] = acck;
+        tp[b] = -P.gravconst * accp;
+    }
+    double outer = tp[nr - 1];
+    double *o = out + (int64_t)h * HP_NFIELDS;
+    double ep;
+    if (count < P.direct_pe_nmax) {
+        ep = ep_direct[h];
+        if (ep != outer && outer != 0.0)
+            for (int b = 0; b < nr; ++b) tp[b] = tp[b] / outer * ep;
+    } else {
+        ep = outer;
+    }
+    double ek = o[HP_EK];
+    double et = ek + ep;
+    o[HP_EP] = ep;
+    o[HP_ET] = et;
+    // det_vir_props_1b4 :1127-1229
+    const int ttab = nr;
+    double amax = S.amax, bmax = S.bmax, cmax = S.cmax;
+    double mvir = tm[ttab - 1];
+    int i1 = 1;
+    for (i1 = ttab - 1; i1 >= 1; --i1)
+        if (tm[i1 - 1] < tm[i1]) break;
+    if (i1 < 1) i1 = 1;
+    double avir = (double)i1 / (ttab - 1) * amax;
+    double bvir = (double)i1 / (ttab - 1) * bmax;
+    double cvir = (double)i1 / (ttab - 1) * cmax;
+    double rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
+    double virth_old = 1.0, virth = 100.0;
+    for (int i0 = ttab - 1; i0 >= 0; --i0) {
+        if ((tk[i0] + tp[i0]) >= 0.0) continue;
+        virth = fabs((2.0 * tk[i0] + tp[i0]) / (tk[i0] + tp[i0]));
+        if (virth < virth_old) {
+            mvir = tm[i0];
+            avir = fmin(avir, (double)i0 / (ttab - 1) * amax);
+            bvir = fmin(bvir, (double)i0 / (ttab - 1) * bmax);
+            cvir = fmin(cvir, (double)i0 / (ttab - 1) * cmax);
+            rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
+            virth_old = virth;
+            if (virth <= 0.20) break;
+        }
+    }
+    double volmin = 4. / 3. * PI * (amax / (ttab - 1)) * (bmax / (ttab - 1)) * (cmax / (ttab - 1));
+    if ((virth > 0.20) || (mvir < P.vir_overdens * P.rho_mean * 4. / 3. * PI * avir * bvir * cvir)) {
+        int ii1;
+        for (ii1 = ttab - 1; ii1 >= 1; --ii1) {
+            double mv = P.vir_overdens * P.rho_mean * volmin * (double)ii1 * (double)ii1 * (double)ii1;
+            if ((tm[ii1] >= mv) && (tm[ii1 - 1] < tm[ttab - 1])) break;
+        }
+        if (ii1 < 1) ii1 = 1;
+        mvir = tm[ii1];
+        avir = (double)ii1 / (ttab - 1) * amax;
+        bvir = (double)ii1 / (ttab - 1) * bmax;
+        cvir = (double)ii1 / (ttab - 1) * cmax;
+        rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
+    }
+    if (mvir > 0.0 && rvir > 0.0) {
+        double rv = fmin(rvir, o[HP_R]);
+        o[HP_RVIR] = rv;
+        o[HP_MVIR] = mvir;
+        o[HP_CVEL] = sqrt(P.gravconst * mvir / rv);
+        o[HP_TVIR] = 35.9 * P.gravconst * mvir / rv;
+    } else {
+        o[HP_RVIR] = o[HP_MVIR] = o[HP_CVEL] = o[HP_TVIR] = NAN;
+    }
+    double hl = sqrt(o[HP_LX] * o[HP_LX] + o[HP_LY] * o[HP_LY] + o[HP_LZ] * o[HP_LZ]);
+    o[HP_SPIN] = hl * sqrt(fabs(et)) / pow(S.m, 2.5) / P.gravconst;
+}
+
+template <typename T>
+T *dalloc(std::vector<void *> &keep, size_t n)
+{
+    void *p = nullptr;
+    AHOP_HIP(hipMalloc(&p, sizeof(T) * (n ? n : 1)));
+    keep.push_back(p);
+    return (T *)p;
+}
+
+struct Tiles {
+    std::vector<int32_t> halo, first;
+    std::vector<int64_t> start, end;
+};
+
+Tiles make_tiles(const std::vector<int64_t> &off, int64_t T)
+{
+    Tiles t;
+    int nh = (int)off.size() - 1;
+    t.first.resize(nh + 1);
+    for (int h = 0; h < nh; ++h) {
+        t.first[h] = (int32_t)t.halo.size();
+        for (int64_t s = off[h]; s < off[h + 1]; s += T) {
+            t.halo.push_back(h);
+            t.start.push_back(s);
+            t.end.push_back(std::min(s + T, off[h + 1]));
+        }
+    }
+    t.first[nh] = (int32_t)t.halo.size();
+    return t;
+}
+
+}  // namespace
+
+// Runs every pass for one tile size / workgroup size; returns device milliseconds.
+template <int BS>
+static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, const double *pos,
+                        const double *vel, const double *mass, const double *dens, int64_t T,
+                        const HaloParamsDev &P, double *d_out, hipStream_t s, bool full)
+{
+    const int nh = (int)hoff.size() - 1;
+    Tiles tl = make_tiles(hoff, T);
+    const int nt = (int)tl.halo.size();
+    std::vector<void *> keep;
+    auto cleanup = [&]() {
+        for (void *p : keep) (void)hipFree(p);
+        keep.clear();
+    };
+    try {
+        int32_t *d_th = dalloc<int32_t>(keep, nt), *d_tf = dalloc<int32_t>(keep, nh + 1);
+        int64_t *d_ts = dalloc<int64_t>(keep, nt), *d_te = dalloc<int64_t>(keep, nt);
+        AHOP_HIP(hipMemcpyAsync(d_th, tl.halo.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice, s));
+        AHOP_HIP(hipMemcpyAsync(d_tf, tl.first.data(), sizeof(int32_t) * (nh + 1), hipMemcpyHostToDevice, s));
+        AHOP_HIP(hipMemcpyAsync(d_ts, tl.start.data(), sizeof(int64_t) * nt, hipMemcpyHostToDevice, s));
+        AHOP_HIP(hipMemcpyAsync(d_te, tl.end.data(), sizeof(int64_t) * nt, hipMemcpyHostToDevice, s));
+        HaloDev D{d_off, pos, vel, mass, dens, d_th, d_ts, d_te, P.lbox, P.hub_pt, P.gravconst};
+        double *part = dalloc<double>(keep, (size_t)nt * NSUM3);
+        double *pkey = dalloc<double>(keep, nt);
+        int64_t *pidx = dalloc<int64_t>(keep, nt);
+        HaloState *st = dalloc<HaloState>(keep, nh);
+        int *missed = dalloc<int>(keep, 1);
+        AHOP_HIP(hipMemsetAsync(missed, 0, sizeof(int), s));
+        const unsigned hb = (unsigned)((nh + 127) / 128);
+        hipEvent_t e0, e1;
+        AHOP_HIP(hipEventCreate(&e0));
+        AHOP_HIP(hipEventCreate(&e1));
+        AHOP_HIP(hipEventRecord(e0, s));
+        hipLaunchKernelGGL(k_pass1<BS>, dim3(nt), dim3(BS), 0, s, D, part);
+        hipLaunchKernelGGL(k_combine1, dim3(hb), dim3(128), 0, s, nh, d_tf, part, d_off, pos, P.lbox, st);
+        hipLaunchKernelGGL(k_pass2<BS>, dim3(nt), dim3(BS), 0, s, D, st, P.cdm, pkey, pidx);
+        hipLaunchKernelGGL(k_combine2, dim3(hb), dim3(128), 0, s, nh, d_tf, pkey, pidx, pos, st);
+        hipLaunchKernelGGL(k_pass3<BS>, dim3(nt), dim3(BS), 0, s, D, st, part);
+        hipLaunchKernelGGL(k_combine3, dim3(hb), dim3(128), 0, s, nh, d_tf, part, P.gravconst, st, d_out);
+        if (full) {
+            hipLaunchKernelGGL(k_pass4a<BS>, dim3(nt), dim3(BS), 0, s, D, st, pkey);
+            hipLaunchKernelGGL(k_combine4a, dim3(hb), dim3(128), 0, s, nh, d_tf, pkey, st);
+            const int nr = P.nshell;
+            double *hist = dalloc<double>(keep, (size_t)nt * 3 * nr);
+            hipLaunchKernelGGL(k_pass4b<BS>, dim3(nt), dim3(BS), sizeof(double) * 3 * nr, s, D, st,
+                               nr, hist, missed);
+            std::vector<int32_t> small;
+            for (int h = 0; h < nh; ++h)
+                if (hoff[h + 1] - hoff[h] < P.direct_pe_nmax) small.push_back(h);
+            double *ep = dalloc<double>(keep, nh);
+            if (!small.empty()) {
+                int32_t *d_small = dalloc<int32_t>(keep, small.size());
+                AHOP_HIP(hipMemcpyAsync(d_small, small.data(), sizeof(int32_t) * small.size(),
+                                        hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL(k_direct_pe<256>, dim3((unsigned)small.size()), dim3(256), 0, s, D,
+                                   d_small, ep);
+            }
+            hipLaunchKernelGGL(k_virial<256>, dim3(nh), dim3(256), sizeof(double) * 4 * nr, s, nh,
+                               d_tf, d_off, hist, nr, st, ep, P, d_out);
+        }
+        AHOP_HIP(hipEventRecord(e1, s));
+        AHOP_HIP(hipStreamSynchronize(s));
+        float ms = 0;
+        AHOP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        int hmissed = 0;
+        AHOP_HIP(hipMemcpy(&hmissed, missed, sizeof(int), hipMemcpyDeviceToHost));
+        cleanup();
+        if (hmissed)
+            throw std::runtime_error("tab_props_inside: " + std::to_string(hmissed) +
+                                     " members fall outside the outer ellipsoid");
+        return ms;
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+}
+
+HaloTuning halo_props_device(const int64_t *h_off, int64_t nhalos, const double *pos,
+                             const double *vel, const double *mass, const double *dens,
+                             const HaloParamsDev &P, double *d_out, int tile, int bs,
+                             hipStream_t s)
+{
+    if (nhalos <= 0) throw std::runtime_error("need at least one halo");
+    if (P.nshell < 2) throw std::runtime_error("nshell must be >= 2");
+    if (!P.cdm && !dens) throw std::runtime_error("densest-member centre needs densities");
+    std::vector<int64_t> hoff(h_off, h_off + nhalos + 1);
+    for (int64_t h = 0; h < nhalos; ++h)
+        if (hoff[h + 1] <= hoff[h]) throw std::runtime_error("every halo needs at least one member");
+    const int64_t M = hoff[nhalos] - hoff[0];
+    int64_t *d_off = nullptr;
+    AHOP_HIP(hipMalloc(&d_off, sizeof(int64_t) * (nhalos + 1)));
+    AHOP_HIP(hipMemcpyAsync(d_off, hoff.data(), sizeof(int64_t) * (nhalos + 1), hipMemcpyHostToDevice, s));
+    HaloTuning tune{};
+    try {
+        if (tile <= 0 || bs <= 0) {
+            // autotune the member-pass schedule on this workload: time passes 1-3 for each
+            // candidate (tile, workgroup) and keep the fastest
+            const int64_t tiles[] = {1024, 4096, 16384, 65536};
+            const int wgs[] = {128, 256};
+            float best = 1e30f;
+            for (int64_t T : tiles)
+                for (int w : wgs) {
+                    if (T > std::max<int64_t>(M, 1024) * 4) continue;
+                    float ms = (w == 128) ? run_passes<128>(hoff, d_off, pos, vel, mass, dens, T, P, d_out, s, false)
+                                          : run_passes<256>(hoff, d_off, pos, vel, mass, dens, T, P, d_out, s, false);
+                    tune.trials++;
+                    if (ms < best) {
+                        best = ms;
+                        tune.tile = (int)T;
+                        tune.bs = w;
+                    }
+                }
+        } else {
+            tune.tile = tile;
+            tune.bs = bs;
+        }
+        if (tune.bs == 128)
+            tune.ms = run_passes<128>(hoff, d_off, pos, vel, mass, dens, tune.tile, P, d_out, s, true);
+        else if (tune.bs == 256)
+            tune.ms = run_passes<256>(hoff, d_off, pos, vel, mass, dens, tune.tile, P, d_out, s, true);
+        else
+            throw std::runtime_error("workgroup size must be 128 or 256");
+    } catch (...) {
+        (void)hipFree(d_off);
+        throw;
+    }
+    (void)hipFree(d_off);
+    return tune;
+}
+
+}  // namespace ahop

@@ -103,6 +103,42 @@ int ahop_run_device(const double *d_pos, const double *d_mass, int64_t n,
 /* Host copies of the saddle CSR of the last ahop_run_device (sizes from its info). */
 int ahop_device_saddle(int32_t *offset, int32_t *isad, double *rho);
 
+/* ---- batched per-halo property reductions ------------------------------------------- */
+
+/* Row layout of the output (one row of AHOP_HALO_NFIELDS doubles per halo):
+ * m, px, py, pz, vx, vy, vz, sigma, Lx, Ly, Lz, r, Ixx, Ixy, Ixz, Iyy, Iyz, Izz,
+ * sha, shb, shc, ek, ep, et, rvir, mvir, cvel, tvir, spin */
+#define AHOP_HALO_NFIELDS 29
+
+typedef struct {
+    double lbox;            /* Lbox_pt, periodic box side in the members' length unit */
+    double hub_pt;          /* Hub_pt */
+    double gravconst;       /* gravconst, 430.1 in (km/s)^2 Mpc / 1e11 Msun */
+    double vir_overdens;    /* vir_overdens of virial_12 */
+    double rho_mean;        /* rho_mean of virial_12 */
+    int32_t cdm;            /* 1: centre = member nearest the centre of mass; 0: densest member */
+    int32_t nshell;         /* ellipsoidal shells (ttab = 1000 in the reference) */
+    int32_t direct_pe_nmax; /* direct pair-sum potential energy below this count (1000) */
+    int32_t tile;           /* members per work tile; <= 0 lets the autotuner choose */
+    int32_t workgroup;      /* 128 or 256; <= 0 lets the autotuner choose */
+} ahop_halo_params;
+
+typedef struct {
+    int32_t tile, workgroup, trials;
+    float ms;               /* device time of the final run, all passes */
+} ahop_halo_info;
+
+/* Properties of every halo of a CSR member layout in one call.  offsets: host array of
+ * nhalos+1 member offsets; d_pos/d_vel (M,3) row-major, d_mass (M), d_density (M, may be
+ * NULL when cdm=1) and d_out (nhalos x AHOP_HALO_NFIELDS) are device pointers.
+ * Replaces the per-halo fork-Pool loop of src/compute_halo_props.py:487-534 over
+ * det_mass_17 (:754), det_center_18 (:901), compute_ang_mom_19 (:783), r_halos_1a (:828),
+ * det_vir_1b (:1241: det_main_axis_1b2, det_halo_energies_1b3, det_vir_props_1b4) and
+ * compute_spin_parameter_1c (:1599). */
+int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
+                           const double *d_vel, const double *d_mass, const double *d_density,
+                           const ahop_halo_params *params, double *d_out, ahop_halo_info *info);
+
 /* Host structure-tree stage alone (create_group_tree f90:885-983) on host inputs, for
  * callers that computed the hot-path stages elsewhere.  Outputs: liste_parts (npart),
  * *nnodes, and the tables, which must have room for 2*ngroups nodes (nnodes <= 2*ngroups,

@@ -113,3 +113,15 @@ def test_host_structure_tree_sensitive_to_threshold():
         ptr(np.ascontiguousarray(g["saddle_rho"])), 80.0, 4.0, 1.0, eps, 2000, 1.0, ptr(lp),
         ctypes.byref(nn), ptr(rt), ptr(it)))
     assert nn.value < int(g["nnodes"])
+
+
+def test_sync_calls_accept_the_bridge_argument_lists():
+    """The three sync_* calls with the argument lists compute_neiKDtree_mod.py passes
+    (init_adaptahop_130 :183-197, change_pos_1310 :207-210, sync_fortran :740-747)."""
+    from compute_adaptahop import neikdtree
+    neikdtree.sync_from_init_adaptahop(1000, 20, 20, 0.7, 0.7, 0.3, 0.3, 1.0, 1.0, 0.7, 70.0, 1.0,
+                                       1.0, 1.0, 1.0, 1.0, 0.5, 0.5, 0.5, 70.0, 1.0,
+                                       np.zeros(3))
+    neikdtree.sync_from_change_pos(np.int64(1000), 1000, 1e-4, 0.001, 1.0, 1.0)
+    neikdtree.sync_others(False, False, 1000, 16, 80.0, 1e-3, 0.0, 16, 32, 4.0, 1.0, "MSM", 30, False)
+    neikdtree.close()

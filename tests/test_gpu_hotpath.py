@@ -136,9 +136,15 @@ def test_c3_full_size_sample_and_invariants(gpu):
     act = dmx > 80.0
     assert int(act.sum()) == imx.nactive
     assert np.all(gmx[~act] == 0) and np.all(gmx[act] >= 1) and gmx.max() == imx.ngroups
-    # group ids number the roots by ascending particle index: first occurrence order
-    first = np.unique(gmx[act], return_index=True)[1]
-    assert np.all(np.diff(np.nonzero(act)[0][first]) > 0)
+    # every hop chain climbs in (density, -index), so a group's root is its densest member
+    # (lowest index on ties); group ids number the roots by ascending index (f90:797-803)
+    idx_act = np.nonzero(act)[0]
+    order = np.lexsort((idx_act, -dmx[idx_act], gmx[idx_act]))
+    g_sorted = gmx[idx_act][order]
+    root_pos = np.nonzero(np.r_[True, g_sorted[1:] != g_sorted[:-1]])[0]
+    roots = idx_act[order][root_pos]
+    assert np.array_equal(g_sorted[root_pos], np.arange(1, imx.ngroups + 1))
+    assert np.all(np.diff(roots) > 0)
 
 
 def test_saddle_symmetric_and_bounded_c2(gpu):
