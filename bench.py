@@ -123,6 +123,12 @@ def main():
                           "knn_density": round(last.t_knn, 3), "fp64_rerun": round(last.t_fallback, 3),
                           "hop_groups": round(last.t_hop, 3)}
     extra["nactive"], extra["ngroups"], extra["nfallback"] = int(last.nactive), int(last.ngroups), int(last.nfallback)
+    w = max(1, last.knn_waves)
+    extra["knn_work_per_wave"] = {"nodes": round(last.knn_nodes / w, 1),
+                                  "buckets": round(last.knn_buckets / w, 1),
+                                  "candidates_per_lane": round(last.knn_candidates / w, 1),
+                                  "insert_steps": round(last.knn_insert_steps / w, 1),
+                                  "inserts_per_lane": round(last.knn_inserts / w / 64, 1)}
 
     achieved = KNN_BYTES_PER_PARTICLE * n / (knn_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -435,6 +435,12 @@ int ahop_run_device(const double *d_pos, const double *d_mass, int64_t n,
             info->t_hop = t.hop;
             info->t_saddle = t.saddle;
             info->t_total = t.total;
+            info->knn_nodes = (int64_t)r.knn_stats[0];
+            info->knn_buckets = (int64_t)r.knn_stats[1];
+            info->knn_candidates = (int64_t)r.knn_stats[2];
+            info->knn_insert_steps = (int64_t)r.knn_stats[3];
+            info->knn_inserts = (int64_t)r.knn_stats[4];
+            info->knn_waves = (int64_t)r.knn_stats[5];
         }
     });
 }

@@ -32,6 +32,7 @@ struct DeviceTree {
     int32_t *parent = nullptr, *leaf_parent = nullptr;
     int32_t *flag = nullptr;
     double *box = nullptr;                      // 6 doubles per internal node: lo xyz, hi xyz
+    float4 *fbox = nullptr;                     // FP32 centre / half-widths (2 per node)
 };
 
 // Per-stage device timings of the last run (milliseconds, HIP events on the run's stream).
@@ -54,6 +55,9 @@ struct DeviceResults {
     double *sad_rho = nullptr;
     double *densityg = nullptr;       // [ngroups]
     int64_t nsaddle = 0;
+    // work counters of the main kNN launch: nodes visited, buckets scanned, candidate
+    // distances per lane, heap-insertion steps, heap insertions, waves
+    unsigned long long knn_stats[6] = {0, 0, 0, 0, 0, 0};
 };
 
 // Growable device allocation owned by an Engine.
@@ -93,7 +97,7 @@ class Engine {
         b_leaf_parent, b_flag, b_box, b_temp, b_rho_s, b_nbr, b_hop_s, b_gid_s, b_flag_o,
         b_rank_o, b_density, b_gid, b_hop, b_counter, b_fallback, b_emit_cnt, b_emit_off,
         b_ekey, b_eval, b_ekey2, b_eval2, b_ukey, b_uval, b_keep, b_keep_off, b_skey, b_sval,
-        b_sad_off, b_sad_isad, b_sad_rho, b_sad_cnt, b_densityg, b_misc, b_roots;
+        b_sad_off, b_sad_isad, b_sad_rho, b_sad_cnt, b_densityg, b_misc, b_roots, b_fbox;
     int64_t nmax_ = 0;
 };
 

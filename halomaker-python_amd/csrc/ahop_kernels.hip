@@ -227,27 +227,99 @@ __device__ __forceinline__ void child_box(int c, const double4 *__restrict__ spo
     }
 }
 
-// Bottom-up bounding boxes: the second thread to reach a node merges its children.
-__global__ void k_refit(int n, const double4 *__restrict__ spos, const int32_t *__restrict__ left,
-                        const int32_t *__restrict__ right, const int32_t *__restrict__ parent,
-                        const int32_t *__restrict__ leaf_parent, int32_t *flag, double *box)
+// Bottom-up bounding boxes in two phases.  A fence per climbing thread at device scope
+// (needed for cross-XCD visibility) costs ~1e2 ms at 16.8M particles, so:
+//  phase 1 (k_refit_local): one workgroup per 256 consecutive leaves merges every node whose
+//    sorted range lies inside its 256-leaf block, communicating through LDS counters and
+//    workgroup-scope fences only;
+//  phase 2 (k_refit_upper): the few nodes whose range crosses a block boundary are merged by
+//    threads climbing from the lowest crossing nodes with device-scope fences.
+constexpr int REFIT_BS = 256;
+
+__device__ __forceinline__ void merge_children(int node, const double4 *__restrict__ spos,
+                                               const int32_t *__restrict__ left,
+                                               const int32_t *__restrict__ right, double *box)
 {
-    int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
+    double a[6], b[6];
+    child_box(left[node], spos, box, a);
+    child_box(right[node], spos, box, b);
+    for (int q = 0; q < 3; ++q) {
+        box[6 * (int64_t)node + q] = fmin(a[q], b[q]);
+        box[6 * (int64_t)node + q + 3] = fmax(a[q + 3], b[q + 3]);
+    }
+}
+
+__global__ __launch_bounds__(REFIT_BS) void k_refit_local(
+    int n, const double4 *__restrict__ spos, const int32_t *__restrict__ first,
+    const int32_t *__restrict__ last, const int32_t *__restrict__ left,
+    const int32_t *__restrict__ right, const int32_t *__restrict__ parent,
+    const int32_t *__restrict__ leaf_parent, double *box)
+{
+    __shared__ int cnt[REFIT_BS];
+    const int B0 = blockIdx.x * REFIT_BS;
+    const int B1 = min(n, B0 + REFIT_BS) - 1;   // last leaf of the block
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int p = B0 + threadIdx.x;
+    if (p > B1) return;
     int node = leaf_parent[p];
-    while (node >= 0) {
-        __threadfence();
-        int old = atomicAdd(&flag[node], 1);
-        if (old == 0) return;
-        __threadfence();
-        double a[6], b[6];
-        child_box(left[node], spos, box, a);
-        child_box(right[node], spos, box, b);
-        for (int q = 0; q < 3; ++q) {
-            box[6 * (int64_t)node + q] = fmin(a[q], b[q]);
-            box[6 * (int64_t)node + q + 3] = fmax(a[q + 3], b[q + 3]);
-        }
+    while (node >= 0 && first[node] >= B0 && last[node] <= B1) {
+        __threadfence_block();
+        if (atomicAdd(&cnt[node - B0], 1) == 0) return;   // sibling subtree not done yet
+        __threadfence_block();
+        merge_children(node, spos, left, right, box);
         node = parent[node];
+    }
+}
+
+// FP32 copy of the node boxes for the traversal tests: centre rounded to nearest,
+// half-widths rounded up (the test's margin covers the centre rounding)
+__global__ void k_float_boxes(int n, const double *__restrict__ box, float4 *__restrict__ fbox)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const double *b = box + 6 * (int64_t)i;
+    float c[3], h[3];
+    for (int a = 0; a < 3; ++a) {
+        c[a] = (float)(0.5 * (b[a] + b[a + 3]));
+        h[a] = __double2float_ru(0.5 * (b[a + 3] - b[a]));
+    }
+    fbox[2 * (int64_t)i] = make_float4(c[0], c[1], c[2], h[0]);
+    fbox[2 * (int64_t)i + 1] = make_float4(h[1], h[2], 0.0f, 0.0f);
+}
+
+// counter of a crossing node = number of its children already merged by phase 1 (or leaves)
+__global__ void k_refit_upper_init(int n, const int32_t *__restrict__ first,
+                                   const int32_t *__restrict__ last, const int32_t *__restrict__ left,
+                                   const int32_t *__restrict__ right, int32_t *flag, int32_t *starts,
+                                   int *nstarts)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    if (first[i] / REFIT_BS == last[i] / REFIT_BS) return;
+    int done = 0;
+    int c[2] = {left[i], right[i]};
+    for (int k = 0; k < 2; ++k)
+        done += (c[k] < 0) || (first[c[k]] / REFIT_BS == last[c[k]] / REFIT_BS);
+    flag[i] = done;
+    if (done == 2) starts[atomicAdd(nstarts, 1)] = i;
+}
+
+__global__ void k_refit_upper(const int32_t *__restrict__ starts, const int *nstarts,
+                              const double4 *__restrict__ spos, const int32_t *__restrict__ left,
+                              const int32_t *__restrict__ right, const int32_t *__restrict__ parent,
+                              int32_t *flag, double *box)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= *nstarts) return;
+    int node = starts[t];
+    while (true) {
+        merge_children(node, spos, left, right, box);
+        node = parent[node];
+        if (node < 0) return;
+        __threadfence();
+        if (atomicAdd(&flag[node], 1) == 0) return;        // the other child is still pending
+        __threadfence();
     }
 }
 
@@ -258,6 +330,7 @@ struct KnnArgs {
     const double4 *spos;
     const int32_t *first, *last, *left, *right;
     const double *box;
+    const float4 *fbox;     // per internal node: (cx, cy, cz, hx), (hy, hz, 0, 0) in FP32
     int n, K, nhop, bucket;
     double Lx, Ly, Lz, xyzl, rho_thr;
     const int32_t *qlist;   // optional explicit queries (sorted slots); null -> all slots
@@ -269,321 +342,56 @@ struct KnnArgs {
     unsigned long long *nact;
     int *overflow;
     int nblocks;
+    unsigned long long *stats;   // optional: per-launch work counters (see KNN_STAT_*)
 };
+
+// work counters accumulated by k_knn when KnnArgs::stats is set
+enum { KNN_STAT_NODES = 0, KNN_STAT_BUCKETS, KNN_STAT_CANDIDATES, KNN_STAT_INSERT_STEPS,
+       KNN_STAT_INSERTS, KNN_STAT_WAVES, KNN_STAT_N };
 
 // Heap of the KH smallest keys per lane, lane-interleaved in LDS (entry e of lane l at e*64+l,
 // so any per-lane entry index is bank-conflict free).
+// 8-ary max-heap: at most 2 levels below the root for up to 73 entries, and each level
+// issues all of its children's key and id loads at once, so a replacement costs two
+// dependent LDS round trips instead of the ~12 of a binary heap with the same capacity.
+constexpr int HEAP_D = 8;
+
 template <typename KeyT>
 __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int size, KeyT d,
                                           int32_t id)
 {
     int e = 0;
     while (true) {
-        int c1 = 2 * e + 1;
-        if (c1 >= size) break;
-        KeyT kc = hk[c1 * WAVE + lane];
-        int ic = c1;
-        if (c1 + 1 < size) {
-            KeyT k2 = hk[(c1 + 1) * WAVE + lane];
-            if (k2 > kc) {
-                kc = k2;
-                ic = c1 + 1;
-            }
+        const int c0 = HEAP_D * e + 1;
+        if (c0 >= size) break;
+        KeyT kk[HEAP_D];
+        int32_t ii[HEAP_D];
+#pragma unroll
+        for (int t = 0; t < HEAP_D; ++t) {
+            const int c = c0 + t < size ? c0 + t : c0;
+            kk[t] = hk[c * WAVE + lane];
+            ii[t] = hi[c * WAVE + lane];
         }
+        KeyT kc = kk[0];
+        int32_t idc = ii[0];
+        int ic = c0;
+#pragma unroll
+        for (int t = 1; t < HEAP_D; ++t)
+            if (c0 + t < size && kk[t] > kc) {
+                kc = kk[t];
+                idc = ii[t];
+                ic = c0 + t;
+            }
         if (!(kc > d)) break;
         hk[e * WAVE + lane] = kc;
-        hi[e * WAVE + lane] = hi[ic * WAVE + lane];
+        hi[e * WAVE + lane] = idc;
         e = ic;
     }
     hk[e * WAVE + lane] = d;
     hi[e * WAVE + lane] = id;
 }
 
-template <int KCAP, bool MIXED>
-__global__ __launch_bounds__(64) void k_knn(KnnArgs A)
-{
-    using KeyT = typename std::conditional<MIXED, float, double>::type;
-    constexpr int KHCAP = MIXED ? KCAP + 1 : KCAP;
-    __shared__ KeyT hk[KHCAP * WAVE];
-    __shared__ int32_t hi[KHCAP * WAVE];
-    __shared__ typename std::conditional<MIXED, float4, double4>::type stage[WAVE];
-    __shared__ int32_t stack[STACK];
-
-    const int lane = threadIdx.x;
-    const int wave = logical_block(A.nblocks);
-    const int slot = wave * WAVE + lane;
-    const bool active = slot < A.nq;
-    const int q = active ? (A.qlist ? A.qlist[slot] : slot) : 0;
-    const int K = A.K;
-    const int KH = MIXED ? K + 1 : K;
-    const double4 P = A.spos[q];
-    const double Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
-
-    // ---- query frame: centre of the wave's queries (orders the traversal; in the mixed
-    // path also the origin of the box-relative FP32 coordinates)
-    double cx, cy, cz;
-    float qx = 0, qy = 0, qz = 0, hQ = 0;
-    bool wave_unsafe = false;
-    {
-        const double BIG = 1e300;
-        double lx = wave_min(active ? P.x : BIG), hx = wave_max(active ? P.x : -BIG);
-        double ly = wave_min(active ? P.y : BIG), hy = wave_max(active ? P.y : -BIG);
-        double lz = wave_min(active ? P.z : BIG), hz = wave_max(active ? P.z : -BIG);
-        cx = 0.5 * (lx + hx);
-        cy = 0.5 * (ly + hy);
-        cz = 0.5 * (lz + hz);
-    }
-    if constexpr (MIXED) {
-        double rx = wrapd(P.x - cx, Lx), ry = wrapd(P.y - cy, Ly), rz = wrapd(P.z - cz, Lz);
-        qx = (float)rx;
-        qy = (float)ry;
-        qz = (float)rz;
-        double hq = active ? fmax(fabs(rx), fmax(fabs(ry), fabs(rz))) : 0.0;
-        hq = wave_max(hq);
-        hQ = (float)hq * (1.0f + 4 * U32);
-        double Lmin = fmin(Lx, fmin(Ly, Lz));
-        wave_unsafe = hq >= 0.25 * Lmin;
-    }
-
-    for (int e = 0; e < KH; ++e) {
-        hk[e * WAVE + lane] = (KeyT)INFINITY;
-        hi[e * WAVE + lane] = -1;
-    }
-    KeyT tau = active ? (KeyT)INFINITY : (KeyT)-1;
-    if (wave_unsafe) tau = (KeyT)-1;
-
-    // exact distance key of candidate slot j for this lane
-    auto cand_key = [&](const double4 &c) -> KeyT {
-        if constexpr (MIXED) {
-            float dx = (float)wrapd(c.x - cx, Lx) - qx;
-            float dy = (float)wrapd(c.y - cy, Ly) - qy;
-            float dz = (float)wrapd(c.z - cz, Lz) - qz;
-            return dx * dx + dy * dy + dz * dz;
-        } else {
-            return d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
-        }
-    };
-
-    // ---- seed each lane with its Morton-order neighbours [q-W, q+W] so that the search
-    // radius is already close to the k-th neighbour distance when the traversal starts;
-    // the traversal then skips these slots
-    const int W = K / 2 + 16;
-    int seed_lo = 1, seed_hi = 0;
-    if (active && !wave_unsafe) {
-        seed_lo = max(0, q - W);
-        seed_hi = min(A.n - 1, q + W);
-    }
-    for (int j = seed_lo; j <= seed_hi; ++j) {
-        if (j == q) continue;
-        KeyT d = cand_key(A.spos[j]);
-        if (d < tau) {
-            heap_sift<KeyT>(hk, hi, lane, KH, d, j);
-            tau = hk[lane];
-        }
-    }
-
-    // per-lane visit test of a node / particle box
-    auto need_box = [&](int c) -> bool {
-        double b[6];
-        child_box(c, A.spos, A.box, b);
-        if constexpr (MIXED) {
-            float dd[3];
-            const double cq[3] = {cx, cy, cz};
-            const float qq[3] = {qx, qy, qz};
-            const double L[3] = {Lx, Ly, Lz};
-            for (int a = 0; a < 3; ++a) {
-                double cn = wrapd(0.5 * (b[a] + b[a + 3]) - cq[a], L[a]);
-                float cnf = (float)cn;
-                float hn = (float)(0.5 * (b[a + 3] - b[a])) * (1.0f + 2 * U32);
-                float Lf = (float)L[a];
-                float t = fabsf(qq[a] - cnf);
-                bool img = t > 0.5f * Lf;
-                float tw = img ? Lf - t : t;
-                float m = 4 * U32 * (fabsf(cnf) + t + hn + hQ + (img ? Lf : 0.0f));
-                float d = tw - hn - m;
-                dd[a] = d > 0.0f ? d : 0.0f;
-            }
-            float dist2 = (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2]) * (1.0f - 4 * U32);
-            float st = sqrtf(tau) + hQ;
-            float E = 8 * U32 * st * st;
-            return dist2 < tau + 2 * E;
-        } else {
-            const double pq[3] = {P.x, P.y, P.z};
-            const double L[3] = {Lx, Ly, Lz};
-            double dist2 = 0;
-            for (int a = 0; a < 3; ++a) {
-                double cn = 0.5 * (b[a] + b[a + 3]);
-                double hn = 0.5 * (b[a + 3] - b[a]);
-                double t = fabs(pq[a] - cn);
-                double tw = fmin(t, L[a] - t);
-                double d = tw - hn - 1e-15 * L[a];
-                d = d > 0.0 ? d : 0.0;
-                dist2 = dist2 + d * d;
-            }
-            return dist2 < tau;
-        }
-    };
-
-    // squared periodic distance from the wave centre to a node box (traversal order only)
-    auto centre_dist2 = [&](int c) -> double {
-        double b[6];
-        child_box(c, A.spos, A.box, b);
-        const double cq[3] = {cx, cy, cz};
-        const double L[3] = {Lx, Ly, Lz};
-        double s = 0;
-        for (int a = 0; a < 3; ++a) {
-            double t = fabs(cq[a] - 0.5 * (b[a] + b[a + 3]));
-            double d = fmin(t, L[a] - t) - 0.5 * (b[a + 3] - b[a]);
-            d = d > 0 ? d : 0;
-            s += d * d;
-        }
-        return s;
-    };
-
-    int sp = 0;
-    stack[0] = 0;
-    sp = 1;
-    const bool root_is_bucket = (A.n <= A.bucket);
-    while (sp > 0) {
-        int node = stack[--sp];
-        int f, l;
-        if (node < 0) f = l = ~node;
-        else {
-            f = A.first[node];
-            l = A.last[node];
-        }
-        bool need = active && need_box(node);
-        if (!__any(need)) continue;
-        if (l - f + 1 <= A.bucket || (root_is_bucket && node == 0)) {
-            // ---- bucket: stage candidates in LDS, test all, then insert accepted ones
-            int cnt = l - f + 1;
-            __syncthreads();
-            if (lane < cnt) {
-                double4 c = A.spos[f + lane];
-                if constexpr (MIXED) {
-                    float rx = (float)wrapd(c.x - cx, Lx);
-                    float ry = (float)wrapd(c.y - cy, Ly);
-                    float rz = (float)wrapd(c.z - cz, Lz);
-                    stage[lane] = make_float4(rx, ry, rz, 0.0f);
-                } else {
-                    stage[lane] = c;
-                }
-            }
-            __syncthreads();
-            uint64_t mask = 0;
-            for (int j = 0; j < cnt; ++j) {
-                auto c = stage[j];
-                KeyT d;
-                if constexpr (MIXED) {
-                    float dx = c.x - qx, dy = c.y - qy, dz = c.z - qz;
-                    d = dx * dx + dy * dy + dz * dz;
-                } else {
-                    d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
-                }
-                const int sj = f + j;
-                if (d < tau && (sj < seed_lo || sj > seed_hi) && sj != q) mask |= (1ull << j);
-            }
-            while (__any(mask != 0)) {
-                if (mask) {
-                    int j = __ffsll((long long)mask) - 1;
-                    mask &= mask - 1;
-                    auto c = stage[j];
-                    KeyT d;
-                    if constexpr (MIXED) {
-                        float dx = c.x - qx, dy = c.y - qy, dz = c.z - qz;
-                        d = dx * dx + dy * dy + dz * dz;
-                    } else {
-                        d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
-                    }
-                    if (d < tau) {
-                        heap_sift<KeyT>(hk, hi, lane, KH, d, f + j);
-                        tau = hk[lane];
-                    }
-                }
-            }
-            continue;
-        }
-        int c1 = A.left[node], c2 = A.right[node];
-        bool n1 = active && need_box(c1);
-        bool n2 = active && need_box(c2);
-        int p1 = __popcll(__ballot(n1)), p2 = __popcll(__ballot(n2));
-        if (sp + 2 > STACK) {
-            if (lane == 0) atomicAdd(A.overflow, 1);
-            break;
-        }
-        if (p1 && p2) {
-            // nearer child (to the wave centre) first
-            if (centre_dist2(c1) <= centre_dist2(c2)) {
-                stack[sp++] = c2;
-                stack[sp++] = c1;
-            } else {
-                stack[sp++] = c1;
-                stack[sp++] = c2;
-            }
-        } else if (p1) {
-            stack[sp++] = c1;
-        } else if (p2) {
-            stack[sp++] = c2;
-        }
-    }
-
-    // ---- heap -> ascending order (in place heapsort, same trip count on every lane)
-    for (int end = KH - 1; end >= 1; --end) {
-        KeyT top = hk[lane];
-        int32_t tid = hi[lane];
-        KeyT mv = hk[end * WAVE + lane];
-        int32_t mid = hi[end * WAVE + lane];
-        hk[end * WAVE + lane] = top;
-        hi[end * WAVE + lane] = tid;
-        heap_sift<KeyT>(hk, hi, lane, end, mv, mid);
-    }
-    if (!active) return;
-
-    bool ok = !wave_unsafe;
-    if constexpr (MIXED) {
-        // certify that FP32 ranking cannot have changed the k-nearest set
-        float lk = hk[K * WAVE + lane], lk1 = hk[(K - 1) * WAVE + lane];
-        float st = sqrtf(lk) + hQ;
-        float E = 8 * U32 * st * st;
-        if (!(lk == INFINITY) && !(lk - lk1 > 2 * E && E < 0.01f * lk)) ok = false;
-    }
-    if (!ok) {
-        int at = atomicAdd(A.fb_count, 1);
-        A.fb_list[at] = q;
-        return;
-    }
-    // ---- density in FP64, reference order (omp_compute_density f90:2816-2838)
-    auto exact_d2 = [&](int i) -> double {
-        if constexpr (MIXED) {
-            double4 c = A.spos[hi[i * WAVE + lane]];
-            return d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
-        } else {
-            return hk[i * WAVE + lane];
-        }
-    };
-    double d2k = exact_d2(K - 1);
-    double r = sqrt(d2k) * 0.5;
-    double r3 = r * r * r;
-    double unsr = 1.0 / r;
-    double contrib = 0.0;
-    double prev = -1.0;
-    for (int i = 0; i < K - 1; ++i) {
-        double d = exact_d2(i);
-        ok = ok && (d >= prev);
-        prev = d;
-        double m = A.spos[hi[i * WAVE + lane]].w;
-        contrib = contrib + m * spline_ref(sqrt(d) * unsr);
-    }
-    ok = ok && (d2k >= prev);
-    if (MIXED && !ok) {
-        int at = atomicAdd(A.fb_count, 1);
-        A.fb_list[at] = q;
-        return;
-    }
-    double rho = A.xyzl * (contrib + P.w) / (PI_REF * r3);
-    A.rho_s[q] = rho;
-    for (int t = 0; t < A.nhop; ++t) A.nbr[(int64_t)q * A.nhop + t] = hi[t * WAVE + lane];
-}
+#include "knn_wave.h"
 
 // active-particle count (density > rho_threshold, f90:712-715)
 __global__ void k_count_active(const double *__restrict__ rho, int64_t n, double thr,
@@ -791,7 +599,7 @@ void Engine::release_all()
                      &b_counter, &b_fallback, &b_emit_cnt, &b_emit_off, &b_ekey, &b_eval,
                      &b_ekey2, &b_eval2, &b_ukey, &b_uval, &b_keep, &b_keep_off, &b_skey,
                      &b_sval, &b_sad_off, &b_sad_isad, &b_sad_rho, &b_sad_cnt, &b_densityg,
-                     &b_misc, &b_roots};
+                     &b_misc, &b_roots, &b_fbox};
     for (auto *b : all) b->release();
     res_ = DeviceResults{};
 }
@@ -842,8 +650,16 @@ void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, co
     hipLaunchKernelGGL(k_radix_tree, dim3(nblk(ni)), dim3(256), 0, s, t_.key, (int)n, t_.first,
                        t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ni)), dim3(256), 0, s, t_.flag, ni, 0);
-    hipLaunchKernelGGL(k_refit, dim3(nblk(n)), dim3(256), 0, s, (int)n, t_.spos, t_.left,
-                       t_.right, t_.parent, t_.leaf_parent, t_.flag, t_.box);
+    hipLaunchKernelGGL(k_refit_local, dim3(nblk(n, REFIT_BS)), dim3(REFIT_BS), 0, s, (int)n,
+                       t_.spos, t_.first, t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent,
+                       t_.box);
+    int32_t *starts = as<int32_t>(b_roots, n);        // scratch: the roots buffer is free here
+    hipLaunchKernelGGL(k_refit_upper_init, dim3(nblk(ni)), dim3(256), 0, s, (int)n, t_.first,
+                       t_.last, t_.left, t_.right, t_.flag, starts, misc + 5);
+    hipLaunchKernelGGL(k_refit_upper, dim3(nblk(n / 2 + 1)), dim3(256), 0, s, starts, misc + 5,
+                       t_.spos, t_.left, t_.right, t_.parent, t_.flag, t_.box);
+    t_.fbox = as<float4>(b_fbox, 2 * ni);
+    hipLaunchKernelGGL(k_float_boxes, dim3(nblk(ni)), dim3(256), 0, s, (int)n, t_.box, t_.fbox);
     AHOP_HIP(hipEventRecord(ev_[2], s));
 }
 
@@ -879,6 +695,7 @@ void Engine::knn_density(const Params &p)
     a.left = t_.left;
     a.right = t_.right;
     a.box = t_.box;
+    a.fbox = t_.fbox;
     a.n = (int)n;
     a.K = p.nvoisins;
     a.nhop = p.nhop;
@@ -896,10 +713,15 @@ void Engine::knn_density(const Params &p)
     a.fb_count = misc + 2;
     a.nact = cnt;
     a.overflow = misc + 1;
+    unsigned long long *stats = as<unsigned long long>(b_counter, 16) + 8;
+    AHOP_HIP(hipMemsetAsync(stats, 0, 8 * sizeof(unsigned long long), s));
+    a.stats = stats;
     unsigned nb = (unsigned)((n + 63) / 64);
     a.nblocks = (int)nb;
     dispatch_knn(p.nvoisins, p.mixed != 0, a, nb, s);
     AHOP_HIP(hipEventRecord(ev_[3], s));
+    AHOP_HIP(hipMemcpyAsync(res_.knn_stats, stats, KNN_STAT_N * sizeof(unsigned long long),
+                            hipMemcpyDeviceToHost, s));
     int host_misc[4];
     AHOP_HIP(hipMemcpyAsync(host_misc, misc, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
     AHOP_HIP(hipStreamSynchronize(s));
@@ -911,6 +733,7 @@ void Engine::knn_density(const Params &p)
         b.nq = host_misc[2];
         unsigned nb2 = (unsigned)((b.nq + 63) / 64);
         b.nblocks = (int)nb2;
+        b.stats = nullptr;
         dispatch_knn(p.nvoisins, false, b, nb2, s);
     }
     AHOP_HIP(hipEventRecord(ev_[4], s));

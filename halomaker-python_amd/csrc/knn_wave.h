@@ -1,0 +1,361 @@
+// knn_wave.h -- the kNN + SPH density kernel (included by ahop_kernels.hip).
+//
+// One wave handles 64 consecutive Morton-sorted query particles, one query per lane.
+// The wave walks the binary radix tree together (a node is entered when any lane needs
+// it), stages the particles of each leaf bucket in LDS and every lane keeps the k (mixed:
+// k+1) nearest candidates seen so far in an 8-ary max-heap in LDS.
+//
+// The per-wave state is kept in plain local variables and the repeated steps are macros:
+// a state struct (or closures capturing by reference) made the compiler spill it to
+// scratch memory.
+#pragma once
+
+
+// conservative FP32 visit test of a node (c >= 0) or particle (c < 0) box: centres rounded
+// to nearest, half-widths rounded up, and the margin mL covers the rounding of the query,
+// the centre and the periodic fold, so the returned squared distance never exceeds the
+// true minimum distance from the query to the box
+__device__ __forceinline__ float box_dist2_lower(int c, const double4 *__restrict__ spos,
+                                                 const float4 *__restrict__ fbox, float px,
+                                                 float py, float pz, float Lx, float Ly,
+                                                 float Lz, float mLx, float mLy, float mLz)
+{
+    float bx, by, bz, hx, hy, hz;
+    if (c < 0) {
+        const double4 p = spos[~c];
+        bx = (float)p.x;
+        by = (float)p.y;
+        bz = (float)p.z;
+        hx = hy = hz = 0.0f;
+    } else {
+        const float4 b0 = fbox[2 * c], b1 = fbox[2 * c + 1];
+        bx = b0.x;
+        by = b0.y;
+        bz = b0.z;
+        hx = b0.w;
+        hy = b1.x;
+        hz = b1.y;
+    }
+    float tx = fabsf(px - bx), ty = fabsf(py - by), tz = fabsf(pz - bz);
+    tx = fminf(tx, Lx - tx);
+    ty = fminf(ty, Ly - ty);
+    tz = fminf(tz, Lz - tz);
+    const float dx = fmaxf(tx - hx - mLx, 0.0f);
+    const float dy = fmaxf(ty - hy - mLy, 0.0f);
+    const float dz = fmaxf(tz - hz - mLz, 0.0f);
+    return (dx * dx + dy * dy + dz * dz) * (1.0f - 4 * U32);
+}
+
+template <int KCAP, bool MIXED>
+__global__ __launch_bounds__(64) void k_knn(KnnArgs A)
+{
+    using KeyT = typename std::conditional<MIXED, float, double>::type;
+    using StageT = typename std::conditional<MIXED, float4, double4>::type;
+    constexpr int KHCAP = MIXED ? KCAP + 1 : KCAP;
+    constexpr int NBUF = 4;   // staged buckets whose insertions are deferred
+    __shared__ KeyT hk[KHCAP * WAVE];
+    __shared__ int32_t hi[KHCAP * WAVE];
+    __shared__ StageT stage[NBUF * WAVE];
+    __shared__ int32_t stack[STACK];
+
+    const int lane = threadIdx.x;
+    const int wave = logical_block(A.nblocks);
+    const int slot = wave * WAVE + lane;
+    const bool active = slot < A.nq;
+    const int q = active ? (A.qlist ? A.qlist[slot] : slot) : 0;
+    const int K = A.K;
+    const int KH = MIXED ? K + 1 : K;
+    const double4 P = A.spos[q];
+    const double Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
+    const double4 *__restrict__ spos = A.spos;
+
+    // centre of the wave's queries: origin of the box-relative FP32 coordinates
+    double cx, cy, cz;
+    {
+        const double BIG = 1e300;
+        const double lx = wave_min(active ? P.x : BIG), hx = wave_max(active ? P.x : -BIG);
+        const double ly = wave_min(active ? P.y : BIG), hy = wave_max(active ? P.y : -BIG);
+        const double lz = wave_min(active ? P.z : BIG), hz = wave_max(active ? P.z : -BIG);
+        cx = 0.5 * (lx + hx);
+        cy = 0.5 * (ly + hy);
+        cz = 0.5 * (lz + hz);
+    }
+    float qx = 0, qy = 0, qz = 0, hQ = 0;
+    bool wave_unsafe = false;
+    if constexpr (MIXED) {
+        const double rx = wrapd(P.x - cx, Lx), ry = wrapd(P.y - cy, Ly), rz = wrapd(P.z - cz, Lz);
+        qx = (float)rx;
+        qy = (float)ry;
+        qz = (float)rz;
+        double hq = active ? fmax(fabs(rx), fmax(fabs(ry), fabs(rz))) : 0.0;
+        hq = wave_max(hq);
+        hQ = (float)hq * (1.0f + 4 * U32);
+        // the FP32 frame is valid while search radius + query spread < L/2
+        wave_unsafe = hq >= 0.25 * fmin(Lx, fmin(Ly, Lz));
+    }
+    for (int e = 0; e < KH; ++e) {
+        hk[e * WAVE + lane] = (KeyT)INFINITY;
+        hi[e * WAVE + lane] = -1;
+    }
+    KeyT tau = (active && !wave_unsafe) ? (KeyT)INFINITY : (KeyT)-1;
+    // Morton-order seed window of this lane: [q-Wd, q+Wd]; the traversal skips it
+    const int Wd = K / 2 + 16;
+    int seed_lo = 1, seed_hi = 0;
+    if (active && !wave_unsafe) {
+        seed_lo = max(0, q - Wd);
+        seed_hi = min(A.n - 1, q + Wd);
+    }
+    const float pxf = (float)P.x, pyf = (float)P.y, pzf = (float)P.z;
+    const float Lxf = (float)Lx, Lyf = (float)Ly, Lzf = (float)Lz;
+    const float mLx = 4.0f * U32 * Lxf, mLy = 4.0f * U32 * Lyf, mLz = 4.0f * U32 * Lzf;
+    const int qmid = A.qlist ? __builtin_amdgcn_readfirstlane(q) : wave * WAVE + WAVE / 2;
+    uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;   // accepted-candidate masks of staged buckets
+    int f0 = 0, f1 = 0, f2 = 0, f3 = 0;         // first slot of each staged bucket
+    int nbuf = 0;
+    unsigned long long st_nodes = 0, st_buckets = 0, st_cand = 0, st_steps = 0, st_ins = 0;
+
+#define KNN_KEY(c, out)                                                                        \
+    do {                                                                                       \
+        if constexpr (MIXED) {                                                                 \
+            const float dx_ = (c).x - qx, dy_ = (c).y - qy, dz_ = (c).z - qz;                  \
+            (out) = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;                                         \
+        } else {                                                                               \
+            (out) = d2_ref(P.x, P.y, P.z, (c).x, (c).y, (c).z, Lx, Ly, Lz);                    \
+        }                                                                                      \
+    } while (0)
+
+#define KNN_INSERT(d, id)                                                                      \
+    do {                                                                                       \
+        heap_sift<KeyT>(hk, hi, lane, KH, (d), (id));                                          \
+        ++st_ins;                                                                              \
+        tau = hk[lane];                                                                        \
+    } while (0)
+
+    // Insert every pending accepted candidate lane by lane: the wave pays for the busiest
+    // lane's total over all staged buckets, not for each bucket's busiest lane.
+#define KNN_FLUSH()                                                                            \
+    do {                                                                                       \
+        while (__any((m0 | m1 | m2 | m3) != 0)) {                                             \
+            ++st_steps;                                                                        \
+            if ((m0 | m1 | m2 | m3) != 0) {                                                    \
+                const bool s0 = m0 != 0, s1 = !s0 && m1 != 0, s2 = !s0 && !s1 && m2 != 0;      \
+                const bool s3 = !s0 && !s1 && !s2;                                             \
+                uint64_t mk = s0 ? m0 : (s1 ? m1 : (s2 ? m2 : m3));                            \
+                const int fsel = s0 ? f0 : (s1 ? f1 : (s2 ? f2 : f3));                         \
+                const int bsel = s0 ? 0 : (s1 ? 1 : (s2 ? 2 : 3));                             \
+                const int j_ = __ffsll((long long)mk) - 1;                                     \
+                mk &= mk - 1;                                                                  \
+                m0 = s0 ? mk : m0;                                                             \
+                m1 = s1 ? mk : m1;                                                             \
+                m2 = s2 ? mk : m2;                                                             \
+                m3 = s3 ? mk : m3;                                                             \
+                KeyT d_;                                                                       \
+                KNN_KEY(stage[bsel * WAVE + j_], d_);                                          \
+                if (d_ < tau) KNN_INSERT(d_, fsel + j_);                                       \
+            }                                                                                  \
+        }                                                                                      \
+        nbuf = 0;                                                                              \
+    } while (0)
+
+    // Stage sorted slots [f, f+cnt) (cnt <= 64) and record each lane's accepted subset;
+    // seeding accepts only the lane's own window, the traversal everything outside it.
+#define KNN_STAGE(fst, cnt, seeding)                                                           \
+    do {                                                                                       \
+        if (nbuf == NBUF) KNN_FLUSH();                                                         \
+        StageT *buf_ = stage + nbuf * WAVE;                                                    \
+        __syncthreads();                                                                       \
+        if (lane < (cnt)) {                                                                    \
+            const double4 c_ = spos[(fst) + lane];                                             \
+            if constexpr (MIXED)                                                               \
+                buf_[lane] = make_float4((float)wrapd(c_.x - cx, Lx), (float)wrapd(c_.y - cy, Ly), \
+                                         (float)wrapd(c_.z - cz, Lz), 0.0f);                   \
+            else                                                                               \
+                buf_[lane] = c_;                                                               \
+        }                                                                                      \
+        __syncthreads();                                                                       \
+        uint64_t mask_ = 0;                                                                    \
+        for (int j_ = 0; j_ < (cnt); ++j_) {                                                   \
+            KeyT d_;                                                                           \
+            KNN_KEY(buf_[j_], d_);                                                             \
+            const int sj_ = (fst) + j_;                                                        \
+            const bool in_seed_ = sj_ >= seed_lo && sj_ <= seed_hi;                            \
+            if (d_ < tau && sj_ != q && ((seeding) ? in_seed_ : !in_seed_)) mask_ |= (1ull << j_); \
+        }                                                                                      \
+        m0 = nbuf == 0 ? mask_ : m0;                                                           \
+        m1 = nbuf == 1 ? mask_ : m1;                                                           \
+        m2 = nbuf == 2 ? mask_ : m2;                                                           \
+        m3 = nbuf == 3 ? mask_ : m3;                                                           \
+        f0 = nbuf == 0 ? (fst) : f0;                                                           \
+        f1 = nbuf == 1 ? (fst) : f1;                                                           \
+        f2 = nbuf == 2 ? (fst) : f2;                                                           \
+        f3 = nbuf == 3 ? (fst) : f3;                                                           \
+        ++nbuf;                                                                                \
+    } while (0)
+
+#define KNN_NEED(c, out)                                                                       \
+    do {                                                                                       \
+        const float bd_ = box_dist2_lower((c), spos, A.fbox, pxf, pyf, pzf, Lxf, Lyf, Lzf,     \
+                                          mLx, mLy, mLz);                                      \
+        if constexpr (MIXED) {                                                                 \
+            const float st_ = sqrtf(tau) + hQ;                                                 \
+            (out) = active && bd_ < tau + 16 * U32 * st_ * st_;                                \
+        } else {                                                                               \
+            (out) = active && (double)bd_ < tau;                                               \
+        }                                                                                      \
+    } while (0)
+
+    // ---- seed: the wave's seed windows cover the contiguous slots [qb-Wd, qb+63+Wd]
+    if (!A.qlist) {
+        const int qb = wave * WAVE;
+        const int s0 = max(0, qb - Wd), s1 = min(A.n - 1, qb + WAVE - 1 + Wd);
+        for (int f = s0; f <= s1; f += WAVE) {
+            const int cnt = min(WAVE, s1 - f + 1);
+            KNN_STAGE(f, cnt, true);
+        }
+    } else {
+        for (int j = seed_lo; j <= seed_hi; ++j) {
+            if (j == q) continue;
+            const double4 c = spos[j];
+            StageT sc;
+            if constexpr (MIXED)
+                sc = make_float4((float)wrapd(c.x - cx, Lx), (float)wrapd(c.y - cy, Ly),
+                                 (float)wrapd(c.z - cz, Lz), 0.0f);
+            else
+                sc = c;
+            KeyT d;
+            KNN_KEY(sc, d);
+            if (d < tau) KNN_INSERT(d, j);
+        }
+    }
+    KNN_FLUSH();
+
+    // ---- wave-coherent traversal of the binary radix tree, nearest child first
+    int sp = 1;
+    stack[0] = 0;
+    const bool root_is_bucket = (A.n <= A.bucket);
+    while (sp > 0) {
+        // the traversal is wave-uniform: say so, so that node data come through scalar loads
+        const int node = __builtin_amdgcn_readfirstlane(stack[--sp]);
+        int f, l;
+        if (node < 0) f = l = ~node;
+        else {
+            f = A.first[node];
+            l = A.last[node];
+        }
+        ++st_nodes;
+        bool need;
+        KNN_NEED(node, need);
+        if (!__any(need)) continue;
+        if (l - f + 1 <= A.bucket || (root_is_bucket && node == 0)) {
+            const int cnt = l - f + 1;
+            ++st_buckets;
+            st_cand += cnt;
+            KNN_STAGE(f, cnt, false);
+            continue;
+        }
+        const int c1 = __builtin_amdgcn_readfirstlane(A.left[node]);
+        const int c2 = __builtin_amdgcn_readfirstlane(A.right[node]);
+        bool n1, n2;
+        KNN_NEED(c1, n1);
+        KNN_NEED(c2, n2);
+        const bool a1 = __any(n1), a2 = __any(n2);
+        if (sp + 2 > STACK) {
+            if (lane == 0) atomicAdd(A.overflow, 1);
+            break;
+        }
+        if (a1 && a2) {
+            // nearest first, in sorted-slot distance to the wave's queries
+            const int f1_ = c1 < 0 ? ~c1 : A.first[c1], l1_ = c1 < 0 ? ~c1 : A.last[c1];
+            const int f2_ = c2 < 0 ? ~c2 : A.first[c2], l2_ = c2 < 0 ? ~c2 : A.last[c2];
+            const int d1 = max(0, max(f1_ - qmid, qmid - l1_));
+            const int d2 = max(0, max(f2_ - qmid, qmid - l2_));
+            if (d1 <= d2) {
+                stack[sp++] = c2;
+                stack[sp++] = c1;
+            } else {
+                stack[sp++] = c1;
+                stack[sp++] = c2;
+            }
+        } else if (a1) {
+            stack[sp++] = c1;
+        } else if (a2) {
+            stack[sp++] = c2;
+        }
+    }
+    KNN_FLUSH();
+#undef KNN_NEED
+#undef KNN_STAGE
+#undef KNN_FLUSH
+#undef KNN_INSERT
+#undef KNN_KEY
+
+    if (A.stats) {
+        unsigned long long ins = st_ins;
+        for (int o = 32; o > 0; o >>= 1) ins += __shfl_xor(ins, o);
+        if (lane == 0) {
+            atomicAdd(&A.stats[KNN_STAT_NODES], st_nodes);
+            atomicAdd(&A.stats[KNN_STAT_BUCKETS], st_buckets);
+            atomicAdd(&A.stats[KNN_STAT_CANDIDATES], st_cand);
+            atomicAdd(&A.stats[KNN_STAT_INSERT_STEPS], st_steps);
+            atomicAdd(&A.stats[KNN_STAT_INSERTS], ins);
+            atomicAdd(&A.stats[KNN_STAT_WAVES], 1ull);
+        }
+    }
+
+    // ---- heap -> ascending order (in-place heapsort, same trip count on every lane)
+    for (int end = KH - 1; end >= 1; --end) {
+        const KeyT top = hk[lane];
+        const int32_t tid = hi[lane];
+        const KeyT mv = hk[end * WAVE + lane];
+        const int32_t mid = hi[end * WAVE + lane];
+        hk[end * WAVE + lane] = top;
+        hi[end * WAVE + lane] = tid;
+        heap_sift<KeyT>(hk, hi, lane, end, mv, mid);
+    }
+    if (!active) return;
+
+    bool ok = !wave_unsafe;
+    if constexpr (MIXED) {
+        // certify that FP32 ranking cannot have changed the k-nearest set: the (k+1)-th
+        // FP32 key must clear the k-th by twice the error bound E of a key of that size
+        const float lk = hk[K * WAVE + lane], lk1 = hk[(K - 1) * WAVE + lane];
+        const float st = sqrtf(lk) + hQ;
+        const float E = 8 * U32 * st * st;
+        if (!(lk == INFINITY) && !(lk - lk1 > 2 * E && E < 0.01f * lk)) ok = false;
+    }
+    if (!ok) {
+        A.fb_list[atomicAdd(A.fb_count, 1)] = q;
+        return;
+    }
+    // ---- density in FP64, reference order (omp_compute_density f90:2816-2838); the mixed
+    // path recomputes every distance in the reference's FP64 arithmetic and re-checks the
+    // order, so its densities are those of the FP64 path
+    double d2k;
+    if constexpr (MIXED) {
+        const double4 c = spos[hi[(K - 1) * WAVE + lane]];
+        d2k = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+    } else {
+        d2k = hk[(K - 1) * WAVE + lane];
+    }
+    const double r = sqrt(d2k) * 0.5;
+    const double r3 = r * r * r;
+    const double unsr = 1.0 / r;
+    double contrib = 0.0;
+    double prev = -1.0;
+    for (int i = 0; i < K - 1; ++i) {
+        const double4 c = spos[hi[i * WAVE + lane]];
+        double d;
+        if constexpr (MIXED) d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+        else d = hk[i * WAVE + lane];
+        ok = ok && (d >= prev);
+        prev = d;
+        contrib = contrib + c.w * spline_ref(sqrt(d) * unsr);
+    }
+    ok = ok && (d2k >= prev);
+    if (MIXED && !ok) {
+        A.fb_list[atomicAdd(A.fb_count, 1)] = q;
+        return;
+    }
+    A.rho_s[q] = A.xyzl * (contrib + P.w) / (PI_REF * r3);
+    for (int t = 0; t < A.nhop; ++t) A.nbr[(int64_t)q * A.nhop + t] = hi[t * WAVE + lane];
+}

@@ -90,6 +90,9 @@ typedef struct {
     int64_t nfallback;   /* mixed path: queries re-run in FP64 */
     int64_t nsaddle;
     float t_sort, t_tree, t_knn, t_fallback, t_hop, t_saddle, t_total;  /* ms, HIP events */
+    /* work counters of the main kNN launch: tree nodes visited, leaf buckets scanned,
+     * candidate distances (per lane), heap-insertion steps, heap insertions, waves */
+    int64_t knn_nodes, knn_buckets, knn_candidates, knn_insert_steps, knn_inserts, knn_waves;
 } ahop_run_info;
 
 /* kNN + SPH density + hop + group ids (+ saddle edges) on device-resident inputs:
