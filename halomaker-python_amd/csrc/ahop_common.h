@@ -97,7 +97,7 @@ class Engine {
         b_leaf_parent, b_flag, b_box, b_temp, b_rho_s, b_nbr, b_hop_s, b_gid_s, b_flag_o,
         b_rank_o, b_density, b_gid, b_hop, b_counter, b_fallback, b_emit_cnt, b_emit_off,
         b_ekey, b_eval, b_ekey2, b_eval2, b_ukey, b_uval, b_keep, b_keep_off, b_skey, b_sval,
-        b_sad_off, b_sad_isad, b_sad_rho, b_sad_cnt, b_densityg, b_misc, b_roots, b_fbox;
+        b_sad_off, b_sad_isad, b_sad_rho, b_sad_cnt, b_densityg, b_misc, b_roots, b_fbox, b_fb_sorted;
     int64_t nmax_ = 0;
 };
 

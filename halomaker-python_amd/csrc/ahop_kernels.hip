@@ -599,7 +599,7 @@ void Engine::release_all()
                      &b_counter, &b_fallback, &b_emit_cnt, &b_emit_off, &b_ekey, &b_eval,
                      &b_ekey2, &b_eval2, &b_ukey, &b_uval, &b_keep, &b_keep_off, &b_skey,
                      &b_sval, &b_sad_off, &b_sad_isad, &b_sad_rho, &b_sad_cnt, &b_densityg,
-                     &b_misc, &b_roots, &b_fbox};
+                     &b_misc, &b_roots, &b_fbox, &b_fb_sorted};
     for (auto *b : all) b->release();
     res_ = DeviceResults{};
 }
@@ -728,9 +728,16 @@ void Engine::knn_density(const Params &p)
     if (host_misc[1] != 0) throw std::runtime_error("device tree deeper than the traversal stack");
     res_.nfallback = host_misc[2];
     if (host_misc[2] > 0) {
+        // re-run the uncertified queries in FP64, in sorted-slot order so that each wave's
+        // queries are as close together as the list allows
+        const int nfb = host_misc[2];
+        int32_t *fb_sorted = as<int32_t>(b_fb_sorted, nfb);
+        size_t tb = 0;
+        AHOP_HIP(rocprim::radix_sort_keys(nullptr, tb, fb, fb_sorted, (size_t)nfb, 0, 32, s));
+        AHOP_HIP(rocprim::radix_sort_keys(b_temp.ensure(tb), tb, fb, fb_sorted, (size_t)nfb, 0, 32, s));
         KnnArgs b = a;
-        b.qlist = fb;
-        b.nq = host_misc[2];
+        b.qlist = fb_sorted;
+        b.nq = nfb;
         unsigned nb2 = (unsigned)((b.nq + 63) / 64);
         b.nblocks = (int)nb2;
         b.stats = nullptr;

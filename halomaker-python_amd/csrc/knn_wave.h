@@ -46,6 +46,19 @@ __device__ __forceinline__ float box_dist2_lower(int c, const double4 *__restric
     return (dx * dx + dy * dy + dz * dz) * (1.0f - 4 * U32);
 }
 
+// Bound on |d2f - d2| for a candidate at true squared distance ~d from a query, when both
+// positions are rounded to FP32 relative to a common origin no farther than hQ from the
+// query: per axis the FP32 difference is off by at most 2u(|dx| + hQ) (u = 2^-24), which
+// with the rounding of the three squares and two sums gives
+//   |d2f - d2| <= 7.01 u d + 6.94 u hQ sqrt(d) + 13 u^2 (sqrt(d) + hQ)^2 .
+// The bound below rounds the constants up, evaluates at a slightly larger d, and adds a
+// term for the FP64 rounding of the box-relative coordinates.
+__device__ __forceinline__ float fp32_key_err(float d, float hQ, float L)
+{
+    const float s = sqrtf(d * 1.0001f) + hQ;
+    return 8 * U32 * sqrtf(d * 1.0001f) * s + 16 * U32 * U32 * s * s + 1e-15f * L * s;
+}
+
 template <int KCAP, bool MIXED>
 __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
 {
@@ -108,6 +121,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     const float pxf = (float)P.x, pyf = (float)P.y, pzf = (float)P.z;
     const float Lxf = (float)Lx, Lyf = (float)Ly, Lzf = (float)Lz;
     const float mLx = 4.0f * U32 * Lxf, mLy = 4.0f * U32 * Lyf, mLz = 4.0f * U32 * Lzf;
+    const float Lmin = fminf(Lxf, fminf(Lyf, Lzf));
     const int qmid = A.qlist ? __builtin_amdgcn_readfirstlane(q) : wave * WAVE + WAVE / 2;
     uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;   // accepted-candidate masks of staged buckets
     int f0 = 0, f1 = 0, f2 = 0, f3 = 0;         // first slot of each staged bucket
@@ -197,8 +211,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         const float bd_ = box_dist2_lower((c), spos, A.fbox, pxf, pyf, pzf, Lxf, Lyf, Lzf,     \
                                           mLx, mLy, mLz);                                      \
         if constexpr (MIXED) {                                                                 \
-            const float st_ = sqrtf(tau) + hQ;                                                 \
-            (out) = active && bd_ < tau + 16 * U32 * st_ * st_;                                \
+            (out) = active && bd_ < tau + 2 * fp32_key_err(tau, hQ, Lmin);                     \
         } else {                                                                               \
             (out) = active && (double)bd_ < tau;                                               \
         }                                                                                      \
@@ -232,6 +245,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     // ---- wave-coherent traversal of the binary radix tree, nearest child first
     int sp = 1;
     stack[0] = 0;
+    bool stack_overflow = false;
     const bool root_is_bucket = (A.n <= A.bucket);
     while (sp > 0) {
         // the traversal is wave-uniform: say so, so that node data come through scalar loads
@@ -260,7 +274,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         KNN_NEED(c2, n2);
         const bool a1 = __any(n1), a2 = __any(n2);
         if (sp + 2 > STACK) {
-            if (lane == 0) atomicAdd(A.overflow, 1);
+            stack_overflow = true;   // reported after the loop (no global atomics inside it)
             break;
         }
         if (a1 && a2) {
@@ -283,6 +297,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         }
     }
     KNN_FLUSH();
+    if (stack_overflow && lane == 0) atomicAdd(A.overflow, 1);
 #undef KNN_NEED
 #undef KNN_STAGE
 #undef KNN_FLUSH
@@ -317,45 +332,67 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     bool ok = !wave_unsafe;
     if constexpr (MIXED) {
         // certify that FP32 ranking cannot have changed the k-nearest set: the (k+1)-th
-        // FP32 key must clear the k-th by twice the error bound E of a key of that size
+        // FP32 key must clear the k-th by twice the error bound E of a key of that size,
+        // and every particle within that radius must lie in the periodic image the
+        // wave-centred frame picks (radius + query spread < L/2; a particle the frame folds
+        // onto another image gets a key no smaller than its true distance, so only the
+        // particles inside the radius need the right image)
         const float lk = hk[K * WAVE + lane], lk1 = hk[(K - 1) * WAVE + lane];
-        const float st = sqrtf(lk) + hQ;
-        const float E = 8 * U32 * st * st;
-        if (!(lk == INFINITY) && !(lk - lk1 > 2 * E && E < 0.01f * lk)) ok = false;
+        const float E = fp32_key_err(lk, hQ, Lmin);
+        if (!(lk == INFINITY) &&
+            !(lk - lk1 > 2 * E && E < 0.01f * lk && sqrtf(lk * 1.0001f) + hQ < 0.4999f * Lmin))
+            ok = false;
     }
     if (!ok) {
         A.fb_list[atomicAdd(A.fb_count, 1)] = q;
         return;
     }
     // ---- density in FP64, reference order (omp_compute_density f90:2816-2838); the mixed
-    // path recomputes every distance in the reference's FP64 arithmetic and re-checks the
-    // order, so its densities are those of the FP64 path
-    double d2k;
-    if constexpr (MIXED) {
-        const double4 c = spos[hi[(K - 1) * WAVE + lane]];
-        d2k = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
-    } else {
-        d2k = hk[(K - 1) * WAVE + lane];
+    // path recomputes every distance in the reference's FP64 arithmetic; where FP32 near-ties
+    // left two entries out of FP64 order, the lane re-sorts its k entries by the FP64
+    // distances and sums again, so its densities are those of the FP64 path
+    double rho = 0.0;
+    for (int pass = 0; pass < 2; ++pass) {
+        double d2k;
+        if constexpr (MIXED) {
+            const double4 c = spos[hi[(K - 1) * WAVE + lane]];
+            d2k = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+        } else {
+            d2k = hk[(K - 1) * WAVE + lane];
+        }
+        const double r = sqrt(d2k) * 0.5;
+        const double r3 = r * r * r;
+        const double unsr = 1.0 / r;
+        double contrib = 0.0;
+        double prev = -1.0;
+        bool sorted = true;
+        for (int i = 0; i < K - 1; ++i) {
+            const double4 c = spos[hi[i * WAVE + lane]];
+            double d;
+            if constexpr (MIXED) d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+            else d = hk[i * WAVE + lane];
+            sorted = sorted && (d >= prev);
+            prev = d;
+            contrib = contrib + c.w * spline_ref(sqrt(d) * unsr);
+        }
+        sorted = sorted && (d2k >= prev);
+        rho = A.xyzl * (contrib + P.w) / (PI_REF * r3);
+        if (!MIXED || sorted) break;
+        // insertion sort of this lane's k entries by FP64 distance (stable on ties)
+        for (int i = 1; i < K; ++i) {
+            const int32_t id = hi[i * WAVE + lane];
+            const double4 ci = spos[id];
+            const double di = d2_ref(P.x, P.y, P.z, ci.x, ci.y, ci.z, Lx, Ly, Lz);
+            int j = i - 1;
+            while (j >= 0) {
+                const double4 cj = spos[hi[j * WAVE + lane]];
+                if (d2_ref(P.x, P.y, P.z, cj.x, cj.y, cj.z, Lx, Ly, Lz) <= di) break;
+                hi[(j + 1) * WAVE + lane] = hi[j * WAVE + lane];
+                --j;
+            }
+            hi[(j + 1) * WAVE + lane] = id;
+        }
     }
-    const double r = sqrt(d2k) * 0.5;
-    const double r3 = r * r * r;
-    const double unsr = 1.0 / r;
-    double contrib = 0.0;
-    double prev = -1.0;
-    for (int i = 0; i < K - 1; ++i) {
-        const double4 c = spos[hi[i * WAVE + lane]];
-        double d;
-        if constexpr (MIXED) d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
-        else d = hk[i * WAVE + lane];
-        ok = ok && (d >= prev);
-        prev = d;
-        contrib = contrib + c.w * spline_ref(sqrt(d) * unsr);
-    }
-    ok = ok && (d2k >= prev);
-    if (MIXED && !ok) {
-        A.fb_list[atomicAdd(A.fb_count, 1)] = q;
-        return;
-    }
-    A.rho_s[q] = A.xyzl * (contrib + P.w) / (PI_REF * r3);
+    A.rho_s[q] = rho;
     for (int t = 0; t < A.nhop; ++t) A.nbr[(int64_t)q * A.nhop + t] = hi[t * WAVE + lane];
 }
