@@ -27,6 +27,9 @@ HBM_PEAK_GBS = 8000.0
 # group-id write 8 B: 56 B per particle)
 KNN_BYTES_PER_PARTICLE = 40
 STEP_BYTES_PER_PARTICLE = 56
+# per member of the batched halo properties: position 24 B, velocity 24 B, mass 8 B and a
+# member index 4 B read once
+HALO_BYTES_PER_MEMBER = 60
 
 
 def log(msg):
@@ -57,16 +60,80 @@ def cpu_baseline(sample_n, nthreads):
                       f"tree+density+local-maxima stages, best of 2, {t:.2f} s"}
 
 
+def halo_props_stage(steps, cpu_sample_halos, with_cpu):
+    """Batched per-halo properties on 50 000 NFW halos (about 30 M members): device time of
+    all member passes and per-halo combines with inputs resident in HBM, the tuned schedule,
+    the roofline of 60 algorithmic bytes per member, and the oracle's NumPy restatement of
+    the reference's per-halo reductions timed on a bounded sample of the same halos."""
+    import numpy as np
+    import torch
+    from amdahop import generate as G, halo_props
+
+    dev = torch.device("cuda:0")
+    t0 = time.perf_counter()
+    hs = G.halo_config()
+    log(f"halo workload: {hs.nhalos} halos, {int(hs.offsets[-1])} members "
+        f"(generated in {time.perf_counter() - t0:.1f} s)")
+    pos = torch.from_numpy(hs.pos).to(dev)
+    vel = torch.from_numpy(hs.vel).to(dev)
+    mass = torch.from_numpy(hs.mass).to(dev)
+    _, info = halo_props(hs.offsets, pos, vel, mass, hub_pt=70.3)      # autotune
+    tile, wg = info.tile, info.workgroup
+    dev_ms = []
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    for _ in range(steps):
+        _, i2 = halo_props(hs.offsets, pos, vel, mass, hub_pt=70.3, tile=tile, workgroup=wg)
+        dev_ms.append(i2.ms)
+    torch.cuda.synchronize()
+    wall_ms = (time.perf_counter() - w0) * 1e3 / steps
+    ms = float(np.median(dev_ms))
+    M = int(hs.offsets[-1])
+    achieved = HALO_BYTES_PER_MEMBER * M / (ms * 1e-3) / 1e9
+    out = {"halos": hs.nhalos, "members": M, "ms": round(ms, 3), "wall_ms": round(wall_ms, 3),
+           "Mmembers_per_s": round(M / (ms * 1e-3) / 1e6, 2), "tile": tile, "workgroup": wg,
+           "autotune_trials": info.trials,
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "bytes_per_member": HALO_BYTES_PER_MEMBER}}
+    log(f"halo props: {ms:.2f} ms device ({wall_ms:.2f} ms wall), tile {tile} wg {wg}")
+    del pos, vel, mass
+    if with_cpu:
+        import oracle.halo_props as HP
+        rng = np.random.default_rng(0)
+        pick = np.sort(rng.choice(hs.nhalos, size=cpu_sample_halos, replace=False))
+        nmem = 0
+        c0 = time.perf_counter()
+        for h in pick:
+            s, e = int(hs.offsets[h]), int(hs.offsets[h + 1])
+            HP.halo_props(hs.pos[s:e], hs.vel[s:e], hs.mass[s:e], hub_pt=70.3)
+            nmem += e - s
+        t = time.perf_counter() - c0
+        out["cpu_baseline"] = {"value": round(nmem / t / 1e6, 4), "unit": "Mmembers/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{cpu_sample_halos} random halos of the same workload "
+                                         f"({nmem} members), NumPy restatement, {t:.1f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3"])
-    ap.add_argument("--bucket", type=int, default=32)
+    ap.add_argument("--bucket", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=2_097_152)
+    ap.add_argument("--no-halo-props", action="store_true")
+    # one plain device copy of this many MiB before the timed steps: under a counter
+    # profiler its FETCH_SIZE / WRITE_SIZE readings calibrate bytes per counter unit
+    ap.add_argument("--copy-calibration-mib", type=int, default=0)
+    ap.add_argument("--traffic-json", default=None,
+                    help="JSON with measured HBM bytes per launch of the kNN kernel "
+                         "({'fetch_bytes': F, 'write_bytes': W}) to report as roofline.traffic")
+    ap.add_argument("--cpu-sample-halos", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -85,6 +152,15 @@ def main():
     mass = torch.from_numpy(ps.mass).to(dev)
     dens = torch.empty(n, dtype=torch.float64, device=dev)
     gid = torch.empty(n, dtype=torch.int32, device=dev)
+
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {nb} bytes read and written")
+        del src, dst
 
     def step(mixed, saddle=False):
         return run_device(pos, mass, k, nhop=16, mixed=mixed, saddle=saddle, bucket=args.bucket,
@@ -136,6 +212,18 @@ def main():
                 "kernel": "k_knn<64,mixed> (kNN + SPH density)", "kernel_ms": round(knn_ms, 3),
                 "bytes_per_particle": KNN_BYTES_PER_PARTICLE,
                 "step_achieved_GBs": round(STEP_BYTES_PER_PARTICLE * n / (ms * 1e-3) / 1e9, 2)}
+    tpath = args.traffic_json or os.path.join(ROOT, "profiles", f"knn_{args.workload}_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("n") == n and tr.get("bucket") == args.bucket:
+            roofline["traffic"] = int(tr["fetch_bytes"] + tr["write_bytes"])
+            roofline["traffic_source"] = os.path.relpath(tpath, ROOT)
+
+    del pos, mass, dens, gid
+    if not args.no_halo_props:
+        extra["halo_props"] = halo_props_stage(max(2, args.steps // 2), args.cpu_sample_halos,
+                                               not args.no_cpu_baseline)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -17,7 +17,7 @@ __all__ = ["run_device", "device_saddle", "HotParams", "RunInfo"]
 
 
 def run_device(pos, mass, nvoisins, nhop=16, rho_threshold=80.0, box=1.0, mixed=True,
-               bucket=32, saddle=True, density_out=None, gid_out=None):
+               bucket=64, saddle=True, density_out=None, gid_out=None):
     """Run kNN + density + hop + group ids (+ saddle edges) on device tensors.
 
     pos: (N,3) float64 contiguous on the GPU; mass: (N,) float64.  Returns
@@ -75,9 +75,15 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
     from ._lib import HaloInfo, HaloParams, ptr
     off = np.ascontiguousarray(offsets, dtype=np.int64)
     nh = off.shape[0] - 1
-    for t in (pos, vel, mass):
+    for t in (pos, vel, mass) + ((density,) if density is not None else ()):
         if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
-            raise TypeError("pos, vel, mass must be contiguous float64 device tensors")
+            raise TypeError("pos, vel, mass, density must be contiguous float64 device tensors")
+    nm = mass.shape[0]
+    if pos.shape != (nm, 3) or vel.shape != (nm, 3) or (density is not None and
+                                                         density.shape != (nm,)):
+        raise ValueError("pos and vel must be (M, 3) and density (M,) for M = len(mass)")
+    if off.ndim != 1 or nh < 1 or off[0] < 0 or off[-1] > nm:
+        raise ValueError("offsets must be a 1-D CSR array inside [0, len(mass)]")
     out = torch.empty((nh, len(HALO_FIELDS)), dtype=torch.float64, device=pos.device)
     p = HaloParams(float(lbox), float(hub_pt), float(gravconst), float(vir_overdens),
                    float(rho_mean), 1 if cdm else 0, int(nshell), int(direct_pe_nmax), int(tile),

@@ -12,7 +12,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libamdahop.so")
+# AMDAHOP_LIB selects another build of the same library (e.g. one built with different
+# compile-time options); by default the in-tree build is loaded
+LIB_PATH = os.environ.get("AMDAHOP_LIB") or os.path.join(HERE, "libamdahop.so")
 
 _i64, _i32, _dbl, _vp, _f32 = ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p, ctypes.c_float
 

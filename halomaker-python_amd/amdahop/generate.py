@@ -19,7 +19,7 @@ from dataclasses import dataclass
 import numpy as np
 
 __all__ = ["ParticleSet", "clumpy_box", "mixed_family_box", "config", "digest",
-           "CONFIGS", "HaloSet", "nfw_halos"]
+           "CONFIGS", "HaloSet", "nfw_halos", "halo_config"]
 
 
 @dataclass
@@ -199,46 +199,55 @@ class HaloSet:
         return int(self.offsets.shape[0] - 1)
 
 
-def _nfw_radii(rng, n, c, rvir):
-    """Radii from the NFW enclosed-mass profile up to rvir (bisection inverse)."""
-    mu = lambda x: np.log1p(x) - x / (1.0 + x)
-    u = rng.uniform(0.0, 1.0, size=n) * mu(c)
-    lo = np.zeros(n)
-    hi = np.full(n, c)
-    for _ in range(60):
-        mid = 0.5 * (lo + hi)
-        f = mu(mid) - u
-        lo = np.where(f < 0, mid, lo)
-        hi = np.where(f < 0, hi, mid)
-    return 0.5 * (lo + hi) * (rvir / c)
+def _nfw_x(y):
+    """x >= 0 with ln(1+x) - x/(1+x) = y: the NFW enclosed-mass profile inverted through
+    the principal branch of Lambert's W, 1 + x = -1 / W0(-exp(-1-y))."""
+    from scipy.special import lambertw
+    w = lambertw(-np.exp(-1.0 - y), 0).real
+    return np.maximum(-1.0 / w - 1.0, 0.0)
 
 
-def nfw_halos(nhalos, seed, alpha=1.9, nmin=20, nmax=2_000_000, mass_sigma_dex=0.0,
-              gravconst=430.1, name="halos"):
-    """``nhalos`` NFW spheres, member counts Pareto(alpha) truncated to [nmin, nmax]."""
+def nfw_halos(nhalos, seed, alpha=1.9, nmin=20, nmax=2_000_000, total=None,
+              mass_sigma_dex=0.0, gravconst=430.1, name="halos"):
+    """``nhalos`` NFW spheres in a unit periodic box.
+
+    Member counts are Pareto(alpha) with minimum ``nmin``, scaled to sum to about
+    ``total`` when it is given, then truncated to [nmin, nmax].  Concentrations are
+    logU[4, 15]; radii follow the NFW enclosed-mass profile up to R = 0.01 (n/1000)^(1/3);
+    centres are U[0,1)^3, positions wrapped and shifted by -0.5; velocities are isotropic
+    Gaussians with sigma^2 = G M / (2 R); masses are equal and sum to 1 (log-normal with
+    ``mass_sigma_dex`` when it is positive).  Vectorised over all members.
+    """
     rng = np.random.default_rng(seed)
     raw = nmin * (1.0 + rng.pareto(alpha, size=nhalos))
-    counts = np.minimum(np.floor(raw).astype(np.int64), nmax)
+    if total is not None:
+        raw *= float(total) / raw.sum()
+    counts = np.clip(np.floor(raw).astype(np.int64), nmin, nmax)
     offsets = np.zeros(nhalos + 1, dtype=np.int64)
     np.cumsum(counts, out=offsets[1:])
     mtot = int(offsets[-1])
-    pos = np.empty((mtot, 3))
-    vel = np.empty((mtot, 3))
     conc = np.exp(rng.uniform(np.log(4.0), np.log(15.0), size=nhalos))
     centres = rng.uniform(0.0, 1.0, size=(nhalos, 3))
-    mass = np.full(mtot, 1.0 / max(mtot, 1))
+    rvir = 0.01 * (counts / 1000.0) ** (1.0 / 3.0)
+    mass = np.full(mtot, 1.0 / mtot)
     if mass_sigma_dex > 0:
         mass *= 10.0 ** rng.normal(0.0, mass_sigma_dex, size=mtot)
-    for h in range(nhalos):
-        s, e = offsets[h], offsets[h + 1]
-        n = int(e - s)
-        rvir = 0.01 * (n / 1000.0) ** (1.0 / 3.0)
-        r = _nfw_radii(rng, n, conc[h], rvir)
-        d = rng.normal(size=(n, 3))
-        d /= np.linalg.norm(d, axis=1, keepdims=True)
-        p = centres[h] + d * r[:, None]
-        pos[s:e] = np.mod(p, 1.0) - 0.5
-        mh = float(mass[s:e].sum())
-        sig = np.sqrt(gravconst * mh / (2.0 * rvir))
-        vel[s:e] = rng.normal(0.0, sig, size=(n, 3))
+    hid = np.repeat(np.arange(nhalos, dtype=np.int32), counts)
+    mu_c = np.log1p(conc) - conc / (1.0 + conc)
+    r = _nfw_x(rng.uniform(0.0, 1.0, size=mtot) * mu_c[hid]) * (rvir / conc)[hid]
+    pos = rng.normal(size=(mtot, 3))
+    pos *= (r / np.linalg.norm(pos, axis=1))[:, None]
+    del r
+    pos += centres[hid]
+    np.mod(pos, 1.0, out=pos)
+    pos -= 0.5
+    pos[pos >= 0.5] -= 1.0
+    sig = np.sqrt(gravconst * np.add.reduceat(mass, offsets[:-1]) / (2.0 * rvir))
+    vel = rng.normal(size=(mtot, 3))
+    vel *= sig[hid][:, None]
     return HaloSet(offsets, pos, vel, mass, name)
+
+
+def halo_config(nhalos=50_000, total=30_000_000, seed=4):
+    """The batched halo-property workload: 50 000 NFW halos, about 30 M members."""
+    return nfw_halos(nhalos, seed, total=total, name=f"nfw{nhalos}")

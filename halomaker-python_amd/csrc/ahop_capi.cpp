@@ -40,7 +40,7 @@ struct ModuleState {
     int32_t nlevelmax = 30;
     bool optimize_nodes = false;
     // extensions
-    int32_t mixed = 1, bucket = 32;
+    int32_t mixed = 1, bucket = 64;
     // outputs (allocated by compute_adaptahop, freed by close)
     bool allocated = false;
     std::vector<int32_t> liste_parts;

@@ -16,7 +16,7 @@ struct Params {
     double rho_threshold = 80.0;
     double xlong = 1.0, ylong = 1.0, zlong = 1.0;
     int mixed = 1;              // 1: FP32 box-relative filter + FP64 re-rank, 0: FP64 throughout
-    int bucket = 32;            // particles per leaf bucket of the device tree (<= 64)
+    int bucket = 64;            // particles per leaf bucket of the device tree (<= 64)
 };
 
 // Device tree built over Morton-sorted particles (binary radix tree, leaves = particles).

@@ -335,6 +335,7 @@ struct KnnArgs {
     double Lx, Ly, Lz, xyzl, rho_thr;
     const int32_t *qlist;   // optional explicit queries (sorted slots); null -> all slots
     int nq;
+    int qper;               // queries per wave: WAVE, or 1 for a short list of scattered queries
     double *rho_s;
     int32_t *nbr;
     int32_t *fb_list;       // mixed path: queries that need the FP64 path
@@ -391,6 +392,9 @@ __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int s
     hi[e * WAVE + lane] = id;
 }
 
+#ifndef AHOP_FP64_NBUF
+#define AHOP_FP64_NBUF 1
+#endif
 #include "knn_wave.h"
 
 // active-particle count (density > rho_threshold, f90:712-715)
@@ -707,6 +711,7 @@ void Engine::knn_density(const Params &p)
     a.rho_thr = p.rho_threshold;
     a.qlist = nullptr;
     a.nq = (int)n;
+    a.qper = WAVE;
     a.rho_s = rho_s;
     a.nbr = nbr;
     a.fb_list = fb;
@@ -738,7 +743,10 @@ void Engine::knn_density(const Params &p)
         KnnArgs b = a;
         b.qlist = fb_sorted;
         b.nq = nfb;
-        unsigned nb2 = (unsigned)((b.nq + 63) / 64);
+        // a short list is spread out over the box: one query per wave, so that no wave
+        // walks the union of 64 unrelated search spheres
+        b.qper = nfb <= 16384 ? 1 : WAVE;
+        unsigned nb2 = (unsigned)((b.nq + b.qper - 1) / b.qper);
         b.nblocks = (int)nb2;
         b.stats = nullptr;
         dispatch_knn(p.nvoisins, false, b, nb2, s);

@@ -176,7 +176,8 @@ __global__ __launch_bounds__(BS) void k_pass2(HaloDev D, const HaloState *st, in
 }
 
 __global__ void k_combine2(int nh, const int32_t *tile_first, const double *pkey,
-                           const int64_t *pidx, const double *pos, HaloState *st)
+                           const int64_t *pidx, const int64_t *off, const double *pos,
+                           HaloState *st)
 {
     int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= nh) return;
@@ -187,6 +188,7 @@ __global__ void k_combine2(int nh, const int32_t *tile_first, const double *pkey
             best = pkey[t];
             bi = pidx[t];
         }
+    if (bi < 0) bi = off[h];   // every key NaN (non-finite input): take the first member
     HaloState &S = st[h];
     S.icenter = bi;
     S.px = pos[3 * bi];
@@ -416,28 +418,45 @@ __global__ __launch_bounds__(BS) void k_pass4b(HaloDev D, const HaloState *st, i
     for (int b = threadIdx.x; b < 3 * nr; b += BS) out[b] = hs[b];
 }
 
-// ---- direct potential energy of halos below the pair-sum limit (det_halo_energies_1b3 :1417-1427)
+// one row of the pair sum: member i against members j > i, per pair -(m_i m_j) / (d_ij / L)
+// with the pair separation pos[j] - pos[i] wrapped strictly at L/2 (:1417-1426)
+__device__ __forceinline__ double pe_row(const double4 *__restrict__ mem, int i, int n, double L)
+{
+    const double4 a = mem[i];
+    double s = 0.0;
+    for (int j = i + 1; j < n; ++j) {
+        const double4 b = mem[j];
+        const double dx = wrap_strict(b.x - a.x, L);
+        const double dy = wrap_strict(b.y - a.y, L);
+        const double dz = wrap_strict(b.z - a.z, L);
+        const double dist = sqrt(dx * dx + dy * dy + dz * dz) / L;
+        s += -(a.w * b.w) / dist;
+    }
+    return s;
+}
+
+// ---- direct potential energy of halos below the pair-sum limit (det_halo_energies_1b3
+// :1417-1427).  The halo's members are staged in LDS (at most PE_NMAX of them); a thread takes
+// rows r and n-2-r of the upper triangle together, so every thread sums about n pairs.
+constexpr int PE_NMAX = 2048;   // 64 KiB of LDS
 template <int BS>
 __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *small_halos,
                                                   double *ep_out)
 {
+    extern __shared__ double4 mem[];
     __shared__ double sh[BS];
     const int h = small_halos[blockIdx.x];
     const int64_t f = D.off[h];
-    const int64_t n = D.off[h + 1] - f;
+    const int n = (int)(D.off[h + 1] - f);
+    for (int i = threadIdx.x; i < n; i += BS)
+        mem[i] = make_double4(D.pos[3 * (f + i)], D.pos[3 * (f + i) + 1], D.pos[3 * (f + i) + 2],
+                              D.mass[f + i]);
+    __syncthreads();
     double acc = 0.0;
-    for (int64_t i = 0; i < n - 1; ++i) {     // pairs (i, j > i), each counted once
-        const int64_t a = f + i;
-        const double ax = D.pos[3 * a], ay = D.pos[3 * a + 1], az = D.pos[3 * a + 2];
-        const double ma = D.mass[a];
-        for (int64_t j = i + 1 + threadIdx.x; j < n; j += BS) {
-            const int64_t b = f + j;
-            double dx = wrap_strict(D.pos[3 * b] - ax, D.L);
-            double dy = wrap_strict(D.pos[3 * b + 1] - ay, D.L);
-            double dz = wrap_strict(D.pos[3 * b + 2] - az, D.L);
-            double dist = sqrt(dx * dx + dy * dy + dz * dz) / D.L;
-            acc += -(ma * D.mass[b]) / dist;
-        }
+    const int R = n - 1;   // rows 0 .. n-2
+    for (int r = threadIdx.x; r < (R + 1) / 2; r += BS) {
+        acc += pe_row(mem, r, n, D.L);
+        if (R - 1 - r != r) acc += pe_row(mem, R - 1 - r, n, D.L);
     }
     const int t = threadIdx.x;
     sh[t] = acc;
@@ -449,6 +468,30 @@ __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *smal
     if (t == 0) ep_out[h] = sh[0] * D.G / D.L;
 }
 
+// In-place inclusive prefix sum added strictly left to right (np.cumsum's order), one thread;
+// loads are batched so that only the additions form the dependent chain.
+__device__ void seq_cumsum(double *a, int n)
+{
+    double acc = 0.0;
+    int b = 0;
+    for (; b + 8 <= n; b += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = a[b + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            acc += v[u];
+            a[b + u] = acc;
+        }
+    }
+    for (; b < n; ++b) {
+        acc += a[b];
+        a[b] = acc;
+    }
+}
+
+constexpr int WAVE_HP = 64;   // first thread of the second wave of a workgroup
+
 // ---- shell tables, potential profile and the virial scan, one workgroup per halo
 template <int BS>
 __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first,
@@ -459,9 +502,9 @@ __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first
     extern __shared__ double work[];          // 4*nr doubles
     const int h = blockIdx.x;
     if (h >= nh) return;
-    double *tm = work;                         // cumulative mass
-    double *tk = tm + nr;                      // cumulative kinetic energy
-    double *cn = tk + nr;                      // shell counts
+    double *tm = work;                         // shell mass -> cumulative mass
+    double *tk = tm + nr;                      // shell kinetic energy -> cumulative
+    double *cn = tk + nr;                      // shell counts -> virialisation ratio per shell
     double *tp = cn + nr;                      // potential profile
     const HaloState &S = st[h];
     const int t0 = tile_first[h], t1 = tile_first[h + 1];
@@ -478,38 +521,47 @@ __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first
         cn[b] = c;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    // cumulative tables, potential profile (shell / count self pairs / profile outer for large)
+    // cumulative tables: sequential left-to-right sums, as np.cumsum adds (:1024-1025)
+    if (threadIdx.x == 0) seq_cumsum(tm, nr);
+    else if (threadIdx.x == WAVE_HP) seq_cumsum(tk, nr);
+    __syncthreads();
+    // shell potential terms in parallel (potential_profile_1b40 :1090-1103): dm from the
+    // cumulative table (np.diff with prepend 0), mass before the shell = cumulative - dm
+    const double reff0 = pow(S.amax * S.bmax * S.cmax, 1.0 / 3.0);
+    for (int b = threadIdx.x; b < nr; b += BS) {
+        const double cum = tm[b];
+        const double dm = cum - (b > 0 ? tm[b - 1] : 0.0);
+        const double before = cum - dm;
+        const double sf = cn[b] > 0 ? 0.5 * (cn[b] - 1.0) / cn[b] : 0.0;
+        const double scale = (b == 0) ? 0.5 / (double)(nr - 1) : (double)b / (double)(nr - 1);
+        tp[b] = (before + sf * dm) * dm / (scale * reff0);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) seq_cumsum(tp, nr);
+    __syncthreads();
     const int64_t count = off[h + 1] - off[h];
-    double reff0 = cbrt(S.amax * S.bmax * S.cmax);
-    double accm = 0, acck = 0, accp = 0;
-    for (int b = 0; b < nr; ++b) {
-        double dm = tm[b];
-        double before = accm;
-        accm += dm;
-        acck += tk[b];
-        double sf = cn[b] > 0 ? 0.5 * (cn[b] - 1.0) / cn[b] : 0.0;
-        double scale = (b == 0) ? 0.5 / (double)(nr - 1) : (double)b / (double)(nr - 1);
-        accp += (before + sf * dm) * dm / (scale * reff0);
-        tm[b] = accm;
-        tk[b] = acck;
-        tp[b] = -P.gravconst * accp;
-    }
-    double outer = tp[nr - 1];
+    const double outer = -P.gravconst * tp[nr - 1];
     double *o = out + (int64_t)h * HP_NFIELDS;
-    double ep;
-    if (count < P.direct_pe_nmax) {
-        ep = ep_direct[h];
-        if (ep != outer && outer != 0.0)
-            for (int b = 0; b < nr; ++b) tp[b] = tp[b] / outer * ep;
-    } else {
-        ep = outer;
+    const bool small = count < P.direct_pe_nmax;
+    const double ep = small ? ep_direct[h] : outer;
+    const bool renorm = small && ep != outer && outer != 0.0;
+    __syncthreads();
+    // profile (-G * cumulative terms, rescaled to the direct energy for small halos) and the
+    // virialisation ratio |2K+W|/|K+W| of each shell; -1 marks an unbound shell (K+W >= 0)
+    for (int b = threadIdx.x; b < nr; b += BS) {
+        double p = -P.gravconst * tp[b];
+        if (renorm) p = p / outer * ep;
+        tp[b] = p;
+        const double kw = tk[b] + p;
+        cn[b] = (kw >= 0.0) ? -1.0 : fabs((2.0 * tk[b] + p) / kw);
     }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     double ek = o[HP_EK];
     double et = ek + ep;
     o[HP_EP] = ep;
     o[HP_ET] = et;
-    // det_vir_props_1b4 :1127-1229
+    // det_vir_props_1b4 :1127-1229 (compare-only scans over the tables)
     const int ttab = nr;
     double amax = S.amax, bmax = S.bmax, cmax = S.cmax;
     double mvir = tm[ttab - 1];
@@ -523,8 +575,8 @@ __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first
     double rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
     double virth_old = 1.0, virth = 100.0;
     for (int i0 = ttab - 1; i0 >= 0; --i0) {
-        if ((tk[i0] + tp[i0]) >= 0.0) continue;
-        virth = fabs((2.0 * tk[i0] + tp[i0]) / (tk[i0] + tp[i0]));
+        if (cn[i0] == -1.0) continue;
+        virth = cn[i0];
         if (virth < virth_old) {
             mvir = tm[i0];
             avir = fmin(avir, (double)i0 / (ttab - 1) * amax);
@@ -539,7 +591,8 @@ __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first
     if ((virth > 0.20) || (mvir < P.vir_overdens * P.rho_mean * 4. / 3. * PI * avir * bvir * cvir)) {
         int ii1;
         for (ii1 = ttab - 1; ii1 >= 1; --ii1) {
-            double mv = P.vir_overdens * P.rho_mean * volmin * (double)ii1 * (double)ii1 * (double)ii1;
+            // vir_overdens * rho_mean * volmin * ii1**3, the cube an exact integer
+            double mv = P.vir_overdens * P.rho_mean * volmin * (double)((int64_t)ii1 * ii1 * ii1);
             if ((tm[ii1] >= mv) && (tm[ii1 - 1] < tm[ttab - 1])) break;
         }
         if (ii1 < 1) ii1 = 1;
@@ -604,6 +657,7 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
     const int nh = (int)hoff.size() - 1;
     Tiles tl = make_tiles(hoff, T);
     const int nt = (int)tl.halo.size();
+    std::vector<int32_t> small;   // halos of the pair-sum branch (host copy kept until the sync)
     std::vector<void *> keep;
     auto cleanup = [&]() {
         for (void *p : keep) (void)hipFree(p);
@@ -631,7 +685,7 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
         hipLaunchKernelGGL(k_pass1<BS>, dim3(nt), dim3(BS), 0, s, D, part);
         hipLaunchKernelGGL(k_combine1, dim3(hb), dim3(128), 0, s, nh, d_tf, part, d_off, pos, P.lbox, st);
         hipLaunchKernelGGL(k_pass2<BS>, dim3(nt), dim3(BS), 0, s, D, st, P.cdm, pkey, pidx);
-        hipLaunchKernelGGL(k_combine2, dim3(hb), dim3(128), 0, s, nh, d_tf, pkey, pidx, pos, st);
+        hipLaunchKernelGGL(k_combine2, dim3(hb), dim3(128), 0, s, nh, d_tf, pkey, pidx, d_off, pos, st);
         hipLaunchKernelGGL(k_pass3<BS>, dim3(nt), dim3(BS), 0, s, D, st, part);
         hipLaunchKernelGGL(k_combine3, dim3(hb), dim3(128), 0, s, nh, d_tf, part, P.gravconst, st, d_out);
         if (full) {
@@ -641,16 +695,20 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
             double *hist = dalloc<double>(keep, (size_t)nt * 3 * nr);
             hipLaunchKernelGGL(k_pass4b<BS>, dim3(nt), dim3(BS), sizeof(double) * 3 * nr, s, D, st,
                                nr, hist, missed);
-            std::vector<int32_t> small;
+            int64_t nsmall_max = 0;
             for (int h = 0; h < nh; ++h)
-                if (hoff[h + 1] - hoff[h] < P.direct_pe_nmax) small.push_back(h);
+                if (hoff[h + 1] - hoff[h] < P.direct_pe_nmax) {
+                    small.push_back(h);
+                    nsmall_max = std::max(nsmall_max, hoff[h + 1] - hoff[h]);
+                }
             double *ep = dalloc<double>(keep, nh);
             if (!small.empty()) {
+                if (nsmall_max > PE_NMAX) throw std::runtime_error("direct_pe_nmax above the LDS limit");
                 int32_t *d_small = dalloc<int32_t>(keep, small.size());
                 AHOP_HIP(hipMemcpyAsync(d_small, small.data(), sizeof(int32_t) * small.size(),
                                         hipMemcpyHostToDevice, s));
-                hipLaunchKernelGGL(k_direct_pe<256>, dim3((unsigned)small.size()), dim3(256), 0, s, D,
-                                   d_small, ep);
+                hipLaunchKernelGGL(k_direct_pe<256>, dim3((unsigned)small.size()), dim3(256),
+                                   sizeof(double4) * nsmall_max, s, D, d_small, ep);
             }
             hipLaunchKernelGGL(k_virial<256>, dim3(nh), dim3(256), sizeof(double) * 4 * nr, s, nh,
                                d_tf, d_off, hist, nr, st, ep, P, d_out);
@@ -680,9 +738,12 @@ HaloTuning halo_props_device(const int64_t *h_off, int64_t nhalos, const double 
                              hipStream_t s)
 {
     if (nhalos <= 0) throw std::runtime_error("need at least one halo");
-    if (P.nshell < 2) throw std::runtime_error("nshell must be >= 2");
+    // the shell tables of one halo live in LDS (4 doubles per shell)
+    if (P.nshell < 2 || P.nshell > 2000) throw std::runtime_error("nshell must be in [2, 2000]");
     if (!P.cdm && !dens) throw std::runtime_error("densest-member centre needs densities");
     std::vector<int64_t> hoff(h_off, h_off + nhalos + 1);
+    if (hoff[0] < 0) throw std::runtime_error("negative member offset");
+    if (nhalos > INT32_MAX / 2) throw std::runtime_error("too many halos");
     for (int64_t h = 0; h < nhalos; ++h)
         if (hoff[h + 1] <= hoff[h]) throw std::runtime_error("every halo needs at least one member");
     const int64_t M = hoff[nhalos] - hoff[0];

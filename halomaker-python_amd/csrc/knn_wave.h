@@ -64,8 +64,13 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
 {
     using KeyT = typename std::conditional<MIXED, float, double>::type;
     using StageT = typename std::conditional<MIXED, float4, double4>::type;
-    constexpr int KHCAP = MIXED ? KCAP + 1 : KCAP;
-    constexpr int NBUF = 4;   // staged buckets whose insertions are deferred
+    // the mixed heap keeps KX entries beyond the k nearest, so that near-ties at the k-th
+    // place are settled in FP64 inside this kernel rather than by the FP64 re-run
+    constexpr int KX = 2;
+    constexpr int KHCAP = MIXED ? KCAP + KX : KCAP;
+    // staged buckets whose insertions are deferred (the FP64 heap is twice as large, so its
+    // stage may be cut down to fit one more wave per CU)
+    constexpr int NBUF = MIXED ? 4 : AHOP_FP64_NBUF;
     __shared__ KeyT hk[KHCAP * WAVE];
     __shared__ int32_t hi[KHCAP * WAVE];
     __shared__ StageT stage[NBUF * WAVE];
@@ -73,11 +78,13 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
 
     const int lane = threadIdx.x;
     const int wave = logical_block(A.nblocks);
-    const int slot = wave * WAVE + lane;
-    const bool active = slot < A.nq;
+    // qper < WAVE only with an explicit query list (the seeding below assumes full waves
+    // of consecutive slots otherwise)
+    const int slot = wave * A.qper + lane;
+    const bool active = lane < A.qper && slot < A.nq;
     const int q = active ? (A.qlist ? A.qlist[slot] : slot) : 0;
     const int K = A.K;
-    const int KH = MIXED ? K + 1 : K;
+    const int KH = MIXED ? K + KX : K;
     const double4 P = A.spos[q];
     const double Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
     const double4 *__restrict__ spos = A.spos;
@@ -330,17 +337,22 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     if (!active) return;
 
     bool ok = !wave_unsafe;
+    // entries that take part in the FP64 selection of the k nearest: all but the last heap
+    // entry (the mixed heap holds k+KX), and no more than the other particles there are
+    const int nsel = MIXED ? min(KH - 1, A.n - 1) : K;
     if constexpr (MIXED) {
-        // certify that FP32 ranking cannot have changed the k-nearest set: the (k+1)-th
-        // FP32 key must clear the k-th by twice the error bound E of a key of that size,
-        // and every particle within that radius must lie in the periodic image the
-        // wave-centred frame picks (radius + query spread < L/2; a particle the frame folds
-        // onto another image gets a key no smaller than its true distance, so only the
-        // particles inside the radius need the right image)
-        const float lk = hk[K * WAVE + lane], lk1 = hk[(K - 1) * WAVE + lane];
-        const float E = fp32_key_err(lk, hQ, Lmin);
-        if (!(lk == INFINITY) &&
-            !(lk - lk1 > 2 * E && E < 0.01f * lk && sqrtf(lk * 1.0001f) + hQ < 0.4999f * Lmin))
+        // certify that FP32 ranking cannot have left one of the k nearest out of the first
+        // k+KX-1 entries: the last FP32 key must clear the k-th by twice the error bound E
+        // of a key of that size (every particle of the true k nearest has a key below the
+        // k-th key + 2E), and every particle within that radius must lie in the periodic
+        // image the wave-centred frame picks (radius + query spread < L/2; a particle the
+        // frame folds onto another image gets a key no smaller than its true distance, so
+        // only the particles inside the radius need the right image).  An unfilled heap
+        // (last key infinite) holds every other particle.
+        const float lkx = hk[(KH - 1) * WAVE + lane], lk1 = hk[(K - 1) * WAVE + lane];
+        const float E = fp32_key_err(lkx, hQ, Lmin);
+        if (!(lkx == INFINITY) &&
+            !(lkx - lk1 > 2 * E && E < 0.01f * lkx && sqrtf(lkx * 1.0001f) + hQ < 0.4999f * Lmin))
             ok = false;
     }
     if (!ok) {
@@ -349,8 +361,9 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     }
     // ---- density in FP64, reference order (omp_compute_density f90:2816-2838); the mixed
     // path recomputes every distance in the reference's FP64 arithmetic; where FP32 near-ties
-    // left two entries out of FP64 order, the lane re-sorts its k entries by the FP64
-    // distances and sums again, so its densities are those of the FP64 path
+    // left entries out of FP64 order (within the k, or an extra entry nearer than the k-th),
+    // the lane re-sorts its nsel entries by the FP64 distances, keeps the first k and sums
+    // again, so its densities are those of the FP64 path
     double rho = 0.0;
     for (int pass = 0; pass < 2; ++pass) {
         double d2k;
@@ -376,10 +389,16 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
             contrib = contrib + c.w * spline_ref(sqrt(d) * unsr);
         }
         sorted = sorted && (d2k >= prev);
+        if constexpr (MIXED) {
+            for (int i = K; i < nsel; ++i) {
+                const double4 c = spos[hi[i * WAVE + lane]];
+                sorted = sorted && (d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz) >= d2k);
+            }
+        }
         rho = A.xyzl * (contrib + P.w) / (PI_REF * r3);
         if (!MIXED || sorted) break;
-        // insertion sort of this lane's k entries by FP64 distance (stable on ties)
-        for (int i = 1; i < K; ++i) {
+        // insertion sort of this lane's nsel entries by FP64 distance (stable on ties)
+        for (int i = 1; i < nsel; ++i) {
             const int32_t id = hi[i * WAVE + lane];
             const double4 ci = spos[id];
             const double di = d2_ref(P.x, P.y, P.z, ci.x, ci.y, ci.z, Lx, Ly, Lz);

@@ -62,7 +62,7 @@ const char *ahop_last_error(void);
 
 /* 1: FP32 box-relative search with certified FP64 re-rank (default); 0: FP64 throughout. */
 int ahop_set_precision(int32_t mixed);
-/* particles per device-tree leaf bucket, 1..64 (default 32) */
+/* particles per device-tree leaf bucket, 1..64 (default 64) */
 int ahop_set_bucket(int32_t bucket);
 
 /* Stage outputs of the last ahop_compute_adaptahop, original particle order:

@@ -7,12 +7,15 @@ SC/DPMMC off (refine_centers_1b :1555 then leaves the centre unchanged), virial
 potential profile 'shell' with self mode 'count' and norm policy
 'profile_outer_for_large' (the defaults at :1035-1047).
 
-Output order of :func:`halo_props` (one float64 row per halo) -- see FIELDS.
+Each reference function has its own function here so that each can be pinned on its
+own; :func:`halo_props` composes them in the reference's order and returns one float64
+row per halo in the order of FIELDS.
 
-Pinning (tests/test_halo_oracle.py): closed forms (two-body angular momentum,
-rigid rotation, the inertia tensor of a symmetric cube shell, a two-particle direct
-potential energy), invariance of L and I under a periodic translation across the box
-edge, eigen-decomposition vs scipy.linalg.eigh used as the reference does.
+Pinning (tests/test_halo_oracle.py): closed forms (two-body and rigid-rotation angular
+momentum, the inertia tensor of a cube shell, direct potential energies of regular
+2-, 3- and 4-member systems, Hubble-flow kinetic energy, the shell profile of a single
+shell, the overdensity branch of a static halo), invariance under a periodic translation
+across the box edge and rotation covariance of L and I, and eigh ordering.
 """
 from __future__ import annotations
 
@@ -24,15 +27,15 @@ FIELDS = ["m", "px", "py", "pz", "vx", "vy", "vz", "sigma", "Lx", "Ly", "Lz", "r
 F = {k: i for i, k in enumerate(FIELDS)}
 
 
-def _wrap(d, L):
+def wrap(d, L):
     """correct_for_periodicity_1d :888-897 (strict > L/2 and < -L/2)."""
-    d = d.copy()
+    d = np.array(d, dtype=np.float64, copy=True)
     d[d > L / 2] -= L
     d[d < -L / 2] += L
     return d
 
 
-def _wrap_scalar(x, L):
+def wrap_scalar(x, L):
     """correct_for_periodicity :856-885 on one coordinate."""
     if x > L / 2:
         x = x - L
@@ -41,104 +44,96 @@ def _wrap_scalar(x, L):
     return x
 
 
-def _shell_scale(nr):
+def det_center(pos, vel, mass, density, L, cdm):
+    """det_center_18 :901-966 -> (centre position (3,), mean velocity (3,), sigma)."""
+    m = np.sum(mass)
+    if cdm:
+        p0 = pos[0]
+        pc = [wrap_scalar(np.sum(mass * wrap(pos[:, d] - p0[d], L)) / m + p0[d], L)
+              for d in range(3)]
+        d2 = sum(wrap(pos[:, d] - pc[d], L) ** 2 for d in range(3))
+        cpos = pos[int(np.argmin(d2))]
+    else:
+        cpos = pos[int(np.argmax(density))]
+    vbar = np.array([np.sum(mass * vel[:, d]) / m for d in range(3)])
+    dv2 = mass * ((vel[:, 0] - vbar[0]) ** 2 + (vel[:, 1] - vbar[1]) ** 2 +
+                  (vel[:, 2] - vbar[2]) ** 2)
+    return np.array(cpos, dtype=np.float64), vbar, np.sqrt(np.sum(dv2) / m)
+
+
+def rel(pos, c, L):
+    """Member offsets from the centre, wrapped (used by :783-1032, :1400-1444, :1932-1951)."""
+    return [wrap(pos[:, d] - c[d], L) for d in range(3)]
+
+
+def ang_mom(pos, vel, mass, c, vbar, L):
+    """compute_ang_mom_19 :783-824: L = sum dr x m (v - vbar)."""
+    drx, dry, drz = rel(pos, c, L)
+    pvx, pvy, pvz = (mass * (vel[:, d] - vbar[d]) for d in range(3))
+    return np.array([np.sum(dry * pvz - drz * pvy), np.sum(drz * pvx - drx * pvz),
+                     np.sum(drx * pvy - dry * pvx)])
+
+
+def r_halo(pos, c, L):
+    """r_halos_1a :828-851: distance of the farthest member from the centre."""
+    drx, dry, drz = rel(pos, c, L)
+    return np.sqrt(np.max(drx ** 2 + dry ** 2 + drz ** 2))
+
+
+def inertia(pos, mass, c, L):
+    """det_inertial_tensor_1b20 :1932-1951: I = sum m dr (x) dr."""
+    dr = rel(pos, c, L)
+    I = np.zeros((3, 3))
+    for a in range(3):
+        for b in range(a, 3):
+            I[a, b] = I[b, a] = np.sum(mass * dr[a] * dr[b])
+    return I
+
+
+def main_axis(I, m):
+    """det_main_axis_1b2 :1954-1967: eigh (ascending) -> axes sqrt(lambda/m), vectors."""
+    from scipy.linalg import eigh
+    d, v = eigh(I)
+    return np.sqrt(d / m), v
+
+
+def direct_pe(pos, mass, L, gravconst):
+    """det_halo_energies_1b3 :1417-1427: pair sum over i<j of -m_i m_j / (d_ij/L) * G/L."""
+    n = mass.shape[0]
+    i, j = np.triu_indices(n, 1)
+    drs = pos[j] - pos[i]
+    drs[drs > L / 2] -= L
+    drs[drs < -L / 2] += L
+    dists = np.sqrt(np.sum(drs ** 2, axis=1)) / L
+    return np.sum(-(mass[i] * mass[j]) / dists) * gravconst / L
+
+
+def kinetic(pos, vel, mass, c, vbar, L, hub_pt):
+    """det_halo_energies_1b3 :1430-1441 -> (ek, per-member |v_t|^2 with the Hubble flow)."""
+    drx, dry, drz = rel(pos, c, L)
+    v2 = ((vel[:, 0] - vbar[0] + drx * hub_pt) ** 2 + (vel[:, 1] - vbar[1] + dry * hub_pt) ** 2 +
+          (vel[:, 2] - vbar[2] + drz * hub_pt) ** 2)
+    return 0.5 * np.sum(mass * v2), v2
+
+
+def shell_scale(nr):
     """_shell_scale :1061-1066."""
     s = np.arange(nr, dtype=np.float64) / np.float64(nr - 1)
     s[0] = 0.5 / np.float64(nr - 1)
     return s
 
 
-def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst=430.1,
-               vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000,
-               eig=None):
-    """All properties of one halo.  ``eig`` optionally supplies (d, v) to use instead of
-    scipy.linalg.eigh (to compare the virial scan on identical axes)."""
-    from scipy.linalg import eigh
-
-    out = np.full(len(FIELDS), np.nan)
-    L = lbox
-    n = mass.shape[0]
-    # det_mass_17 :754-780
-    m = np.sum(mass)
-    out[F["m"]] = m
-    # det_center_18 :901-966
-    if cdm:
-        p0 = pos[0]
-        drx = _wrap(pos[:, 0] - p0[0], L)
-        dry = _wrap(pos[:, 1] - p0[1], L)
-        drz = _wrap(pos[:, 2] - p0[2], L)
-        pc = [np.sum(mass * drx) / m + p0[0], np.sum(mass * dry) / m + p0[1],
-              np.sum(mass * drz) / m + p0[2]]
-        pc = [_wrap_scalar(x, L) for x in pc]
-        drx = _wrap(pos[:, 0] - pc[0], L)
-        dry = _wrap(pos[:, 1] - pc[1], L)
-        drz = _wrap(pos[:, 2] - pc[2], L)
-        d2 = drx ** 2 + dry ** 2 + drz ** 2
-        imin = int(np.argmin(d2))
-        cpos = pos[imin]
-    else:
-        cpos = pos[int(np.argmax(density))]
-    px, py, pz = cpos
-    out[F["px"]], out[F["py"]], out[F["pz"]] = px, py, pz
-    vx = np.sum(mass * vel[:, 0]) / m
-    vy = np.sum(mass * vel[:, 1]) / m
-    vz = np.sum(mass * vel[:, 2]) / m
-    out[F["vx"]], out[F["vy"]], out[F["vz"]] = vx, vy, vz
-    dv2 = mass * ((vel[:, 0] - vx) ** 2 + (vel[:, 1] - vy) ** 2 + (vel[:, 2] - vz) ** 2)
-    out[F["sigma"]] = np.sqrt(np.sum(dv2) / m)
-    # compute_ang_mom_19 :783-824
-    drx = _wrap(pos[:, 0] - px, L)
-    dry = _wrap(pos[:, 1] - py, L)
-    drz = _wrap(pos[:, 2] - pz, L)
-    pvx = mass * (vel[:, 0] - vx)
-    pvy = mass * (vel[:, 1] - vy)
-    pvz = mass * (vel[:, 2] - vz)
-    out[F["Lx"]] = np.sum(dry * pvz - drz * pvy)
-    out[F["Ly"]] = np.sum(drz * pvx - drx * pvz)
-    out[F["Lz"]] = np.sum(drx * pvy - dry * pvx)
-    # r_halos_1a :828-851
-    out[F["r"]] = np.sqrt(np.max(drx ** 2 + dry ** 2 + drz ** 2))
-    # det_inertial_tensor_1b20 :1932-1951
-    I = np.zeros((3, 3))
-    I[0, 0] = np.sum(mass * drx * drx)
-    I[0, 1] = I[1, 0] = np.sum(mass * drx * dry)
-    I[0, 2] = I[2, 0] = np.sum(mass * drx * drz)
-    I[1, 1] = np.sum(mass * dry * dry)
-    I[1, 2] = I[2, 1] = np.sum(mass * dry * drz)
-    I[2, 2] = np.sum(mass * drz * drz)
-    out[F["Ixx"]], out[F["Ixy"]], out[F["Ixz"]] = I[0, 0], I[0, 1], I[0, 2]
-    out[F["Iyy"]], out[F["Iyz"]], out[F["Izz"]] = I[1, 1], I[1, 2], I[2, 2]
-    # det_main_axis_1b2 :1954-1967
-    d, v = eigh(I) if eig is None else eig
-    d = np.sqrt(d / m)
-    sha, shb, shc = d
-    out[F["sha"]], out[F["shb"]], out[F["shc"]] = sha, shb, shc
-    # det_halo_energies_1b3 :1400-1444 (pair sum below direct_pe_nmax; the RF formula is
-    # overwritten by the shell profile for larger halos under the default policy)
-    ep = np.nan
-    if n < direct_pe_nmax:
-        i, j = np.triu_indices(n, 1)
-        drs = pos[j] - pos[i]
-        drs[drs > L / 2] -= L
-        drs[drs < -L / 2] += L
-        dists = np.sqrt(np.sum(drs ** 2, axis=1)) / L
-        ep = np.sum(-(mass[i] * mass[j]) / dists) * gravconst / L
-    vtx = vel[:, 0] - vx + drx * hub_pt
-    vty = vel[:, 1] - vy + dry * hub_pt
-    vtz = vel[:, 2] - vz + drz * hub_pt
-    v2 = vtx ** 2 + vty ** 2 + vtz ** 2
-    ek = 0.5 * np.sum(mass * v2)
-    # tab_props_inside_1b40 :969-1032
-    nr = nshell
-    dra = drx * v[0, 0] + dry * v[1, 0] + drz * v[2, 0]
-    drb = drx * v[0, 1] + dry * v[1, 1] + drz * v[2, 1]
-    drc = drx * v[0, 2] + dry * v[1, 2] + drz * v[2, 2]
-    r_ell = np.sqrt((dra / sha) ** 2 + (drb / shb) ** 2 + (drc / shc) ** 2)
-    rmax = np.max(r_ell)
+def shell_tables(pos, mass, v2, c, axes, vecs, L, nr):
+    """tab_props_inside_1b40 :969-1032 -> (cumulative mass, cumulative kinetic energy,
+    shell counts, amax, bmax, cmax)."""
+    drx, dry, drz = rel(pos, c, L)
+    dra = drx * vecs[0, 0] + dry * vecs[1, 0] + drz * vecs[2, 0]
+    drb = drx * vecs[0, 1] + dry * vecs[1, 1] + drz * vecs[2, 1]
+    drc = drx * vecs[0, 2] + dry * vecs[1, 2] + drz * vecs[2, 2]
+    sha, shb, shc = axes
+    rmax = np.max(np.sqrt((dra / sha) ** 2 + (drb / shb) ** 2 + (drc / shc) ** 2))
     eps = 1e-2
-    amax = rmax * sha * (1.0 + eps)
-    bmax = rmax * shb * (1.0 + eps)
-    cmax = rmax * shc * (1.0 + eps)
+    amax, bmax, cmax = rmax * sha * (1.0 + eps), rmax * shb * (1.0 + eps), rmax * shc * (1.0 + eps)
     r_ell = np.sqrt((dra / amax) ** 2 + (drb / bmax) ** 2 + (drc / cmax) ** 2)
     i_ell = (r_ell * nr).astype(int)
     if np.any(i_ell >= nr):
@@ -148,29 +143,28 @@ def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst
     np.add.at(tabm, i_ell, mass)
     np.add.at(tabk, i_ell, 0.5 * mass * v2)
     cnt = np.bincount(i_ell, minlength=nr).astype(np.float64)
-    tabm = np.cumsum(tabm)
-    tabk = np.cumsum(tabk)
-    # potential_profile_1b40 :1069-1123 (shell / count / profile_outer_for_large)
+    return np.cumsum(tabm), np.cumsum(tabk), cnt, amax, bmax, cmax
+
+
+def potential_profile(tabm, cnt, amax, bmax, cmax, gravconst):
+    """potential_profile_1b40 :1069-1123, mode 'shell', self mode 'count' (before the
+    outer-value normalisation)."""
+    nr = tabm.shape[0]
     dm = np.diff(tabm, prepend=0.0)
     mass_before = tabm - dm
     self_factor = np.zeros(nr)
     occ = cnt > 0
     self_factor[occ] = 0.5 * (cnt[occ] - 1.0) / cnt[occ]
-    r_eff = _shell_scale(nr) * (amax * bmax * cmax) ** (1.0 / 3.0)
-    tabp = -gravconst * np.cumsum((mass_before + self_factor * dm) * dm / r_eff)
-    outer = tabp[nr - 1]
-    if n < direct_pe_nmax:
-        if ep != outer and outer != 0.0:
-            tabp = tabp / outer * ep
-    else:
-        ep = outer
-    et = ek + ep
-    out[F["ek"]], out[F["ep"]], out[F["et"]] = ek, ep, et
-    # det_vir_props_1b4 :1127-1229
-    ttab = nr
+    r_eff = shell_scale(nr) * (amax * bmax * cmax) ** (1.0 / 3.0)
+    return -gravconst * np.cumsum((mass_before + self_factor * dm) * dm / r_eff)
+
+
+def vir_props(tabm, tabk, tabp, amax, bmax, cmax, vir_overdens, rho_mean):
+    """det_vir_props_1b4 :1127-1229 -> (mvir, rvir) before the min with r."""
+    ttab = tabm.shape[0]
     mvir = tabm[ttab - 1]
     i1 = 1
-    for i1 in range(ttab - 1, 0, -1):
+    for i1 in range(ttab - 1, 0, -1):          # frange(ttab-1, 1, -1): down to 1 inclusive
         if tabm[i1 - 1] < tabm[i1]:
             break
     avir = i1 / (ttab - 1) * amax
@@ -179,7 +173,7 @@ def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst
     rvir = (avir * bvir * cvir) ** (1. / 3.)
     virth_old = 1.0
     virth = 100.0
-    for i0 in range(ttab - 1, -1, -1):
+    for i0 in range(ttab - 1, -1, -1):          # frange(ttab-1, 0, -1)
         if (tabk[i0] + tabp[i0]) >= 0.0:
             continue
         virth = abs((2.0 * tabk[i0] + tabp[i0]) / (tabk[i0] + tabp[i0]))
@@ -204,13 +198,52 @@ def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst
         bvir = ii1 / (ttab - 1) * bmax
         cvir = ii1 / (ttab - 1) * cmax
         rvir = (avir * bvir * cvir) ** (1. / 3.)
+    return mvir, rvir
+
+
+def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst=430.1,
+               vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000):
+    """All properties of one halo, composed as det_vir_1b :1241-1259 and the property loop
+    (det_mass_17, det_center_18, compute_ang_mom_19, r_halos_1a, det_main_axis_1b2,
+    det_halo_energies_1b3, det_vir_props_1b4, compute_spin_parameter_1c :1599-1602)."""
+    out = np.full(len(FIELDS), np.nan)
+    L = lbox
+    n = mass.shape[0]
+    m = np.sum(mass)                                          # det_mass_17 :754-780
+    out[F["m"]] = m
+    c, vbar, sigma = det_center(pos, vel, mass, density, L, cdm)
+    out[F["px"]], out[F["py"]], out[F["pz"]] = c
+    out[F["vx"]], out[F["vy"]], out[F["vz"]] = vbar
+    out[F["sigma"]] = sigma
+    Lv = ang_mom(pos, vel, mass, c, vbar, L)
+    out[F["Lx"]], out[F["Ly"]], out[F["Lz"]] = Lv
+    out[F["r"]] = r_halo(pos, c, L)
+    I = inertia(pos, mass, c, L)
+    out[F["Ixx"]], out[F["Ixy"]], out[F["Ixz"]] = I[0, 0], I[0, 1], I[0, 2]
+    out[F["Iyy"]], out[F["Iyz"]], out[F["Izz"]] = I[1, 1], I[1, 2], I[2, 2]
+    axes, vecs = main_axis(I, m)
+    out[F["sha"]], out[F["shb"]], out[F["shc"]] = axes
+    # det_halo_energies_1b3: pair sum below direct_pe_nmax; the RF estimate of larger halos
+    # is overwritten by the profile's outer value under the default norm policy
+    ep = direct_pe(pos, mass, L, gravconst) if n < direct_pe_nmax else np.nan
+    ek, v2 = kinetic(pos, vel, mass, c, vbar, L, hub_pt)
+    tabm, tabk, cnt, amax, bmax, cmax = shell_tables(pos, mass, v2, c, axes, vecs, L, nshell)
+    tabp = potential_profile(tabm, cnt, amax, bmax, cmax, gravconst)
+    outer = tabp[-1]
+    if n < direct_pe_nmax:
+        if ep != outer and outer != 0.0:
+            tabp = tabp / outer * ep
+    else:
+        ep = outer
+    et = ek + ep
+    out[F["ek"]], out[F["ep"]], out[F["et"]] = ek, ep, et
+    mvir, rvir = vir_props(tabm, tabk, tabp, amax, bmax, cmax, vir_overdens, rho_mean)
     if mvir > 0.0 and rvir > 0.0:
         out[F["rvir"]] = min(rvir, out[F["r"]])
         out[F["mvir"]] = mvir
         out[F["cvel"]] = np.sqrt(gravconst * mvir / out[F["rvir"]])
         out[F["tvir"]] = 35.9 * gravconst * mvir / out[F["rvir"]]
-    # compute_spin_parameter_1c :1599-1602
-    hl = np.sqrt(out[F["Lx"]] ** 2 + out[F["Ly"]] ** 2 + out[F["Lz"]] ** 2)
+    hl = np.sqrt(np.sum(Lv ** 2))
     out[F["spin"]] = hl * np.sqrt(np.abs(et)) / m ** 2.5 / gravconst
     return out
 

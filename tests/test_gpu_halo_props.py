@@ -1,0 +1,130 @@
+"""Batched per-halo properties on the GPU (ahop_halo_props_device) vs the FP64 oracle.
+
+The reference sums members with NumPy's pairwise summation in an unstable-argsort
+member order (src/compute_neiKDtree_mod.py:111), so parity is a tolerance: every field
+within 1e-10 of the oracle relative to the field's natural scale (|value| for positive
+quantities, |L| for the components of L, the trace of I for its components, sigma for
+the mean velocity, max(|ek|, |ep|) for et and the spin).  The centre is a member's
+position and must match exactly.  The discrete virial choices (shell bins) are
+compared through rvir/mvir at the same tolerance; a bin flip would fail it.
+"""
+import numpy as np
+import pytest
+
+from amdahop import generate as G
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def scales(ref):
+    from oracle.halo_props import F
+    s = np.abs(ref).copy()
+    Lnorm = np.sqrt(ref[:, F["Lx"]] ** 2 + ref[:, F["Ly"]] ** 2 + ref[:, F["Lz"]] ** 2)
+    trace = ref[:, F["Ixx"]] + ref[:, F["Iyy"]] + ref[:, F["Izz"]]
+    eng = np.maximum(np.abs(ref[:, F["ek"]]), np.abs(ref[:, F["ep"]]))
+    for k in ("Lx", "Ly", "Lz"):
+        s[:, F[k]] = Lnorm
+    for k in ("Ixx", "Ixy", "Ixz", "Iyy", "Iyz", "Izz"):
+        s[:, F[k]] = trace
+    for k in ("vx", "vy", "vz"):
+        s[:, F[k]] = ref[:, F["sigma"]]
+    s[:, F["et"]] = eng
+    s[:, F["spin"]] = ref[:, F["spin"]] * eng / np.abs(ref[:, F["et"]])
+    return s
+
+
+def compare(out, ref, tol=TOL):
+    from oracle.halo_props import F, FIELDS
+    assert out.shape == ref.shape
+    for k in ("px", "py", "pz"):
+        assert np.array_equal(out[:, F[k]], ref[:, F[k]]), f"centre {k} differs"
+    err = np.abs(out - ref) / scales(ref)
+    worst = {k: float(np.nanmax(err[:, F[k]])) for k in FIELDS}
+    print("worst relative error per field:", {k: f"{v:.1e}" for k, v in worst.items()},
+          flush=True)
+    assert np.array_equal(np.isnan(out), np.isnan(ref))
+    bad = {k: v for k, v in worst.items() if v > tol}
+    assert not bad, bad
+
+
+def run(hs, gpu, density=None, **kw):
+    import torch
+    from amdahop import halo_props
+    pos = torch.from_numpy(hs.pos).to(gpu)
+    vel = torch.from_numpy(hs.vel).to(gpu)
+    mass = torch.from_numpy(hs.mass).to(gpu)
+    dens = torch.from_numpy(density).to(gpu) if density is not None else None
+    out, info = halo_props(hs.offsets, pos, vel, mass, dens, **kw)
+    return out.cpu().numpy(), info
+
+
+@pytest.fixture(scope="module")
+def halos():
+    return G.nfw_halos(200, 5, alpha=0.9, nmin=50, nmax=30000, mass_sigma_dex=0.3)
+
+
+@pytest.fixture(scope="module")
+def halos_ref(halos):
+    from oracle.halo_props import all_halos
+    return all_halos(halos.offsets, halos.pos, halos.vel, halos.mass, hub_pt=70.3)
+
+
+def test_halo_props_match_the_oracle(gpu, halos, halos_ref):
+    c = np.diff(halos.offsets)
+    print(f"{halos.nhalos} halos, {c.sum()} members, {(c >= 1000).sum()} at or above the "
+          "pair-sum limit", flush=True)
+    # the set exercises both energy branches and halos across the periodic edge
+    assert (c >= 1000).any() and (c < 1000).any()
+    assert np.any(np.ptp(halos.pos, axis=0) > 0.9)
+    out, info = run(halos, gpu, hub_pt=70.3)
+    print(f"autotuned tile={info.tile} workgroup={info.workgroup} trials={info.trials} "
+          f"ms={info.ms:.3f}", flush=True)
+    assert info.trials > 0 and info.tile > 0 and info.workgroup in (128, 256)
+    compare(out, halos_ref)
+
+
+@pytest.mark.parametrize("tile,wg", [(1024, 128), (65536, 256)])
+def test_tile_schedule_does_not_change_results(gpu, halos, halos_ref, tile, wg):
+    print(f"tile={tile} wg={wg}", flush=True)
+    out, info = run(halos, gpu, hub_pt=70.3, tile=tile, workgroup=wg)
+    assert info.tile == tile and info.workgroup == wg and info.trials == 0
+    compare(out, halos_ref)
+
+
+def test_densest_member_centre(gpu):
+    from oracle.halo_props import all_halos
+    hs = G.nfw_halos(60, 8, alpha=1.0, nmin=30, nmax=5000)
+    rng = np.random.default_rng(8)
+    dens = rng.uniform(1.0, 100.0, size=hs.mass.shape[0])
+    dens[hs.offsets[3]:hs.offsets[3] + 2] = 1e9      # a tie of maxima: first member wins
+    ref = all_halos(hs.offsets, hs.pos, hs.vel, hs.mass, dens, hub_pt=70.3, cdm=False)
+    out, _ = run(hs, gpu, density=dens, hub_pt=70.3, cdm=False)
+    compare(out, ref)
+
+
+def test_non_default_constants(gpu):
+    from oracle.halo_props import all_halos
+    hs = G.nfw_halos(40, 9, alpha=1.0, nmin=40, nmax=3000)
+    kw = dict(hub_pt=0.0, gravconst=1.0, vir_overdens=200.0, rho_mean=3.0, nshell=500,
+              direct_pe_nmax=200)
+    ref = all_halos(hs.offsets, hs.pos, hs.vel, hs.mass, **kw)
+    out, _ = run(hs, gpu, **kw)
+    compare(out, ref)
+
+
+def test_invalid_layouts_raise(gpu, halos):
+    import torch
+    from amdahop import halo_props
+    pos = torch.from_numpy(halos.pos).to(gpu)
+    vel = torch.from_numpy(halos.vel).to(gpu)
+    mass = torch.from_numpy(halos.mass).to(gpu)
+    off = halos.offsets.copy()
+    off[2] = off[1]                                   # an empty halo
+    with pytest.raises(RuntimeError, match="member"):
+        halo_props(off, pos, vel, mass)
+    with pytest.raises(RuntimeError, match="nshell"):
+        halo_props(halos.offsets, pos, vel, mass, nshell=1)
+    with pytest.raises(RuntimeError, match="densit"):
+        halo_props(halos.offsets, pos, vel, mass, cdm=False)

@@ -1,0 +1,243 @@
+"""Pins of the per-halo property oracle (oracle/halo_props.py) by closed forms and
+invariants -- the reference's Python reductions are not imported, so these checks are
+what fixes the restatement (src/compute_halo_props.py:754-1229)."""
+import numpy as np
+import pytest
+
+from oracle import halo_props as HP
+
+F = HP.F
+G = 430.1
+L1 = 1.0
+
+
+def props(pos, vel, mass, **kw):
+    kw.setdefault("hub_pt", 0.0)
+    return HP.halo_props(np.asarray(pos, float), np.asarray(vel, float), np.asarray(mass, float),
+                         **kw)
+
+
+def _static_symmetric_cloud(seed, npairs, scale):
+    # a member at the origin plus point-symmetric pairs: the COM is the origin member
+    rng = np.random.default_rng(seed)
+    p = rng.normal(size=(npairs, 3)) * scale
+    return np.vstack([[0, 0, 0], p, -p])
+
+
+def _cloud(seed, n):
+    rng = np.random.default_rng(seed)
+    pos = rng.normal(size=(n, 3)) * 0.01
+    vel = rng.normal(size=(n, 3)) * 50.0
+    mass = np.exp(rng.normal(size=n) * 0.3) / n
+    return pos, vel, mass
+
+
+def test_two_body_angular_momentum_and_centre():
+    # a pair at +-d on x moving +-v on y inside a static point-symmetric cloud: the centre
+    # is the origin member, mean velocity 0, and only the pair carries angular momentum,
+    # L = 2 * (d x) x (m v y) = 2 d m v z
+    d, v, m, mc = 0.01, 3.0, 0.5, 0.1
+    cloud = _static_symmetric_cloud(0, 20, 0.004)
+    pos = np.vstack([cloud, [[d, 0, 0], [-d, 0, 0]]])
+    vel = np.zeros_like(pos)
+    vel[-2:] = [[0, v, 0], [0, -v, 0]]
+    mass = np.r_[np.full(len(cloud), mc), m, m]
+    o = props(pos, vel, mass)
+    assert (o[F["px"]], o[F["py"]], o[F["pz"]]) == (0.0, 0.0, 0.0)
+    assert (o[F["vx"]], o[F["vy"]], o[F["vz"]]) == (0.0, 0.0, 0.0)
+    assert o[F["Lx"]] == 0 and o[F["Ly"]] == 0
+    assert o[F["Lz"]] == pytest.approx(2 * d * m * v, rel=1e-14)
+    assert o[F["sigma"]] == pytest.approx(v * np.sqrt(2 * m / mass.sum()), rel=1e-14)
+    assert o[F["m"]] == pytest.approx(mass.sum(), rel=1e-15)
+
+
+def test_centre_is_the_member_nearest_the_periodic_com():
+    # COM about the first member, wrapped: two members either side of the x edge
+    pos = np.array([[0.499, 0, 0], [-0.499, 0, 0], [0.4995, 0.001, 0]])
+    c, vbar, _ = HP.det_center(pos, np.zeros((3, 3)), np.ones(3), None, L1, True)
+    # COM in the unwrapped frame of member 0 is x = 0.499 + (0 + 0.002 + 0.0005)/3
+    assert np.array_equal(c, pos[2]) and np.all(vbar == 0)
+    # ties go to the first member
+    pos = np.array([[-0.01, 0, 0], [0.01, 0, 0]])
+    c, _, _ = HP.det_center(pos, np.zeros((2, 3)), np.ones(2), None, L1, True)
+    assert np.array_equal(c, pos[0])
+
+
+def test_rigid_rotation_angular_momentum():
+    # ring of radius R around the centre, v = omega x r: L = omega * sum m R^2 along z
+    R, om, nring, m = 0.02, 5.0, 12, 1e-3
+    phi = 2 * np.pi * np.arange(nring) / nring
+    pos = np.c_[R * np.cos(phi), R * np.sin(phi), np.zeros(nring)]
+    vel = np.c_[-om * R * np.sin(phi), om * R * np.cos(phi), np.zeros(nring)]
+    Lv = HP.ang_mom(pos, vel, np.full(nring, m), np.zeros(3), np.zeros(3), L1)
+    assert Lv[2] == pytest.approx(om * nring * m * R * R, rel=1e-12)
+    assert abs(Lv[0]) < 1e-18 and abs(Lv[1]) < 1e-18
+
+
+def test_inertia_tensor_and_axes_of_a_cube_shell():
+    # 8 cube corners (+-a)^3 and a centre member: I = 8 m a^2 * identity, all three axes
+    # sqrt(8 m a^2 / (9 m)) (det_inertial_tensor_1b20, det_main_axis_1b2)
+    a, m = 0.003, 2.0
+    corners = np.array([[sx, sy, sz] for sx in (-a, a) for sy in (-a, a) for sz in (-a, a)])
+    pos = np.vstack([[0, 0, 0], corners])
+    o = props(pos, np.zeros((9, 3)), np.full(9, m))
+    for k in ("Ixx", "Iyy", "Izz"):
+        assert o[F[k]] == pytest.approx(8 * m * a * a, rel=1e-14)
+    for k in ("Ixy", "Ixz", "Iyz"):
+        assert abs(o[F[k]]) < 1e-20
+    for k in ("sha", "shb", "shc"):
+        assert o[F[k]] == pytest.approx(a * np.sqrt(8 / 9), rel=1e-12)
+    assert o[F["r"]] == pytest.approx(a * np.sqrt(3), rel=1e-15)
+
+
+def test_axes_are_ascending_and_follow_an_elongation():
+    # an elongated cloud along z: sha <= shb <= shc, the largest axis being z's spread
+    rng = np.random.default_rng(0)
+    pos = rng.normal(size=(400, 3)) * [0.001, 0.002, 0.004]
+    I = HP.inertia(pos, np.ones(400), np.zeros(3), L1)
+    axes, vecs = HP.main_axis(I, 400.0)
+    assert axes[0] <= axes[1] <= axes[2]
+    assert axes[2] == pytest.approx(0.004, rel=0.1) and abs(vecs[2, 2]) > 0.99
+
+
+@pytest.mark.parametrize("shape,npairs", [("pair", 1), ("triangle", 3), ("tetrahedron", 6)])
+def test_direct_potential_energy_of_regular_systems(shape, npairs):
+    # every pair at the same separation s: ep = -npairs * G m^2 / s (det_halo_energies_1b3)
+    s, m = 0.004, 0.25
+    if shape == "pair":
+        pos = [[0, 0, 0], [s, 0, 0]]
+    elif shape == "triangle":
+        pos = [[0, 0, 0], [s, 0, 0], [s / 2, s * np.sqrt(3) / 2, 0]]
+    else:
+        pos = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]]) * s / np.sqrt(8)
+    pos = np.asarray(pos, float)
+    n = len(pos)
+    ep = HP.direct_pe(pos, np.full(n, m), L1, G)
+    assert ep == pytest.approx(-npairs * G * m * m / s, rel=1e-12)
+    # the same pair across the periodic edge
+    if shape == "pair":
+        ep2 = HP.direct_pe(np.array([[0.4995, 0, 0], [-0.4995 + s - 0.001, 0, 0]]),
+                           np.full(2, m), L1, G)
+        assert ep2 == pytest.approx(-G * m * m / s, rel=1e-9)
+
+
+def test_kinetic_energy_includes_the_hubble_flow():
+    # members at rest, H = 70: ek = 1/2 sum m (H dr)^2 about the centre
+    pos = np.array([[0, 0, 0], [0.01, 0, 0], [0, -0.02, 0]])
+    ek, _ = HP.kinetic(pos, np.zeros((3, 3)), np.array([1.0, 2.0, 3.0]), np.zeros(3),
+                       np.zeros(3), L1, 70.0)
+    assert ek == pytest.approx(0.5 * (2 * 0.7 ** 2 + 3 * 1.4 ** 2), rel=1e-13)
+
+
+def test_shell_profile_of_single_occupied_shells():
+    # one member per occupied shell: no self pairs, so shell b adds -G m_before m_b / r_b;
+    # two members in one shell add the self term -G (1/4) m^2 / r_b (0.5 (n-1)/n of dm^2)
+    nr = 10
+    tabm = np.zeros(nr)
+    cnt = np.zeros(nr)
+    tabm[3:] = 1.0
+    cnt[3] = 1
+    tabm[6:] = 3.0
+    cnt[6] = 2
+    p = HP.potential_profile(tabm, cnt, 0.9, 0.9, 0.9, G)
+    r6 = 6 / 9 * 0.9
+    assert np.all(p[:6] == 0.0)
+    assert p[6] == pytest.approx(-G * (1.0 * 2.0 + 0.25 * 2.0 * 2.0) / r6, rel=1e-14)
+    assert np.all(p[6:] == p[6])
+
+
+def test_translation_across_the_periodic_edge():
+    # moving a halo so that it straddles the box edge changes only its centre
+    pos, vel, mass = _cloud(1, 300)
+    a = props(pos, vel, mass, hub_pt=70.3)
+    shift = np.array([0.5 - 0.002, -0.5 + 0.001, 0.25])
+    p2 = np.mod(pos + shift + 0.5, 1.0) - 0.5
+    assert np.any(p2[:, 0] < 0) and np.any(p2[:, 0] > 0.4)
+    b = props(p2, vel, mass, hub_pt=70.3)
+    for k in HP.FIELDS:
+        if k in ("px", "py", "pz"):
+            continue
+        assert b[F[k]] == pytest.approx(a[F[k]], rel=1e-9, abs=1e-12 * abs(a[F["m"]])), k
+    c = np.array([b[F["px"]], b[F["py"]], b[F["pz"]]])
+    c0 = np.mod(np.array([a[F["px"]], a[F["py"]], a[F["pz"]]]) + shift + 0.5, 1.0) - 0.5
+    assert np.allclose(c, c0, atol=1e-15)
+
+
+def test_rotation_covariance_of_L_and_I():
+    # rotating positions and velocities rotates L as a vector and I as a tensor
+    pos, vel, mass = _cloud(2, 500)
+    th = 0.7
+    Rm = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+    a = props(pos, vel, mass)
+    b = props(pos @ Rm.T, vel @ Rm.T, mass)
+    La = np.array([a[F["Lx"]], a[F["Ly"]], a[F["Lz"]]])
+    Lb = np.array([b[F["Lx"]], b[F["Ly"]], b[F["Lz"]]])
+    assert np.allclose(Rm @ La, Lb, rtol=1e-10, atol=1e-12 * np.linalg.norm(La))
+
+    def tens(o):
+        return np.array([[o[F["Ixx"]], o[F["Ixy"]], o[F["Ixz"]]],
+                         [o[F["Ixy"]], o[F["Iyy"]], o[F["Iyz"]]],
+                         [o[F["Ixz"]], o[F["Iyz"]], o[F["Izz"]]]])
+    assert np.allclose(Rm @ tens(a) @ Rm.T, tens(b), rtol=1e-10, atol=1e-14)
+    for k in ("sha", "shb", "shc", "ek", "ep", "spin"):
+        assert b[F[k]] == pytest.approx(a[F[k]], rel=1e-9), k
+
+
+def test_virial_overdensity_fallback_on_a_static_halo():
+    # members at rest with no Hubble flow: K = 0, so |2K+W|/|K+W| = 1 never improves on
+    # the initial 1 and the overdensity branch decides.  With a negligible overdensity the
+    # first shell (from outside) whose inner neighbour holds less than the total is the
+    # outermost occupied one: r_ell max = 1/(1+eps) -> bin int(nr/1.01) = 990 of 1000, so
+    # mvir = m and rvir = 990/999 * (amax bmax cmax)^(1/3), amax = 1.01 rmax_ell sha, ...
+    pos, _, mass = _cloud(3, 800)
+    o = props(pos, np.zeros((800, 3)), mass, vir_overdens=1e-30)
+    assert o[F["mvir"]] == pytest.approx(o[F["m"]], rel=1e-14)
+    p0 = np.array([o[F["px"]], o[F["py"]], o[F["pz"]]])
+    I = HP.inertia(pos, mass, p0, L1)
+    lam, V = np.linalg.eigh(I)
+    ax = np.sqrt(lam / o[F["m"]])
+    q = ((pos - p0) @ V) / ax
+    rmax = np.sqrt((q ** 2).sum(1)).max()
+    expect = min(990 / 999 * np.prod(1.01 * rmax * ax) ** (1 / 3), o[F["r"]])
+    assert o[F["rvir"]] == pytest.approx(expect, rel=1e-10)
+    assert o[F["cvel"]] == pytest.approx(np.sqrt(G * o[F["mvir"]] / o[F["rvir"]]), rel=1e-14)
+
+
+def test_virial_scan_takes_the_first_well_virialised_shell():
+    # tables where shell 7 (from outside) is bound with |2K+W|/|K+W| = 0.1: the scan stops
+    # there, mvir = tabm[7], axes scaled by 7/(ttab-1)
+    nr = 10
+    tabm = np.arange(1, nr + 1, dtype=float)
+    tabk = np.full(nr, 5.0)
+    tabp = np.full(nr, 1.0)          # unbound everywhere but shell 7
+    tabk[7], tabp[7] = 9.0, -19.0    # K+W = -10, 2K+W = -1 -> 0.1
+    mvir, rvir = HP.vir_props(tabm, tabk, tabp, 1.0, 2.0, 4.0, 1e-30, 1.0)
+    assert mvir == 8.0
+    assert rvir == pytest.approx(7 / 9 * 2.0, rel=1e-14)
+
+
+def test_large_halo_takes_the_shell_profile_energy():
+    # at or above the pair-sum limit ep is the outer value of the shell profile: for a
+    # bound cloud it is negative and of the order of the direct pair sum
+    pos, vel, mass = _cloud(4, 1200)
+    big = props(pos, vel * 0.01, mass, direct_pe_nmax=1000)
+    direct = props(pos, vel * 0.01, mass, direct_pe_nmax=5000)
+    assert big[F["ep"]] < 0
+    assert big[F["ep"]] == pytest.approx(direct[F["ep"]], rel=0.5)
+    assert big[F["et"]] == pytest.approx(big[F["ek"]] + big[F["ep"]], rel=1e-15)
+    assert direct[F["ep"]] == pytest.approx(HP.direct_pe(pos, mass, L1, G), rel=1e-15)
+
+
+def test_densest_member_centre_takes_the_first_of_equal_maxima():
+    pos = np.array([[0, 0, 0], [0.01, 0, 0], [0.02, 0, 0], [0.03, 0, 0]])
+    dens = np.array([1.0, 5.0, 5.0, 2.0])
+    c, _, _ = HP.det_center(pos, np.zeros((4, 3)), np.ones(4), dens, L1, False)
+    assert np.array_equal(c, pos[1])
+
+
+def test_spin_parameter_closed_form():
+    pos, vel, mass = _cloud(5, 200)
+    o = props(pos, vel, mass, hub_pt=70.3)
+    hl = np.sqrt(o[F["Lx"]] ** 2 + o[F["Ly"]] ** 2 + o[F["Lz"]] ** 2)
+    assert o[F["spin"]] == pytest.approx(hl * np.sqrt(abs(o[F["et"]])) / o[F["m"]] ** 2.5 / G,
+                                         rel=1e-15)
