@@ -30,6 +30,9 @@ STEP_BYTES_PER_PARTICLE = 56
 # per member of the batched halo properties: position 24 B, velocity 24 B, mass 8 B and a
 # member index 4 B read once
 HALO_BYTES_PER_MEMBER = 60
+# per particle of the density adjoint (k=32): position 24, mass 8, upstream gradient 8,
+# density 8, smoothing length 8, neighbour ids 4k = 128, position gradient 24, mass gradient 8
+ADJOINT_BYTES_PER_PARTICLE = 216
 
 
 def log(msg):
@@ -116,6 +119,54 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
     return out
 
 
+def adjoint_stage(steps, with_cpu):
+    """SPH density adjoint on the 2,097,152-particle clumpy box (k=32): forward kNN rows
+    frozen, upstream gradient of L = sum w log rho with w ~ N(0,1) (seed 3); device time
+    of the adjoint with inputs resident in HBM, roofline at 216 algorithmic bytes per
+    particle; the FP64 NumPy gradient timed on a 262,144-particle sample as the CPU leg."""
+    import numpy as np
+    import torch
+    from amdahop import density_adjoint, generate as G, knn_device
+
+    dev = torch.device("cuda:0")
+    ps = G.config("c5")
+    n, k = ps.n, ps.nvoisins
+    pos = torch.from_numpy(ps.pos).to(dev)
+    mass = torch.from_numpy(ps.mass).to(dev)
+    dens, nbr, info = knn_device(pos, mass, k)
+    w = torch.from_numpy(np.random.default_rng(3).normal(size=n)).to(dev)
+    grad_rho = w / dens
+    density_adjoint(pos, mass, nbr, grad_rho)
+    dev_ms = []
+    for _ in range(steps):
+        dev_ms.append(density_adjoint(pos, mass, nbr, grad_rho)[2])
+    ms = float(np.median(dev_ms))
+    achieved = ADJOINT_BYTES_PER_PARTICLE * n / (ms * 1e-3) / 1e9
+    out = {"n": n, "k": k, "ms": round(ms, 3), "Mparticles_per_s": round(n / (ms * 1e-3) / 1e6, 2),
+           "forward_knn_ms": round(info.t_total, 3),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "bytes_per_particle": ADJOINT_BYTES_PER_PARTICLE}}
+    log(f"adjoint: {ms:.3f} ms device on n={n} k={k}")
+    del pos, mass, dens, nbr, w, grad_rho
+    if with_cpu:
+        import oracle.sph_adjoint as SA
+        cs = G.config("c1")     # the same generator at 262,144 particles, k=32
+        cpos = torch.from_numpy(cs.pos).to(dev)
+        cmass = torch.from_numpy(cs.mass).to(dev)
+        cd, cn, _ = knn_device(cpos, cmass, cs.nvoisins)
+        g = np.random.default_rng(3).normal(size=cs.n) / cd.cpu().numpy()
+        nb = cn.cpu().numpy().astype(np.int64)
+        c0 = time.perf_counter()
+        SA.density_grad(cs.pos, cs.mass, nb, g)
+        t = time.perf_counter() - c0
+        out["cpu_baseline"] = {"value": round(cs.n / t / 1e6, 4), "unit": "Mparticles/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"same generator at N={cs.n}, k=32, FP64 NumPy "
+                                         f"gradient with the same frozen rows, {t:.1f} s"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +178,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-sample-n", type=int, default=2_097_152)
     ap.add_argument("--no-halo-props", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     # one plain device copy of this many MiB before the timed steps: under a counter
     # profiler its FETCH_SIZE / WRITE_SIZE readings calibrate bytes per counter unit
     ap.add_argument("--copy-calibration-mib", type=int, default=0)
@@ -224,6 +276,8 @@ def main():
     if not args.no_halo_props:
         extra["halo_props"] = halo_props_stage(max(2, args.steps // 2), args.cpu_sample_halos,
                                                not args.no_cpu_baseline)
+    if not args.no_adjoint:
+        extra["density_adjoint"] = adjoint_stage(max(3, args.steps), not args.no_cpu_baseline)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -13,7 +13,8 @@ import ctypes
 
 from ._lib import HotParams, RunInfo, check, lib
 
-__all__ = ["run_device", "device_saddle", "HotParams", "RunInfo"]
+__all__ = ["run_device", "device_saddle", "knn_device", "density_adjoint", "sph_density",
+           "halo_props", "HALO_FIELDS", "HotParams", "RunInfo"]
 
 
 def run_device(pos, mass, nvoisins, nhop=16, rho_threshold=80.0, box=1.0, mixed=True,
@@ -39,12 +40,117 @@ def run_device(pos, mass, nvoisins, nhop=16, rho_threshold=80.0, box=1.0, mixed=
                   1 if mixed else 0, int(bucket), 1 if saddle else 0)
     dens = density_out if density_out is not None else torch.empty(n, dtype=torch.float64, device=pos.device)
     gid = gid_out if gid_out is not None else torch.empty(n, dtype=torch.int32, device=pos.device)
+    _check_out(dens, (n,), torch.float64, pos.device, "density_out")
+    _check_out(gid, (n,), torch.int32, pos.device, "gid_out")
     info = RunInfo()
     torch.cuda.synchronize(pos.device)
     check(lib().ahop_run_device(ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(mass.data_ptr()), n,
                                 ctypes.byref(p), ctypes.c_void_p(dens.data_ptr()),
                                 ctypes.c_void_p(gid.data_ptr()), ctypes.byref(info)))
     return dens, gid, info
+
+
+def _check_out(t, shape, dtype, device, name):
+    if tuple(t.shape) != tuple(shape) or t.dtype != dtype or t.device != device or \
+            not t.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous {dtype} tensor of shape {shape} on {device}")
+
+
+def _box3(box):
+    return (box, box, box) if not isinstance(box, (tuple, list)) else tuple(box)
+
+
+def _check_particles(pos, mass):
+    import torch
+    if pos.dtype != torch.float64 or mass.dtype != torch.float64:
+        raise TypeError("pos and mass must be float64")
+    if not (pos.is_cuda and mass.is_cuda):
+        raise ValueError("pos and mass must be device tensors")
+    n = int(mass.shape[0])
+    if tuple(pos.shape) != (n, 3) or mass.dim() != 1:
+        raise ValueError("pos must have shape (N,3) and mass (N,)")
+    return pos.contiguous(), mass.contiguous(), n
+
+
+def knn_device(pos, mass, nvoisins, box=1.0, mixed=True, bucket=64):
+    """kNN + SPH density with every neighbour row kept: returns (density (N,) float64,
+    nbr (N, nvoisins) int32 original ids in ascending distance, RunInfo)."""
+    import torch
+
+    pos, mass, n = _check_particles(pos, mass)
+    Lx, Ly, Lz = _box3(box)
+    p = HotParams(int(nvoisins), 0, 0.0, float(Lx), float(Ly), float(Lz), 1 if mixed else 0,
+                  int(bucket), 0)
+    dens = torch.empty(n, dtype=torch.float64, device=pos.device)
+    nbr = torch.empty((n, int(nvoisins)), dtype=torch.int32, device=pos.device)
+    info = RunInfo()
+    torch.cuda.synchronize(pos.device)
+    check(lib().ahop_knn_device(ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(mass.data_ptr()),
+                                n, ctypes.byref(p), ctypes.c_void_p(dens.data_ptr()),
+                                ctypes.c_void_p(nbr.data_ptr()), ctypes.byref(info)))
+    return dens, nbr, info
+
+
+def density_adjoint(pos, mass, nbr, grad_rho, box=1.0):
+    """dLoss/dpos (N,3) and dLoss/dmass (N,) of the SPH density for the upstream gradient
+    ``grad_rho`` = dLoss/drho, with the neighbour rows ``nbr`` (N,k) frozen.  Returns
+    (grad_pos, grad_mass, device milliseconds)."""
+    import torch
+
+    pos, mass, n = _check_particles(pos, mass)
+    if nbr.dtype != torch.int32 or nbr.dim() != 2 or nbr.shape[0] != n or not nbr.is_cuda:
+        raise ValueError("nbr must be an (N, k) int32 device tensor")
+    grad_rho = grad_rho.to(torch.float64).contiguous()
+    if tuple(grad_rho.shape) != (n,) or grad_rho.device != pos.device:
+        raise ValueError("grad_rho must have shape (N,) on the particles' device")
+    nbr = nbr.contiguous()
+    Lx, Ly, Lz = _box3(box)
+    gpos = torch.empty((n, 3), dtype=torch.float64, device=pos.device)
+    gmass = torch.empty(n, dtype=torch.float64, device=pos.device)
+    ms = ctypes.c_float(0.0)
+    torch.cuda.synchronize(pos.device)
+    check(lib().ahop_density_adjoint_device(
+        ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(mass.data_ptr()), n, int(nbr.shape[1]),
+        ctypes.c_void_p(nbr.data_ptr()), ctypes.c_void_p(grad_rho.data_ptr()), float(Lx),
+        float(Ly), float(Lz), ctypes.c_void_p(gpos.data_ptr()), ctypes.c_void_p(gmass.data_ptr()),
+        ctypes.byref(ms)))
+    return gpos, gmass, ms.value
+
+
+def _density_function():
+    import torch
+
+    class SPHDensity(torch.autograd.Function):
+        """rho = SPH density of omp_compute_density (f90:2816-2838); backward is the
+        hand-written adjoint with the forward's neighbour rows frozen."""
+
+        @staticmethod
+        def forward(ctx, pos, mass, nvoisins, box, mixed):
+            dens, nbr, _ = knn_device(pos.detach(), mass.detach(), nvoisins, box, mixed)
+            ctx.save_for_backward(pos, mass, nbr)
+            ctx.box = box
+            ctx.mark_non_differentiable(nbr)
+            return dens, nbr
+
+        @staticmethod
+        def backward(ctx, grad_rho, grad_nbr):
+            pos, mass, nbr = ctx.saved_tensors
+            gpos, gmass, _ = density_adjoint(pos.detach(), mass.detach(), nbr, grad_rho, ctx.box)
+            return gpos, gmass, None, None, None
+
+    return SPHDensity
+
+
+_SPHDensity = None
+
+
+def sph_density(pos, mass, nvoisins, box=1.0, mixed=True):
+    """Differentiable SPH density: returns (rho (N,), nbr (N,k) int32).  Gradients with
+    respect to ``pos`` (N,3) and ``mass`` (N,) come from the hand-written adjoint kernel."""
+    global _SPHDensity
+    if _SPHDensity is None:
+        _SPHDensity = _density_function()
+    return _SPHDensity.apply(pos, mass, int(nvoisins), box, bool(mixed))
 
 
 def device_saddle(info):

@@ -379,22 +379,80 @@ int ahop_get_saddle(int32_t *offset, int32_t *isad, double *rho)
     });
 }
 
+static Params hot_params(const ahop_hot_params *params)
+{
+    if (!params) throw std::runtime_error("null params");
+    Params p;
+    p.nvoisins = params->nvoisins;
+    p.nhop = params->nhop;
+    p.rho_threshold = params->rho_threshold;
+    p.xlong = params->xlong;
+    p.ylong = params->ylong;
+    p.zlong = params->zlong;
+    p.mixed = params->mixed;
+    p.bucket = params->bucket;
+    return p;
+}
+
+static void fill_info(ahop_run_info *info, const Engine &e, bool saddle)
+{
+    if (!info) return;
+    const DeviceResults &r = e.results();
+    const StageTimes &t = e.times();
+    info->nactive = r.nactive;
+    info->ngroups = r.ngroups;
+    info->nfallback = r.nfallback;
+    info->nsaddle = saddle ? r.nsaddle : 0;
+    info->t_sort = t.sort;
+    info->t_tree = t.tree;
+    info->t_knn = t.knn;
+    info->t_fallback = t.fallback;
+    info->t_hop = t.hop;
+    info->t_saddle = t.saddle;
+    info->t_total = t.total;
+    info->knn_nodes = (int64_t)r.knn_stats[0];
+    info->knn_buckets = (int64_t)r.knn_stats[1];
+    info->knn_candidates = (int64_t)r.knn_stats[2];
+    info->knn_insert_steps = (int64_t)r.knn_stats[3];
+    info->knn_inserts = (int64_t)r.knn_stats[4];
+    info->knn_waves = (int64_t)r.knn_stats[5];
+}
+
+int ahop_knn_device(const double *d_pos, const double *d_mass, int64_t n,
+                    const ahop_hot_params *params, double *d_density, int32_t *d_nbr,
+                    ahop_run_info *info)
+{
+    return guarded([&] {
+        Params p = hot_params(params);
+        if (!d_pos || !d_mass || !d_density || !d_nbr) throw std::runtime_error("null array");
+        Engine &e = engine();
+        e.knn_lists(d_pos, d_mass, n, p, d_density, d_nbr);
+        fill_info(info, e, false);
+    });
+}
+
+int ahop_density_adjoint_device(const double *d_pos, const double *d_mass, int64_t n, int32_t k,
+                                const int32_t *d_nbr, const double *d_grad_rho, double xlong,
+                                double ylong, double zlong, double *d_grad_pos,
+                                double *d_grad_mass, float *ms)
+{
+    return guarded([&] {
+        if (!d_pos || !d_mass || !d_nbr || !d_grad_rho || !d_grad_pos || !d_grad_mass)
+            throw std::runtime_error("null array");
+        Engine &e = engine();
+        float t = sph_density_adjoint(d_pos, d_mass, d_nbr, d_grad_rho, n, k, xlong, ylong, zlong,
+                                      d_grad_pos, d_grad_mass, e.stream());
+        if (ms) *ms = t;
+    });
+}
+
 int ahop_run_device(const double *d_pos, const double *d_mass, int64_t n,
                     const ahop_hot_params *params, double *d_density, int32_t *d_gid,
                     ahop_run_info *info)
 {
     return guarded([&] {
-        if (!params) throw std::runtime_error("null params");
+        Params p = hot_params(params);
         if (!d_pos || !d_mass) throw std::runtime_error("null input array");
-        Params p;
-        p.nvoisins = params->nvoisins;
-        p.nhop = params->nhop;
-        p.rho_threshold = params->rho_threshold;
-        p.xlong = params->xlong;
-        p.ylong = params->ylong;
-        p.zlong = params->zlong;
-        p.mixed = params->mixed;
-        p.bucket = params->bucket;
         Engine &e = engine();
         e.run(d_pos, d_mass, n, p, params->saddle != 0);
         const DeviceResults &r = e.results();
@@ -422,26 +480,7 @@ int ahop_run_device(const double *d_pos, const double *d_mass, int64_t n,
             s.dev_sad_rho.clear();
         }
         AHOP_HIP(hipStreamSynchronize(st));
-        if (info) {
-            const StageTimes &t = e.times();
-            info->nactive = r.nactive;
-            info->ngroups = r.ngroups;
-            info->nfallback = r.nfallback;
-            info->nsaddle = params->saddle ? r.nsaddle : 0;
-            info->t_sort = t.sort;
-            info->t_tree = t.tree;
-            info->t_knn = t.knn;
-            info->t_fallback = t.fallback;
-            info->t_hop = t.hop;
-            info->t_saddle = t.saddle;
-            info->t_total = t.total;
-            info->knn_nodes = (int64_t)r.knn_stats[0];
-            info->knn_buckets = (int64_t)r.knn_stats[1];
-            info->knn_candidates = (int64_t)r.knn_stats[2];
-            info->knn_insert_steps = (int64_t)r.knn_stats[3];
-            info->knn_inserts = (int64_t)r.knn_stats[4];
-            info->knn_waves = (int64_t)r.knn_stats[5];
-        }
+        fill_info(info, e, params->saddle != 0);
     });
 }
 

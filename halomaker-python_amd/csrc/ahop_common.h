@@ -76,6 +76,10 @@ class Engine {
     // Full hot path on device-resident inputs: pos (n,3) row-major doubles, mass (n).
     // Fills results (device pointers valid until the next run or close).
     void run(const double *d_pos, const double *d_mass, int64_t n, const Params &p, bool saddle);
+    // kNN + density only, with every particle's k neighbours (original 0-based ids, ascending
+    // distance) written to d_nbr (n*k) and the density to d_rho (n), both in original order
+    void knn_lists(const double *d_pos, const double *d_mass, int64_t n, const Params &p,
+                   double *d_rho, int32_t *d_nbr);
     const DeviceResults &results() const { return res_; }
     const StageTimes &times() const { return times_; }
     hipStream_t stream() const { return stream_; }
@@ -103,6 +107,12 @@ class Engine {
 
 const char *last_error();
 void set_error(const std::string &msg);
+
+// Adjoint of the SPH density with frozen neighbour rows (sph_adjoint.hip): all pointers are
+// device arrays in original order (pos (n,3), nbr (n,K)); returns the device milliseconds.
+float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32_t *d_nbr,
+                          const double *d_grad_rho, int64_t n, int K, double Lx, double Ly,
+                          double Lz, double *d_gpos, double *d_gmass, hipStream_t s);
 
 }  // namespace ahop
 

@@ -904,7 +904,20 @@ void Engine::saddles(const Params &p)
     AHOP_HIP(hipEventRecord(ev_[6], s));
 }
 
-void Engine::run(const double *d_pos, const double *d_mass, int64_t n, const Params &p, bool saddle)
+// sorted-slot results -> original order: density and the full neighbour rows (original ids)
+__global__ void k_lists_to_original(const int32_t *__restrict__ nbr_s,
+                                    const int32_t *__restrict__ perm,
+                                    const double *__restrict__ rho_s, int64_t n, int K,
+                                    int32_t *__restrict__ nbr_o, double *__restrict__ rho_o)
+{
+    int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int64_t o = perm[s];
+    rho_o[o] = rho_s[s];
+    for (int t = 0; t < K; ++t) nbr_o[o * K + t] = perm[nbr_s[s * K + t]];
+}
+
+static void validate(int64_t n, const Params &p)
 {
     if (n < 2) throw std::runtime_error("need at least 2 particles");
     if (n >= ((int64_t)1 << 30)) throw std::runtime_error("particle count must be < 2^30");
@@ -913,6 +926,35 @@ void Engine::run(const double *d_pos, const double *d_mass, int64_t n, const Par
     if (p.nhop < 0 || p.nhop > p.nvoisins) throw std::runtime_error("need 0 <= nhop <= nvoisins");
     if (p.bucket < 1 || p.bucket > 64) throw std::runtime_error("bucket must be in [1, 64]");
     if (!(p.xlong > 0 && p.ylong > 0 && p.zlong > 0)) throw std::runtime_error("box sides must be > 0");
+}
+
+void Engine::knn_lists(const double *d_pos, const double *d_mass, int64_t n, const Params &p,
+                       double *d_rho, int32_t *d_nbr)
+{
+    validate(n, p);
+    Params q = p;
+    q.nhop = p.nvoisins;   // the kNN kernel stores all k ids per particle
+    times_ = StageTimes{};
+    AHOP_HIP(hipEventRecord(ev_[0], stream_));
+    build_tree(d_pos, d_mass, n, q);
+    knn_density(q);
+    hipLaunchKernelGGL(k_lists_to_original, dim3(nblk(n)), dim3(256), 0, stream_,
+                       (const int32_t *)b_nbr.p, t_.perm, (const double *)b_rho_s.p, n, q.nvoisins,
+                       d_nbr, d_rho);
+    AHOP_HIP(hipEventRecord(ev_[5], stream_));
+    AHOP_HIP(hipStreamSynchronize(stream_));
+    AHOP_HIP(hipEventElapsedTime(&times_.sort, ev_[0], ev_[1]));
+    AHOP_HIP(hipEventElapsedTime(&times_.tree, ev_[1], ev_[2]));
+    AHOP_HIP(hipEventElapsedTime(&times_.knn, ev_[2], ev_[3]));
+    AHOP_HIP(hipEventElapsedTime(&times_.fallback, ev_[3], ev_[4]));
+    AHOP_HIP(hipEventElapsedTime(&times_.total, ev_[0], ev_[5]));
+    res_.nactive = 0;
+    res_.ngroups = 0;
+}
+
+void Engine::run(const double *d_pos, const double *d_mass, int64_t n, const Params &p, bool saddle)
+{
+    validate(n, p);
     times_ = StageTimes{};
     AHOP_HIP(hipEventRecord(ev_[0], stream_));
     build_tree(d_pos, d_mass, n, p);

@@ -35,5 +35,4 @@ HaloTuning halo_props_device(const int64_t *h_off, int64_t nhalos, const double 
                              const double *vel, const double *mass, const double *dens,
                              const HaloParamsDev &P, double *d_out, int tile, int bs,
                              hipStream_t s);
-
 }  // namespace ahop

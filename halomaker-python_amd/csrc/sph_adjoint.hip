@@ -1,0 +1,183 @@
+// sph_adjoint.hip -- hand-written adjoint of the SPH density with frozen neighbour lists.
+//
+// Forward (omp_compute_density, fortran/compute_adaptahop.f90:2816-2838, spline :3270-3287),
+// with the neighbour rows j_1..j_k of particle i in ascending distance:
+//   r_i   = sqrt(d2(i, j_k)) / 2
+//   rho_i = A * (sum_{n<k} m_{j_n} W(d_{i j_n} / r_i) + m_i) / r_i^3,   A = L^3 / pi
+// Adjoint, for an upstream gradient g_i = dLoss/drho_i:
+//   dLoss/dm_i     += g_i A / r_i^3                               (own mass)
+//   dLoss/dm_j     += g_i A W(q_ij) / r_i^3                       (each of the first k-1)
+//   dLoss/dx_j     += g_i A m_j W'(q_ij) / r_i^4 * u_ij,  dLoss/dx_i -= the same
+//   dLoss/dx_{j_k} += g_i (drho_i/dr_i) / 2 * u_ik,       dLoss/dx_i -= the same
+//   drho_i/dr_i     = -A / r_i^4 * (3 (C_i + m_i) + S_i / r_i),
+//   C_i = sum m_j W(q_ij),  S_i = sum m_j W'(q_ij) d_ij
+// where u_ij is the unit minimum-image separation x_j - x_i.  Per-pair terms and sums are
+// FP64: with FP32 pair terms the gradient of a mass that many particles list as a neighbour
+// loses up to ~3e-4 relative to cancellation (the upstream weights g_i have both signs).
+// Each particle gathers its own terms and adds its neighbours' terms with FP64 global
+// atomics.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+#include "ahop_common.h"
+
+namespace ahop {
+
+namespace {
+
+__device__ __forceinline__ double min_image(double d, double L)
+{
+    // the minimum-image separation, the signed counterpart of the reference's
+    // min(|d|, L - |d|) (walk_tree2 :2931-2941)
+    const double h = 0.5 * L;
+    if (d > h) return d - L;
+    if (d < -h) return d + L;
+    return d;
+}
+
+// W(q) and W'(q) of the cubic spline (spline :3270-3287)
+__device__ __forceinline__ void spline_wd(double q, double &w, double &wp)
+{
+    if (q <= 1.0) {
+        w = 1.0 - 1.5 * q * q + 0.75 * q * q * q;
+        wp = -3.0 * q + 2.25 * q * q;
+    } else if (q <= 2.0) {
+        const double t = 2.0 - q;
+        w = 0.25 * t * t * t;
+        wp = -0.75 * t * t;
+    } else {
+        w = 0.0;
+        wp = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sph_adjoint(
+    const double *__restrict__ pos, const double *__restrict__ mass,
+    const int32_t *__restrict__ nbr, const double *__restrict__ grad_rho, int64_t n, int K,
+    double Lx, double Ly, double Lz, double A, double *__restrict__ gpos,
+    double *__restrict__ gmass)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double g = grad_rho[i];
+    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+    const int32_t *row = nbr + i * (int64_t)K;
+    // smoothing length from the k-th neighbour, in the reference's FP64 arithmetic
+    const int32_t jk = row[K - 1];
+    const double kx = min_image(pos[3 * (int64_t)jk] - xi, Lx);
+    const double ky = min_image(pos[3 * (int64_t)jk + 1] - yi, Ly);
+    const double kz = min_image(pos[3 * (int64_t)jk + 2] - zi, Lz);
+    const double dk = sqrt((kx * kx + ky * ky) + kz * kz);
+    const double r = 0.5 * dk;
+    const double ir = 1.0 / r;
+    const double ir3 = ir * ir * ir;
+    const double gA3 = g * A * ir3;          // g A / r^3
+    const double gA4 = gA3 * ir;             // g A / r^4 (pair-force scale)
+    double C = 0.0, S = 0.0, ox = 0.0, oy = 0.0, oz = 0.0;
+    for (int t = 0; t < K - 1; ++t) {
+        const int64_t j = row[t];
+        const double dx = min_image(pos[3 * j] - xi, Lx);
+        const double dy = min_image(pos[3 * j + 1] - yi, Ly);
+        const double dz = min_image(pos[3 * j + 2] - zi, Lz);
+        const double d = sqrt(dx * dx + dy * dy + dz * dz);
+        double w, wp;
+        spline_wd(d * ir, w, wp);
+        const double mj = mass[j];
+        C += mj * w;
+        S += mj * wp * d;
+        unsafeAtomicAdd(&gmass[j], gA3 * w);
+        if (d > 0.0) {
+            const double c = gA4 * mj * wp / d;   // along x_j - x_i, per unit length
+            const double fx = c * dx, fy = c * dy, fz = c * dz;
+            ox -= fx;
+            oy -= fy;
+            oz -= fz;
+            unsafeAtomicAdd(&gpos[3 * j], fx);
+            unsafeAtomicAdd(&gpos[3 * j + 1], fy);
+            unsafeAtomicAdd(&gpos[3 * j + 2], fz);
+        }
+    }
+    const double mi = mass[i];
+    unsafeAtomicAdd(&gmass[i], gA3);
+    // the k-th neighbour moves the smoothing length: d r / d x_{j_k} = u_ik / 2
+    if (dk > 0.0) {
+        const double drho_dr = -A * ir3 * ir * (3.0 * (C + mi) + S * ir);
+        const double c = g * drho_dr * 0.5 / dk;
+        const double fx = c * kx, fy = c * ky, fz = c * kz;
+        ox -= fx;
+        oy -= fy;
+        oz -= fz;
+        unsafeAtomicAdd(&gpos[3 * (int64_t)jk], fx);
+        unsafeAtomicAdd(&gpos[3 * (int64_t)jk + 1], fy);
+        unsafeAtomicAdd(&gpos[3 * (int64_t)jk + 2], fz);
+    }
+    unsafeAtomicAdd(&gpos[3 * i], ox);
+    unsafeAtomicAdd(&gpos[3 * i + 1], oy);
+    unsafeAtomicAdd(&gpos[3 * i + 2], oz);
+}
+
+__global__ void k_zero(double *__restrict__ a, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0.0;
+}
+
+// counts neighbour ids outside [0, n) (the adjoint indexes positions with them)
+__global__ void k_count_bad_ids(const int32_t *__restrict__ nbr, int64_t m, int64_t n,
+                                int *__restrict__ bad)
+{
+    const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (e < m && (nbr[e] < 0 || nbr[e] >= n)) atomicAdd(bad, 1);
+}
+
+}  // namespace
+
+// single-rounded pi of the reference's density (f90:58)
+static constexpr double PI_REF_HOST = 3.1415927410125732421875;
+
+float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32_t *d_nbr,
+                          const double *d_grad_rho, int64_t n, int K, double Lx, double Ly,
+                          double Lz, double *d_gpos, double *d_gmass, hipStream_t s)
+{
+    if (n < 2) throw std::runtime_error("need at least 2 particles");
+    if (K < 1 || K >= n) throw std::runtime_error("need 1 <= k < npart");
+    if (!(Lx > 0 && Ly > 0 && Lz > 0)) throw std::runtime_error("box sides must be > 0");
+    const double A = Lx * Ly * Lz / PI_REF_HOST;
+    {
+        int *d_bad = nullptr;
+        AHOP_HIP(hipMalloc(&d_bad, sizeof(int)));
+        int bad = 0;
+        hipError_t e = hipMemcpyAsync(d_bad, &bad, sizeof(int), hipMemcpyHostToDevice, s);
+        if (e == hipSuccess) {
+            const int64_t m = n * (int64_t)K;
+            hipLaunchKernelGGL(k_count_bad_ids, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
+                               d_nbr, m, n, d_bad);
+            e = hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(d_bad);
+        AHOP_HIP(e);
+        if (bad) throw std::runtime_error(std::to_string(bad) + " neighbour ids outside [0, npart)");
+    }
+    hipEvent_t e0, e1;
+    AHOP_HIP(hipEventCreate(&e0));
+    AHOP_HIP(hipEventCreate(&e1));
+    AHOP_HIP(hipEventRecord(e0, s));
+    const unsigned b3 = (unsigned)((3 * n + 255) / 256), b1 = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(k_zero, dim3(b3), dim3(256), 0, s, d_gpos, 3 * n);
+    hipLaunchKernelGGL(k_zero, dim3(b1), dim3(256), 0, s, d_gmass, n);
+    hipLaunchKernelGGL(k_sph_adjoint, dim3(b1), dim3(256), 0, s, d_pos, d_mass, d_nbr, d_grad_rho,
+                       n, K, Lx, Ly, Lz, A, d_gpos, d_gmass);
+    AHOP_HIP(hipEventRecord(e1, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+    float ms = 0;
+    AHOP_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+}  // namespace ahop
