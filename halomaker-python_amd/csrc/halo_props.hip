@@ -418,26 +418,29 @@ __global__ __launch_bounds__(BS) void k_pass4b(HaloDev D, const HaloState *st, i
     for (int b = threadIdx.x; b < 3 * nr; b += BS) out[b] = hs[b];
 }
 
-// one row of the pair sum: member i against members j > i, per pair -(m_i m_j) / (d_ij / L)
-// with the pair separation pos[j] - pos[i] wrapped strictly at L/2 (:1417-1426)
-__device__ __forceinline__ double pe_row(const double4 *__restrict__ mem, int i, int n, double L)
+// first pair index of row i of the upper triangle of an n x n pair matrix (pairs (i, j > i)
+// numbered row by row)
+__device__ __forceinline__ int64_t pair_row_start(int i, int n)
 {
-    const double4 a = mem[i];
-    double s = 0.0;
-    for (int j = i + 1; j < n; ++j) {
-        const double4 b = mem[j];
-        const double dx = wrap_strict(b.x - a.x, L);
-        const double dy = wrap_strict(b.y - a.y, L);
-        const double dz = wrap_strict(b.z - a.z, L);
-        const double dist = sqrt(dx * dx + dy * dy + dz * dz) / L;
-        s += -(a.w * b.w) / dist;
-    }
-    return s;
+    return (int64_t)i * (2 * n - i - 1) / 2;
+}
+
+// row of pair p: the largest i with pair_row_start(i) <= p
+__device__ __forceinline__ int pair_row(int64_t p, int n)
+{
+    const double b = 2.0 * n - 1.0;
+    int i = (int)floor(0.5 * (b - sqrt(fmax(b * b - 8.0 * (double)p, 0.0))));
+    i = max(0, min(i, n - 2));
+    while (i > 0 && pair_row_start(i, n) > p) --i;
+    while (i < n - 2 && pair_row_start(i + 1, n) <= p) ++i;
+    return i;
 }
 
 // ---- direct potential energy of halos below the pair-sum limit (det_halo_energies_1b3
-// :1417-1427).  The halo's members are staged in LDS (at most PE_NMAX of them); a thread takes
-// rows r and n-2-r of the upper triangle together, so every thread sums about n pairs.
+// :1417-1427): ep = sum_{i<j} -(m_i m_j) / (d_ij / L) * G / L = -G sum m_i m_j / d_ij, with
+// d_ij the minimum-image distance (the strict wrap at L/2 gives the same distance as
+// min(|d|, L - |d|)).  The halo's members are staged in LDS (at most PE_NMAX of them) and the
+// n(n-1)/2 pairs are cut into equal contiguous runs, one per thread, walked row by row.
 constexpr int PE_NMAX = 2048;   // 64 KiB of LDS
 template <int BS>
 __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *small_halos,
@@ -452,11 +455,31 @@ __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *smal
         mem[i] = make_double4(D.pos[3 * (f + i)], D.pos[3 * (f + i) + 1], D.pos[3 * (f + i) + 2],
                               D.mass[f + i]);
     __syncthreads();
+    const double L = D.L;
+    const int64_t P = (int64_t)n * (n - 1) / 2;
+    const int64_t p0 = P * threadIdx.x / BS, p1 = P * (threadIdx.x + 1) / BS;
     double acc = 0.0;
-    const int R = n - 1;   // rows 0 .. n-2
-    for (int r = threadIdx.x; r < (R + 1) / 2; r += BS) {
-        acc += pe_row(mem, r, n, D.L);
-        if (R - 1 - r != r) acc += pe_row(mem, R - 1 - r, n, D.L);
+    if (p0 < p1) {
+        int i = pair_row(p0, n);
+        int j = (int)(p0 - pair_row_start(i, n)) + i + 1;
+        double4 a = mem[i];
+        double row = 0.0;   // sum of m_j / d_ij over this thread's part of row i
+        for (int64_t p = p0; p < p1; ++p) {
+            const double4 b = mem[j];
+            double tx = fabs(b.x - a.x), ty = fabs(b.y - a.y), tz = fabs(b.z - a.z);
+            tx = fmin(tx, L - tx);
+            ty = fmin(ty, L - ty);
+            tz = fmin(tz, L - tz);
+            row += b.w * rsqrt(tx * tx + ty * ty + tz * tz);
+            if (++j == n) {
+                acc += a.w * row;
+                row = 0.0;
+                ++i;
+                j = i + 1;
+                if (i < n) a = mem[i];
+            }
+        }
+        acc += a.w * row;
     }
     const int t = threadIdx.x;
     sh[t] = acc;
@@ -465,7 +488,7 @@ __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *smal
         if (t < w) sh[t] += sh[t + w];
         __syncthreads();
     }
-    if (t == 0) ep_out[h] = sh[0] * D.G / D.L;
+    if (t == 0) ep_out[h] = -D.G * sh[0];
 }
 
 // In-place inclusive prefix sum added strictly left to right (np.cumsum's order), one thread;
@@ -556,46 +579,89 @@ __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first
         cn[b] = (kw >= 0.0) ? -1.0 : fabs((2.0 * tk[b] + p) / kw);
     }
     __syncthreads();
+    // det_vir_props_1b4 :1127-1229.  Its scans pick the outermost shell meeting a condition;
+    // on finite tables they are block-wide maximum-index searches:
+    //   i1  -- outermost shell holding mass (tm[i-1] < tm[i]), 1 if none;
+    //   brk -- the virial scan stops at the outermost bound shell with ratio <= 0.2; that
+    //          shell is always a new best, so it sets mvir and (the axes being a running
+    //          minimum of increasing functions of the index) the axes; without such a shell
+    //          the last ratio seen exceeds 0.2 and the overdensity branch decides;
+    //   ii1 -- outermost shell with tm >= vir_overdens rho_mean volmin ii1^3 whose inner
+    //          neighbour holds less than the total, 1 if none.
+    // A non-finite ratio (from non-finite input) takes the serial scans instead.
+    const int ttab = nr;
+    const double amax = S.amax, bmax = S.bmax, cmax = S.cmax;
+    const double volmin = 4. / 3. * PI * (amax / (ttab - 1)) * (bmax / (ttab - 1)) * (cmax / (ttab - 1));
+    // vir_overdens * rho_mean * volmin * ii1**3, the cube an exact integer
+    const double mv0 = P.vir_overdens * P.rho_mean * volmin;
+    const double mtot = tm[ttab - 1];
+    __shared__ int red[4][BS];
+    {
+        int b_i1 = 0, b_brk = -1, b_fb = 0, b_nan = 0;
+        for (int b = threadIdx.x; b < ttab; b += BS) {
+            const double v = cn[b];
+            if (v != v) b_nan = 1;
+            else if (v != -1.0 && v <= 0.20) b_brk = b;
+            if (b >= 1) {
+                if (tm[b - 1] < tm[b]) b_i1 = b;
+                if (tm[b] >= mv0 * (double)((int64_t)b * b * b) && tm[b - 1] < mtot) b_fb = b;
+            }
+        }
+        const int t = threadIdx.x;
+        red[0][t] = b_i1;
+        red[1][t] = b_brk;
+        red[2][t] = b_fb;
+        red[3][t] = b_nan;
+        __syncthreads();
+        for (int w = BS / 2; w > 0; w >>= 1) {
+            if (t < w)
+                for (int q = 0; q < 4; ++q) red[q][t] = max(red[q][t], red[q][t + w]);
+            __syncthreads();
+        }
+    }
     if (threadIdx.x != 0) return;
     double ek = o[HP_EK];
     double et = ek + ep;
     o[HP_EP] = ep;
     o[HP_ET] = et;
-    // det_vir_props_1b4 :1127-1229 (compare-only scans over the tables)
-    const int ttab = nr;
-    double amax = S.amax, bmax = S.bmax, cmax = S.cmax;
-    double mvir = tm[ttab - 1];
-    int i1 = 1;
-    for (i1 = ttab - 1; i1 >= 1; --i1)
-        if (tm[i1 - 1] < tm[i1]) break;
-    if (i1 < 1) i1 = 1;
-    double avir = (double)i1 / (ttab - 1) * amax;
-    double bvir = (double)i1 / (ttab - 1) * bmax;
-    double cvir = (double)i1 / (ttab - 1) * cmax;
-    double rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
-    double virth_old = 1.0, virth = 100.0;
-    for (int i0 = ttab - 1; i0 >= 0; --i0) {
-        if (cn[i0] == -1.0) continue;
-        virth = cn[i0];
-        if (virth < virth_old) {
-            mvir = tm[i0];
-            avir = fmin(avir, (double)i0 / (ttab - 1) * amax);
-            bvir = fmin(bvir, (double)i0 / (ttab - 1) * bmax);
-            cvir = fmin(cvir, (double)i0 / (ttab - 1) * cmax);
+    double mvir = mtot, avir, bvir, cvir, rvir;
+    int i1 = max(red[0][0], 1);
+    avir = (double)i1 / (ttab - 1) * amax;
+    bvir = (double)i1 / (ttab - 1) * bmax;
+    cvir = (double)i1 / (ttab - 1) * cmax;
+    rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
+    bool overdensity;
+    if (!red[3][0]) {
+        const int brk = red[1][0];
+        overdensity = true;
+        if (brk >= 0) {
+            mvir = tm[brk];
+            avir = fmin(avir, (double)brk / (ttab - 1) * amax);
+            bvir = fmin(bvir, (double)brk / (ttab - 1) * bmax);
+            cvir = fmin(cvir, (double)brk / (ttab - 1) * cmax);
             rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
-            virth_old = virth;
-            if (virth <= 0.20) break;
+            overdensity = mvir < P.vir_overdens * P.rho_mean * 4. / 3. * PI * avir * bvir * cvir;
         }
+    } else {
+        double virth_old = 1.0, virth = 100.0;
+        for (int i0 = ttab - 1; i0 >= 0; --i0) {
+            if (cn[i0] == -1.0) continue;
+            virth = cn[i0];
+            if (virth < virth_old) {
+                mvir = tm[i0];
+                avir = fmin(avir, (double)i0 / (ttab - 1) * amax);
+                bvir = fmin(bvir, (double)i0 / (ttab - 1) * bmax);
+                cvir = fmin(cvir, (double)i0 / (ttab - 1) * cmax);
+                rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
+                virth_old = virth;
+                if (virth <= 0.20) break;
+            }
+        }
+        overdensity = (virth > 0.20) ||
+                      (mvir < P.vir_overdens * P.rho_mean * 4. / 3. * PI * avir * bvir * cvir);
     }
-    double volmin = 4. / 3. * PI * (amax / (ttab - 1)) * (bmax / (ttab - 1)) * (cmax / (ttab - 1));
-    if ((virth > 0.20) || (mvir < P.vir_overdens * P.rho_mean * 4. / 3. * PI * avir * bvir * cvir)) {
-        int ii1;
-        for (ii1 = ttab - 1; ii1 >= 1; --ii1) {
-            // vir_overdens * rho_mean * volmin * ii1**3, the cube an exact integer
-            double mv = P.vir_overdens * P.rho_mean * volmin * (double)((int64_t)ii1 * ii1 * ii1);
-            if ((tm[ii1] >= mv) && (tm[ii1 - 1] < tm[ttab - 1])) break;
-        }
-        if (ii1 < 1) ii1 = 1;
+    if (overdensity) {
+        const int ii1 = max(red[2][0], 1);
         mvir = tm[ii1];
         avir = (double)ii1 / (ttab - 1) * amax;
         bvir = (double)ii1 / (ttab - 1) * bmax;
