@@ -357,11 +357,12 @@ enum { KNN_STAT_NODES = 0, KNN_STAT_BUCKETS, KNN_STAT_CANDIDATES, KNN_STAT_INSER
 // dependent LDS round trips instead of the ~12 of a binary heap with the same capacity.
 constexpr int HEAP_D = 8;
 
+// Place (d, id) at heap node e (the root by default) and sift it down a max-heap of `size`
+// entries (HEAP_D-ary, lane-interleaved in LDS).
 template <typename KeyT>
 __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int size, KeyT d,
-                                          int32_t id)
+                                          int32_t id, int e = 0)
 {
-    int e = 0;
     while (true) {
         const int c0 = HEAP_D * e + 1;
         if (c0 >= size) break;
@@ -393,7 +394,13 @@ __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int s
 }
 
 #ifndef AHOP_FP64_NBUF
-#define AHOP_FP64_NBUF 1
+#define AHOP_FP64_NBUF 2
+#endif
+#ifndef AHOP_FP64_FILL
+#define AHOP_FP64_FILL 1
+#endif
+#ifndef AHOP_MIXED_NBUF
+#define AHOP_MIXED_NBUF 4
 #endif
 #include "knn_wave.h"
 
@@ -670,8 +677,12 @@ void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, co
 template <int KCAP>
 static void launch_knn(bool mixed, const KnnArgs &a, unsigned nb, hipStream_t s)
 {
-    if (mixed) hipLaunchKernelGGL((k_knn<KCAP, true>), dim3(nb), dim3(64), 0, s, a);
-    else hipLaunchKernelGGL((k_knn<KCAP, false>), dim3(nb), dim3(64), 0, s, a);
+    if (mixed)
+        hipLaunchKernelGGL((k_knn<KCAP, true>), dim3(nb), dim3(64), 0, s, a, a.first, a.last, a.left,
+                           a.right, a.fbox, a.spos);
+    else
+        hipLaunchKernelGGL((k_knn<KCAP, false>), dim3(nb), dim3(64), 0, s, a, a.first, a.last, a.left,
+                           a.right, a.fbox, a.spos);
 }
 
 static void dispatch_knn(int K, bool mixed, const KnnArgs &a, unsigned nb, hipStream_t s)

@@ -59,8 +59,16 @@ __device__ __forceinline__ float fp32_key_err(float d, float hQ, float L)
     return 8 * U32 * sqrtf(d * 1.0001f) * s + 16 * U32 * U32 * s * s + 1e-15f * L * s;
 }
 
+// The tree arrays and the sorted particles are also passed as restrict-qualified kernel
+// arguments: the kernel never writes them, and saying so lets the wave-uniform node reads
+// go through scalar loads.
 template <int KCAP, bool MIXED>
-__global__ __launch_bounds__(64) void k_knn(KnnArgs A)
+__global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict__ nfirst,
+                                            const int32_t *__restrict__ nlast,
+                                            const int32_t *__restrict__ nleft,
+                                            const int32_t *__restrict__ nright,
+                                            const float4 *__restrict__ nfbox,
+                                            const double4 *__restrict__ spos)
 {
     using KeyT = typename std::conditional<MIXED, float, double>::type;
     using StageT = typename std::conditional<MIXED, float4, double4>::type;
@@ -70,7 +78,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     constexpr int KHCAP = MIXED ? KCAP + KX : KCAP;
     // staged buckets whose insertions are deferred (the FP64 heap is twice as large, so its
     // stage may be cut down to fit one more wave per CU)
-    constexpr int NBUF = MIXED ? 4 : AHOP_FP64_NBUF;
+    constexpr int NBUF = MIXED ? AHOP_MIXED_NBUF : AHOP_FP64_NBUF;
     __shared__ KeyT hk[KHCAP * WAVE];
     __shared__ int32_t hi[KHCAP * WAVE];
     __shared__ StageT stage[NBUF * WAVE];
@@ -85,9 +93,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     const int q = active ? (A.qlist ? A.qlist[slot] : slot) : 0;
     const int K = A.K;
     const int KH = MIXED ? K + KX : K;
-    const double4 P = A.spos[q];
+    const double4 P = spos[q];
     const double Lx = A.Lx, Ly = A.Ly, Lz = A.Lz;
-    const double4 *__restrict__ spos = A.spos;
 
     // centre of the wave's queries: origin of the box-relative FP32 coordinates
     double cx, cy, cz;
@@ -125,6 +132,9 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         seed_lo = max(0, q - Wd);
         seed_hi = min(A.n - 1, q + Wd);
     }
+    // slots that may lie in some lane's seed window or be some lane's query (wave-uniform)
+    const int wseed_lo = A.qlist ? 0 : max(0, wave * WAVE - Wd);
+    const int wseed_hi = A.qlist ? A.n - 1 : min(A.n - 1, wave * WAVE + WAVE - 1 + Wd);
     const float pxf = (float)P.x, pyf = (float)P.y, pzf = (float)P.z;
     const float Lxf = (float)Lx, Lyf = (float)Ly, Lzf = (float)Lz;
     const float mLx = 4.0f * U32 * Lxf, mLy = 4.0f * U32 * Lyf, mLz = 4.0f * U32 * Lzf;
@@ -133,13 +143,18 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
     uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;   // accepted-candidate masks of staged buckets
     int f0 = 0, f1 = 0, f2 = 0, f3 = 0;         // first slot of each staged bucket
     int nbuf = 0;
+    // heap entries written so far by the seeding (the per-lane gather of a query list
+    // inserts into the +inf-filled heap directly)
+    int filled = (A.qlist || (!MIXED && !AHOP_FP64_FILL)) ? KH : 0;
     unsigned long long st_nodes = 0, st_buckets = 0, st_cand = 0, st_steps = 0, st_ins = 0;
 
+    // FP32 keys use fused multiply-adds: one rounding per step, so the error bound of
+    // fp32_key_err (written for separate roundings) still holds
 #define KNN_KEY(c, out)                                                                        \
     do {                                                                                       \
         if constexpr (MIXED) {                                                                 \
             const float dx_ = (c).x - qx, dy_ = (c).y - qy, dz_ = (c).z - qz;                  \
-            (out) = dx_ * dx_ + dy_ * dy_ + dz_ * dz_;                                         \
+            (out) = __builtin_fmaf(dz_, dz_, __builtin_fmaf(dy_, dy_, dx_ * dx_));             \
         } else {                                                                               \
             (out) = d2_ref(P.x, P.y, P.z, (c).x, (c).y, (c).z, Lx, Ly, Lz);                    \
         }                                                                                      \
@@ -152,27 +167,55 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         tau = hk[lane];                                                                        \
     } while (0)
 
+    // Take this lane's next pending candidate (first staged bucket with a set bit, lowest
+    // bit first): its key and sorted slot.
+#define KNN_PICK(dout, idout)                                                                  \
+    do {                                                                                       \
+        const bool s0 = m0 != 0, s1 = !s0 && m1 != 0, s2 = !s0 && !s1 && m2 != 0;              \
+        const bool s3 = !s0 && !s1 && !s2;                                                     \
+        uint64_t mk = s0 ? m0 : (s1 ? m1 : (s2 ? m2 : m3));                                    \
+        const int fsel = s0 ? f0 : (s1 ? f1 : (s2 ? f2 : f3));                                 \
+        const int bsel = s0 ? 0 : (s1 ? 1 : (s2 ? 2 : 3));                                     \
+        const int j_ = __ffsll((long long)mk) - 1;                                             \
+        mk &= mk - 1;                                                                          \
+        m0 = s0 ? mk : m0;                                                                     \
+        m1 = s1 ? mk : m1;                                                                     \
+        m2 = s2 ? mk : m2;                                                                     \
+        m3 = s3 ? mk : m3;                                                                     \
+        KNN_KEY(stage[bsel * WAVE + j_], dout);                                                \
+        (idout) = fsel + j_;                                                                   \
+    } while (0)
+
+    // Bottom-up heapify of this lane's KH entries.
+#define KNN_HEAPIFY()                                                                          \
+    do {                                                                                       \
+        for (int e_ = (KH - 2) / HEAP_D; e_ >= 0; --e_)                                        \
+            heap_sift<KeyT>(hk, hi, lane, KH, hk[e_ * WAVE + lane], hi[e_ * WAVE + lane], e_); \
+    } while (0)
+
     // Insert every pending accepted candidate lane by lane: the wave pays for the busiest
-    // lane's total over all staged buckets, not for each bucket's busiest lane.
+    // lane's total over all staged buckets, not for each bucket's busiest lane.  While the
+    // lane's heap is still being filled (seeding), candidates are written straight into the
+    // next free entry and the full array is heapified once, instead of sifting each one.
 #define KNN_FLUSH()                                                                            \
     do {                                                                                       \
         while (__any((m0 | m1 | m2 | m3) != 0)) {                                             \
             ++st_steps;                                                                        \
             if ((m0 | m1 | m2 | m3) != 0) {                                                    \
-                const bool s0 = m0 != 0, s1 = !s0 && m1 != 0, s2 = !s0 && !s1 && m2 != 0;      \
-                const bool s3 = !s0 && !s1 && !s2;                                             \
-                uint64_t mk = s0 ? m0 : (s1 ? m1 : (s2 ? m2 : m3));                            \
-                const int fsel = s0 ? f0 : (s1 ? f1 : (s2 ? f2 : f3));                         \
-                const int bsel = s0 ? 0 : (s1 ? 1 : (s2 ? 2 : 3));                             \
-                const int j_ = __ffsll((long long)mk) - 1;                                     \
-                mk &= mk - 1;                                                                  \
-                m0 = s0 ? mk : m0;                                                             \
-                m1 = s1 ? mk : m1;                                                             \
-                m2 = s2 ? mk : m2;                                                             \
-                m3 = s3 ? mk : m3;                                                             \
                 KeyT d_;                                                                       \
-                KNN_KEY(stage[bsel * WAVE + j_], d_);                                          \
-                if (d_ < tau) KNN_INSERT(d_, fsel + j_);                                       \
+                int id_;                                                                       \
+                KNN_PICK(d_, id_);                                                             \
+                if (filled < KH) {                                                             \
+                    hk[filled * WAVE + lane] = d_;                                             \
+                    hi[filled * WAVE + lane] = id_;                                            \
+                    ++st_ins;                                                                  \
+                    if (++filled == KH) {                                                      \
+                        KNN_HEAPIFY();                                                         \
+                        tau = hk[lane];                                                        \
+                    }                                                                          \
+                } else if (d_ < tau) {                                                         \
+                    KNN_INSERT(d_, id_);                                                       \
+                }                                                                              \
             }                                                                                  \
         }                                                                                      \
         nbuf = 0;                                                                              \
@@ -195,12 +238,22 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         }                                                                                      \
         __syncthreads();                                                                       \
         uint64_t mask_ = 0;                                                                    \
-        for (int j_ = 0; j_ < (cnt); ++j_) {                                                   \
-            KeyT d_;                                                                           \
-            KNN_KEY(buf_[j_], d_);                                                             \
-            const int sj_ = (fst) + j_;                                                        \
-            const bool in_seed_ = sj_ >= seed_lo && sj_ <= seed_hi;                            \
-            if (d_ < tau && sj_ != q && ((seeding) ? in_seed_ : !in_seed_)) mask_ |= (1ull << j_); \
+        if ((seeding) || ((fst) <= wseed_hi && (fst) + (cnt) - 1 >= wseed_lo)) {               \
+            for (int j_ = 0; j_ < (cnt); ++j_) {                                               \
+                KeyT d_;                                                                       \
+                KNN_KEY(buf_[j_], d_);                                                         \
+                const int sj_ = (fst) + j_;                                                    \
+                const bool in_seed_ = sj_ >= seed_lo && sj_ <= seed_hi;                        \
+                if (d_ < tau && sj_ != q && ((seeding) ? in_seed_ : !in_seed_))                \
+                    mask_ |= (1ull << j_);                                                     \
+            }                                                                                  \
+        } else {                                                                               \
+            /* the bucket lies outside every lane's seed window (and holds no query) */       \
+            for (int j_ = 0; j_ < (cnt); ++j_) {                                               \
+                KeyT d_;                                                                       \
+                KNN_KEY(buf_[j_], d_);                                                         \
+                if (d_ < tau) mask_ |= (1ull << j_);                                           \
+            }                                                                                  \
         }                                                                                      \
         m0 = nbuf == 0 ? mask_ : m0;                                                           \
         m1 = nbuf == 1 ? mask_ : m1;                                                           \
@@ -215,7 +268,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
 
 #define KNN_NEED(c, out)                                                                       \
     do {                                                                                       \
-        const float bd_ = box_dist2_lower((c), spos, A.fbox, pxf, pyf, pzf, Lxf, Lyf, Lzf,     \
+        const float bd_ = box_dist2_lower((c), spos, nfbox, pxf, pyf, pzf, Lxf, Lyf, Lzf,     \
                                           mLx, mLy, mLz);                                      \
         if constexpr (MIXED) {                                                                 \
             (out) = active && bd_ < tau + 2 * fp32_key_err(tau, hQ, Lmin);                     \
@@ -248,6 +301,12 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         }
     }
     KNN_FLUSH();
+    if (filled < KH) {
+        // a clipped window (array ends) or an idle lane: the unfilled entries are +inf
+        KNN_HEAPIFY();
+        filled = KH;
+        tau = (active && !wave_unsafe) ? hk[lane] : (KeyT)-1;
+    }
 
     // ---- wave-coherent traversal of the binary radix tree, nearest child first
     int sp = 1;
@@ -260,8 +319,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         int f, l;
         if (node < 0) f = l = ~node;
         else {
-            f = A.first[node];
-            l = A.last[node];
+            f = nfirst[node];
+            l = nlast[node];
         }
         ++st_nodes;
         bool need;
@@ -274,8 +333,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
             KNN_STAGE(f, cnt, false);
             continue;
         }
-        const int c1 = __builtin_amdgcn_readfirstlane(A.left[node]);
-        const int c2 = __builtin_amdgcn_readfirstlane(A.right[node]);
+        const int c1 = __builtin_amdgcn_readfirstlane(nleft[node]);
+        const int c2 = __builtin_amdgcn_readfirstlane(nright[node]);
         bool n1, n2;
         KNN_NEED(c1, n1);
         KNN_NEED(c2, n2);
@@ -286,8 +345,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A)
         }
         if (a1 && a2) {
             // nearest first, in sorted-slot distance to the wave's queries
-            const int f1_ = c1 < 0 ? ~c1 : A.first[c1], l1_ = c1 < 0 ? ~c1 : A.last[c1];
-            const int f2_ = c2 < 0 ? ~c2 : A.first[c2], l2_ = c2 < 0 ? ~c2 : A.last[c2];
+            const int f1_ = c1 < 0 ? ~c1 : nfirst[c1], l1_ = c1 < 0 ? ~c1 : nlast[c1];
+            const int f2_ = c2 < 0 ? ~c2 : nfirst[c2], l2_ = c2 < 0 ? ~c2 : nlast[c2];
             const int d1 = max(0, max(f1_ - qmid, qmid - l1_));
             const int d2 = max(0, max(f2_ - qmid, qmid - l2_));
             if (d1 <= d2) {
