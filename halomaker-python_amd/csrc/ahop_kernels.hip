@@ -358,8 +358,10 @@ enum { KNN_STAT_NODES = 0, KNN_STAT_BUCKETS, KNN_STAT_CANDIDATES, KNN_STAT_INSER
 constexpr int HEAP_D = 8;
 
 // Place (d, id) at heap node e (the root by default) and sift it down a max-heap of `size`
-// entries (HEAP_D-ary, lane-interleaved in LDS).
-template <typename KeyT>
+// entries (HEAP_D-ary, lane-interleaved in LDS).  CHECKED = false reads all HEAP_D children
+// of an internal node without bounds checks: the caller keeps the entries past `size` up to
+// the last child of the last internal node allocated and at -inf, so they never win.
+template <typename KeyT, bool CHECKED = true>
 __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int size, KeyT d,
                                           int32_t id, int e = 0)
 {
@@ -370,7 +372,7 @@ __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int s
         int32_t ii[HEAP_D];
 #pragma unroll
         for (int t = 0; t < HEAP_D; ++t) {
-            const int c = c0 + t < size ? c0 + t : c0;
+            const int c = (!CHECKED || c0 + t < size) ? c0 + t : c0;
             kk[t] = hk[c * WAVE + lane];
             ii[t] = hi[c * WAVE + lane];
         }
@@ -379,7 +381,7 @@ __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int s
         int ic = c0;
 #pragma unroll
         for (int t = 1; t < HEAP_D; ++t)
-            if (c0 + t < size && kk[t] > kc) {
+            if ((!CHECKED || c0 + t < size) && kk[t] > kc) {
                 kc = kk[t];
                 idc = ii[t];
                 ic = c0 + t;

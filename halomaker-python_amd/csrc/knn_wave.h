@@ -72,10 +72,16 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
 {
     using KeyT = typename std::conditional<MIXED, float, double>::type;
     using StageT = typename std::conditional<MIXED, float4, double4>::type;
-    // the mixed heap keeps KX entries beyond the k nearest, so that near-ties at the k-th
-    // place are settled in FP64 inside this kernel rather than by the FP64 re-run
-    constexpr int KX = 2;
-    constexpr int KHCAP = MIXED ? KCAP + KX : KCAP;
+    // the mixed heap keeps KX entries beyond the k nearest (the certificate needs the
+    // (k+1)-th key) and holds KCAP + 1 entries: with KCAP a multiple of HEAP_D every child
+    // index of an internal node of a heap of at most KCAP + 1 entries lies below KCAP + 1,
+    // so its insertions read all HEAP_D children unchecked, the entries past the heap size
+    // being kept at -inf.  The FP64 heap stays at KCAP entries (one more would cost it a
+    // wave per CU of LDS) and checks child indices.
+    constexpr int KX = 1;
+    constexpr int KHCAP = MIXED ? KCAP + 1 : KCAP;
+    constexpr bool SIFT_CHECKED = !MIXED;
+    static_assert(KCAP % HEAP_D == 0, "heap capacity must be a multiple of the arity");
     // staged buckets whose insertions are deferred (the FP64 heap is twice as large, so its
     // stage may be cut down to fit one more wave per CU)
     constexpr int NBUF = MIXED ? AHOP_MIXED_NBUF : AHOP_FP64_NBUF;
@@ -120,8 +126,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         // the FP32 frame is valid while search radius + query spread < L/2
         wave_unsafe = hq >= 0.25 * fmin(Lx, fmin(Ly, Lz));
     }
-    for (int e = 0; e < KH; ++e) {
-        hk[e * WAVE + lane] = (KeyT)INFINITY;
+    for (int e = 0; e < KHCAP; ++e) {
+        hk[e * WAVE + lane] = e < KH ? (KeyT)INFINITY : (KeyT)-INFINITY;
         hi[e * WAVE + lane] = -1;
     }
     KeyT tau = (active && !wave_unsafe) ? (KeyT)INFINITY : (KeyT)-1;
@@ -162,7 +168,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
 
 #define KNN_INSERT(d, id)                                                                      \
     do {                                                                                       \
-        heap_sift<KeyT>(hk, hi, lane, KH, (d), (id));                                          \
+        heap_sift<KeyT, SIFT_CHECKED>(hk, hi, lane, KH, (d), (id));                            \
         ++st_ins;                                                                              \
         tau = hk[lane];                                                                        \
     } while (0)
@@ -190,7 +196,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
 #define KNN_HEAPIFY()                                                                          \
     do {                                                                                       \
         for (int e_ = (KH - 2) / HEAP_D; e_ >= 0; --e_)                                        \
-            heap_sift<KeyT>(hk, hi, lane, KH, hk[e_ * WAVE + lane], hi[e_ * WAVE + lane], e_); \
+            heap_sift<KeyT, SIFT_CHECKED>(hk, hi, lane, KH, hk[e_ * WAVE + lane],               \
+                                          hi[e_ * WAVE + lane], e_);                    \
     } while (0)
 
     // Insert every pending accepted candidate lane by lane: the wave pays for the busiest
