@@ -401,6 +401,12 @@ __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int s
 #ifndef AHOP_FP64_FILL
 #define AHOP_FP64_FILL 1
 #endif
+#ifndef AHOP_WAVE_SEED
+#define AHOP_WAVE_SEED 1
+#endif
+#ifndef AHOP_SEED_PAD
+#define AHOP_SEED_PAD 16
+#endif
 #ifndef AHOP_MIXED_NBUF
 #define AHOP_MIXED_NBUF 4
 #endif

@@ -132,11 +132,17 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
     }
     KeyT tau = (active && !wave_unsafe) ? (KeyT)INFINITY : (KeyT)-1;
     // Morton-order seed window of this lane: [q-Wd, q+Wd]; the traversal skips it
-    const int Wd = K / 2 + 16;
+    const int Wd = K / 2 + AHOP_SEED_PAD;
     int seed_lo = 1, seed_hi = 0;
     if (active && !wave_unsafe) {
-        seed_lo = max(0, q - Wd);
-        seed_hi = min(A.n - 1, q + Wd);
+        if (AHOP_WAVE_SEED && !A.qlist) {
+            // every lane seeds from the whole window of the wave
+            seed_lo = max(0, wave * WAVE - Wd);
+            seed_hi = min(A.n - 1, wave * WAVE + WAVE - 1 + Wd);
+        } else {
+            seed_lo = max(0, q - Wd);
+            seed_hi = min(A.n - 1, q + Wd);
+        }
     }
     // slots that may lie in some lane's seed window or be some lane's query (wave-uniform)
     const int wseed_lo = A.qlist ? 0 : max(0, wave * WAVE - Wd);
@@ -334,6 +340,8 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         KNN_NEED(node, need);
         if (!__any(need)) continue;
         if (l - f + 1 <= A.bucket || (root_is_bucket && node == 0)) {
+            // a bucket inside the wave-wide seed window was seen by every lane already
+            if (AHOP_WAVE_SEED && !A.qlist && f >= wseed_lo && l <= wseed_hi) continue;
             const int cnt = l - f + 1;
             ++st_buckets;
             st_cand += cnt;
