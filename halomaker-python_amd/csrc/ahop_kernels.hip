@@ -395,6 +395,72 @@ __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int s
     hi[e * WAVE + lane] = id;
 }
 
+// Packed heap of the mixed kernel: one 64-bit LDS word per entry, (FP32 key bits, id), so
+// that a sift step moves and reads an entry with one LDS instruction.
+template <bool CHECKED = true>
+__device__ __forceinline__ void heap_sift_packed(uint2 *hp, int lane, int size, float d,
+                                                 int32_t id, int e = 0)
+{
+    while (true) {
+        const int c0 = HEAP_D * e + 1;
+        if (c0 >= size) break;
+        uint2 ch[HEAP_D];
+#pragma unroll
+        for (int t = 0; t < HEAP_D; ++t) {
+            const int c = (!CHECKED || c0 + t < size) ? c0 + t : c0;
+            ch[t] = hp[c * WAVE + lane];
+        }
+        float kc = __uint_as_float(ch[0].x);
+        uint32_t idc = ch[0].y;
+        int ic = c0;
+#pragma unroll
+        for (int t = 1; t < HEAP_D; ++t) {
+            const float kt = __uint_as_float(ch[t].x);
+            if ((!CHECKED || c0 + t < size) && kt > kc) {
+                kc = kt;
+                idc = ch[t].y;
+                ic = c0 + t;
+            }
+        }
+        if (!(kc > d)) break;
+        hp[e * WAVE + lane] = make_uint2(__float_as_uint(kc), idc);
+        e = ic;
+    }
+    hp[e * WAVE + lane] = make_uint2(__float_as_uint(d), (uint32_t)id);
+}
+
+// Heap entry access for both kernels: packed (mixed) or key and id arrays (FP64).
+template <bool PACKED, typename KeyT>
+__device__ __forceinline__ KeyT heap_key(const uint2 *hp, const KeyT *hk, int idx)
+{
+    if constexpr (PACKED) return __uint_as_float(hp[idx].x);
+    else return hk[idx];
+}
+template <bool PACKED, typename KeyT>
+__device__ __forceinline__ int32_t heap_id(const uint2 *hp, const int32_t *hi, int idx)
+{
+    if constexpr (PACKED) return (int32_t)hp[idx].y;
+    else return hi[idx];
+}
+template <bool PACKED, typename KeyT>
+__device__ __forceinline__ void heap_set(uint2 *hp, KeyT *hk, int32_t *hi, int idx, KeyT k,
+                                         int32_t id)
+{
+    if constexpr (PACKED) {
+        hp[idx] = make_uint2(__float_as_uint((float)k), (uint32_t)id);
+    } else {
+        hk[idx] = k;
+        hi[idx] = id;
+    }
+}
+template <bool PACKED, bool CHECKED, typename KeyT>
+__device__ __forceinline__ void heap_sift_any(uint2 *hp, KeyT *hk, int32_t *hi, int lane,
+                                              int size, KeyT d, int32_t id, int e = 0)
+{
+    if constexpr (PACKED) heap_sift_packed<CHECKED>(hp, lane, size, (float)d, id, e);
+    else heap_sift<KeyT, CHECKED>(hk, hi, lane, size, d, id, e);
+}
+
 #ifndef AHOP_FP64_NBUF
 #define AHOP_FP64_NBUF 2
 #endif

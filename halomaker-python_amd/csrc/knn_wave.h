@@ -85,8 +85,15 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
     // staged buckets whose insertions are deferred (the FP64 heap is twice as large, so its
     // stage may be cut down to fit one more wave per CU)
     constexpr int NBUF = MIXED ? AHOP_MIXED_NBUF : AHOP_FP64_NBUF;
-    __shared__ KeyT hk[KHCAP * WAVE];
-    __shared__ int32_t hi[KHCAP * WAVE];
+    // heap storage: packed (key, id) words for the mixed kernel, key and id arrays for FP64
+    __shared__ uint2 hp[MIXED ? KHCAP * WAVE : 1];
+    __shared__ KeyT hk[MIXED ? 1 : KHCAP * WAVE];
+    __shared__ int32_t hi[MIXED ? 1 : KHCAP * WAVE];
+#define HK(e) heap_key<MIXED, KeyT>(hp, hk, (e) * WAVE + lane)
+#define HI(e) heap_id<MIXED, KeyT>(hp, hi, (e) * WAVE + lane)
+#define HSET(e, k, i) heap_set<MIXED, KeyT>(hp, hk, hi, (e) * WAVE + lane, (k), (i))
+#define HSIFT(size, d, id, e, checked) \
+    heap_sift_any<MIXED, checked, KeyT>(hp, hk, hi, lane, (size), (d), (id), (e))
     __shared__ StageT stage[NBUF * WAVE];
     __shared__ int32_t stack[STACK];
 
@@ -126,10 +133,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         // the FP32 frame is valid while search radius + query spread < L/2
         wave_unsafe = hq >= 0.25 * fmin(Lx, fmin(Ly, Lz));
     }
-    for (int e = 0; e < KHCAP; ++e) {
-        hk[e * WAVE + lane] = e < KH ? (KeyT)INFINITY : (KeyT)-INFINITY;
-        hi[e * WAVE + lane] = -1;
-    }
+    for (int e = 0; e < KHCAP; ++e) HSET(e, e < KH ? (KeyT)INFINITY : (KeyT)-INFINITY, -1);
     KeyT tau = (active && !wave_unsafe) ? (KeyT)INFINITY : (KeyT)-1;
     // Morton-order seed window of this lane: [q-Wd, q+Wd]; the traversal skips it
     const int Wd = K / 2 + AHOP_SEED_PAD;
@@ -174,9 +178,9 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
 
 #define KNN_INSERT(d, id)                                                                      \
     do {                                                                                       \
-        heap_sift<KeyT, SIFT_CHECKED>(hk, hi, lane, KH, (d), (id));                            \
+        HSIFT(KH, (d), (id), 0, SIFT_CHECKED);                                                 \
         ++st_ins;                                                                              \
-        tau = hk[lane];                                                                        \
+        tau = HK(0);                                                                           \
     } while (0)
 
     // Take this lane's next pending candidate (first staged bucket with a set bit, lowest
@@ -202,8 +206,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
 #define KNN_HEAPIFY()                                                                          \
     do {                                                                                       \
         for (int e_ = (KH - 2) / HEAP_D; e_ >= 0; --e_)                                        \
-            heap_sift<KeyT, SIFT_CHECKED>(hk, hi, lane, KH, hk[e_ * WAVE + lane],               \
-                                          hi[e_ * WAVE + lane], e_);                    \
+            HSIFT(KH, HK(e_), HI(e_), e_, SIFT_CHECKED);                                       \
     } while (0)
 
     // Insert every pending accepted candidate lane by lane: the wave pays for the busiest
@@ -219,12 +222,11 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
                 int id_;                                                                       \
                 KNN_PICK(d_, id_);                                                             \
                 if (filled < KH) {                                                             \
-                    hk[filled * WAVE + lane] = d_;                                             \
-                    hi[filled * WAVE + lane] = id_;                                            \
+                    HSET(filled, d_, id_);                                                     \
                     ++st_ins;                                                                  \
                     if (++filled == KH) {                                                      \
                         KNN_HEAPIFY();                                                         \
-                        tau = hk[lane];                                                        \
+                        tau = HK(0);                                                        \
                     }                                                                          \
                 } else if (d_ < tau) {                                                         \
                     KNN_INSERT(d_, id_);                                                       \
@@ -318,7 +320,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         // a clipped window (array ends) or an idle lane: the unfilled entries are +inf
         KNN_HEAPIFY();
         filled = KH;
-        tau = (active && !wave_unsafe) ? hk[lane] : (KeyT)-1;
+        tau = (active && !wave_unsafe) ? HK(0) : (KeyT)-1;
     }
 
     // ---- wave-coherent traversal of the binary radix tree, nearest child first
@@ -400,13 +402,12 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
 
     // ---- heap -> ascending order (in-place heapsort, same trip count on every lane)
     for (int end = KH - 1; end >= 1; --end) {
-        const KeyT top = hk[lane];
-        const int32_t tid = hi[lane];
-        const KeyT mv = hk[end * WAVE + lane];
-        const int32_t mid = hi[end * WAVE + lane];
-        hk[end * WAVE + lane] = top;
-        hi[end * WAVE + lane] = tid;
-        heap_sift<KeyT>(hk, hi, lane, end, mv, mid);
+        const KeyT top = HK(0);
+        const int32_t tid = HI(0);
+        const KeyT mv = HK(end);
+        const int32_t mid = HI(end);
+        HSET(end, top, tid);
+        HSIFT(end, mv, mid, 0, true);
     }
     if (!active) return;
 
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         // frame folds onto another image gets a key no smaller than its true distance, so
         // only the particles inside the radius need the right image).  An unfilled heap
         // (last key infinite) holds every other particle.
-        const float lkx = hk[(KH - 1) * WAVE + lane], lk1 = hk[(K - 1) * WAVE + lane];
+        const float lkx = HK(KH - 1), lk1 = HK(K - 1);
         const float E = fp32_key_err(lkx, hQ, Lmin);
         if (!(lkx == INFINITY) &&
             !(lkx - lk1 > 2 * E && E < 0.01f * lkx && sqrtf(lkx * 1.0001f) + hQ < 0.4999f * Lmin))
@@ -442,10 +443,10 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
     for (int pass = 0; pass < 2; ++pass) {
         double d2k;
         if constexpr (MIXED) {
-            const double4 c = spos[hi[(K - 1) * WAVE + lane]];
+            const double4 c = spos[HI(K - 1)];
             d2k = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
         } else {
-            d2k = hk[(K - 1) * WAVE + lane];
+            d2k = HK(K - 1);
         }
         const double r = sqrt(d2k) * 0.5;
         const double r3 = r * r * r;
@@ -454,10 +455,10 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         double prev = -1.0;
         bool sorted = true;
         for (int i = 0; i < K - 1; ++i) {
-            const double4 c = spos[hi[i * WAVE + lane]];
+            const double4 c = spos[HI(i)];
             double d;
             if constexpr (MIXED) d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
-            else d = hk[i * WAVE + lane];
+            else d = HK(i);
             sorted = sorted && (d >= prev);
             prev = d;
             contrib = contrib + c.w * spline_ref(sqrt(d) * unsr);
@@ -465,7 +466,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         sorted = sorted && (d2k >= prev);
         if constexpr (MIXED) {
             for (int i = K; i < nsel; ++i) {
-                const double4 c = spos[hi[i * WAVE + lane]];
+                const double4 c = spos[HI(i)];
                 sorted = sorted && (d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz) >= d2k);
             }
         }
@@ -473,19 +474,26 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         if (!MIXED || sorted) break;
         // insertion sort of this lane's nsel entries by FP64 distance (stable on ties)
         for (int i = 1; i < nsel; ++i) {
-            const int32_t id = hi[i * WAVE + lane];
+            const int32_t id = HI(i);
+            const KeyT ki = HK(i);
             const double4 ci = spos[id];
             const double di = d2_ref(P.x, P.y, P.z, ci.x, ci.y, ci.z, Lx, Ly, Lz);
             int j = i - 1;
             while (j >= 0) {
-                const double4 cj = spos[hi[j * WAVE + lane]];
+                const double4 cj = spos[HI(j)];
                 if (d2_ref(P.x, P.y, P.z, cj.x, cj.y, cj.z, Lx, Ly, Lz) <= di) break;
-                hi[(j + 1) * WAVE + lane] = hi[j * WAVE + lane];
+                HSET(j + 1, HK(j), HI(j));
                 --j;
             }
-            hi[(j + 1) * WAVE + lane] = id;
+            HSET(j + 1, ki, id);
         }
     }
     A.rho_s[q] = rho;
-    for (int t = 0; t < A.nhop; ++t) A.nbr[(int64_t)q * A.nhop + t] = hi[t * WAVE + lane];
+    for (int t = 0; t < A.nhop; ++t) A.nbr[(int64_t)q * A.nhop + t] = HI(t);
 }
+#undef KNN_PICK
+#undef KNN_HEAPIFY
+#undef HK
+#undef HI
+#undef HSET
+#undef HSIFT
