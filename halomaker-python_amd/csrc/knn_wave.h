@@ -127,9 +127,10 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         qx = (float)rx;
         qy = (float)ry;
         qz = (float)rz;
-        double hq = active ? fmax(fabs(rx), fmax(fabs(ry), fabs(rz))) : 0.0;
-        hq = wave_max(hq);
-        hQ = (float)hq * (1.0f + 4 * U32);
+        const double hql = active ? fmax(fabs(rx), fmax(fabs(ry), fabs(rz))) : 0.0;
+        // the error bounds of this lane's keys need only its own offset from the origin
+        hQ = (float)hql * (1.0f + 4 * U32);
+        const double hq = wave_max(hql);
         // the FP32 frame is valid while search radius + query spread < L/2
         wave_unsafe = hq >= 0.25 * fmin(Lx, fmin(Ly, Lz));
     }
