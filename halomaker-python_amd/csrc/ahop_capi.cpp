@@ -276,6 +276,7 @@ int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_c
         tp.xlongs2 = s.xlongs2;
         tp.ylongs2 = s.ylongs2;
         tp.zlongs2 = s.zlongs2;
+        tp.nthreads = s.nbpes;   // the reference runs this stage on nbPes OpenMP threads
         TreeOutputs to;
         create_group_tree(ti, tp, to);
         auto t2 = std::chrono::steady_clock::now();

@@ -7,11 +7,15 @@
 //   omp_dev_create_nodes_task     :1245-1628
 //   reorder_nodes_deterministic   :987-1131
 //   node2table                    :550-573
-// Child subtrees are processed depth first on one thread.  The reference runs them as
-// OpenMP tasks whose only scheduling-dependent effect is the pre-reorder node numbering,
-// which reorder_nodes_deterministic normalises, so the result is the same.
+// Child subtrees are OpenMP tasks, as in the reference: the groups, member lists, particles
+// and node entries of two sibling subtrees are disjoint, node entries are reserved with an
+// atomic counter, and the only scheduling-dependent effect, the pre-reorder node numbering,
+// is normalised by reorder_nodes_deterministic.  The components of one level are treated
+// in parallel too when there are many (the first level holds every group).
 // All sums run in the reference's linked-list order, so the tables match bit for bit.
 #include "create_nodes.h"
+
+#include <omp.h>
 
 #include <algorithm>
 #include <cmath>
@@ -40,6 +44,24 @@ struct Builder {
     std::vector<int32_t> idgroup, igroupid, idgroup_tmp, color;
     std::vector<Node> node;
     int32_t nnodes = 0, nnodesmax = 0;
+    int nthreads_ = 1;
+    // levels with at least this many colour components treat them in parallel; child
+    // subtrees with at least this many groups become tasks
+    static constexpr int32_t PAR_COMPONENTS = 16, TASK_MIN_GROUPS = 2;
+    // first error raised inside a task (tasks cannot throw across the parallel region)
+    bool failed = false;
+    std::string error;
+
+    void record_error(const char *what)
+    {
+#pragma omp critical(ahop_tree_error)
+        {
+            if (!failed) {
+                failed = true;
+                error = what;
+            }
+        }
+    }
 
     Builder(const TreeInputs &i, const TreeParams &pp) : in(i), p(pp)
     {
@@ -54,17 +76,40 @@ struct Builder {
     double rho_sad(int32_t idx) const { return in.sad_rho[idx - 1]; }
 
     // member linked lists: find_local_maxima f90:847-861 (prepend in index order)
-    void link_members()
+    // Each list is the group's members in decreasing index.  Built in parallel: every thread
+    // prepends the members of its contiguous index range to its own per-group lists, then
+    // each group's lists are chained from the highest range to the lowest.
+    void link_members(int nt)
     {
         firstpart.assign(ngroups + 1, 0);
         idpart_adapt.assign(npart + 1, 0);
-        for (int64_t ipar = 1; ipar <= npart; ++ipar) {
-            if (in.density[ipar - 1] <= p.rho_threshold) continue;
-            int32_t g = in.gid[ipar - 1];
-            if (g > 0) {
-                idpart_adapt[ipar] = firstpart[g];
-                firstpart[g] = (int32_t)ipar;
+        const int T = (int)std::max<int64_t>(1, std::min<int64_t>(nt, npart / 65536));
+        std::vector<int32_t> head((size_t)T * (ngroups + 1), 0), tail((size_t)T * (ngroups + 1), 0);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+        for (int t = 0; t < T; ++t) {
+            const int64_t lo = 1 + npart * t / T, hi = npart * (t + 1) / T;
+            int32_t *h = &head[(size_t)t * (ngroups + 1)], *tl = &tail[(size_t)t * (ngroups + 1)];
+            for (int64_t ipar = lo; ipar <= hi; ++ipar) {
+                if (in.density[ipar - 1] <= p.rho_threshold) continue;
+                int32_t g = in.gid[ipar - 1];
+                if (g > 0) {
+                    if (h[g] == 0) tl[g] = (int32_t)ipar;
+                    idpart_adapt[ipar] = h[g];
+                    h[g] = (int32_t)ipar;
+                }
             }
+        }
+#pragma omp parallel for num_threads(nt) schedule(static, 1024)
+        for (int32_t g = 1; g <= ngroups; ++g) {
+            int32_t first = 0, last = 0;
+            for (int t = T - 1; t >= 0; --t) {
+                const int32_t h = head[(size_t)t * (ngroups + 1) + g];
+                if (h == 0) continue;
+                if (first == 0) first = h;
+                else idpart_adapt[last] = h;
+                last = tail[(size_t)t * (ngroups + 1) + g];
+            }
+            firstpart[g] = first;
         }
     }
 
@@ -180,10 +225,78 @@ struct Builder {
 
     int32_t reserve(int32_t nnew)
     {
-        int32_t first = nnodes + 1;
-        nnodes += nnew;
-        if (nnodes > nnodesmax) throw std::runtime_error("create nodes: nnodes > nnodesmax");
-        return first;
+        int32_t before;
+#pragma omp atomic capture
+        {
+            before = nnodes;
+            nnodes += nnew;
+        }
+        if (before + nnew > nnodesmax) throw std::runtime_error("create nodes: nnodes > nnodesmax");
+        return before + 1;
+    }
+
+    // a child subtree as a task body: errors are recorded, not thrown
+    void create_nodes_task(double rhot, int32_t inode, int32_t igr1, int32_t igr2)
+    {
+        if (failed) return;
+        try {
+            create_nodes(rhot, inode, igr1, igr2, true);
+        } catch (const std::exception &e) {
+            record_error(e.what());
+        }
+    }
+
+    // treat_particles over the groups of one colour component, summed as the reference does
+    void treat_component(int32_t ic, double rhot, const std::vector<int32_t> &igrposnew,
+                         std::vector<double> &posgg, std::vector<int32_t> &massg,
+                         std::vector<double> &truemassg, std::vector<double> &densmaxg,
+                         std::vector<int32_t> &mass_compg, std::vector<double> &rsquare,
+                         std::vector<double> &densmoy, std::vector<char> &ifok)
+    {
+        double posg[3] = {0, 0, 0};
+        double posref[3] = {0, 0, 0};
+        int32_t mass_loc = 0;
+        double truemass = 0.0, rsquareg = 0.0, densmoyg = 0.0;
+        int32_t igr1out = igrposnew[ic - 1] + 1, igr2out = igrposnew[ic];
+        double densmaxgroup = -1.0;
+        int32_t mass_comp = 0;
+        double densmoy_comp_max = -1.0;
+        int32_t igroupref = 0;
+        for (int32_t igr = igr1out; igr <= igr2out; ++igr) {
+            int32_t g = idgroup[igr];
+            densmaxgroup = std::max(densmaxgroup, densityg(g));
+            double posgtmp[3], rsquaretmp, densmoytmp, truemasstmp;
+            int32_t masstmp;
+            treat(g, rhot, posgtmp, masstmp, igroupref, posref, rsquaretmp, densmoytmp,
+                  truemasstmp);
+            posg[0] = posg[0] + posgtmp[0];
+            posg[1] = posg[1] + posgtmp[1];
+            posg[2] = posg[2] + posgtmp[2];
+            rsquareg = rsquareg + rsquaretmp;
+            mass_loc = mass_loc + masstmp;
+            truemass = truemass + truemasstmp;
+            densmoyg = densmoyg + densmoytmp;
+            mass_comp = std::max(mass_comp, masstmp);
+            if (masstmp > 0) {
+                densmoytmp = densmoytmp / (double)masstmp;
+                densmoy_comp_max =
+                    std::max(densmoy_comp_max,
+                             densmoytmp / (1.0 + p.fudge / std::sqrt((double)masstmp)));
+            }
+        }
+        massg[ic] = mass_loc;
+        truemassg[ic] = truemass;
+        for (int d = 0; d < 3; ++d) posgg[3 * ic + d] = posg[d];
+        densmaxg[ic] = densmaxgroup;
+        mass_compg[ic] = mass_comp;
+        rsquare[ic] = std::sqrt(std::fabs(
+            (truemass * rsquareg - (posg[0] * posg[0] + posg[1] * posg[1] + posg[2] * posg[2])) /
+            (truemass * truemass)));
+        densmoy[ic] = densmoyg / (double)mass_loc;
+        ifok[ic] = mass_loc >= p.nmembthresh &&
+                   (densmoy[ic] > rhot * (1.0 + p.fudge / std::sqrt((double)mass_loc)) ||
+                    densmoy_comp_max > rhot) &&
+                   densmaxg[ic] >= p.alphap * densmoy[ic] && rsquare[ic] >= p.epsilon;
     }
 
     // omp_dev_create_nodes_task f90:1245-1628, run depth first
@@ -244,51 +357,24 @@ struct Builder {
             std::vector<int32_t> massg(inc_color_tot + 1), mass_compg(inc_color_tot + 1),
                 ok_color(inc_color_tot + 1);
             std::vector<char> ifok(inc_color_tot + 1, 0);
-            double posref[3] = {0, 0, 0};
-            for (int32_t ic = 1; ic <= inc_color_tot; ++ic) {
-                double posg[3] = {0, 0, 0};
-                int32_t mass_loc = 0;
-                double truemass = 0.0, rsquareg = 0.0, densmoyg = 0.0;
-                int32_t igr1out = igrposnew[ic - 1] + 1, igr2out = igrposnew[ic];
-                double densmaxgroup = -1.0;
-                int32_t mass_comp = 0;
-                double densmoy_comp_max = -1.0;
-                int32_t igroupref = 0;
-                for (int32_t igr = igr1out; igr <= igr2out; ++igr) {
-                    int32_t g = idgroup[igr];
-                    densmaxgroup = std::max(densmaxgroup, densityg(g));
-                    double posgtmp[3], rsquaretmp, densmoytmp, truemasstmp;
-                    int32_t masstmp;
-                    treat(g, rhot, posgtmp, masstmp, igroupref, posref, rsquaretmp, densmoytmp,
-                          truemasstmp);
-                    posg[0] = posg[0] + posgtmp[0];
-                    posg[1] = posg[1] + posgtmp[1];
-                    posg[2] = posg[2] + posgtmp[2];
-                    rsquareg = rsquareg + rsquaretmp;
-                    mass_loc = mass_loc + masstmp;
-                    truemass = truemass + truemasstmp;
-                    densmoyg = densmoyg + densmoytmp;
-                    mass_comp = std::max(mass_comp, masstmp);
-                    if (masstmp > 0) {
-                        densmoytmp = densmoytmp / (double)masstmp;
-                        densmoy_comp_max =
-                            std::max(densmoy_comp_max,
-                                     densmoytmp / (1.0 + p.fudge / std::sqrt((double)masstmp)));
+            if (inc_color_tot >= PAR_COMPONENTS) {
+                // components touch disjoint groups and particles
+#pragma omp taskloop grainsize(4) shared(igrposnew, posgg, massg, truemassg, densmaxg, mass_compg, \
+                                         rsquare, densmoy, ifok)
+                for (int32_t ic = 1; ic <= inc_color_tot; ++ic) {
+                    if (failed) continue;
+                    try {
+                        treat_component(ic, rhot, igrposnew, posgg, massg, truemassg, densmaxg,
+                                        mass_compg, rsquare, densmoy, ifok);
+                    } catch (const std::exception &e) {
+                        record_error(e.what());
                     }
                 }
-                massg[ic] = mass_loc;
-                truemassg[ic] = truemass;
-                for (int d = 0; d < 3; ++d) posgg[3 * ic + d] = posg[d];
-                densmaxg[ic] = densmaxgroup;
-                mass_compg[ic] = mass_comp;
-                rsquare[ic] = std::sqrt(std::fabs(
-                    (truemass * rsquareg - (posg[0] * posg[0] + posg[1] * posg[1] + posg[2] * posg[2])) /
-                    (truemass * truemass)));
-                densmoy[ic] = densmoyg / (double)mass_loc;
-                ifok[ic] = mass_loc >= p.nmembthresh &&
-                           (densmoy[ic] > rhot * (1.0 + p.fudge / std::sqrt((double)mass_loc)) ||
-                            densmoy_comp_max > rhot) &&
-                           densmaxg[ic] >= p.alphap * densmoy[ic] && rsquare[ic] >= p.epsilon;
+                if (failed) throw std::runtime_error(error);
+            } else {
+                for (int32_t ic = 1; ic <= inc_color_tot; ++ic)
+                    treat_component(ic, rhot, igrposnew, posgg, massg, truemassg, densmaxg,
+                                    mass_compg, rsquare, densmoy, ifok);
             }
             int32_t nsisters = 0, icolor_ref = 0;
             for (int32_t ic = 1; ic <= inc_color_tot; ++ic)
@@ -324,8 +410,12 @@ struct Builder {
                 node[inode].firstchild = inode1 + nsisters - 1;
                 for (int32_t is = 1; is <= nsisters; ++is) {
                     int32_t ic = ok_color[is];
-                    double trho = rhot * (1.0 + p.fudge / std::sqrt((double)mass_compg[ic]));
-                    create_nodes(trho, inode1 + is - 1, igrposnew[ic - 1] + 1, igrposnew[ic], true);
+                    const double trho = rhot * (1.0 + p.fudge / std::sqrt((double)mass_compg[ic]));
+                    const int32_t child = inode1 + is - 1;
+                    const int32_t a = igrposnew[ic - 1] + 1, b = igrposnew[ic];
+                    // sibling subtrees are independent; small ones run inline
+#pragma omp task firstprivate(trho, child, a, b) if (b - a + 1 >= TASK_MIN_GROUPS)
+                    create_nodes_task(trho, child, a, b);
                 }
                 return;
             } else if (nsisters == 1) {
@@ -380,6 +470,7 @@ struct Builder {
             if (nd.sister > 0) nd.sister = old_to_new[nd.sister];
             sorted[i] = nd;
         }
+#pragma omp parallel for num_threads(nthreads_) schedule(static, 65536)
         for (int64_t i = 1; i <= npart; ++i)
             if (liste_parts[i] > 0) liste_parts[i] = old_to_new[liste_parts[i]];
         for (int32_t i = 0; i <= nnodes; ++i) node[i] = sorted[i];
@@ -387,7 +478,9 @@ struct Builder {
 
     void run(TreeOutputs &out)
     {
-        link_members();
+        const int nt = p.nthreads > 0 ? p.nthreads : omp_get_max_threads();
+        nthreads_ = nt;
+        link_members(nt);
         nhnei.assign(ngroups + 1, 0);
         offset1.assign(ngroups + 2, 1);
         for (int32_t g = 1; g <= ngroups; ++g) {
@@ -404,6 +497,7 @@ struct Builder {
         double rhot = p.rho_threshold;
         nnodes = 0;
         node[0].mother = 0;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 64)
         for (int32_t g = 1; g <= ngroups; ++g) treat_linkonly(g, rhot);
         node[0].mass = 0;
         node[0].truemass = 0.0;
@@ -420,7 +514,18 @@ struct Builder {
             idgroup[g] = g;
             igroupid[g] = g;
         }
-        if (ngroups > 0) create_nodes(rhot, 0, 1, ngroups, false);
+        if (ngroups > 0) {
+#pragma omp parallel num_threads(nt)
+#pragma omp single
+            {
+                try {
+                    create_nodes(rhot, 0, 1, ngroups, false);
+                } catch (const std::exception &e) {
+                    record_error(e.what());
+                }
+            }   // the end of the parallel region waits for every task
+            if (failed) throw std::runtime_error(error);
+        }
         reorder();
         out.nnodes = nnodes;
         out.liste_parts.assign(liste_parts.begin() + 1, liste_parts.end());

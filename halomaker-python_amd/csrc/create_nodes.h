@@ -10,6 +10,7 @@ struct TreeParams {
     int32_t nmembthresh = 20;
     double xlong = 1.0, ylong = 1.0, zlong = 1.0;
     double xlongs2 = 0.5, ylongs2 = 0.5, zlongs2 = 0.5;
+    int32_t nthreads = 0;   // OpenMP threads of the stage (0: the OpenMP default)
 };
 
 // Inputs in original particle order (host memory).
