@@ -395,70 +395,189 @@ __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int s
     hi[e * WAVE + lane] = id;
 }
 
-// Packed heap of the mixed kernel: one 64-bit LDS word per entry, (FP32 key bits, id), so
-// that a sift step moves and reads an entry with one LDS instruction.
-template <bool CHECKED = true>
-__device__ __forceinline__ void heap_sift_packed(uint2 *hp, int lane, int size, float d,
-                                                 int32_t id, int e = 0)
+// Heap of the mixed kernel: packed (FP32 key bits, id) entries with the top two levels in
+// registers.  Entry 0 (the root: the largest key kept) and its HEAP_D children live in VGPRs
+// (rk/ri, RT entries), entries e >= RT in LDS at hp[(e - RT) * WAVE + lane].  A replacement
+// at the root then finds the largest child without LDS and reads one level of children from
+// LDS (the only level below the registers while the heap has at most 73 entries), and the
+// LDS part is RT entries per lane smaller.  The register arrays are indexed with
+// compile-time indices only (selects over all RT entries), so they stay in VGPRs.
+constexpr int RT = HEAP_D + 1;
+// vector types rather than arrays: element reads and writes are register inserts and
+// extracts, never loads from a stack slot, so the compiler cannot turn a select over
+// entries into an indexed memory access
+typedef float rtk_t __attribute__((ext_vector_type(RT)));
+typedef uint32_t rti_t __attribute__((ext_vector_type(RT)));
+
+__device__ __forceinline__ float rt_key(const rtk_t &rk, const uint2 *hp, int lane, int e)
+{
+    if (e >= RT) return __uint_as_float(hp[(e - RT) * WAVE + lane].x);
+    float v = rk[0];
+#pragma unroll
+    for (int t = 1; t < RT; ++t) v = (e == t) ? rk[t] : v;
+    return v;
+}
+
+__device__ __forceinline__ uint32_t rt_id(const rti_t &ri, const uint2 *hp, int lane, int e)
+{
+    if (e >= RT) return hp[(e - RT) * WAVE + lane].y;
+    uint32_t v = ri[0];
+#pragma unroll
+    for (int t = 1; t < RT; ++t) v = (e == t) ? ri[t] : v;
+    return v;
+}
+
+__device__ __forceinline__ void rt_set(rtk_t &rk, rti_t &ri, uint2 *hp, int lane,
+                                       int e, float k, uint32_t id)
+{
+    if (e >= RT) {
+        hp[(e - RT) * WAVE + lane] = make_uint2(__float_as_uint(k), id);
+    } else {
+#pragma unroll
+        for (int t = 0; t < RT; ++t) {
+            const bool m = e == t;
+            rk[t] = m ? k : rk[t];
+            ri[t] = m ? id : ri[t];
+        }
+    }
+}
+
+// the largest of the HEAP_D children c0.. of an entry, all in LDS (c0 >= RT); CHECKED = false
+// reads them all (the caller keeps entries past `size` at -inf)
+template <bool CHECKED>
+__device__ __forceinline__ void rt_max_children(const uint2 *hp, int lane, int size, int c0,
+                                                float &kc, uint32_t &idc, int &ic)
+{
+    uint2 ch[HEAP_D];
+#pragma unroll
+    for (int t = 0; t < HEAP_D; ++t) {
+        const int c = (!CHECKED || c0 + t < size) ? c0 + t : c0;
+        ch[t] = hp[(c - RT) * WAVE + lane];
+    }
+    kc = __uint_as_float(ch[0].x);
+    idc = ch[0].y;
+    ic = c0;
+#pragma unroll
+    for (int t = 1; t < HEAP_D; ++t) {
+        const float kt = __uint_as_float(ch[t].x);
+        if ((!CHECKED || c0 + t < size) && kt > kc) {
+            kc = kt;
+            idc = ch[t].y;
+            ic = c0 + t;
+        }
+    }
+}
+
+// sift (d, id) down from the LDS entry e >= RT
+template <bool CHECKED>
+__device__ __forceinline__ void rt_sift_lds(uint2 *hp, int lane, int size, float d, uint32_t id,
+                                            int e)
 {
     while (true) {
         const int c0 = HEAP_D * e + 1;
         if (c0 >= size) break;
-        uint2 ch[HEAP_D];
-#pragma unroll
-        for (int t = 0; t < HEAP_D; ++t) {
-            const int c = (!CHECKED || c0 + t < size) ? c0 + t : c0;
-            ch[t] = hp[c * WAVE + lane];
-        }
-        float kc = __uint_as_float(ch[0].x);
-        uint32_t idc = ch[0].y;
-        int ic = c0;
-#pragma unroll
-        for (int t = 1; t < HEAP_D; ++t) {
-            const float kt = __uint_as_float(ch[t].x);
-            if ((!CHECKED || c0 + t < size) && kt > kc) {
-                kc = kt;
-                idc = ch[t].y;
-                ic = c0 + t;
-            }
-        }
+        float kc;
+        uint32_t idc;
+        int ic;
+        rt_max_children<CHECKED>(hp, lane, size, c0, kc, idc, ic);
         if (!(kc > d)) break;
-        hp[e * WAVE + lane] = make_uint2(__float_as_uint(kc), idc);
+        hp[(e - RT) * WAVE + lane] = make_uint2(__float_as_uint(kc), idc);
         e = ic;
     }
-    hp[e * WAVE + lane] = make_uint2(__float_as_uint(d), (uint32_t)id);
+    hp[(e - RT) * WAVE + lane] = make_uint2(__float_as_uint(d), id);
 }
 
-// Heap entry access for both kernels: packed (mixed) or key and id arrays (FP64).
-template <bool PACKED, typename KeyT>
-__device__ __forceinline__ KeyT heap_key(const uint2 *hp, const KeyT *hk, int idx)
+// sift (d, id) down from the register entry c, 1 <= c < RT (its children are in LDS)
+template <bool CHECKED>
+__device__ __forceinline__ void rt_sift_l1(rtk_t &rk, rti_t &ri, uint2 *hp,
+                                           int lane, int size, float d, uint32_t id, int c)
 {
-    if constexpr (PACKED) return __uint_as_float(hp[idx].x);
-    else return hk[idx];
+    const int c0 = HEAP_D * c + 1;
+    float vk = d;
+    uint32_t vi = id;
+    if (c0 < size) {
+        float kc;
+        uint32_t idc;
+        int ic;
+        rt_max_children<CHECKED>(hp, lane, size, c0, kc, idc, ic);
+        if (kc > d) {
+            vk = kc;
+            vi = idc;
+            rt_sift_lds<CHECKED>(hp, lane, size, d, id, ic);
+        }
+    }
+#pragma unroll
+    for (int t = 1; t < RT; ++t) {
+        const bool m = c == t;
+        rk[t] = m ? vk : rk[t];
+        ri[t] = m ? vi : ri[t];
+    }
+}
+
+// sift (d, id) down from the root
+template <bool CHECKED>
+__device__ __forceinline__ void rt_sift_root(rtk_t &rk, rti_t &ri, uint2 *hp,
+                                             int lane, int size, float d, uint32_t id)
+{
+    float kc = (!CHECKED || size > 1) ? rk[1] : -INFINITY;
+    uint32_t idc = ri[1];
+    int ic = 1;
+#pragma unroll
+    for (int t = 2; t < RT; ++t)
+        if ((!CHECKED || t < size) && rk[t] > kc) {
+            kc = rk[t];
+            idc = ri[t];
+            ic = t;
+        }
+    if (kc > d) {
+        rk[0] = kc;
+        ri[0] = idc;
+        rt_sift_l1<CHECKED>(rk, ri, hp, lane, size, d, id, ic);
+    } else {
+        rk[0] = d;
+        ri[0] = id;
+    }
+}
+
+// Heap entry access for both kernels: register-top packed (mixed) or key and id arrays
+// (FP64).
+template <bool PACKED, typename KeyT>
+__device__ __forceinline__ KeyT heap_key(const rtk_t &rk, const uint2 *hp, const KeyT *hk,
+                                         int lane, int e)
+{
+    if constexpr (PACKED) return rt_key(rk, hp, lane, e);
+    else return hk[e * WAVE + lane];
 }
 template <bool PACKED, typename KeyT>
-__device__ __forceinline__ int32_t heap_id(const uint2 *hp, const int32_t *hi, int idx)
+__device__ __forceinline__ int32_t heap_id(const rti_t &ri, const uint2 *hp,
+                                           const int32_t *hi, int lane, int e)
 {
-    if constexpr (PACKED) return (int32_t)hp[idx].y;
-    else return hi[idx];
+    if constexpr (PACKED) return (int32_t)rt_id(ri, hp, lane, e);
+    else return hi[e * WAVE + lane];
 }
 template <bool PACKED, typename KeyT>
-__device__ __forceinline__ void heap_set(uint2 *hp, KeyT *hk, int32_t *hi, int idx, KeyT k,
-                                         int32_t id)
+__device__ __forceinline__ void heap_set(rtk_t &rk, rti_t &ri, uint2 *hp, KeyT *hk,
+                                         int32_t *hi, int lane, int e, KeyT k, int32_t id)
 {
     if constexpr (PACKED) {
-        hp[idx] = make_uint2(__float_as_uint((float)k), (uint32_t)id);
+        rt_set(rk, ri, hp, lane, e, (float)k, (uint32_t)id);
     } else {
-        hk[idx] = k;
-        hi[idx] = id;
+        hk[e * WAVE + lane] = k;
+        hi[e * WAVE + lane] = id;
     }
 }
 template <bool PACKED, bool CHECKED, typename KeyT>
-__device__ __forceinline__ void heap_sift_any(uint2 *hp, KeyT *hk, int32_t *hi, int lane,
-                                              int size, KeyT d, int32_t id, int e = 0)
+__device__ __forceinline__ void heap_sift_any(rtk_t &rk, rti_t &ri, uint2 *hp,
+                                              KeyT *hk, int32_t *hi, int lane, int size, KeyT d,
+                                              int32_t id, int e)
 {
-    if constexpr (PACKED) heap_sift_packed<CHECKED>(hp, lane, size, (float)d, id, e);
-    else heap_sift<KeyT, CHECKED>(hk, hi, lane, size, d, id, e);
+    if constexpr (PACKED) {
+        if (e == 0) rt_sift_root<CHECKED>(rk, ri, hp, lane, size, (float)d, (uint32_t)id);
+        else if (e < RT) rt_sift_l1<CHECKED>(rk, ri, hp, lane, size, (float)d, (uint32_t)id, e);
+        else rt_sift_lds<CHECKED>(hp, lane, size, (float)d, (uint32_t)id, e);
+    } else {
+        heap_sift<KeyT, CHECKED>(hk, hi, lane, size, d, id, e);
+    }
 }
 
 #ifndef AHOP_FP64_NBUF

@@ -59,6 +59,15 @@ __device__ __forceinline__ float fp32_key_err(float d, float hQ, float L)
     return 8 * U32 * sqrtf(d * 1.0001f) * s + 16 * U32 * U32 * s * s + 1e-15f * L * s;
 }
 
+// a staged candidate: FP32 coordinates from the x, y, z arrays (mixed) or the FP64 double4
+template <bool MIXED>
+__device__ __forceinline__ auto stage_at(const float *sx, const float *sy, const float *sz,
+                                         const double4 *st, int idx)
+{
+    if constexpr (MIXED) return make_float4(sx[idx], sy[idx], sz[idx], 0.0f);
+    else return st[idx];
+}
+
 // The tree arrays and the sorted particles are also passed as restrict-qualified kernel
 // arguments: the kernel never writes them, and saying so lets the wave-uniform node reads
 // go through scalar loads.
@@ -71,7 +80,6 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
                                             const double4 *__restrict__ spos)
 {
     using KeyT = typename std::conditional<MIXED, float, double>::type;
-    using StageT = typename std::conditional<MIXED, float4, double4>::type;
     // the mixed heap keeps KX entries beyond the k nearest (the certificate needs the
     // (k+1)-th key) and holds KCAP + 1 entries: with KCAP a multiple of HEAP_D every child
     // index of an internal node of a heap of at most KCAP + 1 entries lies below KCAP + 1,
@@ -85,16 +93,25 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
     // staged buckets whose insertions are deferred (the FP64 heap is twice as large, so its
     // stage may be cut down to fit one more wave per CU)
     constexpr int NBUF = MIXED ? AHOP_MIXED_NBUF : AHOP_FP64_NBUF;
-    // heap storage: packed (key, id) words for the mixed kernel, key and id arrays for FP64
-    __shared__ uint2 hp[MIXED ? KHCAP * WAVE : 1];
+    // heap storage: for the mixed kernel the root and its children in registers (rk, ri) and
+    // packed (key, id) words for the other entries in LDS; key and id arrays for FP64
+    static_assert(!MIXED || KHCAP > RT, "the mixed heap keeps its top RT entries in registers");
+    __shared__ uint2 hp[MIXED ? (KHCAP - RT) * WAVE : 1];
     __shared__ KeyT hk[MIXED ? 1 : KHCAP * WAVE];
     __shared__ int32_t hi[MIXED ? 1 : KHCAP * WAVE];
-#define HK(e) heap_key<MIXED, KeyT>(hp, hk, (e) * WAVE + lane)
-#define HI(e) heap_id<MIXED, KeyT>(hp, hi, (e) * WAVE + lane)
-#define HSET(e, k, i) heap_set<MIXED, KeyT>(hp, hk, hi, (e) * WAVE + lane, (k), (i))
+    rtk_t rk = -INFINITY;
+    rti_t ri = 0u;
+#define HK(e) heap_key<MIXED, KeyT>(rk, hp, hk, lane, (e))
+#define HI(e) heap_id<MIXED, KeyT>(ri, hp, hi, lane, (e))
+#define HSET(e, k, i) heap_set<MIXED, KeyT>(rk, ri, hp, hk, hi, lane, (e), (k), (i))
 #define HSIFT(size, d, id, e, checked) \
-    heap_sift_any<MIXED, checked, KeyT>(hp, hk, hi, lane, (size), (d), (id), (e))
-    __shared__ StageT stage[NBUF * WAVE];
+    heap_sift_any<MIXED, checked, KeyT>(rk, ri, hp, hk, hi, lane, (size), (d), (id), (e))
+    // staged bucket coordinates: FP32 x, y, z arrays (mixed; three reads of four candidates
+    // each serve four candidates) or FP64 double4 (FP64)
+    __shared__ __align__(16) float sx[MIXED ? NBUF * WAVE : 4];
+    __shared__ __align__(16) float sy[MIXED ? NBUF * WAVE : 4];
+    __shared__ __align__(16) float sz[MIXED ? NBUF * WAVE : 4];
+    __shared__ double4 stage[MIXED ? 1 : NBUF * WAVE];
     __shared__ int32_t stack[STACK];
 
     const int lane = threadIdx.x;
@@ -199,7 +216,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         m1 = s1 ? mk : m1;                                                                     \
         m2 = s2 ? mk : m2;                                                                     \
         m3 = s3 ? mk : m3;                                                                     \
-        KNN_KEY(stage[bsel * WAVE + j_], dout);                                                \
+        KNN_KEY(stage_at<MIXED>(sx, sy, sz, stage, bsel * WAVE + j_), dout);                   \
         (idout) = fsel + j_;                                                                   \
     } while (0)
 
@@ -237,39 +254,65 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         nbuf = 0;                                                                              \
     } while (0)
 
+    // Evaluate the staged candidates j = 0..cnt-1 of buffer b (slots fst + j) and set bit j of
+    // mask_ where ACCEPT holds (ACCEPT sees the key d_ and the slot sj_).  The mixed kernel
+    // reads the x, y, z arrays four candidates at a time; the bits of the stale entries past
+    // cnt (every buffer has WAVE entries) are cleared afterwards.
+#define KNN_SCAN(b, fst, cnt, ACCEPT)                                                          \
+    do {                                                                                       \
+        if constexpr (MIXED) {                                                                 \
+            for (int j4_ = 0; j4_ < (cnt); j4_ += 4) {                                         \
+                const float4 X_ = *reinterpret_cast<const float4 *>(sx + (b) * WAVE + j4_);    \
+                const float4 Y_ = *reinterpret_cast<const float4 *>(sy + (b) * WAVE + j4_);    \
+                const float4 Z_ = *reinterpret_cast<const float4 *>(sz + (b) * WAVE + j4_);    \
+                const float xs_[4] = {X_.x, X_.y, X_.z, X_.w};                                 \
+                const float ys_[4] = {Y_.x, Y_.y, Y_.z, Y_.w};                                 \
+                const float zs_[4] = {Z_.x, Z_.y, Z_.z, Z_.w};                                 \
+                _Pragma("unroll") for (int t_ = 0; t_ < 4; ++t_) {                             \
+                    const int j_ = j4_ + t_;                                                   \
+                    const int sj_ = (fst) + j_;                                                \
+                    KeyT d_;                                                                   \
+                    KNN_KEY(make_float4(xs_[t_], ys_[t_], zs_[t_], 0.0f), d_);                 \
+                    if (ACCEPT) mask_ |= (1ull << j_);                                         \
+                }                                                                              \
+            }                                                                                  \
+            if ((cnt) < WAVE) mask_ &= (1ull << (cnt)) - 1ull;                                 \
+        } else {                                                                               \
+            for (int j_ = 0; j_ < (cnt); ++j_) {                                               \
+                const int sj_ = (fst) + j_;                                                    \
+                KeyT d_;                                                                       \
+                KNN_KEY(stage[(b) * WAVE + j_], d_);                                           \
+                if (ACCEPT) mask_ |= (1ull << j_);                                             \
+            }                                                                                  \
+        }                                                                                      \
+    } while (0)
+
     // Stage sorted slots [f, f+cnt) (cnt <= 64) and record each lane's accepted subset;
     // seeding accepts only the lane's own window, the traversal everything outside it.
 #define KNN_STAGE(fst, cnt, seeding)                                                           \
     do {                                                                                       \
         if (nbuf == NBUF) KNN_FLUSH();                                                         \
-        StageT *buf_ = stage + nbuf * WAVE;                                                    \
         __syncthreads();                                                                       \
         if (lane < (cnt)) {                                                                    \
             const double4 c_ = spos[(fst) + lane];                                             \
-            if constexpr (MIXED)                                                               \
-                buf_[lane] = make_float4((float)wrapd(c_.x - cx, Lx), (float)wrapd(c_.y - cy, Ly), \
-                                         (float)wrapd(c_.z - cz, Lz), 0.0f);                   \
-            else                                                                               \
-                buf_[lane] = c_;                                                               \
+            if constexpr (MIXED) {                                                             \
+                sx[nbuf * WAVE + lane] = (float)wrapd(c_.x - cx, Lx);                          \
+                sy[nbuf * WAVE + lane] = (float)wrapd(c_.y - cy, Ly);                          \
+                sz[nbuf * WAVE + lane] = (float)wrapd(c_.z - cz, Lz);                          \
+            } else {                                                                           \
+                stage[nbuf * WAVE + lane] = c_;                                                \
+            }                                                                                  \
         }                                                                                      \
         __syncthreads();                                                                       \
         uint64_t mask_ = 0;                                                                    \
         if ((seeding) || ((fst) <= wseed_hi && (fst) + (cnt) - 1 >= wseed_lo)) {               \
-            for (int j_ = 0; j_ < (cnt); ++j_) {                                               \
-                KeyT d_;                                                                       \
-                KNN_KEY(buf_[j_], d_);                                                         \
-                const int sj_ = (fst) + j_;                                                    \
-                const bool in_seed_ = sj_ >= seed_lo && sj_ <= seed_hi;                        \
-                if (d_ < tau && sj_ != q && ((seeding) ? in_seed_ : !in_seed_))                \
-                    mask_ |= (1ull << j_);                                                     \
-            }                                                                                  \
+            KNN_SCAN(nbuf, fst, cnt,                                                           \
+                     d_ < tau && sj_ != q &&                                                   \
+                         ((seeding) ? (sj_ >= seed_lo && sj_ <= seed_hi)                       \
+                                    : !(sj_ >= seed_lo && sj_ <= seed_hi)));                   \
         } else {                                                                               \
             /* the bucket lies outside every lane's seed window (and holds no query) */       \
-            for (int j_ = 0; j_ < (cnt); ++j_) {                                               \
-                KeyT d_;                                                                       \
-                KNN_KEY(buf_[j_], d_);                                                         \
-                if (d_ < tau) mask_ |= (1ull << j_);                                           \
-            }                                                                                  \
+            KNN_SCAN(nbuf, fst, cnt, d_ < tau);                                                \
         }                                                                                      \
         m0 = nbuf == 0 ? mask_ : m0;                                                           \
         m1 = nbuf == 1 ? mask_ : m1;                                                           \
@@ -305,14 +348,13 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         for (int j = seed_lo; j <= seed_hi; ++j) {
             if (j == q) continue;
             const double4 c = spos[j];
-            StageT sc;
-            if constexpr (MIXED)
-                sc = make_float4((float)wrapd(c.x - cx, Lx), (float)wrapd(c.y - cy, Ly),
-                                 (float)wrapd(c.z - cz, Lz), 0.0f);
-            else
-                sc = c;
             KeyT d;
-            KNN_KEY(sc, d);
+            if constexpr (MIXED)
+                KNN_KEY(make_float4((float)wrapd(c.x - cx, Lx), (float)wrapd(c.y - cy, Ly),
+                                    (float)wrapd(c.z - cz, Lz), 0.0f),
+                        d);
+            else
+                KNN_KEY(c, d);
             if (d < tau) KNN_INSERT(d, j);
         }
     }
@@ -384,6 +426,7 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
     if (stack_overflow && lane == 0) atomicAdd(A.overflow, 1);
 #undef KNN_NEED
 #undef KNN_STAGE
+#undef KNN_SCAN
 #undef KNN_FLUSH
 #undef KNN_INSERT
 #undef KNN_KEY
