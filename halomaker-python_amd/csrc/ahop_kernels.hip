@@ -448,12 +448,11 @@ template <bool CHECKED>
 __device__ __forceinline__ void rt_max_children(const uint2 *hp, int lane, int size, int c0,
                                                 float &kc, uint32_t &idc, int &ic)
 {
+    // one base address and constant offsets for the unchecked reads
+    const uint2 *base = hp + (c0 - RT) * WAVE + lane;
     uint2 ch[HEAP_D];
 #pragma unroll
-    for (int t = 0; t < HEAP_D; ++t) {
-        const int c = (!CHECKED || c0 + t < size) ? c0 + t : c0;
-        ch[t] = hp[(c - RT) * WAVE + lane];
-    }
+    for (int t = 0; t < HEAP_D; ++t) ch[t] = base[((!CHECKED || c0 + t < size) ? t : 0) * WAVE];
     kc = __uint_as_float(ch[0].x);
     idc = ch[0].y;
     ic = c0;
@@ -592,8 +591,12 @@ __device__ __forceinline__ void heap_sift_any(rtk_t &rk, rti_t &ri, uint2 *hp,
 #ifndef AHOP_SEED_PAD
 #define AHOP_SEED_PAD 16
 #endif
+// Staged buckets of the mixed kernel.  LDS is allocated per wave in granules of 1280 B
+// (160 KB per CU), so 25 granules (32000 B) are the budget for 5 waves per CU: at k = 64 the
+// LDS heap (56 x 64 x 8 B), the stack (512 B) and 3 buckets of x, y, z (2304 B) take
+// 31488 B; a 4th bucket (32256 B) fits only 4 waves and measured 24% slower.
 #ifndef AHOP_MIXED_NBUF
-#define AHOP_MIXED_NBUF 4
+#define AHOP_MIXED_NBUF 3
 #endif
 #include "knn_wave.h"
 
