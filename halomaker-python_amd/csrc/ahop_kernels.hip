@@ -395,185 +395,171 @@ __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int s
     hi[e * WAVE + lane] = id;
 }
 
-// Heap of the mixed kernel: packed (FP32 key bits, id) entries with the top two levels in
-// registers.  Entry 0 (the root: the largest key kept) and its HEAP_D children live in VGPRs
-// (rk/ri, RT entries), entries e >= RT in LDS at hp[(e - RT) * WAVE + lane].  A replacement
-// at the root then finds the largest child without LDS and reads one level of children from
-// LDS (the only level below the registers while the heap has at most 73 entries), and the
-// LDS part is RT entries per lane smaller.  The register arrays are indexed with
-// compile-time indices only (selects over all RT entries), so they stay in VGPRs.
-constexpr int RT = HEAP_D + 1;
+// Heap of the mixed kernel: packed (FP32 key bits, id) entries, the top two levels in
+// registers.  The root (the largest key kept) is entry 0 (k0, i0); its RC = 16 children are
+// entries 1..RC (rk, ri); every register child c has LC = 8 children, entries
+// LC * c + RT - LC + t (t < LC), which are leaves and live in LDS at
+// hp[(e - RT) * WAVE + lane].  Up to RT + LC * RC = 145 entries fit in these three levels.
+// A replacement at the root finds the largest register child without LDS and, only when
+// that child has children, reads them from LDS in one round trip; with k + 1 = 65 entries
+// only 6 of the 16 register children have children, and the LDS part is 48 entries per
+// lane.  The register vectors are indexed with compile-time indices only (selects).
+constexpr int RC = 16;
+constexpr int RT = RC + 1;
+constexpr int LC = 8;
 // vector types rather than arrays: element reads and writes are register inserts and
 // extracts, never loads from a stack slot, so the compiler cannot turn a select over
 // entries into an indexed memory access
-typedef float rtk_t __attribute__((ext_vector_type(RT)));
-typedef uint32_t rti_t __attribute__((ext_vector_type(RT)));
+typedef float rtk_t __attribute__((ext_vector_type(RC)));
+typedef uint32_t rti_t __attribute__((ext_vector_type(RC)));
 
-__device__ __forceinline__ float rt_key(const rtk_t &rk, const uint2 *hp, int lane, int e)
+// first child of register entry c (1 <= c <= RC)
+__device__ __forceinline__ int rt_child0(int c) { return LC * c + RT - LC; }
+
+struct RegTop {
+    float k0;
+    uint32_t i0;
+    rtk_t rk;
+    rti_t ri;
+};
+
+__device__ __forceinline__ float rt_key(const RegTop &h, const uint2 *hp, int lane, int e)
 {
     if (e >= RT) return __uint_as_float(hp[(e - RT) * WAVE + lane].x);
-    float v = rk[0];
+    float v = h.k0;
 #pragma unroll
-    for (int t = 1; t < RT; ++t) v = (e == t) ? rk[t] : v;
+    for (int t = 0; t < RC; ++t) v = (e == t + 1) ? h.rk[t] : v;
     return v;
 }
 
-__device__ __forceinline__ uint32_t rt_id(const rti_t &ri, const uint2 *hp, int lane, int e)
+__device__ __forceinline__ uint32_t rt_id(const RegTop &h, const uint2 *hp, int lane, int e)
 {
     if (e >= RT) return hp[(e - RT) * WAVE + lane].y;
-    uint32_t v = ri[0];
+    uint32_t v = h.i0;
 #pragma unroll
-    for (int t = 1; t < RT; ++t) v = (e == t) ? ri[t] : v;
+    for (int t = 0; t < RC; ++t) v = (e == t + 1) ? h.ri[t] : v;
     return v;
 }
 
-__device__ __forceinline__ void rt_set(rtk_t &rk, rti_t &ri, uint2 *hp, int lane,
-                                       int e, float k, uint32_t id)
+__device__ __forceinline__ void rt_set(RegTop &h, uint2 *hp, int lane, int e, float k, uint32_t id)
 {
     if (e >= RT) {
         hp[(e - RT) * WAVE + lane] = make_uint2(__float_as_uint(k), id);
     } else {
+        h.k0 = e == 0 ? k : h.k0;
+        h.i0 = e == 0 ? id : h.i0;
 #pragma unroll
-        for (int t = 0; t < RT; ++t) {
-            const bool m = e == t;
-            rk[t] = m ? k : rk[t];
-            ri[t] = m ? id : ri[t];
+        for (int t = 0; t < RC; ++t) {
+            const bool m = e == t + 1;
+            h.rk[t] = m ? k : h.rk[t];
+            h.ri[t] = m ? id : h.ri[t];
         }
     }
 }
 
-// the largest of the HEAP_D children c0.. of an entry, all in LDS (c0 >= RT); CHECKED = false
-// reads them all (the caller keeps entries past `size` at -inf)
+// sift (d, id) down from register entry c, 1 <= c <= RC: the largest of its LDS children
+// (read all LC of them unchecked: the caller keeps entries past `size` at -inf) moves up
+// if it exceeds d, and d takes its place
 template <bool CHECKED>
-__device__ __forceinline__ void rt_max_children(const uint2 *hp, int lane, int size, int c0,
-                                                float &kc, uint32_t &idc, int &ic)
+__device__ __forceinline__ void rt_sift_l1(RegTop &h, uint2 *hp, int lane, int size, float d,
+                                           uint32_t id, int c)
 {
-    // one base address and constant offsets for the unchecked reads
-    const uint2 *base = hp + (c0 - RT) * WAVE + lane;
-    uint2 ch[HEAP_D];
-#pragma unroll
-    for (int t = 0; t < HEAP_D; ++t) ch[t] = base[((!CHECKED || c0 + t < size) ? t : 0) * WAVE];
-    kc = __uint_as_float(ch[0].x);
-    idc = ch[0].y;
-    ic = c0;
-#pragma unroll
-    for (int t = 1; t < HEAP_D; ++t) {
-        const float kt = __uint_as_float(ch[t].x);
-        if ((!CHECKED || c0 + t < size) && kt > kc) {
-            kc = kt;
-            idc = ch[t].y;
-            ic = c0 + t;
-        }
-    }
-}
-
-// sift (d, id) down from the LDS entry e >= RT
-template <bool CHECKED>
-__device__ __forceinline__ void rt_sift_lds(uint2 *hp, int lane, int size, float d, uint32_t id,
-                                            int e)
-{
-    while (true) {
-        const int c0 = HEAP_D * e + 1;
-        if (c0 >= size) break;
-        float kc;
-        uint32_t idc;
-        int ic;
-        rt_max_children<CHECKED>(hp, lane, size, c0, kc, idc, ic);
-        if (!(kc > d)) break;
-        hp[(e - RT) * WAVE + lane] = make_uint2(__float_as_uint(kc), idc);
-        e = ic;
-    }
-    hp[(e - RT) * WAVE + lane] = make_uint2(__float_as_uint(d), id);
-}
-
-// sift (d, id) down from the register entry c, 1 <= c < RT (its children are in LDS)
-template <bool CHECKED>
-__device__ __forceinline__ void rt_sift_l1(rtk_t &rk, rti_t &ri, uint2 *hp,
-                                           int lane, int size, float d, uint32_t id, int c)
-{
-    const int c0 = HEAP_D * c + 1;
+    const int c0 = rt_child0(c);
     float vk = d;
     uint32_t vi = id;
     if (c0 < size) {
-        float kc;
-        uint32_t idc;
-        int ic;
-        rt_max_children<CHECKED>(hp, lane, size, c0, kc, idc, ic);
+        // one base address and constant offsets for the unchecked reads
+        const uint2 *base = hp + (c0 - RT) * WAVE + lane;
+        uint2 ch[LC];
+#pragma unroll
+        for (int t = 0; t < LC; ++t) ch[t] = base[((!CHECKED || c0 + t < size) ? t : 0) * WAVE];
+        float kc = __uint_as_float(ch[0].x);
+        uint32_t idc = ch[0].y;
+        int tc = 0;
+#pragma unroll
+        for (int t = 1; t < LC; ++t) {
+            const float kt = __uint_as_float(ch[t].x);
+            if ((!CHECKED || c0 + t < size) && kt > kc) {
+                kc = kt;
+                idc = ch[t].y;
+                tc = t;
+            }
+        }
         if (kc > d) {
             vk = kc;
             vi = idc;
-            rt_sift_lds<CHECKED>(hp, lane, size, d, id, ic);
+            hp[(c0 + tc - RT) * WAVE + lane] = make_uint2(__float_as_uint(d), id);
         }
     }
 #pragma unroll
-    for (int t = 1; t < RT; ++t) {
-        const bool m = c == t;
-        rk[t] = m ? vk : rk[t];
-        ri[t] = m ? vi : ri[t];
+    for (int t = 0; t < RC; ++t) {
+        const bool m = c == t + 1;
+        h.rk[t] = m ? vk : h.rk[t];
+        h.ri[t] = m ? vi : h.ri[t];
     }
 }
 
 // sift (d, id) down from the root
 template <bool CHECKED>
-__device__ __forceinline__ void rt_sift_root(rtk_t &rk, rti_t &ri, uint2 *hp,
-                                             int lane, int size, float d, uint32_t id)
+__device__ __forceinline__ void rt_sift_root(RegTop &h, uint2 *hp, int lane, int size, float d,
+                                             uint32_t id)
 {
-    float kc = (!CHECKED || size > 1) ? rk[1] : -INFINITY;
-    uint32_t idc = ri[1];
+    float kc = (!CHECKED || size > 1) ? h.rk[0] : -INFINITY;
+    uint32_t idc = h.ri[0];
     int ic = 1;
 #pragma unroll
-    for (int t = 2; t < RT; ++t)
-        if ((!CHECKED || t < size) && rk[t] > kc) {
-            kc = rk[t];
-            idc = ri[t];
-            ic = t;
+    for (int t = 1; t < RC; ++t)
+        if ((!CHECKED || t + 1 < size) && h.rk[t] > kc) {
+            kc = h.rk[t];
+            idc = h.ri[t];
+            ic = t + 1;
         }
     if (kc > d) {
-        rk[0] = kc;
-        ri[0] = idc;
-        rt_sift_l1<CHECKED>(rk, ri, hp, lane, size, d, id, ic);
+        h.k0 = kc;
+        h.i0 = idc;
+        rt_sift_l1<CHECKED>(h, hp, lane, size, d, id, ic);
     } else {
-        rk[0] = d;
-        ri[0] = id;
+        h.k0 = d;
+        h.i0 = id;
     }
 }
 
 // Heap entry access for both kernels: register-top packed (mixed) or key and id arrays
 // (FP64).
 template <bool PACKED, typename KeyT>
-__device__ __forceinline__ KeyT heap_key(const rtk_t &rk, const uint2 *hp, const KeyT *hk,
+__device__ __forceinline__ KeyT heap_key(const RegTop &h, const uint2 *hp, const KeyT *hk,
                                          int lane, int e)
 {
-    if constexpr (PACKED) return rt_key(rk, hp, lane, e);
+    if constexpr (PACKED) return rt_key(h, hp, lane, e);
     else return hk[e * WAVE + lane];
 }
 template <bool PACKED, typename KeyT>
-__device__ __forceinline__ int32_t heap_id(const rti_t &ri, const uint2 *hp,
-                                           const int32_t *hi, int lane, int e)
+__device__ __forceinline__ int32_t heap_id(const RegTop &h, const uint2 *hp, const int32_t *hi,
+                                           int lane, int e)
 {
-    if constexpr (PACKED) return (int32_t)rt_id(ri, hp, lane, e);
+    if constexpr (PACKED) return (int32_t)rt_id(h, hp, lane, e);
     else return hi[e * WAVE + lane];
 }
 template <bool PACKED, typename KeyT>
-__device__ __forceinline__ void heap_set(rtk_t &rk, rti_t &ri, uint2 *hp, KeyT *hk,
-                                         int32_t *hi, int lane, int e, KeyT k, int32_t id)
+__device__ __forceinline__ void heap_set(RegTop &h, uint2 *hp, KeyT *hk, int32_t *hi, int lane,
+                                         int e, KeyT k, int32_t id)
 {
     if constexpr (PACKED) {
-        rt_set(rk, ri, hp, lane, e, (float)k, (uint32_t)id);
+        rt_set(h, hp, lane, e, (float)k, (uint32_t)id);
     } else {
         hk[e * WAVE + lane] = k;
         hi[e * WAVE + lane] = id;
     }
 }
+// sift from entry e: the root, a register child, or (FP64) any entry; the mixed kernel's
+// LDS entries are leaves and are never sifted from
 template <bool PACKED, bool CHECKED, typename KeyT>
-__device__ __forceinline__ void heap_sift_any(rtk_t &rk, rti_t &ri, uint2 *hp,
-                                              KeyT *hk, int32_t *hi, int lane, int size, KeyT d,
-                                              int32_t id, int e)
+__device__ __forceinline__ void heap_sift_any(RegTop &h, uint2 *hp, KeyT *hk, int32_t *hi,
+                                              int lane, int size, KeyT d, int32_t id, int e)
 {
     if constexpr (PACKED) {
-        if (e == 0) rt_sift_root<CHECKED>(rk, ri, hp, lane, size, (float)d, (uint32_t)id);
-        else if (e < RT) rt_sift_l1<CHECKED>(rk, ri, hp, lane, size, (float)d, (uint32_t)id, e);
-        else rt_sift_lds<CHECKED>(hp, lane, size, (float)d, (uint32_t)id, e);
+        if (e == 0) rt_sift_root<CHECKED>(h, hp, lane, size, (float)d, (uint32_t)id);
+        else rt_sift_l1<CHECKED>(h, hp, lane, size, (float)d, (uint32_t)id, e);
     } else {
         heap_sift<KeyT, CHECKED>(hk, hi, lane, size, d, id, e);
     }
@@ -592,11 +578,12 @@ __device__ __forceinline__ void heap_sift_any(rtk_t &rk, rti_t &ri, uint2 *hp,
 #define AHOP_SEED_PAD 16
 #endif
 // Staged buckets of the mixed kernel.  LDS is allocated per wave in granules of 1280 B
-// (160 KB per CU), so 25 granules (32000 B) are the budget for 5 waves per CU: at k = 64 the
-// LDS heap (56 x 64 x 8 B), the stack (512 B) and 3 buckets of x, y, z (2304 B) take
-// 31488 B; a 4th bucket (32256 B) fits only 4 waves and measured 24% slower.
+// (160 KB per CU): 21 granules (26880 B) are the budget for 6 waves per CU, 25 for 5.  At
+// k = 64 the LDS heap (48 x 64 x 8 B), the stack (512 B) and 2 buckets of x, y, z (1536 B)
+// take 26624 B, 6 waves; a 3rd bucket fits only 5 and measured 25% slower (at C3,
+// 16.8 M particles: 138 ms against 172 ms).
 #ifndef AHOP_MIXED_NBUF
-#define AHOP_MIXED_NBUF 3
+#define AHOP_MIXED_NBUF 2
 #endif
 #include "knn_wave.h"
 

@@ -3,7 +3,8 @@
 // One wave handles 64 consecutive Morton-sorted query particles, one query per lane.
 // The wave walks the binary radix tree together (a node is entered when any lane needs
 // it), stages the particles of each leaf bucket in LDS and every lane keeps the k (mixed:
-// k+1) nearest candidates seen so far in an 8-ary max-heap in LDS.
+// k+1) nearest candidates seen so far in a max-heap: mixed, the root and its 16 children
+// in registers and their children in LDS; FP64, an 8-ary heap in LDS.
 //
 // The per-wave state is kept in plain local variables and the repeated steps are macros:
 // a state struct (or closures capturing by reference) made the compiler spill it to
@@ -70,9 +71,11 @@ __device__ __forceinline__ auto stage_at(const float *sx, const float *sy, const
 
 // The tree arrays and the sorted particles are also passed as restrict-qualified kernel
 // arguments: the kernel never writes them, and saying so lets the wave-uniform node reads
-// go through scalar loads.
+// go through scalar loads.  Two waves per SIMD caps the registers at 256 per lane, so that
+// the 6 waves per CU the mixed kernel's LDS allows can run (without the cap it takes 258
+// and runs 4 waves per CU).
 template <int KCAP, bool MIXED>
-__global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict__ nfirst,
+__global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restrict__ nfirst,
                                             const int32_t *__restrict__ nlast,
                                             const int32_t *__restrict__ nleft,
                                             const int32_t *__restrict__ nright,
@@ -81,11 +84,11 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
 {
     using KeyT = typename std::conditional<MIXED, float, double>::type;
     // the mixed heap keeps KX entries beyond the k nearest (the certificate needs the
-    // (k+1)-th key) and holds KCAP + 1 entries: with KCAP a multiple of HEAP_D every child
-    // index of an internal node of a heap of at most KCAP + 1 entries lies below KCAP + 1,
-    // so its insertions read all HEAP_D children unchecked, the entries past the heap size
-    // being kept at -inf.  The FP64 heap stays at KCAP entries (one more would cost it a
-    // wave per CU of LDS) and checks child indices.
+    // (k+1)-th key) and holds KCAP + 1 entries: its LDS part (KCAP + 1 - RT entries) is
+    // then a whole number of LC-child groups, so every register entry with children has
+    // all LC of them allocated and insertions read them unchecked, the entries past the
+    // heap size being kept at -inf.  The FP64 heap stays at KCAP entries (one more would
+    // cost it a wave per CU of LDS) and checks child indices.
     constexpr int KX = 1;
     constexpr int KHCAP = MIXED ? KCAP + 1 : KCAP;
     constexpr bool SIFT_CHECKED = !MIXED;
@@ -93,19 +96,24 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
     // staged buckets whose insertions are deferred (the FP64 heap is twice as large, so its
     // stage may be cut down to fit one more wave per CU)
     constexpr int NBUF = MIXED ? AHOP_MIXED_NBUF : AHOP_FP64_NBUF;
-    // heap storage: for the mixed kernel the root and its children in registers (rk, ri) and
-    // packed (key, id) words for the other entries in LDS; key and id arrays for FP64
-    static_assert(!MIXED || KHCAP > RT, "the mixed heap keeps its top RT entries in registers");
-    __shared__ uint2 hp[MIXED ? (KHCAP - RT) * WAVE : 1];
+    // heap storage: for the mixed kernel the root and its RC children in registers (rt) and
+    // their children, packed (key, id) words, in LDS; key and id arrays for FP64
+    static_assert(!MIXED || KHCAP <= RT + LC * RC, "the mixed heap has three levels");
+    static_assert(!MIXED || (KHCAP - RT) % LC == 0, "every internal register entry has LC children");
+    constexpr int NLDS = MIXED ? (KHCAP > RT ? KHCAP - RT : 1) : 1;
+    __shared__ uint2 hp[NLDS * WAVE];
     __shared__ KeyT hk[MIXED ? 1 : KHCAP * WAVE];
     __shared__ int32_t hi[MIXED ? 1 : KHCAP * WAVE];
-    rtk_t rk = -INFINITY;
-    rti_t ri = 0u;
-#define HK(e) heap_key<MIXED, KeyT>(rk, hp, hk, lane, (e))
-#define HI(e) heap_id<MIXED, KeyT>(ri, hp, hi, lane, (e))
-#define HSET(e, k, i) heap_set<MIXED, KeyT>(rk, ri, hp, hk, hi, lane, (e), (k), (i))
+    RegTop rt;
+    rt.k0 = -INFINITY;
+    rt.i0 = 0u;
+    rt.rk = -INFINITY;
+    rt.ri = 0u;
+#define HK(e) heap_key<MIXED, KeyT>(rt, hp, hk, lane, (e))
+#define HI(e) heap_id<MIXED, KeyT>(rt, hp, hi, lane, (e))
+#define HSET(e, k, i) heap_set<MIXED, KeyT>(rt, hp, hk, hi, lane, (e), (k), (i))
 #define HSIFT(size, d, id, e, checked) \
-    heap_sift_any<MIXED, checked, KeyT>(rk, ri, hp, hk, hi, lane, (size), (d), (id), (e))
+    heap_sift_any<MIXED, checked, KeyT>(rt, hp, hk, hi, lane, (size), (d), (id), (e))
     // staged bucket coordinates: FP32 x, y, z arrays (mixed; three reads of four candidates
     // each serve four candidates) or FP64 double4 (FP64)
     __shared__ __align__(16) float sx[MIXED ? NBUF * WAVE : 4];
@@ -220,11 +228,17 @@ __global__ __launch_bounds__(64) void k_knn(KnnArgs A, const int32_t *__restrict
         (idout) = fsel + j_;                                                                   \
     } while (0)
 
-    // Bottom-up heapify of this lane's KH entries.
+    // Bottom-up heapify of this lane's KH entries (mixed: the register children that have
+    // children, then the root).
 #define KNN_HEAPIFY()                                                                          \
     do {                                                                                       \
-        for (int e_ = (KH - 2) / HEAP_D; e_ >= 0; --e_)                                        \
-            HSIFT(KH, HK(e_), HI(e_), e_, SIFT_CHECKED);                                       \
+        if constexpr (MIXED) {                                                                 \
+            for (int e_ = RC; e_ >= 0; --e_)                                                   \
+                if (e_ == 0 || rt_child0(e_) < KH) HSIFT(KH, HK(e_), HI(e_), e_, SIFT_CHECKED); \
+        } else {                                                                               \
+            for (int e_ = (KH - 2) / HEAP_D; e_ >= 0; --e_)                                    \
+                HSIFT(KH, HK(e_), HI(e_), e_, SIFT_CHECKED);                                   \
+        }                                                                                      \
     } while (0)
 
     // Insert every pending accepted candidate lane by lane: the wave pays for the busiest
