@@ -473,23 +473,42 @@ __device__ __forceinline__ void rt_sift_l1(RegTop &h, uint2 *hp, int lane, int s
         uint2 ch[LC];
 #pragma unroll
         for (int t = 0; t < LC; ++t) ch[t] = base[((!CHECKED || c0 + t < size) ? t : 0) * WAVE];
-        float kc = __uint_as_float(ch[0].x);
-        uint32_t idc = ch[0].y;
-        int tc = 0;
+        float kc;
+        uint32_t idc;
+        int tc;
+        if constexpr (!CHECKED) {
+            // the maximum by max instructions, then the first entry equal to it (the same
+            // entry as a strict-greater scan): no chain of dependent compare-select pairs
+            kc = __uint_as_float(ch[0].x);
 #pragma unroll
-        for (int t = 1; t < LC; ++t) {
-            const float kt = __uint_as_float(ch[t].x);
-            if ((!CHECKED || c0 + t < size) && kt > kc) {
-                kc = kt;
-                idc = ch[t].y;
-                tc = t;
+            for (int t = 1; t < LC; ++t) kc = fmaxf(kc, __uint_as_float(ch[t].x));
+            tc = LC - 1;
+            idc = ch[LC - 1].y;
+#pragma unroll
+            for (int t = LC - 2; t >= 0; --t) {
+                const bool m = __uint_as_float(ch[t].x) == kc;
+                tc = m ? t : tc;
+                idc = m ? ch[t].y : idc;
+            }
+        } else {
+            kc = __uint_as_float(ch[0].x);
+            idc = ch[0].y;
+            tc = 0;
+#pragma unroll
+            for (int t = 1; t < LC; ++t) {
+                const float kt = __uint_as_float(ch[t].x);
+                if (c0 + t < size && kt > kc) {
+                    kc = kt;
+                    idc = ch[t].y;
+                    tc = t;
+                }
             }
         }
-        if (kc > d) {
-            vk = kc;
-            vi = idc;
-            hp[(c0 + tc - RT) * WAVE + lane] = make_uint2(__float_as_uint(d), id);
-        }
+        // the id is used on both sides of the test, so key and id come in one 64-bit read
+        const bool up = kc > d;
+        vk = up ? kc : d;
+        vi = up ? idc : id;
+        if (up) hp[(c0 + tc - RT) * WAVE + lane] = make_uint2(__float_as_uint(d), id);
     }
 #pragma unroll
     for (int t = 0; t < RC; ++t) {
@@ -504,16 +523,34 @@ template <bool CHECKED>
 __device__ __forceinline__ void rt_sift_root(RegTop &h, uint2 *hp, int lane, int size, float d,
                                              uint32_t id)
 {
-    float kc = (!CHECKED || size > 1) ? h.rk[0] : -INFINITY;
-    uint32_t idc = h.ri[0];
-    int ic = 1;
+    float kc;
+    uint32_t idc;
+    int ic;
+    if constexpr (!CHECKED) {
+        // the maximum by max instructions, then the first entry equal to it
+        kc = h.rk[0];
 #pragma unroll
-    for (int t = 1; t < RC; ++t)
-        if ((!CHECKED || t + 1 < size) && h.rk[t] > kc) {
-            kc = h.rk[t];
-            idc = h.ri[t];
-            ic = t + 1;
+        for (int t = 1; t < RC; ++t) kc = fmaxf(kc, h.rk[t]);
+        ic = RC;
+        idc = h.ri[RC - 1];
+#pragma unroll
+        for (int t = RC - 2; t >= 0; --t) {
+            const bool m = h.rk[t] == kc;
+            ic = m ? t + 1 : ic;
+            idc = m ? h.ri[t] : idc;
         }
+    } else {
+        kc = size > 1 ? h.rk[0] : -INFINITY;
+        idc = h.ri[0];
+        ic = 1;
+#pragma unroll
+        for (int t = 1; t < RC; ++t)
+            if (t + 1 < size && h.rk[t] > kc) {
+                kc = h.rk[t];
+                idc = h.ri[t];
+                ic = t + 1;
+            }
+    }
     if (kc > d) {
         h.k0 = kc;
         h.i0 = idc;
