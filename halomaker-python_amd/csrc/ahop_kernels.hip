@@ -122,6 +122,15 @@ __device__ __forceinline__ int logical_block(int nblocks)
     return x * g8 + min(x, rem) + b / 8;
 }
 
+// a value that is the same on every lane, as a wave-uniform (scalar) value
+__device__ __forceinline__ double uniform_d(double v)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)b);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_max(T v)
 {
@@ -476,33 +485,23 @@ __device__ __forceinline__ void rt_sift_l1(RegTop &h, uint2 *hp, int lane, int s
         float kc;
         uint32_t idc;
         int tc;
-        if constexpr (!CHECKED) {
-            // the maximum by max instructions, then the first entry equal to it (the same
-            // entry as a strict-greater scan): no chain of dependent compare-select pairs
-            kc = __uint_as_float(ch[0].x);
+        // the maximum by max instructions, then the first entry equal to it (the same entry
+        // as a strict-greater scan): no chain of dependent compare-select pairs.  Checked,
+        // the entries past the heap size count as -inf (the first child is inside it).
+        float kt[LC];
 #pragma unroll
-            for (int t = 1; t < LC; ++t) kc = fmaxf(kc, __uint_as_float(ch[t].x));
-            tc = LC - 1;
-            idc = ch[LC - 1].y;
+        for (int t = 0; t < LC; ++t)
+            kt[t] = (!CHECKED || c0 + t < size) ? __uint_as_float(ch[t].x) : -INFINITY;
+        kc = kt[0];
 #pragma unroll
-            for (int t = LC - 2; t >= 0; --t) {
-                const bool m = __uint_as_float(ch[t].x) == kc;
-                tc = m ? t : tc;
-                idc = m ? ch[t].y : idc;
-            }
-        } else {
-            kc = __uint_as_float(ch[0].x);
-            idc = ch[0].y;
-            tc = 0;
+        for (int t = 1; t < LC; ++t) kc = fmaxf(kc, kt[t]);
+        tc = LC - 1;
+        idc = ch[LC - 1].y;
 #pragma unroll
-            for (int t = 1; t < LC; ++t) {
-                const float kt = __uint_as_float(ch[t].x);
-                if (c0 + t < size && kt > kc) {
-                    kc = kt;
-                    idc = ch[t].y;
-                    tc = t;
-                }
-            }
+        for (int t = LC - 2; t >= 0; --t) {
+            const bool m = kt[t] == kc;
+            tc = m ? t : tc;
+            idc = m ? ch[t].y : idc;
         }
         // the id is used on both sides of the test, so key and id come in one 64-bit read
         const bool up = kc > d;
@@ -526,30 +525,21 @@ __device__ __forceinline__ void rt_sift_root(RegTop &h, uint2 *hp, int lane, int
     float kc;
     uint32_t idc;
     int ic;
-    if constexpr (!CHECKED) {
-        // the maximum by max instructions, then the first entry equal to it
-        kc = h.rk[0];
+    // the maximum by max instructions, then the first entry equal to it; checked, the
+    // entries past the heap size count as -inf (with none left, kc = -inf and d stays)
+    float kt[RC];
 #pragma unroll
-        for (int t = 1; t < RC; ++t) kc = fmaxf(kc, h.rk[t]);
-        ic = RC;
-        idc = h.ri[RC - 1];
+    for (int t = 0; t < RC; ++t) kt[t] = (!CHECKED || t + 1 < size) ? h.rk[t] : -INFINITY;
+    kc = kt[0];
 #pragma unroll
-        for (int t = RC - 2; t >= 0; --t) {
-            const bool m = h.rk[t] == kc;
-            ic = m ? t + 1 : ic;
-            idc = m ? h.ri[t] : idc;
-        }
-    } else {
-        kc = size > 1 ? h.rk[0] : -INFINITY;
-        idc = h.ri[0];
-        ic = 1;
+    for (int t = 1; t < RC; ++t) kc = fmaxf(kc, kt[t]);
+    ic = RC;
+    idc = h.ri[RC - 1];
 #pragma unroll
-        for (int t = 1; t < RC; ++t)
-            if (t + 1 < size && h.rk[t] > kc) {
-                kc = h.rk[t];
-                idc = h.ri[t];
-                ic = t + 1;
-            }
+    for (int t = RC - 2; t >= 0; --t) {
+        const bool m = kt[t] == kc;
+        ic = m ? t + 1 : ic;
+        idc = m ? h.ri[t] : idc;
     }
     if (kc > d) {
         h.k0 = kc;

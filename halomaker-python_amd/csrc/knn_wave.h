@@ -141,9 +141,10 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         const double lx = wave_min(active ? P.x : BIG), hx = wave_max(active ? P.x : -BIG);
         const double ly = wave_min(active ? P.y : BIG), hy = wave_max(active ? P.y : -BIG);
         const double lz = wave_min(active ? P.z : BIG), hz = wave_max(active ? P.z : -BIG);
-        cx = 0.5 * (lx + hx);
-        cy = 0.5 * (ly + hy);
-        cz = 0.5 * (lz + hz);
+        // the same on every lane: say so, so that it lives in scalar registers
+        cx = uniform_d(0.5 * (lx + hx));
+        cy = uniform_d(0.5 * (ly + hy));
+        cz = uniform_d(0.5 * (lz + hz));
     }
     float qx = 0, qy = 0, qz = 0, hQ = 0;
     bool wave_unsafe = false;
@@ -188,7 +189,8 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     // heap entries written so far by the seeding (the per-lane gather of a query list
     // inserts into the +inf-filled heap directly)
     int filled = (A.qlist || (!MIXED && !AHOP_FP64_FILL)) ? KH : 0;
-    unsigned long long st_nodes = 0, st_buckets = 0, st_cand = 0, st_steps = 0, st_ins = 0;
+    unsigned long long st_nodes = 0, st_buckets = 0, st_cand = 0, st_steps = 0;
+    unsigned st_ins = 0;   // per lane
 
     // FP32 keys use fused multiply-adds: one rounding per step, so the error bound of
     // fp32_key_err (written for separate roundings) still holds
@@ -213,8 +215,10 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     // bit first): its key and sorted slot.
 #define KNN_PICK(dout, idout)                                                                  \
     do {                                                                                       \
-        const bool s0 = m0 != 0, s1 = !s0 && m1 != 0, s2 = !s0 && !s1 && m2 != 0;              \
-        const bool s3 = !s0 && !s1 && !s2;                                                     \
+        const bool s0 = m0 != 0;                                                               \
+        const bool s1 = !s0 && (NBUF == 2 || m1 != 0);                                         \
+        const bool s2 = NBUF > 2 && !s0 && !s1 && (NBUF == 3 || m2 != 0);                      \
+        const bool s3 = NBUF > 3 && !s0 && !s1 && !s2;                                         \
         uint64_t mk = s0 ? m0 : (s1 ? m1 : (s2 ? m2 : m3));                                    \
         const int fsel = s0 ? f0 : (s1 ? f1 : (s2 ? f2 : f3));                                 \
         const int bsel = s0 ? 0 : (s1 ? 1 : (s2 ? 2 : 3));                                     \
@@ -330,12 +334,12 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         }                                                                                      \
         m0 = nbuf == 0 ? mask_ : m0;                                                           \
         m1 = nbuf == 1 ? mask_ : m1;                                                           \
-        m2 = nbuf == 2 ? mask_ : m2;                                                           \
-        m3 = nbuf == 3 ? mask_ : m3;                                                           \
+        if constexpr (NBUF > 2) m2 = nbuf == 2 ? mask_ : m2;                                   \
+        if constexpr (NBUF > 3) m3 = nbuf == 3 ? mask_ : m3;                                   \
         f0 = nbuf == 0 ? (fst) : f0;                                                           \
         f1 = nbuf == 1 ? (fst) : f1;                                                           \
-        f2 = nbuf == 2 ? (fst) : f2;                                                           \
-        f3 = nbuf == 3 ? (fst) : f3;                                                           \
+        if constexpr (NBUF > 2) f2 = nbuf == 2 ? (fst) : f2;                                   \
+        if constexpr (NBUF > 3) f3 = nbuf == 3 ? (fst) : f3;                                   \
         ++nbuf;                                                                                \
     } while (0)
 
@@ -497,12 +501,15 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     // left entries out of FP64 order (within the k, or an extra entry nearer than the k-th),
     // the lane re-sorts its nsel entries by the FP64 distances, keeps the first k and sums
     // again, so its densities are those of the FP64 path
+    // the query's coordinates and mass, read again here rather than held in registers
+    // through the search (the FP64 kernel's keys use them throughout)
+    const double4 Q = MIXED ? spos[q] : P;
     double rho = 0.0;
     for (int pass = 0; pass < 2; ++pass) {
         double d2k;
         if constexpr (MIXED) {
             const double4 c = spos[HI(K - 1)];
-            d2k = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+            d2k = d2_ref(Q.x, Q.y, Q.z, c.x, c.y, c.z, Lx, Ly, Lz);
         } else {
             d2k = HK(K - 1);
         }
@@ -515,7 +522,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         for (int i = 0; i < K - 1; ++i) {
             const double4 c = spos[HI(i)];
             double d;
-            if constexpr (MIXED) d = d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz);
+            if constexpr (MIXED) d = d2_ref(Q.x, Q.y, Q.z, c.x, c.y, c.z, Lx, Ly, Lz);
             else d = HK(i);
             sorted = sorted && (d >= prev);
             prev = d;
@@ -525,21 +532,21 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         if constexpr (MIXED) {
             for (int i = K; i < nsel; ++i) {
                 const double4 c = spos[HI(i)];
-                sorted = sorted && (d2_ref(P.x, P.y, P.z, c.x, c.y, c.z, Lx, Ly, Lz) >= d2k);
+                sorted = sorted && (d2_ref(Q.x, Q.y, Q.z, c.x, c.y, c.z, Lx, Ly, Lz) >= d2k);
             }
         }
-        rho = A.xyzl * (contrib + P.w) / (PI_REF * r3);
+        rho = A.xyzl * (contrib + Q.w) / (PI_REF * r3);
         if (!MIXED || sorted) break;
         // insertion sort of this lane's nsel entries by FP64 distance (stable on ties)
         for (int i = 1; i < nsel; ++i) {
             const int32_t id = HI(i);
             const KeyT ki = HK(i);
             const double4 ci = spos[id];
-            const double di = d2_ref(P.x, P.y, P.z, ci.x, ci.y, ci.z, Lx, Ly, Lz);
+            const double di = d2_ref(Q.x, Q.y, Q.z, ci.x, ci.y, ci.z, Lx, Ly, Lz);
             int j = i - 1;
             while (j >= 0) {
                 const double4 cj = spos[HI(j)];
-                if (d2_ref(P.x, P.y, P.z, cj.x, cj.y, cj.z, Lx, Ly, Lz) <= di) break;
+                if (d2_ref(Q.x, Q.y, Q.z, cj.x, cj.y, cj.z, Lx, Ly, Lz) <= di) break;
                 HSET(j + 1, HK(j), HI(j));
                 --j;
             }
