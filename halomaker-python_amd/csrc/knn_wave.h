@@ -247,9 +247,10 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
 
     // Insert every pending accepted candidate lane by lane: the wave pays for the busiest
     // lane's total over all staged buckets, not for each bucket's busiest lane.  While the
-    // lane's heap is still being filled (seeding), candidates are written straight into the
-    // next free entry and the full array is heapified once, instead of sifting each one.
-#define KNN_FLUSH()                                                                            \
+    // lane's heap is still being filled (seeding, FILL true), candidates are written straight
+    // into the next free entry and the full array is heapified once, instead of sifting each
+    // one; the traversal's flushes (FILL false) carry no fill code.
+#define KNN_FLUSH(FILL)                                                                        \
     do {                                                                                       \
         while (__any((m0 | m1 | m2 | m3) != 0)) {                                             \
             ++st_steps;                                                                        \
@@ -257,7 +258,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
                 KeyT d_;                                                                       \
                 int id_;                                                                       \
                 KNN_PICK(d_, id_);                                                             \
-                if (filled < KH) {                                                             \
+                if ((FILL) && filled < KH) {                                                   \
                     HSET(filled, d_, id_);                                                     \
                     ++st_ins;                                                                  \
                     if (++filled == KH) {                                                      \
@@ -309,7 +310,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     // seeding accepts only the lane's own window, the traversal everything outside it.
 #define KNN_STAGE(fst, cnt, seeding)                                                           \
     do {                                                                                       \
-        if (nbuf == NBUF) KNN_FLUSH();                                                         \
+        if (nbuf == NBUF) KNN_FLUSH(seeding);                                                  \
         __syncthreads();                                                                       \
         if (lane < (cnt)) {                                                                    \
             const double4 c_ = spos[(fst) + lane];                                             \
@@ -376,7 +377,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
             if (d < tau) KNN_INSERT(d, j);
         }
     }
-    KNN_FLUSH();
+    KNN_FLUSH(true);
     if (filled < KH) {
         // a clipped window (array ends) or an idle lane: the unfilled entries are +inf
         KNN_HEAPIFY();
@@ -440,7 +441,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
             stack[sp++] = c2;
         }
     }
-    KNN_FLUSH();
+    KNN_FLUSH(false);
     if (stack_overflow && lane == 0) atomicAdd(A.overflow, 1);
 #undef KNN_NEED
 #undef KNN_STAGE
