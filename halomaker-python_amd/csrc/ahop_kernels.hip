@@ -897,6 +897,8 @@ static void launch_knn(bool mixed, const KnnArgs &a, unsigned nb, hipStream_t s)
 
 static void dispatch_knn(int K, bool mixed, const KnnArgs &a, unsigned nb, hipStream_t s)
 {
+    // the mixed kernel is compiled without the explicit-query-list path
+    if (mixed && a.qlist) throw std::logic_error("the mixed kNN kernel takes no query list");
     if (K <= 16) launch_knn<16>(mixed, a, nb, s);
     else if (K <= 32) launch_knn<32>(mixed, a, nb, s);
     else if (K <= 64) launch_knn<64>(mixed, a, nb, s);

@@ -124,11 +124,14 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
 
     const int lane = threadIdx.x;
     const int wave = logical_block(A.nblocks);
+    // explicit query lists are for the FP64 rerun only (the host checks it), so the mixed
+    // kernel is compiled without that path
+    const int32_t *const qlist = MIXED ? nullptr : A.qlist;
     // qper < WAVE only with an explicit query list (the seeding below assumes full waves
     // of consecutive slots otherwise)
     const int slot = wave * A.qper + lane;
     const bool active = lane < A.qper && slot < A.nq;
-    const int q = active ? (A.qlist ? A.qlist[slot] : slot) : 0;
+    const int q = active ? (qlist ? qlist[slot] : slot) : 0;
     const int K = A.K;
     const int KH = MIXED ? K + KX : K;
     const double4 P = spos[q];
@@ -165,30 +168,29 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     // Morton-order seed window of this lane: [q-Wd, q+Wd]; the traversal skips it
     const int Wd = K / 2 + AHOP_SEED_PAD;
     int seed_lo = 1, seed_hi = 0;
-    if (active && !wave_unsafe) {
-        if (AHOP_WAVE_SEED && !A.qlist) {
-            // every lane seeds from the whole window of the wave
-            seed_lo = max(0, wave * WAVE - Wd);
-            seed_hi = min(A.n - 1, wave * WAVE + WAVE - 1 + Wd);
-        } else {
-            seed_lo = max(0, q - Wd);
-            seed_hi = min(A.n - 1, q + Wd);
-        }
+    if (AHOP_WAVE_SEED && !qlist) {
+        // every lane seeds from the whole window of the wave (wave-uniform; a lane without a
+        // search accepts nothing, its tau being -1)
+        seed_lo = max(0, wave * WAVE - Wd);
+        seed_hi = min(A.n - 1, wave * WAVE + WAVE - 1 + Wd);
+    } else if (active && !wave_unsafe) {
+        seed_lo = max(0, q - Wd);
+        seed_hi = min(A.n - 1, q + Wd);
     }
     // slots that may lie in some lane's seed window or be some lane's query (wave-uniform)
-    const int wseed_lo = A.qlist ? 0 : max(0, wave * WAVE - Wd);
-    const int wseed_hi = A.qlist ? A.n - 1 : min(A.n - 1, wave * WAVE + WAVE - 1 + Wd);
+    const int wseed_lo = qlist ? 0 : max(0, wave * WAVE - Wd);
+    const int wseed_hi = qlist ? A.n - 1 : min(A.n - 1, wave * WAVE + WAVE - 1 + Wd);
     const float pxf = (float)P.x, pyf = (float)P.y, pzf = (float)P.z;
     const float Lxf = (float)Lx, Lyf = (float)Ly, Lzf = (float)Lz;
     const float mLx = 4.0f * U32 * Lxf, mLy = 4.0f * U32 * Lyf, mLz = 4.0f * U32 * Lzf;
     const float Lmin = fminf(Lxf, fminf(Lyf, Lzf));
-    const int qmid = A.qlist ? __builtin_amdgcn_readfirstlane(q) : wave * WAVE + WAVE / 2;
+    const int qmid = qlist ? __builtin_amdgcn_readfirstlane(q) : wave * WAVE + WAVE / 2;
     uint64_t m0 = 0, m1 = 0, m2 = 0, m3 = 0;   // accepted-candidate masks of staged buckets
     int f0 = 0, f1 = 0, f2 = 0, f3 = 0;         // first slot of each staged bucket
     int nbuf = 0;
     // heap entries written so far by the seeding (the per-lane gather of a query list
     // inserts into the +inf-filled heap directly)
-    int filled = (A.qlist || (!MIXED && !AHOP_FP64_FILL)) ? KH : 0;
+    int filled = (qlist || (!MIXED && !AHOP_FP64_FILL)) ? KH : 0;
     unsigned long long st_nodes = 0, st_buckets = 0, st_cand = 0, st_steps = 0;
     unsigned st_ins = 0;   // per lane
 
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     } while (0)
 
     // ---- seed: the wave's seed windows cover the contiguous slots [qb-Wd, qb+63+Wd]
-    if (!A.qlist) {
+    if (!qlist) {
         const int qb = wave * WAVE;
         const int s0 = max(0, qb - Wd), s1 = min(A.n - 1, qb + WAVE - 1 + Wd);
         for (int f = s0; f <= s1; f += WAVE) {
@@ -405,7 +407,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         if (!__any(need)) continue;
         if (l - f + 1 <= A.bucket || (root_is_bucket && node == 0)) {
             // a bucket inside the wave-wide seed window was seen by every lane already
-            if (AHOP_WAVE_SEED && !A.qlist && f >= wseed_lo && l <= wseed_hi) continue;
+            if (AHOP_WAVE_SEED && !qlist && f >= wseed_lo && l <= wseed_hi) continue;
             const int cnt = l - f + 1;
             ++st_buckets;
             st_cand += cnt;
