@@ -615,13 +615,17 @@ __device__ __forceinline__ void heap_sift_any(RegTop &h, uint2 *hp, KeyT *hk, in
 #include "knn_wave.h"
 
 // active-particle count (density > rho_threshold, f90:712-715)
+// (grid-stride: each wave sums its share and adds it with one atomic, so a few thousand
+// atomics serve the whole array)
 __global__ void k_count_active(const double *__restrict__ rho, int64_t n, double thr,
                                unsigned long long *cnt)
 {
-    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    bool a = i < n && rho[i] > thr;
-    unsigned long long b = __ballot(a);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, (unsigned long long)__popcll(b));
+    unsigned c = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        c += rho[i] > thr ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, (unsigned long long)c);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -975,8 +979,8 @@ void Engine::knn_density(const Params &p)
         dispatch_knn(p.nvoisins, false, b, nb2, s);
     }
     AHOP_HIP(hipEventRecord(ev_[4], s));
-    hipLaunchKernelGGL(k_count_active, dim3(nblk(n)), dim3(256), 0, s, rho_s, n, p.rho_threshold,
-                       cnt);
+    hipLaunchKernelGGL(k_count_active, dim3(std::min(nblk(n), 2048u)), dim3(256), 0, s, rho_s, n,
+                       p.rho_threshold, cnt);
 }
 
 void Engine::hop_groups(const Params &p)
