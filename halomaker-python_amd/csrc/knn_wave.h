@@ -282,21 +282,26 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
 #define KNN_SCAN(b, fst, cnt, ACCEPT)                                                          \
     do {                                                                                       \
         if constexpr (MIXED) {                                                                 \
-            for (int j4_ = 0; j4_ < (cnt); j4_ += 4) {                                         \
-                const float4 X_ = *reinterpret_cast<const float4 *>(sx + (b) * WAVE + j4_);    \
-                const float4 Y_ = *reinterpret_cast<const float4 *>(sy + (b) * WAVE + j4_);    \
-                const float4 Z_ = *reinterpret_cast<const float4 *>(sz + (b) * WAVE + j4_);    \
-                const float xs_[4] = {X_.x, X_.y, X_.z, X_.w};                                 \
-                const float ys_[4] = {Y_.x, Y_.y, Y_.z, Y_.w};                                 \
-                const float zs_[4] = {Z_.x, Z_.y, Z_.z, Z_.w};                                 \
-                _Pragma("unroll") for (int t_ = 0; t_ < 4; ++t_) {                             \
-                    const int j_ = j4_ + t_;                                                   \
-                    const int sj_ = (fst) + j_;                                                \
-                    KeyT d_;                                                                   \
-                    KNN_KEY(make_float4(xs_[t_], ys_[t_], zs_[t_], 0.0f), d_);                 \
-                    if (ACCEPT) mask_ |= (1ull << j_);                                         \
+            /* the two 32-candidate halves build 32-bit masks (one select and one or a bit) */ \
+            uint32_t hm_[2] = {0u, 0u};                                                        \
+            _Pragma("unroll") for (int h_ = 0; h_ < 2; ++h_) {                                 \
+                for (int j4_ = 32 * h_; j4_ < min((cnt), 32 * h_ + 32); j4_ += 4) {            \
+                    const float4 X_ = *reinterpret_cast<const float4 *>(sx + (b) * WAVE + j4_); \
+                    const float4 Y_ = *reinterpret_cast<const float4 *>(sy + (b) * WAVE + j4_); \
+                    const float4 Z_ = *reinterpret_cast<const float4 *>(sz + (b) * WAVE + j4_); \
+                    const float xs_[4] = {X_.x, X_.y, X_.z, X_.w};                             \
+                    const float ys_[4] = {Y_.x, Y_.y, Y_.z, Y_.w};                             \
+                    const float zs_[4] = {Z_.x, Z_.y, Z_.z, Z_.w};                             \
+                    _Pragma("unroll") for (int t_ = 0; t_ < 4; ++t_) {                         \
+                        const int j_ = j4_ + t_;                                               \
+                        const int sj_ = (fst) + j_;                                            \
+                        KeyT d_;                                                               \
+                        KNN_KEY(make_float4(xs_[t_], ys_[t_], zs_[t_], 0.0f), d_);             \
+                        if (ACCEPT) hm_[h_] |= 1u << (j_ - 32 * h_);                           \
+                    }                                                                          \
                 }                                                                              \
             }                                                                                  \
+            mask_ = ((uint64_t)hm_[1] << 32) | hm_[0];                                         \
             if ((cnt) < WAVE) mask_ &= (1ull << (cnt)) - 1ull;                                 \
         } else {                                                                               \
             for (int j_ = 0; j_ < (cnt); ++j_) {                                               \
