@@ -236,65 +236,102 @@ __device__ __forceinline__ void child_box(int c, const double4 *__restrict__ spo
     }
 }
 
+// FP32 copy of a node box for the traversal tests: centre rounded to nearest, half-widths
+// rounded up (the test's margin covers the centre rounding)
+__device__ __forceinline__ void store_box(int node, const double lo[3], const double hi[3],
+                                          double *box, float4 *fbox)
+{
+    float c[3], h[3];
+    for (int a = 0; a < 3; ++a) {
+        box[6 * (int64_t)node + a] = lo[a];
+        box[6 * (int64_t)node + a + 3] = hi[a];
+        c[a] = (float)(0.5 * (lo[a] + hi[a]));
+        h[a] = __double2float_ru(0.5 * (hi[a] - lo[a]));
+    }
+    fbox[2 * (int64_t)node] = make_float4(c[0], c[1], c[2], h[0]);
+    fbox[2 * (int64_t)node + 1] = make_float4(h[1], h[2], 0.0f, 0.0f);
+}
+
 // Bottom-up bounding boxes in two phases.  A fence per climbing thread at device scope
 // (needed for cross-XCD visibility) costs ~1e2 ms at 16.8M particles, so:
 //  phase 1 (k_refit_local): one workgroup per 256 consecutive leaves merges every node whose
-//    sorted range lies inside its 256-leaf block, communicating through LDS counters and
-//    workgroup-scope fences only;
+//    sorted range lies inside its 256-leaf block.  Such a node's index lies in the block too
+//    (an internal node is one end of its own range), so the block preloads the links of its
+//    node indices and its leaves' positions into LDS and merges boxes there, communicating
+//    through LDS counters and workgroup-scope fences only;
 //  phase 2 (k_refit_upper): the few nodes whose range crosses a block boundary are merged by
 //    threads climbing from the lowest crossing nodes with device-scope fences.
+// Both write the FP64 box and its FP32 copy.
 constexpr int REFIT_BS = 256;
 
 __device__ __forceinline__ void merge_children(int node, const double4 *__restrict__ spos,
                                                const int32_t *__restrict__ left,
-                                               const int32_t *__restrict__ right, double *box)
+                                               const int32_t *__restrict__ right, double *box,
+                                               float4 *fbox)
 {
-    double a[6], b[6];
+    double a[6], b[6], lo[3], hi[3];
     child_box(left[node], spos, box, a);
     child_box(right[node], spos, box, b);
     for (int q = 0; q < 3; ++q) {
-        box[6 * (int64_t)node + q] = fmin(a[q], b[q]);
-        box[6 * (int64_t)node + q + 3] = fmax(a[q + 3], b[q + 3]);
+        lo[q] = fmin(a[q], b[q]);
+        hi[q] = fmax(a[q + 3], b[q + 3]);
     }
+    store_box(node, lo, hi, box, fbox);
 }
 
 __global__ __launch_bounds__(REFIT_BS) void k_refit_local(
     int n, const double4 *__restrict__ spos, const int32_t *__restrict__ first,
     const int32_t *__restrict__ last, const int32_t *__restrict__ left,
     const int32_t *__restrict__ right, const int32_t *__restrict__ parent,
-    const int32_t *__restrict__ leaf_parent, double *box)
+    const int32_t *__restrict__ leaf_parent, double *box, float4 *fbox)
 {
     __shared__ int cnt[REFIT_BS];
+    __shared__ int lf[REFIT_BS], ll[REFIT_BS], lc0[REFIT_BS], lc1[REFIT_BS], lpar[REFIT_BS];
+    __shared__ double lpos[3][REFIT_BS];      // the block's leaves
+    __shared__ double lbox[6][REFIT_BS];      // boxes of the block's local nodes
     const int B0 = blockIdx.x * REFIT_BS;
     const int B1 = min(n, B0 + REFIT_BS) - 1;   // last leaf of the block
-    cnt[threadIdx.x] = 0;
+    const int t = threadIdx.x;
+    const int p = B0 + t;
+    cnt[t] = 0;
+    if (p <= B1) {
+        const double4 q = spos[p];
+        lpos[0][t] = q.x;
+        lpos[1][t] = q.y;
+        lpos[2][t] = q.z;
+        if (p < n - 1) {
+            lf[t] = first[p];
+            ll[t] = last[p];
+            lc0[t] = left[p];
+            lc1[t] = right[p];
+            lpar[t] = parent[p];
+        }
+    }
     __syncthreads();
-    const int p = B0 + threadIdx.x;
     if (p > B1) return;
     int node = leaf_parent[p];
-    while (node >= 0 && first[node] >= B0 && last[node] <= B1) {
+    while (node >= B0 && node <= B1 && node < n - 1 && lf[node - B0] >= B0 &&
+           ll[node - B0] <= B1) {
+        const int u = node - B0;
         __threadfence_block();
-        if (atomicAdd(&cnt[node - B0], 1) == 0) return;   // sibling subtree not done yet
+        if (atomicAdd(&cnt[u], 1) == 0) return;   // sibling subtree not done yet
         __threadfence_block();
-        merge_children(node, spos, left, right, box);
-        node = parent[node];
+        double lo[3], hi[3];
+        const int c0 = lc0[u], c1 = lc1[u];
+        // children of a local node are the block's leaves or local nodes
+        for (int a = 0; a < 3; ++a) {
+            const double a0 = c0 < 0 ? lpos[a][~c0 - B0] : lbox[a][c0 - B0];
+            const double a1 = c0 < 0 ? lpos[a][~c0 - B0] : lbox[a + 3][c0 - B0];
+            const double b0 = c1 < 0 ? lpos[a][~c1 - B0] : lbox[a][c1 - B0];
+            const double b1 = c1 < 0 ? lpos[a][~c1 - B0] : lbox[a + 3][c1 - B0];
+            lo[a] = fmin(a0, b0);
+            hi[a] = fmax(a1, b1);
+            lbox[a][u] = lo[a];
+            lbox[a + 3][u] = hi[a];
+        }
+        store_box(node, lo, hi, box, fbox);
+        node = lpar[u];
     }
-}
-
-// FP32 copy of the node boxes for the traversal tests: centre rounded to nearest,
-// half-widths rounded up (the test's margin covers the centre rounding)
-__global__ void k_float_boxes(int n, const double *__restrict__ box, float4 *__restrict__ fbox)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    const double *b = box + 6 * (int64_t)i;
-    float c[3], h[3];
-    for (int a = 0; a < 3; ++a) {
-        c[a] = (float)(0.5 * (b[a] + b[a + 3]));
-        h[a] = __double2float_ru(0.5 * (b[a + 3] - b[a]));
-    }
-    fbox[2 * (int64_t)i] = make_float4(c[0], c[1], c[2], h[0]);
-    fbox[2 * (int64_t)i + 1] = make_float4(h[1], h[2], 0.0f, 0.0f);
 }
 
 // counter of a crossing node = number of its children already merged by phase 1 (or leaves)
@@ -317,13 +354,13 @@ __global__ void k_refit_upper_init(int n, const int32_t *__restrict__ first,
 __global__ void k_refit_upper(const int32_t *__restrict__ starts, const int *nstarts,
                               const double4 *__restrict__ spos, const int32_t *__restrict__ left,
                               const int32_t *__restrict__ right, const int32_t *__restrict__ parent,
-                              int32_t *flag, double *box)
+                              int32_t *flag, double *box, float4 *fbox)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= *nstarts) return;
     int node = starts[t];
     while (true) {
-        merge_children(node, spos, left, right, box);
+        merge_children(node, spos, left, right, box, fbox);
         node = parent[node];
         if (node < 0) return;
         __threadfence();
@@ -875,16 +912,15 @@ void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, co
     hipLaunchKernelGGL(k_radix_tree, dim3(nblk(ni)), dim3(256), 0, s, t_.key, (int)n, t_.first,
                        t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ni)), dim3(256), 0, s, t_.flag, ni, 0);
+    t_.fbox = as<float4>(b_fbox, 2 * ni);
     hipLaunchKernelGGL(k_refit_local, dim3(nblk(n, REFIT_BS)), dim3(REFIT_BS), 0, s, (int)n,
                        t_.spos, t_.first, t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent,
-                       t_.box);
+                       t_.box, t_.fbox);
     int32_t *starts = as<int32_t>(b_roots, n);        // scratch: the roots buffer is free here
     hipLaunchKernelGGL(k_refit_upper_init, dim3(nblk(ni)), dim3(256), 0, s, (int)n, t_.first,
                        t_.last, t_.left, t_.right, t_.flag, starts, misc + 5);
     hipLaunchKernelGGL(k_refit_upper, dim3(nblk(n / 2 + 1)), dim3(256), 0, s, starts, misc + 5,
-                       t_.spos, t_.left, t_.right, t_.parent, t_.flag, t_.box);
-    t_.fbox = as<float4>(b_fbox, 2 * ni);
-    hipLaunchKernelGGL(k_float_boxes, dim3(nblk(ni)), dim3(256), 0, s, (int)n, t_.box, t_.fbox);
+                       t_.spos, t_.left, t_.right, t_.parent, t_.flag, t_.box, t_.fbox);
     AHOP_HIP(hipEventRecord(ev_[2], s));
 }
 
