@@ -341,14 +341,24 @@ __global__ void k_refit_upper_init(int n, const int32_t *__restrict__ first,
                                    int *nstarts)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    if (first[i] / REFIT_BS == last[i] / REFIT_BS) return;
-    int done = 0;
-    int c[2] = {left[i], right[i]};
-    for (int k = 0; k < 2; ++k)
-        done += (c[k] < 0) || (first[c[k]] / REFIT_BS == last[c[k]] / REFIT_BS);
-    flag[i] = done;
-    if (done == 2) starts[atomicAdd(nstarts, 1)] = i;
+    bool start = false;
+    if (i < n - 1 && first[i] / REFIT_BS != last[i] / REFIT_BS) {
+        int done = 0;
+        int c[2] = {left[i], right[i]};
+        for (int k = 0; k < 2; ++k)
+            done += (c[k] < 0) || (first[c[k]] / REFIT_BS == last[c[k]] / REFIT_BS);
+        flag[i] = done;
+        start = done == 2;
+    }
+    // one atomic per wave reserves the wave's start slots
+    const unsigned long long b = __ballot(start);
+    if (!b) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)b) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(nstarts, __popcll(b));
+    base = __shfl(base, leader);
+    if (start) starts[base + __popcll(b & ((1ull << lane) - 1))] = i;
 }
 
 __global__ void k_refit_upper(const int32_t *__restrict__ starts, const int *nstarts,
@@ -919,7 +929,8 @@ void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, co
     int32_t *starts = as<int32_t>(b_roots, n);        // scratch: the roots buffer is free here
     hipLaunchKernelGGL(k_refit_upper_init, dim3(nblk(ni)), dim3(256), 0, s, (int)n, t_.first,
                        t_.last, t_.left, t_.right, t_.flag, starts, misc + 5);
-    hipLaunchKernelGGL(k_refit_upper, dim3(nblk(n / 2 + 1)), dim3(256), 0, s, starts, misc + 5,
+    // a start is the lowest node crossing some block boundary: at most one per boundary
+    hipLaunchKernelGGL(k_refit_upper, dim3(nblk(n / REFIT_BS + 1)), dim3(256), 0, s, starts, misc + 5,
                        t_.spos, t_.left, t_.right, t_.parent, t_.flag, t_.box, t_.fbox);
     AHOP_HIP(hipEventRecord(ev_[2], s));
 }
