@@ -98,6 +98,30 @@ def test_tree_shape_does_not_change_results(gpu, c1_oracle, bucket):
     assert np.array_equal(r["gid"], gid)
 
 
+@pytest.mark.parametrize("k", [1, 7, 16, 20, 40, 63, 100, 128])
+def test_every_heap_shape(gpu, k):
+    """k from 1 to 128 covers every heap capacity of the kNN kernel (k <= 16: the whole mixed
+    heap in registers; 32, 64, 128: register entries with groups of LDS children) and k values
+    that leave groups partly used.  Densities and neighbour rows bit-exact, both precisions."""
+    import torch
+    from amdahop import knn_device
+    ps = G.config("c1", n=6000, nclumps=5, seed=11)
+    log(f"heap shapes k={k}")
+    o = O.Oracle(ps.pos, ps.mass, k, nhop=min(16, k))
+    rho, _ = o.density()
+    rows = {int(i): o.knn(int(i))[1]
+            for i in np.random.default_rng(k).choice(ps.n, 100, replace=False)}
+    o.close()
+    pos = torch.from_numpy(ps.pos).cuda()
+    mass = torch.from_numpy(ps.mass).cuda()
+    for mixed in (True, False):
+        dens, nbr, _ = knn_device(pos, mass, k, mixed=mixed)
+        assert np.array_equal(dens.cpu().numpy(), rho), f"mixed={mixed}"
+        nb = nbr.cpu().numpy()
+        for i, ids in rows.items():
+            assert np.array_equal(nb[i], ids), f"mixed={mixed} particle {i}"
+
+
 def _device_run(ps, mixed, saddle=True, **kw):
     import torch
     from amdahop import run_device
