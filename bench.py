@@ -122,8 +122,9 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
 def adjoint_stage(steps, with_cpu):
     """SPH density adjoint on the 2,097,152-particle clumpy box (k=32): forward kNN rows
     frozen, upstream gradient of L = sum w log rho with w ~ N(0,1) (seed 3); device time
-    of the adjoint with inputs resident in HBM, roofline at 216 algorithmic bytes per
-    particle; the FP64 NumPy gradient timed on a 262,144-particle sample as the CPU leg."""
+    of the adjoint (in the forward's Morton order) with inputs resident in HBM, roofline at
+    216 algorithmic bytes per particle; the FP64 NumPy gradient timed on a 262,144-particle
+    sample as the CPU leg."""
     import numpy as np
     import torch
     from amdahop import density_adjoint, generate as G, knn_device
@@ -133,22 +134,26 @@ def adjoint_stage(steps, with_cpu):
     n, k = ps.n, ps.nvoisins
     pos = torch.from_numpy(ps.pos).to(dev)
     mass = torch.from_numpy(ps.mass).to(dev)
-    dens, nbr, info = knn_device(pos, mass, k)
+    dens, nbr, info, order = knn_device(pos, mass, k, return_order=True)
     w = torch.from_numpy(np.random.default_rng(3).normal(size=n)).to(dev)
     grad_rho = w / dens
-    density_adjoint(pos, mass, nbr, grad_rho)
-    dev_ms = []
+    # the forward's Morton order (as sph_density's backward passes it); the adjoint without
+    # an order is timed beside it
+    density_adjoint(pos, mass, nbr, grad_rho, order=order)
+    dev_ms, plain_ms = [], []
     for _ in range(steps):
-        dev_ms.append(density_adjoint(pos, mass, nbr, grad_rho)[2])
+        dev_ms.append(density_adjoint(pos, mass, nbr, grad_rho, order=order)[2])
+        plain_ms.append(density_adjoint(pos, mass, nbr, grad_rho)[2])
     ms = float(np.median(dev_ms))
     achieved = ADJOINT_BYTES_PER_PARTICLE * n / (ms * 1e-3) / 1e9
     out = {"n": n, "k": k, "ms": round(ms, 3), "Mparticles_per_s": round(n / (ms * 1e-3) / 1e6, 2),
            "forward_knn_ms": round(info.t_total, 3),
+           "ms_without_order": round(float(np.median(plain_ms)), 3),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                         "bytes_per_particle": ADJOINT_BYTES_PER_PARTICLE}}
     log(f"adjoint: {ms:.3f} ms device on n={n} k={k}")
-    del pos, mass, dens, nbr, w, grad_rho
+    del pos, mass, dens, nbr, order, w, grad_rho
     if with_cpu:
         import oracle.sph_adjoint as SA
         cs = G.config("c1")     # the same generator at 262,144 particles, k=32

@@ -72,9 +72,10 @@ def _check_particles(pos, mass):
     return pos.contiguous(), mass.contiguous(), n
 
 
-def knn_device(pos, mass, nvoisins, box=1.0, mixed=True, bucket=64):
+def knn_device(pos, mass, nvoisins, box=1.0, mixed=True, bucket=64, return_order=False):
     """kNN + SPH density with every neighbour row kept: returns (density (N,) float64,
-    nbr (N, nvoisins) int32 original ids in ascending distance, RunInfo)."""
+    nbr (N, nvoisins) int32 original ids in ascending distance, RunInfo), and with
+    ``return_order`` also the Morton order (N,) int32 (sorted position -> original index)."""
     import torch
 
     pos, mass, n = _check_particles(pos, mass)
@@ -83,23 +84,34 @@ def knn_device(pos, mass, nvoisins, box=1.0, mixed=True, bucket=64):
                   int(bucket), 0)
     dens = torch.empty(n, dtype=torch.float64, device=pos.device)
     nbr = torch.empty((n, int(nvoisins)), dtype=torch.int32, device=pos.device)
+    order = torch.empty(n, dtype=torch.int32, device=pos.device) if return_order else None
     info = RunInfo()
     torch.cuda.synchronize(pos.device)
     check(lib().ahop_knn_device(ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(mass.data_ptr()),
                                 n, ctypes.byref(p), ctypes.c_void_p(dens.data_ptr()),
-                                ctypes.c_void_p(nbr.data_ptr()), ctypes.byref(info)))
-    return dens, nbr, info
+                                ctypes.c_void_p(nbr.data_ptr()),
+                                ctypes.c_void_p(order.data_ptr() if return_order else None),
+                                ctypes.byref(info)))
+    return (dens, nbr, info, order) if return_order else (dens, nbr, info)
 
 
-def density_adjoint(pos, mass, nbr, grad_rho, box=1.0):
+def density_adjoint(pos, mass, nbr, grad_rho, box=1.0, order=None):
     """dLoss/dpos (N,3) and dLoss/dmass (N,) of the SPH density for the upstream gradient
-    ``grad_rho`` = dLoss/drho, with the neighbour rows ``nbr`` (N,k) frozen.  Returns
+    ``grad_rho`` = dLoss/drho, with the neighbour rows ``nbr`` (N,k) frozen.  ``order`` (N,)
+    int32, optional: a permutation of the particles that keeps neighbours close, the Morton
+    order from :func:`knn_device` (``return_order=True``); with it the neighbours' terms are
+    summed per workgroup in LDS first, and far fewer global atomics are needed.  Returns
     (grad_pos, grad_mass, device milliseconds)."""
     import torch
 
     pos, mass, n = _check_particles(pos, mass)
     if nbr.dtype != torch.int32 or nbr.dim() != 2 or nbr.shape[0] != n or not nbr.is_cuda:
         raise ValueError("nbr must be an (N, k) int32 device tensor")
+    if order is not None:
+        if (order.dtype != torch.int32 or tuple(order.shape) != (n,)
+                or order.device != pos.device):
+            raise ValueError("order must be an (N,) int32 tensor on the particles' device")
+        order = order.contiguous()
     grad_rho = grad_rho.to(torch.float64).contiguous()
     if tuple(grad_rho.shape) != (n,) or grad_rho.device != pos.device:
         raise ValueError("grad_rho must have shape (N,) on the particles' device")
@@ -111,7 +123,9 @@ def density_adjoint(pos, mass, nbr, grad_rho, box=1.0):
     torch.cuda.synchronize(pos.device)
     check(lib().ahop_density_adjoint_device(
         ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(mass.data_ptr()), n, int(nbr.shape[1]),
-        ctypes.c_void_p(nbr.data_ptr()), ctypes.c_void_p(grad_rho.data_ptr()), float(Lx),
+        ctypes.c_void_p(nbr.data_ptr()),
+        ctypes.c_void_p(order.data_ptr() if order is not None else None),
+        ctypes.c_void_p(grad_rho.data_ptr()), float(Lx),
         float(Ly), float(Lz), ctypes.c_void_p(gpos.data_ptr()), ctypes.c_void_p(gmass.data_ptr()),
         ctypes.byref(ms)))
     return gpos, gmass, ms.value
@@ -126,16 +140,18 @@ def _density_function():
 
         @staticmethod
         def forward(ctx, pos, mass, nvoisins, box, mixed):
-            dens, nbr, _ = knn_device(pos.detach(), mass.detach(), nvoisins, box, mixed)
-            ctx.save_for_backward(pos, mass, nbr)
+            dens, nbr, _, order = knn_device(pos.detach(), mass.detach(), nvoisins, box, mixed,
+                                             return_order=True)
+            ctx.save_for_backward(pos, mass, nbr, order)
             ctx.box = box
             ctx.mark_non_differentiable(nbr)
             return dens, nbr
 
         @staticmethod
         def backward(ctx, grad_rho, grad_nbr):
-            pos, mass, nbr = ctx.saved_tensors
-            gpos, gmass, _ = density_adjoint(pos.detach(), mass.detach(), nbr, grad_rho, ctx.box)
+            pos, mass, nbr, order = ctx.saved_tensors
+            gpos, gmass, _ = density_adjoint(pos.detach(), mass.detach(), nbr, grad_rho, ctx.box,
+                                             order=order)
             return gpos, gmass, None, None, None
 
     return SPHDensity

@@ -421,19 +421,20 @@ static void fill_info(ahop_run_info *info, const Engine &e, bool saddle)
 
 int ahop_knn_device(const double *d_pos, const double *d_mass, int64_t n,
                     const ahop_hot_params *params, double *d_density, int32_t *d_nbr,
-                    ahop_run_info *info)
+                    int32_t *d_order, ahop_run_info *info)
 {
     return guarded([&] {
         Params p = hot_params(params);
         if (!d_pos || !d_mass || !d_density || !d_nbr) throw std::runtime_error("null array");
         Engine &e = engine();
-        e.knn_lists(d_pos, d_mass, n, p, d_density, d_nbr);
+        e.knn_lists(d_pos, d_mass, n, p, d_density, d_nbr, d_order);
         fill_info(info, e, false);
     });
 }
 
 int ahop_density_adjoint_device(const double *d_pos, const double *d_mass, int64_t n, int32_t k,
-                                const int32_t *d_nbr, const double *d_grad_rho, double xlong,
+                                const int32_t *d_nbr, const int32_t *d_order,
+                                const double *d_grad_rho, double xlong,
                                 double ylong, double zlong, double *d_grad_pos,
                                 double *d_grad_mass, float *ms)
 {
@@ -441,8 +442,8 @@ int ahop_density_adjoint_device(const double *d_pos, const double *d_mass, int64
         if (!d_pos || !d_mass || !d_nbr || !d_grad_rho || !d_grad_pos || !d_grad_mass)
             throw std::runtime_error("null array");
         Engine &e = engine();
-        float t = sph_density_adjoint(d_pos, d_mass, d_nbr, d_grad_rho, n, k, xlong, ylong, zlong,
-                                      d_grad_pos, d_grad_mass, e.stream());
+        float t = sph_density_adjoint(d_pos, d_mass, d_nbr, d_order, d_grad_rho, n, k, xlong,
+                                      ylong, zlong, d_grad_pos, d_grad_mass, e.stream());
         if (ms) *ms = t;
     });
 }

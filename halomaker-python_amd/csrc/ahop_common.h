@@ -78,8 +78,9 @@ class Engine {
     void run(const double *d_pos, const double *d_mass, int64_t n, const Params &p, bool saddle);
     // kNN + density only, with every particle's k neighbours (original 0-based ids, ascending
     // distance) written to d_nbr (n*k) and the density to d_rho (n), both in original order
+    // d_order (n, may be null) receives the Morton order (sorted slot -> original index)
     void knn_lists(const double *d_pos, const double *d_mass, int64_t n, const Params &p,
-                   double *d_rho, int32_t *d_nbr);
+                   double *d_rho, int32_t *d_nbr, int32_t *d_order);
     const DeviceResults &results() const { return res_; }
     const StageTimes &times() const { return times_; }
     hipStream_t stream() const { return stream_; }
@@ -110,9 +111,13 @@ void set_error(const std::string &msg);
 
 // Adjoint of the SPH density with frozen neighbour rows (sph_adjoint.hip): all pointers are
 // device arrays in original order (pos (n,3), nbr (n,K)); returns the device milliseconds.
+// d_order (n, may be null): a permutation of the particles that keeps neighbours close (the
+// Morton order of the forward); with it, each workgroup takes consecutive positions of that
+// order and adds its neighbours' terms into a window of the order in LDS first.
 float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32_t *d_nbr,
-                          const double *d_grad_rho, int64_t n, int K, double Lx, double Ly,
-                          double Lz, double *d_gpos, double *d_gmass, hipStream_t s);
+                          const int32_t *d_order, const double *d_grad_rho, int64_t n, int K,
+                          double Lx, double Ly, double Lz, double *d_gpos, double *d_gmass,
+                          hipStream_t s);
 
 }  // namespace ahop
 

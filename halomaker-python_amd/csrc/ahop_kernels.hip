@@ -1203,7 +1203,7 @@ static void validate(int64_t n, const Params &p)
 }
 
 void Engine::knn_lists(const double *d_pos, const double *d_mass, int64_t n, const Params &p,
-                       double *d_rho, int32_t *d_nbr)
+                       double *d_rho, int32_t *d_nbr, int32_t *d_order)
 {
     validate(n, p);
     Params q = p;
@@ -1215,6 +1215,9 @@ void Engine::knn_lists(const double *d_pos, const double *d_mass, int64_t n, con
     hipLaunchKernelGGL(k_lists_to_original, dim3(nblk(n)), dim3(256), 0, stream_,
                        (const int32_t *)b_nbr.p, t_.perm, (const double *)b_rho_s.p, n, q.nvoisins,
                        d_nbr, d_rho);
+    if (d_order)
+        AHOP_HIP(hipMemcpyAsync(d_order, t_.perm, sizeof(int32_t) * n, hipMemcpyDeviceToDevice,
+                                stream_));
     AHOP_HIP(hipEventRecord(ev_[5], stream_));
     AHOP_HIP(hipStreamSynchronize(stream_));
     AHOP_HIP(hipEventElapsedTime(&times_.sort, ev_[0], ev_[1]));

@@ -14,8 +14,7 @@
 // where u_ij is the unit minimum-image separation x_j - x_i.  Per-pair terms and sums are
 // FP64: with FP32 pair terms the gradient of a mass that many particles list as a neighbour
 // loses up to ~3e-4 relative to cancellation (the upstream weights g_i have both signs).
-// Each particle gathers its own terms and adds its neighbours' terms with FP64 global
-// atomics.
+// Each particle gathers its own terms and adds its neighbours' terms with FP64 atomics.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -54,69 +53,139 @@ __device__ __forceinline__ void spline_wd(double q, double &w, double &wp)
     }
 }
 
-__global__ __launch_bounds__(256) void k_sph_adjoint(
+// With an order (WINDOW): workgroup b takes positions [b*ADJ_BS, (b+1)*ADJ_BS) of a
+// permutation that keeps neighbours close (the forward's Morton order), and the terms of
+// neighbours whose position lies within ADJ_HALF of the workgroup's range are first added into
+// LDS (FP64 LDS atomics) and then added to global memory once per position; only the others
+// take a global atomic per term.  Without an order, thread i takes particle i and every
+// neighbour term is a global atomic.
+constexpr int ADJ_BS = 256;
+constexpr int ADJ_HALF = 256;
+constexpr int ADJ_WIN = ADJ_BS + 2 * ADJ_HALF;   // 768 positions: 24 KiB of (x, y, z, m) sums
+
+template <bool WINDOW>
+__global__ __launch_bounds__(ADJ_BS) void k_sph_adjoint(
     const double *__restrict__ pos, const double *__restrict__ mass,
-    const int32_t *__restrict__ nbr, const double *__restrict__ grad_rho, int64_t n, int K,
+    const int32_t *__restrict__ nbr, const int32_t *__restrict__ order,
+    const int32_t *__restrict__ slot_of, const double *__restrict__ grad_rho, int64_t n, int K,
     double Lx, double Ly, double Lz, double A, double *__restrict__ gpos,
     double *__restrict__ gmass)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double g = grad_rho[i];
-    const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
-    const int32_t *row = nbr + i * (int64_t)K;
-    // smoothing length from the k-th neighbour, in the reference's FP64 arithmetic
-    const int32_t jk = row[K - 1];
-    const double kx = min_image(pos[3 * (int64_t)jk] - xi, Lx);
-    const double ky = min_image(pos[3 * (int64_t)jk + 1] - yi, Ly);
-    const double kz = min_image(pos[3 * (int64_t)jk + 2] - zi, Lz);
-    const double dk = sqrt((kx * kx + ky * ky) + kz * kz);
-    const double r = 0.5 * dk;
-    const double ir = 1.0 / r;
-    const double ir3 = ir * ir * ir;
-    const double gA3 = g * A * ir3;          // g A / r^3
-    const double gA4 = gA3 * ir;             // g A / r^4 (pair-force scale)
-    double C = 0.0, S = 0.0, ox = 0.0, oy = 0.0, oz = 0.0;
-    for (int t = 0; t < K - 1; ++t) {
-        const int64_t j = row[t];
-        const double dx = min_image(pos[3 * j] - xi, Lx);
-        const double dy = min_image(pos[3 * j + 1] - yi, Ly);
-        const double dz = min_image(pos[3 * j + 2] - zi, Lz);
-        const double d = sqrt(dx * dx + dy * dy + dz * dz);
-        double w, wp;
-        spline_wd(d * ir, w, wp);
-        const double mj = mass[j];
-        C += mj * w;
-        S += mj * wp * d;
-        unsafeAtomicAdd(&gmass[j], gA3 * w);
-        if (d > 0.0) {
-            const double c = gA4 * mj * wp / d;   // along x_j - x_i, per unit length
-            const double fx = c * dx, fy = c * dy, fz = c * dz;
+    __shared__ double acc[WINDOW ? 4 * ADJ_WIN : 1];   // [component][window position]
+    const int64_t s = blockIdx.x * (int64_t)ADJ_BS + threadIdx.x;
+    const int64_t w0 = blockIdx.x * (int64_t)ADJ_BS - ADJ_HALF;
+    if constexpr (WINDOW) {
+        for (int u = threadIdx.x; u < 4 * ADJ_WIN; u += ADJ_BS) acc[u] = 0.0;
+        __syncthreads();
+    }
+    // add (fx, fy, fz) to the position gradient and fm to the mass gradient of particle j;
+    // uj is j's window position (meaningful with WINDOW only)
+    auto add = [&](int64_t j, int64_t uj, bool with_pos, double fx, double fy, double fz,
+                   double fm) {
+        if (WINDOW && uj >= 0 && uj < ADJ_WIN) {
+            atomicAdd(&acc[3 * ADJ_WIN + uj], fm);
+            if (with_pos) {
+                atomicAdd(&acc[uj], fx);
+                atomicAdd(&acc[ADJ_WIN + uj], fy);
+                atomicAdd(&acc[2 * ADJ_WIN + uj], fz);
+            }
+        } else {
+            unsafeAtomicAdd(&gmass[j], fm);
+            if (with_pos) {
+                unsafeAtomicAdd(&gpos[3 * j], fx);
+                unsafeAtomicAdd(&gpos[3 * j + 1], fy);
+                unsafeAtomicAdd(&gpos[3 * j + 2], fz);
+            }
+        }
+    };
+    if (s < n) {
+        const int64_t i = WINDOW ? (int64_t)order[s] : s;
+        const double g = grad_rho[i];
+        const double xi = pos[3 * i], yi = pos[3 * i + 1], zi = pos[3 * i + 2];
+        const int32_t *row = nbr + i * (int64_t)K;
+        // smoothing length from the k-th neighbour, in the reference's FP64 arithmetic
+        const int32_t jk = row[K - 1];
+        const double kx = min_image(pos[3 * (int64_t)jk] - xi, Lx);
+        const double ky = min_image(pos[3 * (int64_t)jk + 1] - yi, Ly);
+        const double kz = min_image(pos[3 * (int64_t)jk + 2] - zi, Lz);
+        const double dk = sqrt((kx * kx + ky * ky) + kz * kz);
+        const double r = 0.5 * dk;
+        const double ir = 1.0 / r;
+        const double ir3 = ir * ir * ir;
+        const double gA3 = g * A * ir3;          // g A / r^3
+        const double gA4 = gA3 * ir;             // g A / r^4 (pair-force scale)
+        double C = 0.0, S = 0.0, ox = 0.0, oy = 0.0, oz = 0.0;
+        for (int t = 0; t < K - 1; ++t) {
+            const int64_t j = row[t];
+            const int64_t uj = WINDOW ? (int64_t)slot_of[j] - w0 : -1;
+            const double dx = min_image(pos[3 * j] - xi, Lx);
+            const double dy = min_image(pos[3 * j + 1] - yi, Ly);
+            const double dz = min_image(pos[3 * j + 2] - zi, Lz);
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            double w, wp;
+            spline_wd(d * ir, w, wp);
+            const double mj = mass[j];
+            C += mj * w;
+            S += mj * wp * d;
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            if (d > 0.0) {
+                const double c = gA4 * mj * wp / d;   // along x_j - x_i, per unit length
+                fx = c * dx;
+                fy = c * dy;
+                fz = c * dz;
+                ox -= fx;
+                oy -= fy;
+                oz -= fz;
+            }
+            add(j, uj, d > 0.0, fx, fy, fz, gA3 * w);
+        }
+        const double mi = mass[i];
+        // the k-th neighbour moves the smoothing length: d r / d x_{j_k} = u_ik / 2
+        if (dk > 0.0) {
+            const double drho_dr = -A * ir3 * ir * (3.0 * (C + mi) + S * ir);
+            const double c = g * drho_dr * 0.5 / dk;
+            const double fx = c * kx, fy = c * ky, fz = c * kz;
             ox -= fx;
             oy -= fy;
             oz -= fz;
-            unsafeAtomicAdd(&gpos[3 * j], fx);
-            unsafeAtomicAdd(&gpos[3 * j + 1], fy);
-            unsafeAtomicAdd(&gpos[3 * j + 2], fz);
+            add(jk, WINDOW ? (int64_t)slot_of[jk] - w0 : -1, true, fx, fy, fz, 0.0);
+        }
+        add(i, s - w0, true, ox, oy, oz, gA3);   // own terms (i's window position is s - w0)
+    }
+    if constexpr (WINDOW) {
+        __syncthreads();
+        for (int u = threadIdx.x; u < ADJ_WIN; u += ADJ_BS) {
+            const int64_t sj = w0 + u;
+            if (sj < 0 || sj >= n) continue;
+            const int64_t j = order[sj];
+            const double fm = acc[3 * ADJ_WIN + u], fx = acc[u], fy = acc[ADJ_WIN + u],
+                         fz = acc[2 * ADJ_WIN + u];
+            if (fm != 0.0) unsafeAtomicAdd(&gmass[j], fm);
+            if (fx != 0.0) unsafeAtomicAdd(&gpos[3 * j], fx);
+            if (fy != 0.0) unsafeAtomicAdd(&gpos[3 * j + 1], fy);
+            if (fz != 0.0) unsafeAtomicAdd(&gpos[3 * j + 2], fz);
         }
     }
-    const double mi = mass[i];
-    unsafeAtomicAdd(&gmass[i], gA3);
-    // the k-th neighbour moves the smoothing length: d r / d x_{j_k} = u_ik / 2
-    if (dk > 0.0) {
-        const double drho_dr = -A * ir3 * ir * (3.0 * (C + mi) + S * ir);
-        const double c = g * drho_dr * 0.5 / dk;
-        const double fx = c * kx, fy = c * ky, fz = c * kz;
-        ox -= fx;
-        oy -= fy;
-        oz -= fz;
-        unsafeAtomicAdd(&gpos[3 * (int64_t)jk], fx);
-        unsafeAtomicAdd(&gpos[3 * (int64_t)jk + 1], fy);
-        unsafeAtomicAdd(&gpos[3 * (int64_t)jk + 2], fz);
-    }
-    unsafeAtomicAdd(&gpos[3 * i], ox);
-    unsafeAtomicAdd(&gpos[3 * i + 1], oy);
-    unsafeAtomicAdd(&gpos[3 * i + 2], oz);
+}
+
+// slot_of[order[s]] = s (slot_of starts at -1, so entries left at -1 reveal a non-permutation)
+__global__ void k_inverse_order(const int32_t *__restrict__ order, int64_t n,
+                                int32_t *__restrict__ slot_of)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s < n) slot_of[order[s]] = (int32_t)s;
+}
+
+__global__ void k_fill_m1(int32_t *__restrict__ a, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) a[i] = -1;
+}
+
+__global__ void k_count_negative(const int32_t *__restrict__ a, int64_t n, int *__restrict__ bad)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n && a[i] < 0) atomicAdd(bad, 1);
 }
 
 __global__ void k_zero(double *__restrict__ a, int64_t n)
@@ -139,40 +208,74 @@ __global__ void k_count_bad_ids(const int32_t *__restrict__ nbr, int64_t m, int6
 static constexpr double PI_REF_HOST = 3.1415927410125732421875;
 
 float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32_t *d_nbr,
-                          const double *d_grad_rho, int64_t n, int K, double Lx, double Ly,
-                          double Lz, double *d_gpos, double *d_gmass, hipStream_t s)
+                          const int32_t *d_order, const double *d_grad_rho, int64_t n, int K,
+                          double Lx, double Ly, double Lz, double *d_gpos, double *d_gmass,
+                          hipStream_t s)
 {
     if (n < 2) throw std::runtime_error("need at least 2 particles");
     if (K < 1 || K >= n) throw std::runtime_error("need 1 <= k < npart");
     if (!(Lx > 0 && Ly > 0 && Lz > 0)) throw std::runtime_error("box sides must be > 0");
+    if (n > INT32_MAX) throw std::runtime_error("npart must fit in int32");
     const double A = Lx * Ly * Lz / PI_REF_HOST;
+    const unsigned b1 = (unsigned)((n + 255) / 256);
+    int32_t *d_slot = nullptr;
     {
+        // neighbour ids in [0, n), and the order (if any) a permutation whose inverse is
+        // built here; the kernels index positions and LDS with them
         int *d_bad = nullptr;
         AHOP_HIP(hipMalloc(&d_bad, sizeof(int)));
-        int bad = 0;
-        hipError_t e = hipMemcpyAsync(d_bad, &bad, sizeof(int), hipMemcpyHostToDevice, s);
+        hipError_t e = d_order ? hipMalloc(&d_slot, sizeof(int32_t) * n) : hipSuccess;
+        int bad[2] = {0, 0};
+        if (e == hipSuccess) e = hipMemcpyAsync(d_bad, &bad[0], sizeof(int), hipMemcpyHostToDevice, s);
         if (e == hipSuccess) {
             const int64_t m = n * (int64_t)K;
             hipLaunchKernelGGL(k_count_bad_ids, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
                                d_nbr, m, n, d_bad);
-            e = hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s);
+            e = hipMemcpyAsync(&bad[0], d_bad, sizeof(int), hipMemcpyDeviceToHost, s);
+        }
+        if (e == hipSuccess && d_order) {
+            e = hipMemcpyAsync(d_bad, &bad[1], sizeof(int), hipMemcpyHostToDevice, s);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_count_bad_ids, dim3(b1), dim3(256), 0, s, d_order, n, n, d_bad);
+                e = hipMemcpyAsync(&bad[1], d_bad, sizeof(int), hipMemcpyDeviceToHost, s);
+            }
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e == hipSuccess && bad[1] == 0) {
+                hipLaunchKernelGGL(k_fill_m1, dim3(b1), dim3(256), 0, s, d_slot, n);
+                hipLaunchKernelGGL(k_inverse_order, dim3(b1), dim3(256), 0, s, d_order, n, d_slot);
+                hipLaunchKernelGGL(k_count_negative, dim3(b1), dim3(256), 0, s, d_slot, n, d_bad);
+                e = hipMemcpyAsync(&bad[1], d_bad, sizeof(int), hipMemcpyDeviceToHost, s);
+            }
         }
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         (void)hipFree(d_bad);
-        AHOP_HIP(e);
-        if (bad) throw std::runtime_error(std::to_string(bad) + " neighbour ids outside [0, npart)");
+        if (e != hipSuccess || bad[0] || bad[1]) {
+            if (d_slot) (void)hipFree(d_slot);
+            AHOP_HIP(e);
+            if (bad[0])
+                throw std::runtime_error(std::to_string(bad[0]) + " neighbour ids outside [0, npart)");
+            throw std::runtime_error("order is not a permutation of [0, npart)");
+        }
     }
     hipEvent_t e0, e1;
     AHOP_HIP(hipEventCreate(&e0));
     AHOP_HIP(hipEventCreate(&e1));
     AHOP_HIP(hipEventRecord(e0, s));
-    const unsigned b3 = (unsigned)((3 * n + 255) / 256), b1 = (unsigned)((n + 255) / 256);
+    const unsigned b3 = (unsigned)((3 * n + 255) / 256);
     hipLaunchKernelGGL(k_zero, dim3(b3), dim3(256), 0, s, d_gpos, 3 * n);
     hipLaunchKernelGGL(k_zero, dim3(b1), dim3(256), 0, s, d_gmass, n);
-    hipLaunchKernelGGL(k_sph_adjoint, dim3(b1), dim3(256), 0, s, d_pos, d_mass, d_nbr, d_grad_rho,
-                       n, K, Lx, Ly, Lz, A, d_gpos, d_gmass);
+    if (d_order)
+        hipLaunchKernelGGL(k_sph_adjoint<true>, dim3((unsigned)((n + ADJ_BS - 1) / ADJ_BS)),
+                           dim3(ADJ_BS), 0, s, d_pos, d_mass, d_nbr, d_order, d_slot, d_grad_rho, n,
+                           K, Lx, Ly, Lz, A, d_gpos, d_gmass);
+    else
+        hipLaunchKernelGGL(k_sph_adjoint<false>, dim3((unsigned)((n + ADJ_BS - 1) / ADJ_BS)),
+                           dim3(ADJ_BS), 0, s, d_pos, d_mass, d_nbr, nullptr, nullptr, d_grad_rho,
+                           n, K, Lx, Ly, Lz, A, d_gpos, d_gmass);
     AHOP_HIP(hipEventRecord(e1, s));
-    AHOP_HIP(hipStreamSynchronize(s));
+    hipError_t e = hipStreamSynchronize(s);
+    if (d_slot) (void)hipFree(d_slot);
+    AHOP_HIP(e);
     float ms = 0;
     AHOP_HIP(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);

@@ -108,22 +108,28 @@ int ahop_device_saddle(int32_t *offset, int32_t *isad, double *rho);
 
 /* kNN + SPH density alone, keeping every particle's neighbour row: d_density (n) and
  * d_nbr (n x nvoisins int32, original 0-based ids in ascending distance, self excluded),
- * both device pointers in original order (params->nhop and ->saddle are ignored).  The
+ * both device pointers in original order (params->nhop and ->saddle are ignored), and the
+ * Morton order of the particles to d_order (n int32, sorted position -> original index; may
+ * be NULL).  The
  * per-particle lists the reference builds in omp_find_nearest_parts / walk_tree2
  * (f90:2841-2962, iparnei) and sums in omp_compute_density (f90:2816-2838), exported so a
  * caller can freeze them (e.g. for ahop_density_adjoint_device). */
 int ahop_knn_device(const double *d_pos, const double *d_mass, int64_t n,
                     const ahop_hot_params *params, double *d_density, int32_t *d_nbr,
-                    ahop_run_info *info);
+                    int32_t *d_order, ahop_run_info *info);
 
 /* Adjoint of the SPH density of omp_compute_density (f90:2816-2838) with the neighbour rows
  * frozen: given d_grad_rho (n) = dLoss/drho, writes dLoss/dpos to d_grad_pos (n,3) and
- * dLoss/dmass to d_grad_mass (n).  d_nbr (n x k) as ahop_knn_device writes it.  No
+ * dLoss/dmass to d_grad_mass (n).  d_nbr (n x k) as ahop_knn_device writes it.  d_order (n,
+ * may be NULL): a permutation of the particles, the d_order of ahop_knn_device; with it the
+ * neighbours' terms are first summed per workgroup over a window of that order, which cuts
+ * the global atomics several-fold.  No
  * counterpart in the reference (which has no gradients); it differentiates the density the
  * reference computes.  FP64 throughout; neighbour contributions are added with atomics,
  * so repeated calls agree to rounding.  *ms = device time (may be NULL).  Synchronous. */
 int ahop_density_adjoint_device(const double *d_pos, const double *d_mass, int64_t n, int32_t k,
-                                const int32_t *d_nbr, const double *d_grad_rho, double xlong,
+                                const int32_t *d_nbr, const int32_t *d_order,
+                                const double *d_grad_rho, double xlong,
                                 double ylong, double zlong, double *d_grad_pos,
                                 double *d_grad_mass, float *ms);
 

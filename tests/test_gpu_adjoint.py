@@ -57,18 +57,22 @@ def test_adjoint_matches_the_fp64_gradient(gpu):
     ps = G.config("c1")
     log(f"adjoint c1 n={ps.n} k=32")
     pos, mass = to_dev(ps, gpu)
-    dens, nbr, _ = knn_device(pos, mass, 32)
+    dens, nbr, _, order = knn_device(pos, mass, 32, return_order=True)
+    # the Morton order is a permutation of the particles
+    assert np.array_equal(np.sort(order.cpu().numpy()), np.arange(ps.n))
     w = np.random.default_rng(3).normal(size=ps.n)
     rho = dens.cpu().numpy()
     grad_rho = w / rho
-    gp, gm, ms = density_adjoint(pos, mass, nbr, torch.from_numpy(grad_rho).to(gpu))
-    log(f"adjoint device ms {ms:.3f}")
     ref_p, ref_m = SA.density_grad(ps.pos, ps.mass, nbr.cpu().numpy().astype(np.int64), grad_rho)
-    ep = gradient_error(gp.cpu().numpy(), ref_p)
-    em = gradient_error(gm.cpu().numpy(), ref_m)
-    log(f"position gradient error max {ep[0]:.2e} median {ep[1]:.2e}; "
-        f"mass gradient error max {em[0]:.2e} median {em[1]:.2e}")
-    assert ep[0] < TOL and em[0] < TOL
+    # per-thread global atomics, and per-workgroup LDS sums over a window of the order
+    for o in (None, order):
+        gp, gm, ms = density_adjoint(pos, mass, nbr, torch.from_numpy(grad_rho).to(gpu), order=o)
+        log(f"adjoint device ms {ms:.3f} (Morton order: {o is not None})")
+        ep = gradient_error(gp.cpu().numpy(), ref_p)
+        em = gradient_error(gm.cpu().numpy(), ref_m)
+        log(f"position gradient error max {ep[0]:.2e} median {ep[1]:.2e}; "
+            f"mass gradient error max {em[0]:.2e} median {em[1]:.2e}")
+        assert ep[0] < TOL and em[0] < TOL
 
 
 def test_adjoint_matches_central_differences_on_1000_particles(gpu):
@@ -146,3 +150,14 @@ def test_bad_neighbour_rows_are_rejected(gpu):
     nbr[7, 3] = ps.n                       # one id past the end
     with pytest.raises(RuntimeError, match="outside"):
         density_adjoint(pos, mass, nbr, torch.ones(ps.n, dtype=torch.float64, device=gpu))
+    # an order with an id outside the particles, and one that repeats an id
+    nbr[7, 3] = 0
+    order = torch.arange(ps.n, dtype=torch.int32, device=gpu)
+    order[11] = -1
+    with pytest.raises(RuntimeError, match="permutation"):
+        density_adjoint(pos, mass, nbr, torch.ones(ps.n, dtype=torch.float64, device=gpu),
+                        order=order)
+    order[11] = 12
+    with pytest.raises(RuntimeError, match="permutation"):
+        density_adjoint(pos, mass, nbr, torch.ones(ps.n, dtype=torch.float64, device=gpu),
+                        order=order)
