@@ -448,6 +448,20 @@ int ahop_density_adjoint_device(const double *d_pos, const double *d_mass, int64
     });
 }
 
+int ahop_group_members_device(const int32_t *d_node_of_part, int64_t n,
+                              const int32_t *d_node_to_struct, int32_t nnodes, int32_t nstruct,
+                              int32_t *d_struct_of_part, int32_t *d_pids, int64_t *d_offsets,
+                              float *ms)
+{
+    return guarded([&] {
+        if (!d_node_of_part || !d_pids || !d_offsets) throw std::runtime_error("null array");
+        Engine &e = engine();
+        float t = group_members(d_node_of_part, n, d_node_to_struct, nnodes, nstruct,
+                                d_struct_of_part, d_pids, d_offsets, e.stream());
+        if (ms) *ms = t;
+    });
+}
+
 int ahop_run_device(const double *d_pos, const double *d_mass, int64_t n,
                     const ahop_hot_params *params, double *d_density, int32_t *d_gid,
                     ahop_run_info *info)

@@ -114,6 +114,15 @@ void set_error(const std::string &msg);
 // d_order (n, may be null): a permutation of the particles that keeps neighbours close (the
 // Morton order of the forward); with it, each workgroup takes consecutive positions of that
 // order and adds its neighbours' terms into a window of the order in LDS first.
+// Particles grouped by label: labels = node_to_struct[node - 1] for node > 0 (0 stays 0; the
+// node ids themselves when node_to_struct is null, nlab = nnodes).  Writes the labels
+// (d_labels_out, may be null), the particle indices sorted by label, ascending within a label
+// (d_pids, n), and CSR offsets (d_offsets, nlab + 2: label l holds d_pids[off[l], off[l+1])).
+// Returns the device time in ms.
+float group_members(const int32_t *d_node, int64_t n, const int32_t *d_node_to_struct,
+                    int32_t nnodes, int32_t nlab, int32_t *d_labels_out, int32_t *d_pids,
+                    int64_t *d_offsets, hipStream_t s);
+
 float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32_t *d_nbr,
                           const int32_t *d_order, const double *d_grad_rho, int64_t n, int K,
                           double Lx, double Ly, double Lz, double *d_gpos, double *d_gmass,

@@ -133,6 +133,23 @@ int ahop_density_adjoint_device(const double *d_pos, const double *d_mass, int64
                                 double ylong, double zlong, double *d_grad_pos,
                                 double *d_grad_mass, float *ms);
 
+/* ---- particles grouped by structure ------------------------------------------------- */
+
+/* Members of every structure in one call, on device arrays.  d_node_of_part (n): the node id
+ * of each particle (liste_parts, 1-based, 0 = in no node).  d_node_to_struct (nnodes, may be
+ * NULL): 1-based structure id of each node, as select_with_MS_method_1319 builds it
+ * (src/compute_neiKDtree_mod.py:341-348); with NULL the labels are the node ids and nstruct
+ * must equal nnodes.  Outputs: d_struct_of_part (n, may be NULL) = the relabelled particles
+ * (:376-379); d_pids (n) = 0-based particle indices sorted by label, ascending within a label;
+ * d_offsets (nstruct + 2, int64): label l (0 = unassigned) owns d_pids[off[l] .. off[l+1]).
+ * Replaces the relabelling of :376-379 and the grouping of compute_adaptahop_131 (:109-119:
+ * argsort, np.unique first indices and counts), whose argsort is unstable (member order
+ * within a structure is not part of the contract).  *ms = device time (may be NULL). */
+int ahop_group_members_device(const int32_t *d_node_of_part, int64_t n,
+                              const int32_t *d_node_to_struct, int32_t nnodes, int32_t nstruct,
+                              int32_t *d_struct_of_part, int32_t *d_pids, int64_t *d_offsets,
+                              float *ms);
+
 /* ---- batched per-halo property reductions ------------------------------------------- */
 
 /* Row layout of the output (one row of AHOP_HALO_NFIELDS doubles per halo):
