@@ -6,7 +6,9 @@ box (k=64, nhop=16): Morton sort, device tree, kNN + density, hop targets, group
 Inputs are in HBM before the timed region; outputs (density, group ids) stay in HBM.
 Prints one JSON line.  Also reports, in the same line, the plain FP64 device path, the
 saddle-edge stage, the roofline of the dominant kernel (kNN + density) and the
-reference's own OpenMP Fortran path timed on this host's cores on a bounded sample.
+reference's own OpenMP Fortran path timed on this host's cores on a bounded sample, and
+the stages beside the path: particles grouped by structure, the batched halo properties and
+the density adjoint.
 """
 import argparse
 import json
@@ -33,6 +35,9 @@ HALO_BYTES_PER_MEMBER = 60
 # per particle of the density adjoint (k=32): position 24, mass 8, upstream gradient 8,
 # density 8, smoothing length 8, neighbour ids 4k = 128, position gradient 24, mass gradient 8
 ADJOINT_BYTES_PER_PARTICLE = 216
+# per particle of the grouping by structure: node id 4 B read, label 4 B and grouped particle
+# index 4 B written
+GROUP_BYTES_PER_PARTICLE = 12
 
 
 def log(msg):
@@ -119,6 +124,55 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
     return out
 
 
+def grouping_stage(gid, ngroups, steps, with_cpu):
+    """Particles grouped by structure on the headline box: the step's group ids stand for the
+    node ids of liste_parts (0 = in none), grouped by id and through a node-to-structure map
+    onto a third as many structures (as MSM merges nodes); device time with the ids resident
+    in HBM, roofline at 12 algorithmic bytes per particle; the reference's own NumPy grouping
+    (argsort + np.unique, compute_neiKDtree_mod.py:109-119) on the same ids as the CPU leg."""
+    import numpy as np
+    import torch
+    from amdahop.structures import group_members_device
+
+    dev = gid.device
+    n = int(gid.shape[0])
+    nstruct = max(1, ngroups // 3)
+    mp = torch.from_numpy(np.random.default_rng(5).integers(1, nstruct + 1, size=max(1, ngroups))
+                          .astype(np.int32)).to(dev)
+    labels = torch.empty(n, dtype=torch.int32, device=dev)
+    pids = torch.empty(n, dtype=torch.int32, device=dev)
+    off_n = torch.empty(ngroups + 2, dtype=torch.int64, device=dev)
+    off_s = torch.empty(nstruct + 2, dtype=torch.int64, device=dev)
+    group_members_device(gid, labels, pids, off_n, nnodes=ngroups)
+    by_node, by_struct = [], []
+    for _ in range(steps):
+        by_node.append(group_members_device(gid, labels, pids, off_n, nnodes=ngroups))
+        by_struct.append(group_members_device(gid, labels, pids, off_s, node_to_struct=mp,
+                                              nstruct=nstruct))
+    ms = float(np.median(by_struct))
+    achieved = GROUP_BYTES_PER_PARTICLE * n / (ms * 1e-3) / 1e9
+    out = {"n": n, "nodes": ngroups, "structures": nstruct, "ms": round(ms, 3),
+           "ms_by_node_id": round(float(np.median(by_node)), 3),
+           "Mparticles_per_s": round(n / (ms * 1e-3) / 1e6, 2),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "bytes_per_particle": GROUP_BYTES_PER_PARTICLE}}
+    log(f"grouping: {ms:.3f} ms device on n={n}, {ngroups} nodes -> {nstruct} structures")
+    if with_cpu:
+        node = gid.cpu().numpy()
+        lab = np.where(node > 0, mp.cpu().numpy()[np.maximum(node, 1) - 1], 0)
+        c0 = time.perf_counter()
+        order = np.argsort(lab)
+        np.unique(lab[order], return_index=True, return_counts=True)
+        t = time.perf_counter() - c0
+        out["cpu_baseline"] = {"value": round(n / t / 1e6, 4), "unit": "Mparticles/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"the whole workload ({n} labels), the reference's "
+                                         f"NumPy argsort + np.unique, {t:.1f} s"}
+    del labels, pids, off_n, off_s, mp
+    return out
+
+
 def adjoint_stage(steps, with_cpu):
     """SPH density adjoint on the 2,097,152-particle clumpy box (k=32): forward kNN rows
     frozen, upstream gradient of L = sum w log rho with w ~ N(0,1) (seed 3); device time
@@ -184,6 +238,7 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=2_097_152)
     ap.add_argument("--no-halo-props", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-grouping", action="store_true")
     # one plain device copy of this many MiB before the timed steps: under a counter
     # profiler its FETCH_SIZE / WRITE_SIZE readings calibrate bytes per counter unit
     ap.add_argument("--copy-calibration-mib", type=int, default=0)
@@ -295,6 +350,9 @@ def main():
             roofline["traffic"] = int(tr["fetch_bytes"] + tr["write_bytes"])
             roofline["traffic_source"] = os.path.relpath(tpath, ROOT)
 
+    if not args.no_grouping:
+        extra["grouping"] = grouping_stage(gid, int(last.ngroups), max(3, args.steps),
+                                           not args.no_cpu_baseline)
     del pos, mass, dens, gid
     if not args.no_halo_props:
         extra["halo_props"] = halo_props_stage(max(2, args.steps // 2), args.cpu_sample_halos,

@@ -163,6 +163,34 @@ def select_with_ms_method(node_0, strict=False):
                          first_daughter, level)
 
 
+def group_members_device(node, labels, pids, offsets, node_to_struct=None, nnodes=None,
+                         nstruct=None):
+    """Device form of :func:`group_members`: every argument a CUDA tensor already in place
+    (``node``, ``labels``, ``pids`` (N,) int32; ``offsets`` (nlab + 2,) int64, where nlab is
+    ``nstruct`` with a map and ``nnodes`` without).  Writes the labels, the grouped particle
+    indices and ``offsets[l]`` = first grouped position of label l (so label l holds
+    ``pids[offsets[l]:offsets[l + 1]]``); returns the device time in ms."""
+    n = int(node.shape[0])
+    nn = int(node_to_struct.shape[0]) if node_to_struct is not None else int(nnodes)
+    nlab = int(nstruct) if node_to_struct is not None else nn
+    for t, name, dt, shape in ((node, "node", "int32", (n,)), (labels, "labels", "int32", (n,)),
+                               (pids, "pids", "int32", (n,)),
+                               (offsets, "offsets", "int64", (nlab + 2,))):
+        if not t.is_cuda or str(t.dtype) != f"torch.{dt}" or tuple(t.shape) != shape \
+                or not t.is_contiguous():
+            raise ValueError(f"{name} must be a contiguous CUDA {dt} tensor of shape {shape}")
+    import torch
+    # the library runs on its own stream: finish torch's work on the inputs first
+    torch.cuda.synchronize(node.device)
+    ms = ctypes.c_float(0.0)
+    check(lib().ahop_group_members_device(
+        ctypes.c_void_p(node.data_ptr()), n,
+        ctypes.c_void_p(node_to_struct.data_ptr() if node_to_struct is not None else None),
+        nn, nlab, ctypes.c_void_p(labels.data_ptr()), ctypes.c_void_p(pids.data_ptr()),
+        ctypes.c_void_p(offsets.data_ptr()), ctypes.byref(ms)))
+    return ms.value
+
+
 def group_members(node_of_part, node_to_struct=None, nnodes=None):
     """Relabel particles by structure and group them (ahop_group_members_device).
 
@@ -192,14 +220,10 @@ def group_members(node_of_part, node_to_struct=None, nnodes=None):
     labels = torch.empty(n, dtype=torch.int32, device=dev)
     pids = torch.empty(n, dtype=torch.int32, device=dev)
     offsets = torch.empty(nlab + 2, dtype=torch.int64, device=dev)
-    ms = ctypes.c_float(0.0)
-    torch.cuda.synchronize()
-    check(lib().ahop_group_members_device(
-        ctypes.c_void_p(node.data_ptr()), n, ctypes.c_void_p(mp.data_ptr() if mp is not None else None),
-        nn, nlab, ctypes.c_void_p(labels.data_ptr()), ctypes.c_void_p(pids.data_ptr()),
-        ctypes.c_void_p(offsets.data_ptr()), ctypes.byref(ms)))
+    ms = group_members_device(node, labels, pids, offsets, node_to_struct=mp, nnodes=nn,
+                              nstruct=nlab)
     off = offsets.cpu().numpy()
     counts = np.diff(off)[:nlab + 1]
     present = np.nonzero(counts > 0)[0]
     return (labels.cpu().numpy(), pids.cpu().numpy(), present.astype(np.int32),
-            off[present].astype(np.int32), counts[present].astype(np.int32), ms.value)
+            off[present].astype(np.int32), counts[present].astype(np.int32), ms)
