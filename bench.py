@@ -105,6 +105,13 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                         "bytes_per_member": HALO_BYTES_PER_MEMBER}}
     log(f"halo props: {ms:.2f} ms device ({wall_ms:.2f} ms wall), tile {tile} wg {wg}")
+    # the same with the SC recentring of the reference's example parameter file (eps_SC 0.3,
+    # dcell_min 0.001 Mpc: 1e-5 of a 100 Mpc box, the unit length here)
+    sc = dict(sc=True, dcell_min=1e-5, eps_sc=0.3)
+    sc_ms = [halo_props(hs.offsets, pos, vel, mass, hub_pt=70.3, tile=tile, workgroup=wg, **sc)[1].ms
+             for _ in range(steps)]
+    out["with_sc_recentring"] = {"ms": round(float(np.median(sc_ms)), 3), **sc}
+    log(f"halo props with SC: {out['with_sc_recentring']['ms']:.2f} ms")
     del pos, vel, mass
     if with_cpu:
         import oracle.halo_props as HP

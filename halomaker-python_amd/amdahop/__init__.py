@@ -188,9 +188,11 @@ HALO_FIELDS = ["m", "px", "py", "pz", "vx", "vy", "vz", "sigma", "Lx", "Ly", "Lz
 
 def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst=430.1,
                vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000,
-               tile=0, workgroup=0):
+               tile=0, workgroup=0, sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0):
     """Per-halo properties of a CSR member layout (device tensors), one row per halo in
-    the order of HALO_FIELDS.  Returns (out tensor (H, 29) float64, HaloInfo)."""
+    the order of HALO_FIELDS.  ``sc`` / ``dpmmc`` recentre each halo as the reference's
+    SC / DPMMC options do (with ``dcell_min`` and ``eps_sc``; DPMMC needs ``density``).
+    Returns (out tensor (H, 29) float64, HaloInfo)."""
     import numpy as np
     import torch
 
@@ -209,7 +211,8 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
     out = torch.empty((nh, len(HALO_FIELDS)), dtype=torch.float64, device=pos.device)
     p = HaloParams(float(lbox), float(hub_pt), float(gravconst), float(vir_overdens),
                    float(rho_mean), 1 if cdm else 0, int(nshell), int(direct_pe_nmax), int(tile),
-                   int(workgroup))
+                   int(workgroup), 1 if sc else 0, 1 if dpmmc else 0, float(dcell_min),
+                   float(eps_sc))
     info = HaloInfo()
     torch.cuda.synchronize(pos.device)
     dptr = ctypes.c_void_p(density.data_ptr()) if density is not None else None

@@ -28,7 +28,8 @@ class HotParams(ctypes.Structure):
 class HaloParams(ctypes.Structure):
     _fields_ = [("lbox", _dbl), ("hub_pt", _dbl), ("gravconst", _dbl), ("vir_overdens", _dbl),
                 ("rho_mean", _dbl), ("cdm", _i32), ("nshell", _i32), ("direct_pe_nmax", _i32),
-                ("tile", _i32), ("workgroup", _i32)]
+                ("tile", _i32), ("workgroup", _i32), ("sc", _i32), ("dpmmc", _i32),
+                ("dcell_min", _dbl), ("eps_sc", _dbl)]
 
 
 class HaloInfo(ctypes.Structure):

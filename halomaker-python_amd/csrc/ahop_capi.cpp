@@ -530,6 +530,10 @@ int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double 
         P.cdm = params->cdm;
         P.nshell = params->nshell;
         P.direct_pe_nmax = params->direct_pe_nmax;
+        P.sc = params->sc;
+        P.dpmmc = params->dpmmc;
+        P.dcell_min = params->dcell_min;
+        P.eps_sc = params->eps_sc;
         if (!(P.lbox > 0)) throw std::runtime_error("lbox must be > 0");
         Engine &e = engine();
         HaloTuning t = halo_props_device(offsets, nhalos, d_pos, d_vel, d_mass, d_density, P,

@@ -22,6 +22,10 @@ struct HaloParamsDev {
     int cdm = 1;                // 1: member nearest the COM (det_center_18 cdm branch); 0: densest
     int nshell = 1000;          // ellipsoidal shells of tab_props_inside_1b40 (ttab)
     int direct_pe_nmax = 1000;  // pair-sum potential energy below this member count
+    int sc = 0;                 // SC: shrinking-sphere recentring (refine_centers_1b)
+    int dpmmc = 0;              // DPMMC: densest member of the most massive cell, then SC if on
+    double dcell_min = 0.0;     // smallest cell / sphere radius of the recentring
+    double eps_sc = 0.0;        // sphere radius shrink fraction of SC
 };
 
 struct HaloTuning {

@@ -3,6 +3,7 @@
 // Restates, for every halo of a CSR member layout at once, reference
 // src/compute_halo_props.py:
 //   det_mass_17 :754-780 (m), det_center_18 :901-966 (centre, mean velocity, sigma),
+//   refine_centers_1b :1555-1567 (DPMMC :1447-1495 and SC :1498-1540 recentring),
 //   compute_ang_mom_19 :783-824 (L), r_halos_1a :828-851 (r),
 //   det_inertial_tensor_1b20 :1932-1951 + det_main_axis_1b2 :1954-1967 (I, axes),
 //   det_halo_energies_1b3 :1400-1444 (ek, direct-sum ep below 1000 members),
@@ -189,6 +190,459 @@ __global__ void k_combine2(int nh, const int32_t *tile_first, const double *pkey
             bi = pidx[t];
         }
     if (bi < 0) bi = off[h];   // every key NaN (non-finite input): take the first member
+    HaloState &S = st[h];
+    S.icenter = bi;
+    S.px = pos[3 * bi];
+    S.py = pos[3 * bi + 1];
+    S.pz = pos[3 * bi + 2];
+}
+
+// ---- recentring (refine_centers_1b :1555-1567): DPMMC, the multiscale grid
+// (det_halo_center_multiscale_1b0 :1447-1495), then SC, the shrinking sphere
+// (det_halo_center_sphere_1b1 :1498-1540).  One workgroup per halo; every grid level and
+// every sphere is one block-wide pass over the halo's members (read from L2 after the first).
+enum RecenterError { RC_DPMMC_INDEX = 1, RC_SC_DISTMIN = 2, RC_TOO_DEEP = 4 };
+constexpr int64_t SC_TILED = 8192;     // SC alone: halos from this size on go sphere by sphere
+                                       // over their member tiles
+constexpr int RC_MAX_LEVELS = 256;     // grid levels (each shrinks the cube by 3)
+constexpr int RC_MAX_SPHERES = 1000;   // spheres: the reference recurses once per sphere
+
+// Block-wide reductions: lanes of a wave by cross-lane shuffles, then the waves in order
+// through LDS (deterministic: a fixed combination order).  sh holds 2 * BS / 64 entries.
+template <int BS>
+__device__ double block_max_nan(double v, double *sh)
+{
+    // np.max: a NaN anywhere makes the maximum NaN
+    auto mx = [](double a, double b) { return (a == a && (b != b || b > a)) ? b : a; };
+    for (int o = 32; o > 0; o >>= 1) v = mx(v, __shfl_xor(v, o));
+    if (BS > 64) {
+        constexpr int NW = BS / 64;
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+        __syncthreads();
+        v = sh[0];
+        for (int w = 1; w < NW; ++w) v = mx(v, sh[w]);
+        __syncthreads();
+    }
+    return v;
+}
+
+// smallest key, lowest index on ties (np.argmin's first occurrence)
+template <int BS>
+__device__ void block_argmin(double &key, int64_t &idx, double *sk, int64_t *si)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        const double k2 = __shfl_xor(key, o);
+        const int64_t i2 = __shfl_xor(idx, o);
+        if (k2 < key || (k2 == key && i2 < idx)) {
+            key = k2;
+            idx = i2;
+        }
+    }
+    if (BS > 64) {
+        constexpr int NW = BS / 64;
+        if ((threadIdx.x & 63) == 0) {
+            sk[threadIdx.x >> 6] = key;
+            si[threadIdx.x >> 6] = idx;
+        }
+        __syncthreads();
+        key = sk[0];
+        idx = si[0];
+        for (int w = 1; w < NW; ++w)
+            if (sk[w] < key || (sk[w] == key && si[w] < idx)) {
+                key = sk[w];
+                idx = si[w];
+            }
+        __syncthreads();
+    }
+}
+
+template <int BS, int NQ>
+__device__ void block_sum_shfl(double (&v)[NQ], double *sh)
+{
+    for (int q = 0; q < NQ; ++q)
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_xor(v[q], o);
+    if (BS > 64) {
+        constexpr int NW = BS / 64;
+        if ((threadIdx.x & 63) == 0)
+            for (int q = 0; q < NQ; ++q) sh[q * NW + (threadIdx.x >> 6)] = v[q];
+        __syncthreads();
+        for (int q = 0; q < NQ; ++q) {
+            double a = sh[q * NW];
+            for (int w = 1; w < NW; ++w) a += sh[q * NW + w];
+            v[q] = a;
+        }
+        __syncthreads();
+    }
+}
+
+// NGP cell of a position in an nx^3 grid from xmin with cells dx wide (:1460-1465): the
+// quotient truncated toward zero must lie in [0, nx), i.e. the quotient in (-1, nx)
+__device__ __forceinline__ bool ngp_cell(const double *p, const double *xmin, double dx, int nx,
+                                         int &cell)
+{
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+        const double q = (p[d] - xmin[d]) / dx;
+        if (!(q > -1.0 && q < (double)nx)) return false;
+        c[d] = (int)q;
+    }
+    cell = (c[0] * nx + c[1]) * nx + c[2];
+    return true;
+}
+
+// One workgroup per halo of the list; BS is sized to the halos' member counts by the host.
+// grid: dynamic LDS of 9^3 doubles, present when dpmmc is on.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_recenter(HaloDev D, HaloState *st, const int32_t *halos,
+                                                 int sc, int dpmmc, double dcell_min,
+                                                 double eps_sc, int *err)
+{
+    constexpr int NW = BS / 64;
+    __shared__ double sh[4 * NW];
+    __shared__ int64_t si[NW];
+    __shared__ int scan[BS];
+    __shared__ int64_t found;
+    extern __shared__ double grid[];
+    const int h = halos[blockIdx.x];
+    const int t = threadIdx.x;
+    const int64_t f = D.off[h], e = D.off[h + 1];
+    const double L = D.L;
+    HaloState &S = st[h];
+    double c[3] = {S.px, S.py, S.pz};
+    int64_t ic = S.icenter;
+
+    // _radius_from_center :1543-1548
+    auto radius = [&]() {
+        double v = 0.0;
+#pragma unroll 4
+        for (int64_t i = f + t; i < e; i += BS) {
+            const double dx = wrap_strict(D.pos[3 * i] - c[0], L);
+            const double dy = wrap_strict(D.pos[3 * i + 1] - c[1], L);
+            const double dz = wrap_strict(D.pos[3 * i + 2] - c[2], L);
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (v == v && (d2 != d2 || d2 > v)) v = d2;
+        }
+        return sqrt(block_max_nan<BS>(v, sh));
+    };
+
+    double r0 = radius();
+    if (!(isfinite(r0) && r0 > 0.0)) return;       // block-uniform
+
+    if (dpmmc) {
+        // the densest member (first on ties): its index within the halo picks the centre
+        double bk = INFINITY;
+        int64_t bi = INT64_MAX;
+        for (int64_t i = f + t; i < e; i += BS)
+            if (-D.dens[i] < bk) {
+                bk = -D.dens[i];
+                bi = i;
+            }
+        block_argmin<BS>(bk, bi, sh, si);
+        const int64_t a = bi == INT64_MAX ? 0 : bi - f;
+        double x0[3] = {c[0], c[1], c[2]};
+        double rr = r0;
+        int nx = 9;
+        int64_t target = -2;                           // -2: still descending
+        for (int level = 0; level < RC_MAX_LEVELS; ++level) {
+            const double dx = 2 * rr / nx;
+            const double xmin[3] = {x0[0] - rr, x0[1] - rr, x0[2] - rr};
+            const int ncell = nx * nx * nx;
+            for (int q = t; q < ncell; q += BS) grid[q] = 0.0;
+            __syncthreads();
+            for (int64_t i = f + t; i < e; i += BS) {
+                int cell;
+                if (ngp_cell(D.pos + 3 * i, xmin, dx, nx, cell)) atomicAdd(&grid[cell], D.mass[i]);
+            }
+            __syncthreads();
+            // first cell of maximum mass in (i, j, k) order (np.where's first, :1475-1477)
+            double ck = INFINITY;
+            int64_t cb = INT64_MAX;
+            for (int q = t; q < ncell; q += BS)
+                if (-grid[q] < ck) {
+                    ck = -grid[q];
+                    cb = q;
+                }
+            block_argmin<BS>(ck, cb, sh, si);
+            if (dx > 3.0 * dcell_min) {
+                const int cell = cb == INT64_MAX ? 0 : (int)cb;
+                const int cx = cell / (nx * nx), cy = (cell / nx) % nx, cz = cell % nx;
+                x0[0] = xmin[0] + (cx + 0.5) * dx;
+                x0[1] = xmin[1] + (cy + 0.5) * dx;
+                x0[2] = xmin[2] + (cz + 0.5) * dx;
+                rr = dx;
+                nx = 3;
+                continue;
+            }
+            // member number a among the members inside this grid, in member order (:1486-1490)
+            target = -1;
+            int64_t seen = 0;
+            for (int64_t base = f; base < e; base += BS) {
+                const int64_t i = base + t;
+                int cell;
+                const int in = (i < e) && ngp_cell(D.pos + 3 * i, xmin, dx, nx, cell);
+                if (t == 0) found = -1;
+                scan[t] = in;
+                __syncthreads();
+                for (int w = 1; w < BS; w <<= 1) {
+                    const int add = t >= w ? scan[t - w] : 0;
+                    __syncthreads();
+                    scan[t] += add;
+                    __syncthreads();
+                }
+                if (in && seen + scan[t] - 1 == a) found = i;
+                __syncthreads();
+                seen += scan[BS - 1];
+                const int64_t fi = found;
+                __syncthreads();
+                if (fi >= 0) {
+                    target = fi;
+                    break;
+                }
+            }
+            break;
+        }
+        if (target < 0) {
+            if (t == 0) atomicOr(err, target == -1 ? RC_DPMMC_INDEX : RC_TOO_DEEP);
+            return;
+        }
+        ic = target;
+        c[0] = D.pos[3 * target];
+        c[1] = D.pos[3 * target + 1];
+        c[2] = D.pos[3 * target + 2];
+        r0 = radius();
+    }
+
+    if (sc) {
+        double x0[3] = {c[0], c[1], c[2]};
+        double r = r0;
+        int64_t target = -1;
+        for (int it = 0; it < RC_MAX_SPHERES; ++it) {
+            const double r2 = r * r;
+            double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+            for (int64_t i = f + t; i < e; i += BS) {
+                const double dx = wrap_strict(D.pos[3 * i] - x0[0], L);
+                const double dy = wrap_strict(D.pos[3 * i + 1] - x0[1], L);
+                const double dz = wrap_strict(D.pos[3 * i + 2] - x0[2], L);
+                if (dx * dx + dy * dy + dz * dz <= r2) {
+                    const double m = D.mass[i];
+                    v[0] += m;
+                    v[1] += m * dx;
+                    v[2] += m * dy;
+                    v[3] += m * dz;
+                }
+            }
+            block_sum_shfl<BS, 4>(v, sh);
+            double xc[3] = {x0[0], x0[1], x0[2]};
+            if (v[0] > 0.0)
+                for (int d = 0; d < 3; ++d) xc[d] = v[1 + d] / v[0] + x0[d];
+            if (r > dcell_min && v[0] > 0.0) {
+                for (int d = 0; d < 3; ++d) x0[d] = xc[d];
+                r = (1 - eps_sc) * r;
+                continue;
+            }
+            // the member nearest the last centre of mass (first on ties)
+            double bk = INFINITY;
+            int64_t bi = INT64_MAX;
+#pragma unroll 4
+            for (int64_t i = f + t; i < e; i += BS) {
+                const double dx = wrap_strict(D.pos[3 * i] - xc[0], L);
+                const double dy = wrap_strict(D.pos[3 * i + 1] - xc[1], L);
+                const double dz = wrap_strict(D.pos[3 * i + 2] - xc[2], L);
+                const double d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 < bk) {
+                    bk = d2;
+                    bi = i;
+                }
+            }
+            block_argmin<BS>(bk, bi, sh, si);
+            if (bi == INT64_MAX) bi = f;                // every distance NaN: np.argmin gives 0
+            if (bk > 1.0) {
+                if (t == 0) atomicOr(err, RC_SC_DISTMIN);
+                return;
+            }
+            target = bi;
+            break;
+        }
+        if (target < 0) {
+            if (t == 0) atomicOr(err, RC_TOO_DEEP);
+            return;
+        }
+        ic = target;
+        c[0] = D.pos[3 * target];
+        c[1] = D.pos[3 * target + 1];
+        c[2] = D.pos[3 * target + 2];
+    }
+    if (t == 0) {
+        S.px = c[0];
+        S.py = c[1];
+        S.pz = c[2];
+        S.icenter = ic;
+    }
+}
+
+// ---- SC for the largest halos, one sphere at a time over the halo's member tiles: every
+// sphere is a partial-sum launch over the tiles and a per-halo combine in tile order.  The
+// host replays the radius sequence (r0, (1-eps) r0, ...) to know how many spheres to launch.
+struct ScState {
+    double x[3];   // current centre; after the last sphere, its centre of mass
+    double r;      // current radius
+    int phase;     // 0: shrinking, 1: last centre of mass in x, 2: no recentring
+};
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sc_radius_tiles(HaloDev D, const HaloState *st,
+                                                        const int32_t *tiles, double *pmax)
+{
+    __shared__ double sh[BS / 64];
+    const int tile = tiles[blockIdx.x];
+    const int h = D.tile_halo[tile];
+    const int64_t s = D.tile_start[tile], e = D.tile_end[tile];
+    const HaloState &S = st[h];
+    double v = 0.0;
+#pragma unroll 4
+    for (int64_t i = s + threadIdx.x; i < e; i += BS) {
+        const double dx = wrap_strict(D.pos[3 * i] - S.px, D.L);
+        const double dy = wrap_strict(D.pos[3 * i + 1] - S.py, D.L);
+        const double dz = wrap_strict(D.pos[3 * i + 2] - S.pz, D.L);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (v == v && (d2 != d2 || d2 > v)) v = d2;
+    }
+    v = block_max_nan<BS>(v, sh);
+    if (threadIdx.x == 0) pmax[tile] = v;
+}
+
+__global__ void k_sc_init(const int32_t *halos, int nb, const int32_t *tile_first,
+                          const double *pmax, const HaloState *st, ScState *sc, double *r0_out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const int h = halos[k];
+    double m = pmax[tile_first[h]];
+    for (int t = tile_first[h] + 1; t < tile_first[h + 1]; ++t) {
+        const double b = pmax[t];
+        if (m == m && (b != b || b > m)) m = b;
+    }
+    const double r0 = sqrt(m);
+    ScState &C = sc[h];
+    C.x[0] = st[h].px;
+    C.x[1] = st[h].py;
+    C.x[2] = st[h].pz;
+    C.r = r0;
+    C.phase = (isfinite(r0) && r0 > 0.0) ? 0 : 2;
+    r0_out[k] = r0;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sc_sphere_tiles(HaloDev D, const ScState *sc,
+                                                        const int32_t *tiles, double *part)
+{
+    __shared__ double sh[4 * (BS / 64)];
+    const int tile = tiles[blockIdx.x];
+    const int h = D.tile_halo[tile];
+    const ScState &C = sc[h];
+    if (C.phase != 0) return;                      // block-uniform
+    const int64_t s = D.tile_start[tile], e = D.tile_end[tile];
+    const double r2 = C.r * C.r;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+    for (int64_t i = s + threadIdx.x; i < e; i += BS) {
+        const double dx = wrap_strict(D.pos[3 * i] - C.x[0], D.L);
+        const double dy = wrap_strict(D.pos[3 * i + 1] - C.x[1], D.L);
+        const double dz = wrap_strict(D.pos[3 * i + 2] - C.x[2], D.L);
+        if (dx * dx + dy * dy + dz * dz <= r2) {
+            const double m = D.mass[i];
+            v[0] += m;
+            v[1] += m * dx;
+            v[2] += m * dy;
+            v[3] += m * dz;
+        }
+    }
+    block_sum_shfl<BS, 4>(v, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) part[(int64_t)tile * 4 + q] = v[q];
+}
+
+__global__ void k_sc_sphere_combine(const int32_t *halos, int nb, const int32_t *tile_first,
+                                    const double *part, ScState *sc, double dcell_min,
+                                    double eps_sc)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const int h = halos[k];
+    ScState &C = sc[h];
+    if (C.phase != 0) return;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int t = tile_first[h]; t < tile_first[h + 1]; ++t)
+        for (int q = 0; q < 4; ++q) a[q] += part[(int64_t)t * 4 + q];
+    double xc[3] = {C.x[0], C.x[1], C.x[2]};
+    if (a[0] > 0.0)
+        for (int d = 0; d < 3; ++d) xc[d] = a[1 + d] / a[0] + C.x[d];
+    for (int d = 0; d < 3; ++d) C.x[d] = xc[d];
+    if (C.r > dcell_min && a[0] > 0.0)
+        C.r = (1 - eps_sc) * C.r;
+    else
+        C.phase = 1;
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_sc_nearest_tiles(HaloDev D, const ScState *sc,
+                                                         const int32_t *tiles, double *pkey,
+                                                         int64_t *pidx)
+{
+    __shared__ double sk[BS / 64];
+    __shared__ int64_t si[BS / 64];
+    const int tile = tiles[blockIdx.x];
+    const int h = D.tile_halo[tile];
+    const ScState &C = sc[h];
+    if (C.phase != 1) return;                      // block-uniform
+    const int64_t s = D.tile_start[tile], e = D.tile_end[tile];
+    double bk = INFINITY;
+    int64_t bi = INT64_MAX;
+#pragma unroll 4
+    for (int64_t i = s + threadIdx.x; i < e; i += BS) {
+        const double dx = wrap_strict(D.pos[3 * i] - C.x[0], D.L);
+        const double dy = wrap_strict(D.pos[3 * i + 1] - C.x[1], D.L);
+        const double dz = wrap_strict(D.pos[3 * i + 2] - C.x[2], D.L);
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 < bk) {
+            bk = d2;
+            bi = i;
+        }
+    }
+    block_argmin<BS>(bk, bi, sk, si);
+    if (threadIdx.x == 0) {
+        pkey[tile] = bk;
+        pidx[tile] = bi;
+    }
+}
+
+__global__ void k_sc_nearest_combine(const int32_t *halos, int nb, const int32_t *tile_first,
+                                     const double *pkey, const int64_t *pidx, const int64_t *off,
+                                     const double *pos, const ScState *sc, HaloState *st,
+                                     int *err)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nb) return;
+    const int h = halos[k];
+    const ScState &C = sc[h];
+    if (C.phase == 2) return;
+    if (C.phase == 0) {                            // more spheres than the host launched
+        atomicOr(err, RC_TOO_DEEP);
+        return;
+    }
+    double best = INFINITY;
+    int64_t bi = INT64_MAX;
+    for (int t = tile_first[h]; t < tile_first[h + 1]; ++t)
+        if (pkey[t] < best || (pkey[t] == best && pidx[t] < bi)) {
+            best = pkey[t];
+            bi = pidx[t];
+        }
+    if (bi == INT64_MAX) bi = off[h];              // every distance NaN: np.argmin gives 0
+    if (best > 1.0) {
+        atomicOr(err, RC_SC_DISTMIN);
+        return;
+    }
     HaloState &S = st[h];
     S.icenter = bi;
     S.px = pos[3 * bi];
@@ -714,6 +1168,56 @@ Tiles make_tiles(const std::vector<int64_t> &off, int64_t T)
 
 }  // namespace
 
+// SC on the listed halos through their member tiles (ScState per halo; part / pkey / pidx
+// are the per-tile scratch of the member passes, free at this point).  One stream sync: the
+// starting radii come back to the host, which counts the spheres as the kernels will.
+static void sc_tiled(const std::vector<int32_t> &big, const Tiles &tl, const HaloDev &D,
+                     const int32_t *d_tf, const int64_t *d_off, const double *pos, HaloState *st,
+                     const HaloParamsDev &P, double *part, double *pkey, int64_t *pidx,
+                     std::vector<void *> &keep, std::vector<int32_t> &tiles, int *err,
+                     hipStream_t s)
+{
+    const int nb = (int)big.size();
+    for (int32_t h : big)
+        for (int32_t t = tl.first[h]; t < tl.first[h + 1]; ++t) tiles.push_back(t);
+    const unsigned ntl = (unsigned)tiles.size(), cb = (unsigned)((nb + 63) / 64);
+    int32_t *d_big = dalloc<int32_t>(keep, nb), *d_tiles = dalloc<int32_t>(keep, ntl);
+    ScState *sc = dalloc<ScState>(keep, (size_t)tl.first.size());
+    double *d_r0 = dalloc<double>(keep, nb);
+    AHOP_HIP(hipMemcpyAsync(d_big, big.data(), sizeof(int32_t) * nb, hipMemcpyHostToDevice, s));
+    AHOP_HIP(hipMemcpyAsync(d_tiles, tiles.data(), sizeof(int32_t) * ntl, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_sc_radius_tiles<256>, dim3(ntl), dim3(256), 0, s, D, st, d_tiles, pkey);
+    hipLaunchKernelGGL(k_sc_init, dim3(cb), dim3(64), 0, s, d_big, nb, d_tf, pkey, st, sc, d_r0);
+    std::vector<double> r0(nb);
+    AHOP_HIP(hipMemcpyAsync(r0.data(), d_r0, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+    // spheres while r > dcell_min, then the last one: the same double arithmetic as the kernels
+    int nsph = 0;
+    for (double r : r0) {
+        if (!(std::isfinite(r) && r > 0.0)) continue;
+        int n = 0;
+        while (r > P.dcell_min && n < RC_MAX_SPHERES) {
+            ++n;
+            r = (1 - P.eps_sc) * r;
+        }
+        if (n >= RC_MAX_SPHERES) {
+            const int flag = RC_TOO_DEEP;
+            AHOP_HIP(hipMemcpyAsync(err, &flag, sizeof(int), hipMemcpyHostToDevice, s));
+            AHOP_HIP(hipStreamSynchronize(s));
+            return;
+        }
+        nsph = std::max(nsph, n);
+    }
+    for (int it = 0; it <= nsph; ++it) {
+        hipLaunchKernelGGL(k_sc_sphere_tiles<256>, dim3(ntl), dim3(256), 0, s, D, sc, d_tiles, part);
+        hipLaunchKernelGGL(k_sc_sphere_combine, dim3(cb), dim3(64), 0, s, d_big, nb, d_tf, part, sc,
+                           P.dcell_min, P.eps_sc);
+    }
+    hipLaunchKernelGGL(k_sc_nearest_tiles<256>, dim3(ntl), dim3(256), 0, s, D, sc, d_tiles, pkey, pidx);
+    hipLaunchKernelGGL(k_sc_nearest_combine, dim3(cb), dim3(64), 0, s, d_big, nb, d_tf, pkey, pidx,
+                       d_off, pos, sc, st, err);
+}
+
 // Runs every pass for one tile size / workgroup size; returns device milliseconds.
 template <int BS>
 static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, const double *pos,
@@ -724,6 +1228,8 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
     Tiles tl = make_tiles(hoff, T);
     const int nt = (int)tl.halo.size();
     std::vector<int32_t> small;   // halos of the pair-sum branch (host copy kept until the sync)
+    std::vector<int32_t> cls[4];  // recentring size classes (likewise)
+    std::vector<int32_t> sc_tiles;
     std::vector<void *> keep;
     auto cleanup = [&]() {
         for (void *p : keep) (void)hipFree(p);
@@ -741,8 +1247,8 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
         double *pkey = dalloc<double>(keep, nt);
         int64_t *pidx = dalloc<int64_t>(keep, nt);
         HaloState *st = dalloc<HaloState>(keep, nh);
-        int *missed = dalloc<int>(keep, 1);
-        AHOP_HIP(hipMemsetAsync(missed, 0, sizeof(int), s));
+        int *missed = dalloc<int>(keep, 2);            // [0]: shell misses, [1]: recentring errors
+        AHOP_HIP(hipMemsetAsync(missed, 0, 2 * sizeof(int), s));
         const unsigned hb = (unsigned)((nh + 127) / 128);
         hipEvent_t e0, e1;
         AHOP_HIP(hipEventCreate(&e0));
@@ -752,6 +1258,38 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
         hipLaunchKernelGGL(k_combine1, dim3(hb), dim3(128), 0, s, nh, d_tf, part, d_off, pos, P.lbox, st);
         hipLaunchKernelGGL(k_pass2<BS>, dim3(nt), dim3(BS), 0, s, D, st, P.cdm, pkey, pidx);
         hipLaunchKernelGGL(k_combine2, dim3(hb), dim3(128), 0, s, nh, d_tf, pkey, pidx, d_off, pos, st);
+        if (full && (P.sc || P.dpmmc)) {
+            // SC alone: halos from SC_TILED members on take the tiled sphere path (cls[3]);
+            // the others one workgroup per halo, sized to the halo: one wave below 2048
+            // members, 4 waves below 32768, 16 above (every sphere is a pass over the halo)
+            const bool tiled = P.sc && !P.dpmmc;
+            for (int h = 0; h < nh; ++h) {
+                const int64_t c = hoff[h + 1] - hoff[h];
+                cls[(tiled && c >= SC_TILED) ? 3 : c < 2048 ? 0 : c < 32768 ? 1 : 2].push_back(h);
+            }
+            if (!cls[3].empty())
+                sc_tiled(cls[3], tl, D, d_tf, d_off, pos, st, P, part, pkey, pidx, keep, sc_tiles,
+                         missed + 1, s);
+            int32_t *d_cls = dalloc<int32_t>(keep, nh);
+            const size_t glds = P.dpmmc ? sizeof(double) * 729 : 0;
+            int pos0 = 0;
+            for (int k = 2; k >= 0; --k) {             // largest first: they set the tail
+                const unsigned nk = (unsigned)cls[k].size();
+                if (!nk) continue;
+                AHOP_HIP(hipMemcpyAsync(d_cls + pos0, cls[k].data(), sizeof(int32_t) * nk,
+                                        hipMemcpyHostToDevice, s));
+                if (k == 2)
+                    hipLaunchKernelGGL(k_recenter<1024>, dim3(nk), dim3(1024), glds, s, D, st,
+                                       d_cls + pos0, P.sc, P.dpmmc, P.dcell_min, P.eps_sc, missed + 1);
+                else if (k == 1)
+                    hipLaunchKernelGGL(k_recenter<256>, dim3(nk), dim3(256), glds, s, D, st,
+                                       d_cls + pos0, P.sc, P.dpmmc, P.dcell_min, P.eps_sc, missed + 1);
+                else
+                    hipLaunchKernelGGL(k_recenter<64>, dim3(nk), dim3(64), glds, s, D, st,
+                                       d_cls + pos0, P.sc, P.dpmmc, P.dcell_min, P.eps_sc, missed + 1);
+                pos0 += (int)nk;
+            }
+        }
         hipLaunchKernelGGL(k_pass3<BS>, dim3(nt), dim3(BS), 0, s, D, st, part);
         hipLaunchKernelGGL(k_combine3, dim3(hb), dim3(128), 0, s, nh, d_tf, part, P.gravconst, st, d_out);
         if (full) {
@@ -785,11 +1323,21 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
         AHOP_HIP(hipEventElapsedTime(&ms, e0, e1));
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        int hmissed = 0;
-        AHOP_HIP(hipMemcpy(&hmissed, missed, sizeof(int), hipMemcpyDeviceToHost));
+        int hmissed[2] = {0, 0};
+        AHOP_HIP(hipMemcpy(hmissed, missed, 2 * sizeof(int), hipMemcpyDeviceToHost));
         cleanup();
-        if (hmissed)
-            throw std::runtime_error("tab_props_inside: " + std::to_string(hmissed) +
+        if (hmissed[1] & RC_DPMMC_INDEX)
+            throw std::runtime_error("DPMMC centre: the densest member's index is beyond the "
+                                     "members inside the last grid (the reference's IndexError)");
+        if (hmissed[1] & RC_SC_DISTMIN)
+            throw std::runtime_error("SC centre: squared distance to the nearest member above 1 "
+                                     "(the reference's ValueError)");
+        if (hmissed[1] & RC_TOO_DEEP)
+            throw std::runtime_error("recentring: more than " + std::to_string(RC_MAX_LEVELS) +
+                                     " grid levels or " + std::to_string(RC_MAX_SPHERES) +
+                                     " spheres (raise dcell_min or eps_SC)");
+        if (hmissed[0])
+            throw std::runtime_error("tab_props_inside: " + std::to_string(hmissed[0]) +
                                      " members fall outside the outer ellipsoid");
         return ms;
     } catch (...) {
@@ -807,6 +1355,11 @@ HaloTuning halo_props_device(const int64_t *h_off, int64_t nhalos, const double 
     // the shell tables of one halo live in LDS (4 doubles per shell)
     if (P.nshell < 2 || P.nshell > 2000) throw std::runtime_error("nshell must be in [2, 2000]");
     if (!P.cdm && !dens) throw std::runtime_error("densest-member centre needs densities");
+    if (P.dpmmc && !dens) throw std::runtime_error("DPMMC recentring needs densities");
+    if ((P.sc || P.dpmmc) && !(P.dcell_min > 0))
+        throw std::runtime_error("SC and DPMMC recentring need dcell_min > 0");
+    if (P.sc && !(P.eps_sc > 0 && P.eps_sc < 1))
+        throw std::runtime_error("SC recentring needs 0 < eps_SC < 1");
     std::vector<int64_t> hoff(h_off, h_off + nhalos + 1);
     if (hoff[0] < 0) throw std::runtime_error("negative member offset");
     if (nhalos > INT32_MAX / 2) throw std::runtime_error("too many halos");

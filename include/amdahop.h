@@ -168,6 +168,10 @@ typedef struct {
     int32_t direct_pe_nmax; /* direct pair-sum potential energy below this count (1000) */
     int32_t tile;           /* members per work tile; <= 0 lets the autotuner choose */
     int32_t workgroup;      /* 128 or 256; <= 0 lets the autotuner choose */
+    int32_t sc;             /* SC of refine_centers_1b (:1555-1596): shrinking-sphere recentring */
+    int32_t dpmmc;          /* DPMMC: multiscale-grid recentring before SC (needs d_density) */
+    double dcell_min;       /* dcell_min: smallest cell / sphere radius, > 0 when SC or DPMMC */
+    double eps_sc;          /* eps_SC: sphere shrink fraction, in (0, 1) when SC */
 } ahop_halo_params;
 
 typedef struct {
@@ -179,7 +183,8 @@ typedef struct {
  * nhalos+1 member offsets; d_pos/d_vel (M,3) row-major, d_mass (M), d_density (M, may be
  * NULL when cdm=1) and d_out (nhalos x AHOP_HALO_NFIELDS) are device pointers.
  * Replaces the per-halo fork-Pool loop of src/compute_halo_props.py:487-534 over
- * det_mass_17 (:754), det_center_18 (:901), compute_ang_mom_19 (:783), r_halos_1a (:828),
+ * det_mass_17 (:754), det_center_18 (:901), refine_centers_1b (:1555, total centre; DPMMC
+ * :1447 and SC :1498), compute_ang_mom_19 (:783), r_halos_1a (:828),
  * det_vir_1b (:1241: det_main_axis_1b2, det_halo_energies_1b3, det_vir_props_1b4) and
  * compute_spin_parameter_1c (:1599). */
 int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,

@@ -2,10 +2,10 @@
 
 Follows reference src/compute_halo_props.py per function (cited below) for one halo
 given its members (positions, velocities, masses, densities) in physical units, in the
-configuration the hot path covers: periodic box (FlagPeriod=1), no stellar payload,
-SC/DPMMC off (refine_centers_1b :1555 then leaves the centre unchanged), virial
-potential profile 'shell' with self mode 'count' and norm policy
-'profile_outer_for_large' (the defaults at :1035-1047).
+configuration the hot path covers: periodic box (FlagPeriod=1), no stellar payload, an
+AdaptaHOP selection method (not FOF, so refine_centers_1b :1555-1596 recentres with the
+DPMMC and SC options when they are on), virial potential profile 'shell' with self mode
+'count' and norm policy 'profile_outer_for_large' (the defaults at :1035-1047).
 
 Each reference function has its own function here so that each can be pinned on its
 own; :func:`halo_props` composes them in the reference's order and returns one float64
@@ -15,7 +15,10 @@ Pinning (tests/test_halo_oracle.py): closed forms (two-body and rigid-rotation a
 momentum, the inertia tensor of a cube shell, direct potential energies of regular
 2-, 3- and 4-member systems, Hubble-flow kinetic energy, the shell profile of a single
 shell, the overdensity branch of a static halo), invariance under a periodic translation
-across the box edge and rotation covariance of L and I, and eigh ordering.
+across the box edge and rotation covariance of L and I, and eigh ordering; for the
+recentring, the cases where it reduces to the centre of det_center_18, a core found
+through a decoy, the grid's cell choice, and the reference's indexing of the densest
+member (tests/test_halo_oracle.py).
 """
 from __future__ import annotations
 
@@ -59,6 +62,94 @@ def det_center(pos, vel, mass, density, L, cdm):
     dv2 = mass * ((vel[:, 0] - vbar[0]) ** 2 + (vel[:, 1] - vbar[1]) ** 2 +
                   (vel[:, 2] - vbar[2]) ** 2)
     return np.array(cpos, dtype=np.float64), vbar, np.sqrt(np.sum(dv2) / m)
+
+
+def radius_from_center(pos, c, L):
+    """_radius_from_center :1543-1548: distance of the farthest member, wrapped."""
+    if len(pos) == 0:
+        return np.float64(np.nan)
+    d = rel(pos, c, L)
+    return np.sqrt(np.max(d[0] ** 2 + d[1] ** 2 + d[2] ** 2))
+
+
+def center_multiscale(pos, mass, density, c, r0, nx, dcell_min, max_levels=256):
+    """det_halo_center_multiscale_1b0 :1447-1495, the recursion unrolled.
+
+    Each level assigns member masses to an nx^3 NGP grid over the cube [c - r0, c + r0]
+    (positions not wrapped; the index is the float quotient truncated toward zero, so
+    quotients in (-1, 0) land in cell 0, :1460-1465), takes the first cell of maximum mass
+    in (i, j, k) order (:1475-1477) and, while the cell is wider than 3 dcell_min, recurses
+    on a 3^3 grid over the cube of half-width one cell around that cell's centre.  The last
+    level returns the position of member number a among the members inside its grid, in
+    member order, where a is the index of the densest member of the halo (first on ties):
+    :1486-1490 index the in-grid members with the argmax over all members.  IndexError where
+    that index is out of range, as in the reference."""
+    x0 = np.array(c, dtype=np.float64)
+    r0 = np.float64(r0)
+    for _ in range(max_levels):
+        xmin = x0 - r0
+        deltax = 2 * r0 / nx
+        ijk = ((pos - xmin) / deltax).astype(np.int64)
+        valid = np.all((ijk >= 0) & (ijk < nx), axis=1)
+        grid = np.zeros((nx, nx, nx))
+        np.add.at(grid, (ijk[valid, 0], ijk[valid, 1], ijk[valid, 2]), mass[valid])
+        cell = np.argwhere(grid == np.max(grid))[0]
+        if deltax > 3 * dcell_min:
+            x0 = xmin + (cell + 0.5) * deltax
+            r0 = deltax
+            nx = 3
+            continue
+        a = int(np.argmax(density))
+        inside = np.nonzero(valid)[0]
+        if a >= len(inside):
+            raise IndexError(f"densest member {a} beyond the {len(inside)} members of the cell grid")
+        return np.array(pos[inside[a]], dtype=np.float64)
+    raise RecursionError("multiscale centre: too many levels")
+
+
+def center_sphere(pos, mass, c, r0, L, dcell_min, eps_sc, max_iter=1000):
+    """det_halo_center_sphere_1b1 :1498-1540, the recursion unrolled: the centre of mass of
+    the members within r of the current centre (wrapped offsets, dr^2 <= r^2) becomes the
+    centre and r shrinks by (1 - eps_SC) while r > dcell_min and the sphere holds mass; then
+    the member nearest that centre of mass (first on ties).  ValueError where the reference
+    raises (squared distance above 1)."""
+    x0 = np.array(c, dtype=np.float64)
+    r = np.float64(r0)
+    for _ in range(max_iter):
+        drx, dry, drz = rel(pos, x0, L)
+        inside = drx ** 2 + dry ** 2 + drz ** 2 <= r * r
+        m = mass[inside]
+        mtot = np.sum(m)
+        if mtot > 0:
+            xc = np.array([np.sum(m * drx[inside]) / mtot + x0[0],
+                           np.sum(m * dry[inside]) / mtot + x0[1],
+                           np.sum(m * drz[inside]) / mtot + x0[2]])
+        else:
+            xc = x0
+        if r > dcell_min and mtot > 0:
+            x0 = xc
+            r = (1 - eps_sc) * r
+            continue
+        drx, dry, drz = rel(pos, xc, L)
+        d2 = drx ** 2 + dry ** 2 + drz ** 2
+        i = int(np.argmin(d2))
+        if d2[i] > 1.0:
+            raise ValueError(f"distmin(={d2[i]}) > 1")
+        return np.array(pos[i], dtype=np.float64)
+    raise RecursionError("sphere centre: too many iterations")
+
+
+def refine_centers(pos, mass, density, c, L, sc, dpmmc, dcell_min, eps_sc):
+    """refine_centers_1b :1555-1567 (total centre; the stellar centre needs the stellar
+    payload): DPMMC then SC, each from the radius of the farthest member."""
+    r0 = radius_from_center(pos, c, L)
+    if np.isfinite(r0) and r0 > 0:
+        if dpmmc:
+            c = center_multiscale(pos, mass, density, c, r0, 9, dcell_min)
+            r0 = radius_from_center(pos, c, L)
+        if sc:
+            c = center_sphere(pos, mass, c, r0, L, dcell_min, eps_sc)
+    return c
 
 
 def rel(pos, c, L):
@@ -202,16 +293,20 @@ def vir_props(tabm, tabk, tabp, amax, bmax, cmax, vir_overdens, rho_mean):
 
 
 def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst=430.1,
-               vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000):
+               vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000,
+               sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0):
     """All properties of one halo, composed as det_vir_1b :1241-1259 and the property loop
-    (det_mass_17, det_center_18, compute_ang_mom_19, r_halos_1a, det_main_axis_1b2,
-    det_halo_energies_1b3, det_vir_props_1b4, compute_spin_parameter_1c :1599-1602)."""
+    (det_mass_17, det_center_18, refine_centers_1b, compute_ang_mom_19, r_halos_1a,
+    det_main_axis_1b2, det_halo_energies_1b3, det_vir_props_1b4, compute_spin_parameter_1c
+    :1599-1602; the order of _compute_halo_props :270-290)."""
     out = np.full(len(FIELDS), np.nan)
     L = lbox
     n = mass.shape[0]
     m = np.sum(mass)                                          # det_mass_17 :754-780
     out[F["m"]] = m
     c, vbar, sigma = det_center(pos, vel, mass, density, L, cdm)
+    if sc or dpmmc:
+        c = refine_centers(pos, mass, density, c, L, sc, dpmmc, dcell_min, eps_sc)
     out[F["px"]], out[F["py"]], out[F["pz"]] = c
     out[F["vx"]], out[F["vy"]], out[F["vz"]] = vbar
     out[F["sigma"]] = sigma

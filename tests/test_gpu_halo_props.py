@@ -128,3 +128,78 @@ def test_invalid_layouts_raise(gpu, halos):
         halo_props(halos.offsets, pos, vel, mass, nshell=1)
     with pytest.raises(RuntimeError, match="densit"):
         halo_props(halos.offsets, pos, vel, mass, cdm=False)
+
+
+# ---- recentring (refine_centers_1b :1555-1567): SC and DPMMC
+
+SC = dict(sc=True, dcell_min=1e-4, eps_sc=0.3)
+
+
+def dpmmc_density(hs):
+    """Densities whose maximum is member h % 3 of halo h, so that the reference's DPMMC
+    indexing (the densest member's index among the last grid's members) stays in range."""
+    dens = np.random.default_rng(12).uniform(1.0, 10.0, size=hs.mass.shape[0])
+    for h in range(hs.nhalos):
+        dens[hs.offsets[h] + h % 3] = 100.0
+    return dens
+
+
+@pytest.mark.parametrize("tile", [0, 1024])
+def test_sc_recentring_matches_the_oracle(gpu, tile):
+    from oracle.halo_props import F, all_halos
+    hs = G.nfw_halos(80, 7, alpha=0.6, nmin=100, nmax=40000, mass_sigma_dex=0.3)
+    # both schedules: one workgroup per halo (below 2048 members, below 8192), and sphere by
+    # sphere over the member tiles from 8192 members on
+    c = np.diff(hs.offsets)
+    assert (c >= 8192).sum() >= 3 and (c < 2048).any() and ((c >= 2048) & (c < 8192)).any()
+    ref = all_halos(hs.offsets, hs.pos, hs.vel, hs.mass, hub_pt=70.3, **SC)
+    base = all_halos(hs.offsets, hs.pos, hs.vel, hs.mass, hub_pt=70.3)
+    cols = [F["px"], F["py"], F["pz"]]
+    moved = np.any(ref[:, cols] != base[:, cols], axis=1)
+    print(f"SC moved {int(moved.sum())} of {hs.nhalos} centres, "
+          f"{int(moved[c >= 8192].sum())} of the {int((c >= 8192).sum())} largest", flush=True)
+    assert moved.sum() > hs.nhalos // 2 and moved[c >= 8192].sum() >= 2
+    out, info = run(hs, gpu, hub_pt=70.3, tile=tile, workgroup=256 if tile else 0, **SC)
+    print(f"with SC: {info.ms:.3f} ms, tile {info.tile}", flush=True)
+    compare(out, ref)
+
+
+@pytest.mark.parametrize("sc", [False, True])
+def test_dpmmc_recentring_matches_the_oracle(gpu, halos, sc):
+    from oracle.halo_props import all_halos
+    dens = dpmmc_density(halos)
+    kw = dict(dpmmc=True, sc=sc, dcell_min=5e-4, eps_sc=0.3)
+    print(f"DPMMC sc={sc}", flush=True)
+    ref = all_halos(halos.offsets, halos.pos, halos.vel, halos.mass, dens, hub_pt=70.3, **kw)
+    out, _ = run(halos, gpu, density=dens, hub_pt=70.3, **kw)
+    compare(out, ref)
+
+
+def test_recentring_errors(gpu, halos):
+    import torch
+    from amdahop import halo_props
+    from oracle.halo_props import det_center, refine_centers
+    pos = torch.from_numpy(halos.pos).to(gpu)
+    vel = torch.from_numpy(halos.vel).to(gpu)
+    mass = torch.from_numpy(halos.mass).to(gpu)
+    # the largest halo's last member densest: beyond the last grid's members, where the
+    # reference raises IndexError
+    h = int(np.argmax(np.diff(halos.offsets)))
+    s, e = int(halos.offsets[h]), int(halos.offsets[h + 1])
+    dens = dpmmc_density(halos)
+    dens[e - 1] = 1e3
+    c0, _, _ = det_center(halos.pos[s:e], halos.vel[s:e], halos.mass[s:e], None, 1.0, True)
+    with pytest.raises(IndexError):
+        refine_centers(halos.pos[s:e], halos.mass[s:e], dens[s:e], c0, 1.0, False, True, 5e-4, 0.3)
+    d = torch.from_numpy(dens).to(gpu)
+    with pytest.raises(RuntimeError, match="DPMMC centre"):
+        halo_props(halos.offsets, pos, vel, mass, d, dpmmc=True, dcell_min=5e-4)
+    with pytest.raises(RuntimeError, match="dcell_min"):
+        halo_props(halos.offsets, pos, vel, mass, sc=True, eps_sc=0.3)
+    with pytest.raises(RuntimeError, match="eps_SC"):
+        halo_props(halos.offsets, pos, vel, mass, sc=True, dcell_min=1e-4, eps_sc=1.0)
+    with pytest.raises(RuntimeError, match="densities"):
+        halo_props(halos.offsets, pos, vel, mass, dpmmc=True, dcell_min=1e-4)
+    # the library is still usable after the errors
+    out, _ = halo_props(halos.offsets, pos, vel, mass, hub_pt=70.3, **SC)
+    assert torch.isfinite(out[:, 1]).all()

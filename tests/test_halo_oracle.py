@@ -241,3 +241,119 @@ def test_spin_parameter_closed_form():
     hl = np.sqrt(o[F["Lx"]] ** 2 + o[F["Ly"]] ** 2 + o[F["Lz"]] ** 2)
     assert o[F["spin"]] == pytest.approx(hl * np.sqrt(abs(o[F["et"]])) / o[F["m"]] ** 2.5 / G,
                                          rel=1e-15)
+
+
+# ---- recentring (refine_centers_1b :1555-1596)
+
+def _line(xs, L=10.0):
+    pos = np.zeros((len(xs), 3))
+    pos[:, 0] = xs
+    return pos
+
+
+def test_shrinking_sphere_hand_worked():
+    """det_halo_center_sphere_1b1 :1498-1540 on seven members on a line, eps_SC=0.75,
+    dcell_min=0.5, starting on member 0 (r0 = 2, the farthest member at x=-2).
+    Sphere 1 (r=2, dr^2 <= r^2 keeps x=-2): masses 17, sum m x = 2.1, centre 0.123529...;
+    r becomes 0.5, not above dcell_min, so the last sphere: members 0, 5, 6 (masses 2, 2, 3)
+    lie within 0.5 of it, their centre of mass is -0.00714..., nearest member 0.
+    A strict < (x=-2 dropped), a shrink by eps instead of 1-eps, the nearest member to
+    the previous centre, or >= dcell_min give members 3, 5, 5 and 6 instead."""
+    xs = [0.0, 1.69, -2.0, 0.69, 0.85, 0.17, -0.13]
+    m = np.array([2.0, 3.0, 3.0, 2.0, 2.0, 2.0, 3.0])
+    pos = _line(xs)
+    c = HP.refine_centers(pos, m, None, pos[0], 10.0, True, False, 0.5, 0.75)
+    assert np.array_equal(c, pos[0])
+    # the same members moved across the periodic edge: the same member, wrapped
+    shifted = pos.copy()
+    shifted[:, 0] += 4.0
+    shifted[shifted[:, 0] >= 5.0, 0] -= 10.0
+    c = HP.refine_centers(shifted, m, None, shifted[0], 10.0, True, False, 0.5, 0.75)
+    assert np.array_equal(c, shifted[0])
+
+
+def test_shrinking_sphere_finds_a_core_through_a_decoy():
+    """A compact core (a member at P plus point-symmetric pairs within 1e-4) inside a broad
+    cloud centred 0.02 away, across the periodic edge: starting from the cloud's
+    centre-of-mass member, the shrinking sphere ends on the member at P."""
+    rng = np.random.default_rng(3)
+    P = np.array([0.4995, -0.2, 0.1])
+    q = rng.normal(size=(40, 3)) * 1e-4
+    core = np.vstack([P, P + q, P - q])
+    cloud = rng.normal(size=(300, 3)) * 0.01 + P + np.array([0.02, 0.0, 0.0])
+    cloud = cloud[np.linalg.norm(HP.wrap(cloud - P, 1.0), axis=1) > 3e-3]
+    pos = np.vstack([cloud, core])
+    pos = (pos + 0.5) % 1.0 - 0.5
+    m = np.full(len(pos), 1.0)
+    c0, _, _ = HP.det_center(pos, np.zeros_like(pos), m, None, 1.0, True)
+    assert np.linalg.norm(HP.wrap(c0 - P, 1.0)) > 5e-3        # the plain centre is elsewhere
+    c = HP.refine_centers(pos, m, None, c0, 1.0, True, False, 1e-3, 0.3)
+    assert np.array_equal(c, pos[len(cloud)])
+
+
+def test_shrinking_sphere_above_the_cell_size_is_the_com_centre():
+    """dcell_min above the starting radius: one sphere holding every member, then the member
+    nearest its centre of mass -- the centre det_center_18 chose."""
+    pos, vel, mass = _cloud(5, 200)
+    c0, _, _ = HP.det_center(pos, vel, mass, None, 1.0, True)
+    c = HP.refine_centers(pos, mass, None, pos[17], 1.0, True, False, 10.0, 0.3)
+    assert np.array_equal(c, c0)
+
+
+def _grid_halo():
+    """Members around the origin for a 9^3 grid of half-width 1 (cells 2/9 wide): far
+    members 1-3 and a clump of six (members 4-9) in cell (6, 6, 6)."""
+    far = [[0.9, -0.8, 0.1], [-0.7, 0.8, -0.6], [0.2, -0.9, 0.85]]
+    d = np.array([[0.0, 0.01, -0.02], [0.02, -0.01, 0.0], [-0.02, 0.0, 0.01],
+                  [0.01, 0.02, 0.02], [-0.01, -0.02, -0.01], [0.03, 0.03, -0.03]])
+    pos = np.vstack([[0.0, 0.0, 0.0], far, 0.45 + d])
+    return pos, np.ones(len(pos))
+
+
+def test_multiscale_centre_indexes_the_cell_members_with_the_densest_member():
+    """det_halo_center_multiscale_1b0 :1447-1495 with dcell_min=0.06: level 0 (cells 2/9 >
+    0.18) picks the clump's cell (mass 6) and recurses on 3^3 cells over [0.222, 0.667]^3,
+    where only the clump lies; cells 0.148 wide end the recursion.  The densest member of
+    the halo is member 1, so the centre is the clump's member number 1 (member 5): the
+    reference indexes the in-grid members with the argmax over all members (:1486-1490).
+    Not the densest member (1), nor the densest in the cell (member 8)."""
+    pos, m = _grid_halo()
+    dens = np.array([1.0, 100.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 50.0, 8.0])
+    c = HP.center_multiscale(pos, m, dens, pos[0], 1.0, 9, 0.06)
+    assert np.array_equal(c, pos[5])
+    # one level (cells not above 3 dcell_min): every member but the far corner ones lies in
+    # the grid, in member order, so the densest member's own position
+    c = HP.center_multiscale(pos, m, dens, pos[0], 1.0, 9, 1.0)
+    assert np.array_equal(c, pos[1])
+    # densest member 9 beyond the six members of the last grid: the reference's IndexError
+    dens[9] = 1e3
+    with pytest.raises(IndexError):
+        HP.center_multiscale(pos, m, dens, pos[0], 1.0, 9, 0.06)
+
+
+def test_multiscale_takes_the_first_cell_of_equal_maximum_mass_in_ijk_order():
+    """Two clumps of three in cells (6, 2, 6) (members 1-3) and (2, 6, 6) (members 4-6):
+    np.where's first maximum is (2, 6, 6) (C order; Fortran order would pick (6, 2, 6)).
+    The densest member is member 0, so the centre is the first member of the chosen clump."""
+    a = np.array([[0.45, -0.45, 0.45]]) + np.array([[0, 0.01, 0], [0.01, 0, 0], [0, 0, 0.01]])
+    b = np.array([[-0.45, 0.45, 0.45]]) + np.array([[0, 0.01, 0], [0.01, 0, 0], [0, 0, 0.01]])
+    pos = np.vstack([[0.0, 0.0, 0.0], a, b])
+    m = np.ones(len(pos))
+    dens = np.array([9.0, 1, 1, 1, 1, 1, 1])
+    c = HP.center_multiscale(pos, m, dens, pos[0], 1.0, 9, 0.06)
+    assert np.array_equal(c, pos[4])
+
+
+def test_refine_centers_runs_dpmmc_then_sc():
+    """refine_centers_1b :1555-1567: DPMMC, the radius from its centre, then SC from there."""
+    pos, m = _grid_halo()
+    dens = np.array([1.0, 100.0, 2.0, 3.0, 4.0, 5.0, 6.0, 7.0, 50.0, 8.0])
+    c1 = HP.center_multiscale(pos, m, dens, pos[0], HP.radius_from_center(pos, pos[0], 10.0),
+                              9, 0.06)
+    r1 = HP.radius_from_center(pos, c1, 10.0)
+    c2 = HP.center_sphere(pos, m, c1, r1, 10.0, 0.06, 0.3)
+    c = HP.refine_centers(pos, m, dens, pos[0], 10.0, True, True, 0.06, 0.3)
+    assert np.array_equal(c, c2)
+    # off: the centre is left alone
+    assert np.array_equal(HP.refine_centers(pos, m, dens, pos[3], 10.0, False, False, 0.06, 0.3),
+                          pos[3])
