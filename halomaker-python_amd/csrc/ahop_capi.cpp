@@ -18,6 +18,32 @@ using namespace ahop;
 
 namespace {
 
+// Host output buffer in pinned (page-locked) memory: device-to-host copies into it run at
+// link speed instead of through pageable staging.  The capacity is kept across calls (pinning
+// is costly) and released only at process end; clear() only resets the size.
+template <typename T>
+struct PinnedVec {
+    T *p = nullptr;
+    size_t n = 0, cap = 0;
+    void resize(size_t m)
+    {
+        if (m > cap) {
+            if (p) AHOP_HIP(hipHostFree(p));
+            p = nullptr;
+            cap = 0;
+            AHOP_HIP(hipHostMalloc((void **)&p, sizeof(T) * m, hipHostMallocDefault));
+            cap = m;
+        }
+        n = m;
+    }
+    void clear() { n = 0; }
+    T *data() { return p; }
+    const T *data() const { return p; }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+    // no destructor: the HIP runtime may already be gone when static state is destroyed
+};
+
 struct ModuleState {
     // sync_from_init_adaptahop (f90:591-641)
     int64_t npart = 0;
@@ -44,11 +70,12 @@ struct ModuleState {
     // outputs (allocated by compute_adaptahop, freed by close)
     bool allocated = false;
     std::vector<int32_t> liste_parts;
-    std::vector<double> density;
+    PinnedVec<double> density;
     int32_t nnodes = 0;
     std::vector<double> real_table;
     std::vector<int32_t> integer_table;
-    std::vector<int32_t> gid, hop, sad_offset, sad_isad;
+    PinnedVec<int32_t> gid, hop;
+    std::vector<int32_t> sad_offset, sad_isad;
     std::vector<double> sad_rho, densityg;
     int32_t ngroups = 0;
     // last device run (ahop_run_device)
@@ -97,7 +124,6 @@ void free_outputs(ModuleState &s)
     s.liste_parts.clear();
     s.liste_parts.shrink_to_fit();
     s.density.clear();
-    s.density.shrink_to_fit();
     s.real_table.clear();
     s.integer_table.clear();
     s.nnodes = 0;
@@ -280,8 +306,12 @@ int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_c
         TreeOutputs to;
         create_group_tree(ti, tp, to);
         auto t2 = std::chrono::steady_clock::now();
-        s.liste_parts.assign(s.nusedpart, 0);
-        std::copy(to.liste_parts.begin(), to.liste_parts.end(), s.liste_parts.begin());
+        if ((int64_t)to.liste_parts.size() == (int64_t)s.nusedpart) {
+            s.liste_parts = std::move(to.liste_parts);
+        } else {
+            s.liste_parts.assign(s.nusedpart, 0);
+            std::copy(to.liste_parts.begin(), to.liste_parts.end(), s.liste_parts.begin());
+        }
         s.nnodes = to.nnodes;
         s.real_table = std::move(to.real_table);
         s.integer_table = std::move(to.integer_table);
