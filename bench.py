@@ -75,7 +75,7 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
     the reference's per-halo reductions timed on a bounded sample of the same halos."""
     import numpy as np
     import torch
-    from amdahop import generate as G, halo_props
+    from amdahop import generate as G, halo_profiles, halo_props
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
@@ -112,6 +112,23 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
              for _ in range(steps)]
     out["with_sc_recentring"] = {"ms": round(float(np.median(sc_ms)), 3), **sc}
     log(f"halo props with SC: {out['with_sc_recentring']['ms']:.2f} ms")
+    # extended profiles (mode 'full'): peak circular velocity, NFW fit, inner slope
+    props, _ = halo_props(hs.offsets, pos, vel, mass, hub_pt=70.3, tile=tile, workgroup=wg)
+    halo_profiles(hs.offsets, pos, mass, props)
+    pr_ms, pr_wall = [], []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        w1 = time.perf_counter()
+        prof, dms = halo_profiles(hs.offsets, pos, mass, props)
+        torch.cuda.synchronize()
+        pr_wall.append((time.perf_counter() - w1) * 1e3)
+        pr_ms.append(dms)
+    nfit = int(torch.isfinite(prof[:, 2]).sum().item())
+    out["extended_profiles"] = {"ms": round(float(np.median(pr_ms)), 3),
+                                "wall_ms": round(float(np.median(pr_wall)), 3),
+                                "nfw_fits": nfit}
+    log(f"extended profiles: {out['extended_profiles']['ms']:.2f} ms device, "
+        f"{out['extended_profiles']['wall_ms']:.2f} ms wall, {nfit} NFW fits")
     del pos, vel, mass
     if with_cpu:
         import oracle.halo_props as HP
@@ -128,6 +145,16 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
                                "cores": 1, "kind": "port",
                                "sample": f"{cpu_sample_halos} random halos of the same workload "
                                          f"({nmem} members), NumPy restatement, {t:.1f} s"}
+        import oracle.halo_profiles as HPF
+        pr = props.cpu().numpy()
+        c0 = time.perf_counter()
+        for h in pick:
+            s, e = int(hs.offsets[h]), int(hs.offsets[h + 1])
+            HPF.extended_profiles(hs.pos[s:e], hs.mass[s:e], pr[h, 1:4], pr[h, 24], 1.0, 430.1)
+        t = time.perf_counter() - c0
+        out["extended_profiles"]["cpu_baseline"] = {
+            "value": round(len(pick) / t, 1), "unit": "halos/s", "cores": 1, "kind": "port",
+            "sample": f"the same {cpu_sample_halos} halos, NumPy/SciPy restatement, {t:.1f} s"}
     return out
 
 

@@ -14,7 +14,8 @@ import ctypes
 from ._lib import HotParams, RunInfo, check, lib
 
 __all__ = ["run_device", "device_saddle", "knn_device", "density_adjoint", "sph_density",
-           "halo_props", "HALO_FIELDS", "HotParams", "RunInfo"]
+           "halo_props", "HALO_FIELDS", "halo_profiles", "PROFILE_FIELDS", "HotParams",
+           "RunInfo"]
 
 
 def run_device(pos, mass, nvoisins, nhop=16, rho_threshold=80.0, box=1.0, mixed=True,
@@ -221,3 +222,81 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
                                        ctypes.c_void_p(mass.data_ptr()), dptr, ctypes.byref(p),
                                        ctypes.c_void_p(out.data_ptr()), ctypes.byref(info)))
     return out, info
+
+
+PROFILE_FIELDS = ["vmaxcir", "rmaxcir", "cNFW", "cNFWerr", "inslope", "inslopeerr"]
+
+
+def profile_bin_edges(first, last, nin):
+    """Log-spaced bin edges of every halo eligible for the profile fits, computed as the
+    reference's np.logspace(np.log10(first), np.log10(last), nbins + 1) with
+    nbins = max(8, min(80, int(np.sqrt(nin)))) (compute_halo_props.py:1862-1863), vectorised
+    over halos with the same element operations (linspace: i * step + start, last point =
+    stop; then np.power(10., y)), so that the end members fall inside or outside the
+    histogram exactly as there.  ``first`` / ``last`` / ``nin``: arrays over the eligible
+    halos.  Returns (edges, per-halo edge counts)."""
+    import numpy as np
+    nin = np.asarray(nin, dtype=np.int64)
+    nb = np.maximum(8, np.minimum(80, np.sqrt(nin.astype(np.float64)).astype(np.int64)))
+    num = nb + 1
+    start = np.log10(np.asarray(first, dtype=np.float64))
+    stop = np.log10(np.asarray(last, dtype=np.float64))
+    step = (stop - start) / (num - 1)
+    eoff = np.zeros(len(num) + 1, dtype=np.int64)
+    np.cumsum(num, out=eoff[1:])
+    i = (np.arange(eoff[-1], dtype=np.int64) - np.repeat(eoff[:-1], num)).astype(np.float64)
+    y = i * np.repeat(step, num)
+    y += np.repeat(start, num)
+    y[eoff[1:] - 1] = stop
+    return np.power(10.0, y), num
+
+
+def halo_profiles(offsets, pos, mass, props, *, lbox=1.0, gravconst=430.1):
+    """Extended profiles (compute_extended_profiles_1e, compute_halo_props.py:1819-1926) of
+    every halo of a CSR member layout whose members are all dark matter, about the centres
+    and rvir of ``props`` (the rows of :func:`halo_props`).  Device tensors in, one row per
+    halo in the order of PROFILE_FIELDS out.  Returns (tensor (H, 6) float64, device ms)."""
+    import numpy as np
+    import torch
+
+    from ._lib import ptr
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nh = off.shape[0] - 1
+    for t in (pos, mass, props):
+        if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+            raise TypeError("pos, mass, props must be contiguous float64 device tensors")
+    nm = mass.shape[0]
+    if pos.shape != (nm, 3) or props.shape != (nh, len(HALO_FIELDS)):
+        raise ValueError("pos must be (M, 3) and props (nhalos, %d)" % len(HALO_FIELDS))
+    if off.ndim != 1 or nh < 1 or off[0] != 0 or off[-1] != nm:
+        raise ValueError("offsets must be a 1-D CSR array over all of mass, from 0")
+    dev = pos.device
+    sdist = torch.empty(nm, dtype=torch.float64, device=dev)
+    smass = torch.empty(nm, dtype=torch.float64, device=dev)
+    aux = torch.empty((nh, 4), dtype=torch.float64, device=dev)
+    prof = torch.empty((nh, len(PROFILE_FIELDS)), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize(dev)
+    ms1, ms2 = ctypes.c_float(0.0), ctypes.c_float(0.0)
+    vp = ctypes.c_void_p
+    check(lib().ahop_halo_profile_radii_device(
+        nh, ptr(off), vp(pos.data_ptr()), vp(mass.data_ptr()), vp(props.data_ptr()),
+        float(lbox), float(gravconst), vp(sdist.data_ptr()), vp(smass.data_ptr()),
+        vp(aux.data_ptr()), vp(prof.data_ptr()), ctypes.byref(ms1)))
+    a = aux.cpu().numpy()
+    rvir = props[:, HALO_FIELDS.index("rvir")].cpu().numpy()
+    nin = a[:, 1].astype(np.int64)
+    # the reference's early returns (:1842-1860): fewer than 8 non-zero distances, rvir not
+    # > 0, fewer than 8 members inside rvir
+    elig = (np.diff(off) - a[:, 0] >= 8) & (rvir > 0) & (nin >= 8)
+    edges, num = profile_bin_edges(a[elig, 2], a[elig, 3], nin[elig])
+    counts = np.zeros(nh, dtype=np.int64)
+    counts[elig] = num
+    eoff = np.zeros(nh + 1, dtype=np.int64)
+    np.cumsum(counts, out=eoff[1:])
+    d_edges = torch.from_numpy(edges).to(dev) if len(edges) else None
+    torch.cuda.synchronize(dev)
+    check(lib().ahop_halo_profile_fits_device(
+        nh, ptr(off), vp(sdist.data_ptr()), vp(smass.data_ptr()), vp(props.data_ptr()),
+        vp(aux.data_ptr()), ptr(eoff), vp(d_edges.data_ptr()) if d_edges is not None else None,
+        vp(prof.data_ptr()), ctypes.byref(ms2)))
+    return prof, ms1.value + ms2.value

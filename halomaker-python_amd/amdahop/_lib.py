@@ -71,6 +71,10 @@ SIGNATURES = {
                                        ctypes.POINTER(RunInfo)]),
     "ahop_density_adjoint_device": (ctypes.c_int, [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _dbl,
                                                    _dbl, _dbl, _vp, _vp, ctypes.POINTER(_f32)]),
+    "ahop_halo_profile_radii_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp,
+                                                       _vp, _vp, _vp, ctypes.POINTER(_f32)]),
+    "ahop_halo_profile_fits_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                      _vp, ctypes.POINTER(_f32)]),
     "ahop_group_members_device": (ctypes.c_int, [_vp, _i64, _vp, _i32, _i32, _vp, _vp, _vp,
                                                  ctypes.POINTER(_f32)]),
     "ahop_halo_props_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp,

@@ -123,6 +123,20 @@ float group_members(const int32_t *d_node, int64_t n, const int32_t *d_node_to_s
                     int32_t nnodes, int32_t nlab, int32_t *d_labels_out, int32_t *d_pids,
                     int64_t *d_offsets, hipStream_t s);
 
+// Extended profiles (halo_profiles.hip).  radii: distances from each halo's centre sorted per
+// halo (d_sdist, d_smass: M), per halo aux (first non-zero slot, members inside rvir, first
+// and last such distance) and vmaxcir, rmaxcir in d_prof (nh x 6; the fit columns NaN).
+// fits: with the caller's bin edges (h_eoff: host nh+1 offsets into d_edges; 0 edges = no
+// fits) the NFW and inner-slope fits.  props: nh x 29 halo property rows.  Device ms.
+float halo_profile_radii(const int64_t *h_off, int64_t nh, const double *d_pos,
+                         const double *d_mass, const double *d_props, double L, double G,
+                         double *d_sdist, double *d_smass, double *d_aux, double *d_prof,
+                         hipStream_t s);
+float halo_profile_fits(const int64_t *h_off, int64_t nh, const double *d_sdist,
+                        const double *d_smass, const double *d_props, const double *d_aux,
+                        const int64_t *h_eoff, const double *d_edges, double *d_prof,
+                        hipStream_t s);
+
 float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32_t *d_nbr,
                           const int32_t *d_order, const double *d_grad_rho, int64_t n, int K,
                           double Lx, double Ly, double Lz, double *d_gpos, double *d_gmass,
