@@ -2,8 +2,8 @@
 // compute_extended_profiles_1e (src/compute_halo_props.py:1819-1926) for halos whose members
 // are all dark matter.
 //
-//   radii:  member distances from the halo centre, sorted per halo (one segmented radix sort of
-//           (distance, mass) pairs), then per halo the circular-velocity peak over the
+//   radii:  member distances from the halo centre, sorted per halo (a stable radix sort by
+//           distance, then one by halo), then per halo the circular-velocity peak over the
 //           cumulative mass (vmaxcir, rmaxcir, :1827-1840) and the members inside rvir.
 //   fits:   with the log-spaced bin edges of each halo (computed by the caller in NumPy, whose
 //           log10 and power decide which end members fall inside, :1862-1863), the weighted
@@ -12,12 +12,13 @@
 //           inslopeerr, :1893-1926).  One wave per halo.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <stdexcept>
 #include <vector>
 
-#include <rocprim/device/device_segmented_radix_sort.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "ahop_common.h"
 
@@ -61,9 +62,10 @@ __device__ int64_t bound(const double *d, int64_t lo, int64_t hi, double x, bool
     return lo;
 }
 
-// distance of every member from its halo's centre (wrapped as correct_for_periodicity_1d)
-__global__ void k_prof_dist(const int64_t *off, const double *pos, const double *mass,
-                            const double *props, double L, double *key, double *val)
+// distance of every member from its halo's centre (wrapped as correct_for_periodicity_1d),
+// with the member's index and halo: the keys of the two sorts below
+__global__ void k_prof_keys(const int64_t *off, const double *pos, const double *props,
+                            double L, double *dist, int32_t *idx, uint32_t *hid)
 {
     const int h = blockIdx.x;
     const double *c = props + (int64_t)h * PROPS_STRIDE + P_PX;
@@ -71,8 +73,27 @@ __global__ void k_prof_dist(const int64_t *off, const double *pos, const double 
         const double dx = wrap_strict(pos[3 * i] - c[0], L);
         const double dy = wrap_strict(pos[3 * i + 1] - c[1], L);
         const double dz = wrap_strict(pos[3 * i + 2] - c[2], L);
-        key[i] = sqrt(dx * dx + dy * dy + dz * dz);
-        val[i] = mass[i];
+        dist[i] = sqrt(dx * dx + dy * dy + dz * dz);
+        idx[i] = (int32_t)i;
+        hid[i] = (uint32_t)h;
+    }
+}
+
+__global__ void k_prof_gather_hid(const int32_t *idx, const uint32_t *hid, int64_t M,
+                                  uint32_t *out)
+{
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p < M) out[p] = hid[idx[p]];
+}
+
+__global__ void k_prof_gather(const int32_t *idx, const double *dist, const double *mass,
+                              int64_t M, double *sd, double *sm)
+{
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p < M) {
+        const int32_t i = idx[p];
+        sd[p] = dist[i];
+        sm[p] = mass[i];
     }
 }
 
@@ -366,24 +387,36 @@ float halo_profile_radii(const int64_t *h_off, int64_t nh, const double *d_pos,
     for (int64_t h = 0; h < nh; ++h)
         if (h_off[h + 1] <= h_off[h]) throw std::runtime_error("every halo needs at least one member");
     const int64_t M = h_off[nh];
-    DevBuf doff, kin, vin, temp;
+    if (M > INT32_MAX) throw std::runtime_error("at most 2^31-1 members");
+    // sorted per halo by two stable device-wide radix sorts: by distance, then by halo (the
+    // second keeps the first's order within a halo), so no halo waits on one workgroup
+    DevBuf doff, ddist, dd2, didx, di2, dhid, dh2, temp;
     int64_t *d_off = (int64_t *)doff.ensure(sizeof(int64_t) * (nh + 1));
-    double *k0 = (double *)kin.ensure(sizeof(double) * M);
-    double *v0 = (double *)vin.ensure(sizeof(double) * M);
+    double *dist = (double *)ddist.ensure(sizeof(double) * M);
+    double *dist_s = (double *)dd2.ensure(sizeof(double) * M);
+    int32_t *idx = (int32_t *)didx.ensure(sizeof(int32_t) * M);
+    int32_t *idx_s = (int32_t *)di2.ensure(sizeof(int32_t) * M);
+    uint32_t *hid = (uint32_t *)dhid.ensure(sizeof(uint32_t) * M);
+    uint32_t *hid_s = (uint32_t *)dh2.ensure(sizeof(uint32_t) * M);
+    int hbits = 1;
+    while (hbits < 32 && ((nh - 1) >> hbits) != 0) ++hbits;
     hipEvent_t e0, e1;
     AHOP_HIP(hipEventCreate(&e0));
     AHOP_HIP(hipEventCreate(&e1));
     AHOP_HIP(hipMemcpyAsync(d_off, h_off, sizeof(int64_t) * (nh + 1), hipMemcpyHostToDevice, s));
     AHOP_HIP(hipEventRecord(e0, s));
-    hipLaunchKernelGGL(k_prof_dist, dim3((unsigned)nh), dim3(256), 0, s, d_off, d_pos, d_mass,
-                       d_props, L, k0, v0);
-    size_t tb = 0;
-    AHOP_HIP(rocprim::segmented_radix_sort_pairs(nullptr, tb, k0, d_sdist, v0, d_smass,
-                                                 (unsigned int)M, (unsigned int)nh, d_off,
-                                                 d_off + 1, 0, 64, s));
-    AHOP_HIP(rocprim::segmented_radix_sort_pairs(temp.ensure(tb), tb, k0, d_sdist, v0, d_smass,
-                                                 (unsigned int)M, (unsigned int)nh, d_off,
-                                                 d_off + 1, 0, 64, s));
+    const unsigned gb = (unsigned)((M + 255) / 256);
+    hipLaunchKernelGGL(k_prof_keys, dim3((unsigned)nh), dim3(256), 0, s, d_off, d_pos, d_props,
+                       L, dist, idx, hid);
+    size_t tb1 = 0, tb2 = 0;
+    AHOP_HIP(rocprim::radix_sort_pairs(nullptr, tb1, dist, dist_s, idx, idx_s, (size_t)M, 0, 64, s));
+    AHOP_HIP(rocprim::radix_sort_pairs(nullptr, tb2, hid_s, hid, idx_s, idx, (size_t)M, 0, hbits, s));
+    void *tmp = temp.ensure(std::max(tb1, tb2));
+    AHOP_HIP(rocprim::radix_sort_pairs(tmp, tb1, dist, dist_s, idx, idx_s, (size_t)M, 0, 64, s));
+    hipLaunchKernelGGL(k_prof_gather_hid, dim3(gb), dim3(256), 0, s, idx_s, hid, (int64_t)M, hid_s);
+    AHOP_HIP(rocprim::radix_sort_pairs(tmp, tb2, hid_s, hid, idx_s, idx, (size_t)M, 0, hbits, s));
+    hipLaunchKernelGGL(k_prof_gather, dim3(gb), dim3(256), 0, s, idx, dist, d_mass, (int64_t)M,
+                       d_sdist, d_smass);
     hipLaunchKernelGGL(k_prof_vmax<256>, dim3((unsigned)nh), dim3(256), 0, s, d_off, d_sdist,
                        d_smass, d_props, G, d_aux, d_prof);
     AHOP_HIP(hipEventRecord(e1, s));
