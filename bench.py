@@ -75,7 +75,7 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
     the reference's per-halo reductions timed on a bounded sample of the same halos."""
     import numpy as np
     import torch
-    from amdahop import generate as G, halo_profiles, halo_props
+    from amdahop import generate as G, halo_profiles, halo_props, halo_stellar
 
     dev = torch.device("cuda:0")
     t0 = time.perf_counter()
@@ -129,6 +129,22 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
                                 "nfw_fits": nfit}
     log(f"extended profiles: {out['extended_profiles']['ms']:.2f} ms device, "
         f"{out['extended_profiles']['wall_ms']:.2f} ms wall, {nfit} NFW fits")
+    # stellar properties: the first fifth of each halo's members (random positions) as stars,
+    # ages 1 Myr - 10 Gyr log-uniform, random metallicities and densities (seed 6)
+    rng = np.random.default_rng(6)
+    local = np.arange(M) - np.repeat(hs.offsets[:-1], np.diff(hs.offsets))
+    star_h = local < np.repeat(np.maximum(1, np.diff(hs.offsets) // 5), np.diff(hs.offsets))
+    age_h = 10 ** rng.uniform(-3, 1, M)
+    metal_h = rng.uniform(0, 0.04, M)
+    dens_h = rng.uniform(1, 100, M)
+    star_d = torch.from_numpy(star_h).to(dev)
+    age_d, metal_d, dens_d = (torch.from_numpy(a).to(dev) for a in (age_h, metal_h, dens_h))
+    halo_stellar(hs.offsets, pos, vel, mass, dens_d, star_d, age_d, metal_d, props)
+    st_ms = [halo_stellar(hs.offsets, pos, vel, mass, dens_d, star_d, age_d, metal_d, props)[1]
+             for _ in range(steps)]
+    out["stellar"] = {"ms": round(float(np.median(st_ms)), 3), "stars": int(star_h.sum())}
+    log(f"stellar: {out['stellar']['ms']:.2f} ms for {int(star_h.sum())} stars")
+    del star_d, age_d, metal_d, dens_d
     del pos, vel, mass
     if with_cpu:
         import oracle.halo_props as HP
@@ -152,6 +168,17 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
             s, e = int(hs.offsets[h]), int(hs.offsets[h + 1])
             HPF.extended_profiles(hs.pos[s:e], hs.mass[s:e], pr[h, 1:4], pr[h, 24], 1.0, 430.1)
         t = time.perf_counter() - c0
+        import oracle.halo_stellar as HSt
+        pv = props.cpu().numpy()[:, 4:7]
+        c0 = time.perf_counter()
+        for h in pick:
+            s, e = int(hs.offsets[h]), int(hs.offsets[h + 1])
+            HSt.stellar_props(hs.pos[s:e], hs.vel[s:e], hs.mass[s:e], dens_h[s:e], star_h[s:e],
+                              age_h[s:e], metal_h[s:e], pv[h])
+        ts = time.perf_counter() - c0
+        out["stellar"]["cpu_baseline"] = {
+            "value": round(len(pick) / ts, 1), "unit": "halos/s", "cores": 1, "kind": "port",
+            "sample": f"the same {cpu_sample_halos} halos, NumPy restatement, {ts:.1f} s"}
         out["extended_profiles"]["cpu_baseline"] = {
             "value": round(len(pick) / t, 1), "unit": "halos/s", "cores": 1, "kind": "port",
             "sample": f"the same {cpu_sample_halos} halos, NumPy/SciPy restatement, {t:.1f} s"}

@@ -14,8 +14,8 @@ import ctypes
 from ._lib import HotParams, RunInfo, check, lib
 
 __all__ = ["run_device", "device_saddle", "knn_device", "density_adjoint", "sph_density",
-           "halo_props", "HALO_FIELDS", "halo_profiles", "PROFILE_FIELDS", "HotParams",
-           "RunInfo"]
+           "halo_props", "HALO_FIELDS", "halo_profiles", "PROFILE_FIELDS", "halo_stellar",
+           "STELLAR_FIELDS", "HotParams", "RunInfo"]
 
 
 def run_device(pos, mass, nvoisins, nhop=16, rho_threshold=80.0, box=1.0, mixed=True,
@@ -189,10 +189,12 @@ HALO_FIELDS = ["m", "px", "py", "pz", "vx", "vy", "vz", "sigma", "Lx", "Ly", "Lz
 
 def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst=430.1,
                vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000,
-               tile=0, workgroup=0, sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0):
+               tile=0, workgroup=0, sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0,
+               light=False):
     """Per-halo properties of a CSR member layout (device tensors), one row per halo in
     the order of HALO_FIELDS.  ``sc`` / ``dpmmc`` recentre each halo as the reference's
     SC / DPMMC options do (with ``dcell_min`` and ``eps_sc``; DPMMC needs ``density``).
+    ``light`` skips the shell tables, energies and virial scan (those columns read NaN).
     Returns (out tensor (H, 29) float64, HaloInfo)."""
     import numpy as np
     import torch
@@ -213,7 +215,7 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
     p = HaloParams(float(lbox), float(hub_pt), float(gravconst), float(vir_overdens),
                    float(rho_mean), 1 if cdm else 0, int(nshell), int(direct_pe_nmax), int(tile),
                    int(workgroup), 1 if sc else 0, 1 if dpmmc else 0, float(dcell_min),
-                   float(eps_sc))
+                   float(eps_sc), 1 if light else 0)
     info = HaloInfo()
     torch.cuda.synchronize(pos.device)
     dptr = ctypes.c_void_p(density.data_ptr()) if density is not None else None
@@ -300,3 +302,50 @@ def halo_profiles(offsets, pos, mass, props, *, lbox=1.0, gravconst=430.1):
         vp(aux.data_ptr()), ptr(eoff), vp(d_edges.data_ptr()) if d_edges is not None else None,
         vp(prof.data_ptr()), ctypes.byref(ms2)))
     return prof, ms1.value + ms2.value
+
+
+STELLAR_FIELDS = ["ndm", "nstar", "mdm", "m*", "mcontam", "sigma_dm", "sigma*",
+                  "px*", "py*", "pz*", "Lx*", "Ly*", "Lz*", "r*", "r50", "r90", "age", "metal",
+                  "SFR", "SFR_r50", "SFR_r90", "SFR10", "SFR10_r50", "SFR10_r90",
+                  "vrot", "sig3d", "sigcyl", "vrot_r50", "sig3d_r50", "sigcyl_r50",
+                  "vrot_r90", "sig3d_r90", "sigcyl_r90"]
+
+
+def halo_stellar(offsets, pos, vel, mass, density, is_star, age, metal, props, *, lbox=1.0,
+                 gravconst=430.1, massp=0.0, sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0):
+    """Stellar properties (mode 'full') of every halo of a CSR member layout: device tensors
+    pos/vel (M, 3), mass/density/age/metal (M,) float64, is_star (M,) bool or uint8 (the
+    reference's indexps >= ndm), props the rows of :func:`halo_props` (the mean velocity).
+    The SC / DPMMC options refine the stellar centre as the reference's do.  Returns
+    (tensor (H, 33) in the order of STELLAR_FIELDS, device ms)."""
+    import numpy as np
+    import torch
+
+    from ._lib import HaloParams, ptr
+    off = np.ascontiguousarray(offsets, dtype=np.int64)
+    nh = off.shape[0] - 1
+    nm = mass.shape[0]
+    for t in (pos, vel, mass, density, age, metal, props):
+        if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+            raise TypeError("pos, vel, mass, density, age, metal, props must be contiguous "
+                            "float64 device tensors")
+    if pos.shape != (nm, 3) or vel.shape != (nm, 3) or props.shape != (nh, len(HALO_FIELDS)):
+        raise ValueError("pos and vel must be (M, 3) and props (nhalos, %d)" % len(HALO_FIELDS))
+    for t in (density, age, metal, is_star):
+        if tuple(t.shape) != (nm,):
+            raise ValueError("density, age, metal and is_star must be (M,)")
+    if off.ndim != 1 or nh < 1 or off[0] != 0 or off[-1] != nm:
+        raise ValueError("offsets must be a 1-D CSR array over all of mass, from 0")
+    star = is_star.to(torch.uint8).contiguous()
+    out = torch.empty((nh, len(STELLAR_FIELDS)), dtype=torch.float64, device=pos.device)
+    p = HaloParams(float(lbox), 70.3, float(gravconst), 340.0, 1.0, 0, 1000, 1000, 0, 0,
+                   1 if sc else 0, 1 if dpmmc else 0, float(dcell_min), float(eps_sc), 1)
+    ms = ctypes.c_float(0.0)
+    vp = ctypes.c_void_p
+    torch.cuda.synchronize(pos.device)
+    check(lib().ahop_halo_stellar_device(
+        nh, ptr(off), vp(pos.data_ptr()), vp(vel.data_ptr()), vp(mass.data_ptr()),
+        vp(density.data_ptr()), vp(star.data_ptr()), vp(age.data_ptr()), vp(metal.data_ptr()),
+        vp(props.data_ptr()), ctypes.byref(p), float(massp), vp(out.data_ptr()),
+        ctypes.byref(ms)))
+    return out, ms.value

@@ -29,7 +29,7 @@ class HaloParams(ctypes.Structure):
     _fields_ = [("lbox", _dbl), ("hub_pt", _dbl), ("gravconst", _dbl), ("vir_overdens", _dbl),
                 ("rho_mean", _dbl), ("cdm", _i32), ("nshell", _i32), ("direct_pe_nmax", _i32),
                 ("tile", _i32), ("workgroup", _i32), ("sc", _i32), ("dpmmc", _i32),
-                ("dcell_min", _dbl), ("eps_sc", _dbl)]
+                ("dcell_min", _dbl), ("eps_sc", _dbl), ("light", _i32)]
 
 
 class HaloInfo(ctypes.Structure):
@@ -71,6 +71,9 @@ SIGNATURES = {
                                        ctypes.POINTER(RunInfo)]),
     "ahop_density_adjoint_device": (ctypes.c_int, [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _dbl,
                                                    _dbl, _dbl, _vp, _vp, ctypes.POINTER(_f32)]),
+    "ahop_halo_stellar_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                                 ctypes.POINTER(HaloParams), _dbl, _vp,
+                                                 ctypes.POINTER(_f32)]),
     "ahop_halo_profile_radii_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp,
                                                        _vp, _vp, _vp, ctypes.POINTER(_f32)]),
     "ahop_halo_profile_fits_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

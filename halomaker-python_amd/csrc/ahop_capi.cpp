@@ -492,6 +492,33 @@ int ahop_group_members_device(const int32_t *d_node_of_part, int64_t n,
     });
 }
 
+int ahop_halo_stellar_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
+                             const double *d_vel, const double *d_mass, const double *d_density,
+                             const uint8_t *d_is_star, const double *d_age,
+                             const double *d_metal, const double *d_props,
+                             const ahop_halo_params *params, double massp, double *d_out,
+                             float *ms)
+{
+    return guarded([&] {
+        if (!offsets || !d_pos || !d_vel || !d_mass || !d_is_star || !d_age || !d_metal ||
+            !d_props || !params || !d_out)
+            throw std::runtime_error("null argument");
+        static_assert(AHOP_STELLAR_NFIELDS == 33, "stellar field count");
+        HaloParamsDev P;
+        P.lbox = params->lbox;
+        P.gravconst = params->gravconst;
+        P.sc = params->sc;
+        P.dpmmc = params->dpmmc;
+        P.dcell_min = params->dcell_min;
+        P.eps_sc = params->eps_sc;
+        if (!(P.lbox > 0)) throw std::runtime_error("lbox must be > 0");
+        Engine &e = engine();
+        float t = halo_stellar_device(offsets, nhalos, d_pos, d_vel, d_mass, d_density, d_is_star,
+                                      d_age, d_metal, d_props, P, massp, d_out, e.stream());
+        if (ms) *ms = t;
+    });
+}
+
 int ahop_halo_profile_radii_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
                                    const double *d_mass, const double *d_props, double lbox,
                                    double gravconst, double *d_sdist, double *d_smass,
@@ -598,6 +625,7 @@ int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double 
         P.dpmmc = params->dpmmc;
         P.dcell_min = params->dcell_min;
         P.eps_sc = params->eps_sc;
+        P.light = params->light;
         if (!(P.lbox > 0)) throw std::runtime_error("lbox must be > 0");
         Engine &e = engine();
         HaloTuning t = halo_props_device(offsets, nhalos, d_pos, d_vel, d_mass, d_density, P,

@@ -26,12 +26,21 @@ struct HaloParamsDev {
     int dpmmc = 0;              // DPMMC: densest member of the most massive cell, then SC if on
     double dcell_min = 0.0;     // smallest cell / sphere radius of the recentring
     double eps_sc = 0.0;        // sphere radius shrink fraction of SC
+    int light = 0;              // 1: mass, centre, mean velocity, sigma, L, r, I, axes, ek only
+                                // (ep, et, rvir, mvir, cvel, tvir, spin left NaN)
 };
 
 struct HaloTuning {
     int tile = 0, bs = 0, trials = 0;
     float ms = 0;   // device time of the final run, all passes
 };
+
+// Stellar properties (halo_stellar.hip): is_star / age / metal per member, props the
+// nhalos x HP_NFIELDS rows of halo_props_device (mean velocity), out nhalos x 33.
+float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, const double *vel,
+                          const double *mass, const double *dens, const uint8_t *is_star,
+                          const double *age, const double *metal, const double *props,
+                          const HaloParamsDev &P, double massp, double *out, hipStream_t s);
 
 // pos/vel (M,3), mass (M), dens (M, may be null when cdm) are device pointers in CSR order;
 // h_off (nhalos+1) host offsets; d_out device (nhalos x HP_NFIELDS).  tile/bs <= 0 autotune.

@@ -1254,6 +1254,8 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
         AHOP_HIP(hipEventCreate(&e0));
         AHOP_HIP(hipEventCreate(&e1));
         AHOP_HIP(hipEventRecord(e0, s));
+        if (P.light)   // the columns of the skipped stages read NaN (all-ones bytes)
+            AHOP_HIP(hipMemsetAsync(d_out, 0xFF, sizeof(double) * HP_NFIELDS * nh, s));
         hipLaunchKernelGGL(k_pass1<BS>, dim3(nt), dim3(BS), 0, s, D, part);
         hipLaunchKernelGGL(k_combine1, dim3(hb), dim3(128), 0, s, nh, d_tf, part, d_off, pos, P.lbox, st);
         hipLaunchKernelGGL(k_pass2<BS>, dim3(nt), dim3(BS), 0, s, D, st, P.cdm, pkey, pidx);
@@ -1292,7 +1294,7 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
         }
         hipLaunchKernelGGL(k_pass3<BS>, dim3(nt), dim3(BS), 0, s, D, st, part);
         hipLaunchKernelGGL(k_combine3, dim3(hb), dim3(128), 0, s, nh, d_tf, part, P.gravconst, st, d_out);
-        if (full) {
+        if (full && !P.light) {
             hipLaunchKernelGGL(k_pass4a<BS>, dim3(nt), dim3(BS), 0, s, D, st, pkey);
             hipLaunchKernelGGL(k_combine4a, dim3(hb), dim3(128), 0, s, nh, d_tf, pkey, st);
             const int nr = P.nshell;

@@ -172,6 +172,8 @@ typedef struct {
     int32_t dpmmc;          /* DPMMC: multiscale-grid recentring before SC (needs d_density) */
     double dcell_min;       /* dcell_min: smallest cell / sphere radius, > 0 when SC or DPMMC */
     double eps_sc;          /* eps_SC: sphere shrink fraction, in (0, 1) when SC */
+    int32_t light;          /* 1: skip the shell tables, energies and virial scan (ep, et,
+                               rvir, mvir, cvel, tvir, spin read NaN) */
 } ahop_halo_params;
 
 typedef struct {
@@ -190,6 +192,29 @@ typedef struct {
 int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
                            const double *d_vel, const double *d_mass, const double *d_density,
                            const ahop_halo_params *params, double *d_out, ahop_halo_info *info);
+
+/* ---- stellar properties (mode 'full', stellar payload) ----------------------------- */
+
+/* Columns of d_out (nhalos rows):
+ * ndm, nstar, mdm, m*, mcontam, sigma_dm, sigma*, px*, py*, pz*, Lx*, Ly*, Lz*, r*, r50, r90,
+ * age, metal, SFR, SFR_r50, SFR_r90, SFR10, SFR10_r50, SFR10_r90, vrot, sig3d, sigcyl,
+ * vrot_r50, sig3d_r50, sigcyl_r50, vrot_r90, sig3d_r90, sigcyl_r90 */
+#define AHOP_STELLAR_NFIELDS 33
+
+/* Stellar properties of every halo of a CSR member layout: det_mass_17 (:765-780),
+ * det_center_18 (:963-966), the stellar parts of refine_centers_1b (:1569-1593),
+ * compute_ang_mom_19 (:806-822) and r_halos_1a (:842-849), and compute_stellar_1d
+ * (:1635-1751, without the optional chemistry), src/compute_halo_props.py.  d_is_star (M,
+ * uint8: the reference's indexps >= ndm), d_age (Gyr) and d_metal (M, read for stars), d_density
+ * (M, needed when any member is a star), d_props the ahop_halo_props_device rows (the mean
+ * velocity); params: lbox and the SC / DPMMC options (cdm and light ignored); massp for
+ * mcontam.  Halos without stars keep the values of the reference's clear_halo (NaN, SFRs 0). */
+int ahop_halo_stellar_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
+                             const double *d_vel, const double *d_mass, const double *d_density,
+                             const uint8_t *d_is_star, const double *d_age,
+                             const double *d_metal, const double *d_props,
+                             const ahop_halo_params *params, double massp, double *d_out,
+                             float *ms);
 
 /* ---- extended profiles (compute_extended_profiles_1e, src/compute_halo_props.py:1819-1926),
  * halos whose members are all dark matter --------------------------------------------- */
