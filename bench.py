@@ -234,6 +234,57 @@ def grouping_stage(gid, ngroups, steps, with_cpu):
     return out
 
 
+def ramses_stage(with_cpu):
+    """The RAMSES particle reader on a 4,194,304-particle Ra3 output in 16 part files (a tenth
+    stars), written to a temporary directory first (so read from the page cache): the native
+    reader on 16 threads against the reference's per-file reading restated with NumPy/SciPy
+    on one core."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    from amdahop import ramses
+
+    n, ncpu, nst = 4_194_304, 16, 419_430
+    rng = np.random.default_rng(7)
+    top = tempfile.mkdtemp(prefix="amdahop_ramses_", dir="/tmp")
+    rep = os.path.join(top, "output_00001")
+    os.makedirs(rep)
+    try:
+        ids = np.concatenate([rng.permutation(n - nst) + 1, rng.permutation(nst) + 1])
+        birth = np.concatenate([np.zeros(n - nst), -rng.uniform(0.5, 3.0, nst)])
+        order = rng.permutation(n)
+        nbytes = 0
+        for icpu, part in enumerate(np.array_split(order, ncpu), start=1):
+            m = len(part)
+            path = os.path.join(rep, f"part_00001.out{icpu:05d}")
+            ramses.write_part_file(path, ncpu, rng.uniform(0, 1, (m, 3)),
+                                   rng.normal(size=(m, 3)), np.full(m, 1.0 / n), ids[part],
+                                   "Ra3", birth[part], rng.uniform(0, 0.03, m))
+            nbytes += os.path.getsize(path)
+        ramses.read_particles(rep, "Ra3", "all", scale_l=1.0, scale_t=1.0, nthreads=16)
+        t0 = time.perf_counter()
+        r = ramses.read_particles(rep, "Ra3", "all", scale_l=1.0, scale_t=1.0, nthreads=16)
+        t = time.perf_counter() - t0
+        out = {"particles": n, "files": ncpu, "bytes": nbytes, "ms": round(t * 1e3, 2),
+               "GB_per_s": round(nbytes / t / 1e9, 3), "threads": 16,
+               "Mparticles_per_s": round(n / t / 1e6, 2)}
+        assert r["npart"] == n
+        log(f"ramses reader: {t * 1e3:.1f} ms for {nbytes / 1e6:.0f} MB ({n} particles)")
+        if with_cpu:
+            import oracle.ramses_reader as ORR
+            c0 = time.perf_counter()
+            ORR.read_particles(rep, "00001", ncpu, "Ra3", "all", True)
+            tc = time.perf_counter() - c0
+            out["cpu_baseline"] = {"value": round(n / tc / 1e6, 3), "unit": "Mparticles/s",
+                                   "cores": 1, "kind": "port",
+                                   "sample": f"the same files, NumPy/SciPy FortranFile reading "
+                                             f"as the reference, {tc:.1f} s"}
+    finally:
+        shutil.rmtree(top, ignore_errors=True)
+    return out
+
+
 def adjoint_stage(steps, with_cpu):
     """SPH density adjoint on the 2,097,152-particle clumpy box (k=32): forward kNN rows
     frozen, upstream gradient of L = sum w log rho with w ~ N(0,1) (seed 3); device time
@@ -300,6 +351,7 @@ def main():
     ap.add_argument("--no-halo-props", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-grouping", action="store_true")
+    ap.add_argument("--no-ramses", action="store_true")
     # one plain device copy of this many MiB before the timed steps: under a counter
     # profiler its FETCH_SIZE / WRITE_SIZE readings calibrate bytes per counter unit
     ap.add_argument("--copy-calibration-mib", type=int, default=0)
@@ -420,6 +472,8 @@ def main():
                                                not args.no_cpu_baseline)
     if not args.no_adjoint:
         extra["density_adjoint"] = adjoint_stage(max(3, args.steps), not args.no_cpu_baseline)
+    if not args.no_ramses:
+        extra["ramses_reader"] = ramses_stage(not args.no_cpu_baseline)
 
     cpu = None
     if not args.no_cpu_baseline:

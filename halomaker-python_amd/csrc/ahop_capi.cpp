@@ -5,6 +5,7 @@
 #include "ahop_common.h"
 #include "create_nodes.h"
 #include "halo_props.h"
+#include "ramses_reader.h"
 
 #include <chrono>
 #include <cstdio>
@@ -489,6 +490,62 @@ int ahop_group_members_device(const int32_t *d_node_of_part, int64_t n,
         float t = group_members(d_node_of_part, n, d_node_to_struct, nnodes, nstruct,
                                 d_struct_of_part, d_pids, d_offsets, e.stream());
         if (ms) *ms = t;
+    });
+}
+
+static RamsesArgs ramses_args(const char *repository, const char *nchar, int32_t ncpu,
+                              int32_t rver, int32_t family, int32_t stellar, int32_t nthreads)
+{
+    if (!repository || !nchar) throw std::runtime_error("null path");
+    if (family < 0 || family > 2) throw std::runtime_error("family must be 0 (all), 1 (dm) or 2 (star)");
+    RamsesArgs A;
+    A.repository = repository;
+    A.nchar = nchar;
+    A.ncpu = ncpu;
+    A.rver = rver;
+    A.family = family;
+    A.stellar = stellar != 0;
+    A.nthreads = nthreads;
+    return A;
+}
+
+int ahop_ramses_count(const char *repository, const char *nchar, int32_t ncpu, int32_t rver,
+                      int32_t family, int32_t stellar, int32_t nthreads,
+                      ahop_ramses_counts *per_file)
+{
+    return guarded([&] {
+        if (!per_file) throw std::runtime_error("null per_file");
+        RamsesArgs A = ramses_args(repository, nchar, ncpu, rver, family, stellar, nthreads);
+        std::vector<RamsesFileCount> c(ncpu > 0 ? ncpu : 0);
+        ramses_count(A, c.data());
+        for (int i = 0; i < ncpu; ++i) per_file[i] = {c[i].ndm, c[i].nstar, c[i].nselected};
+    });
+}
+
+int ahop_ramses_read(const char *repository, const char *nchar, int32_t ncpu, int32_t rver,
+                     int32_t family, int32_t stellar, int32_t nthreads, int64_t ndm_total,
+                     int64_t nselected_total, int64_t nstar_total, double scale_l,
+                     double scale_t, double *pos, double *vel, double *mass,
+                     int64_t *source_pid, double *birth, double *metal,
+                     ahop_ramses_counts *per_file)
+{
+    return guarded([&] {
+        RamsesArgs A = ramses_args(repository, nchar, ncpu, rver, family, stellar, nthreads);
+        A.ndm_total = ndm_total;
+        A.nselected_total = nselected_total;
+        A.nstar_total = nstar_total;
+        A.scale_l = scale_l;
+        A.scale_t = scale_t;
+        A.pos = pos;
+        A.vel = vel;
+        A.mass = mass;
+        A.source_pid = source_pid;
+        A.birth = birth;
+        A.metal = metal;
+        std::vector<RamsesFileCount> c(ncpu > 0 ? ncpu : 0);
+        ramses_read(A, c.data());
+        if (per_file)
+            for (int i = 0; i < ncpu; ++i) per_file[i] = {c[i].ndm, c[i].nstar, c[i].nselected};
     });
 }
 

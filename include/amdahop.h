@@ -244,6 +244,34 @@ int ahop_halo_profile_fits_device(int64_t nhalos, const int64_t *offsets, const 
                                   const double *d_aux, const int64_t *edge_offsets,
                                   const double *d_edges, double *d_prof, float *ms);
 
+/* ---- RAMSES particle files (host, one OpenMP thread per file) ------------------------ */
+
+typedef struct {
+    int64_t ndm, nstar, nselected;   /* per file: dark matter, stars, selected by family */
+} ahop_ramses_counts;
+
+/* The counting sweep of read_ramses_new_101 (src/input_output.py:842-890) over
+ * repository/part_<nchar>.out<icpu:05d>, icpu = 1..ncpu: per-file counts into per_file
+ * (ncpu entries).  rver 3 (Ra3: families from birth time and id sign) or 4 (Ra4: family
+ * byte); family 0 all, 1 dm, 2 star; stellar: the stellar payload flag; nthreads <= 0: all. */
+int ahop_ramses_count(const char *repository, const char *nchar, int32_t ncpu, int32_t rver,
+                      int32_t family, int32_t stellar, int32_t nthreads,
+                      ahop_ramses_counts *per_file);
+
+/* The reading sweep (_read_ramses_new_1010, :624-760): selected particles scattered to their
+ * dense-id slots (stars after ndm_total dark-matter slots with family all): pos (nselected x
+ * 3, shifted by -0.5), vel (x 3, times scale_l / scale_t * 1e-5), mass; source_pid (int64,
+ * families dm / star, may be NULL); birth and metal (nstar_total star slots, may be NULL):
+ * the raw birth epoch, which the caller turns into an age with its cosmology
+ * (snapshot_age - epoch_to_age(birth), :748-749).  Ids outside the dense range or repeated
+ * within a file are errors, as in the reference. */
+int ahop_ramses_read(const char *repository, const char *nchar, int32_t ncpu, int32_t rver,
+                     int32_t family, int32_t stellar, int32_t nthreads, int64_t ndm_total,
+                     int64_t nselected_total, int64_t nstar_total, double scale_l,
+                     double scale_t, double *pos, double *vel, double *mass,
+                     int64_t *source_pid, double *birth, double *metal,
+                     ahop_ramses_counts *per_file);
+
 /* Host structure-tree stage alone (create_group_tree f90:885-983) on host inputs, for
  * callers that computed the hot-path stages elsewhere.  Outputs: liste_parts (npart),
  * *nnodes, and the tables, which must have room for 2*ngroups nodes (nnodes <= 2*ngroups,
