@@ -177,32 +177,52 @@ __global__ void k_gather_sorted(const int32_t *idx, const double *dr2, const dou
 }
 
 // r50 / r90 (:1662-1668): the cumulative mass in distance order summed left to right as
-// np.cumsum does, the first slot reaching half (90%) of the total as np.searchsorted
-__global__ void k_star_r50(int nsh, const int64_t *soff, const double *sd2, const double *sm,
-                           double *r50, double *r90)
+// np.cumsum does (one thread over chunks staged in LDS by the block), then the first slot
+// reaching half (90%) of the total by binary search, as np.searchsorted
+constexpr int CUM_CHUNK = 2048;
+
+__global__ __launch_bounds__(BS) void k_star_r50(const int64_t *soff, const double *sd2,
+                                                 const double *sm, double *cum, double *r50,
+                                                 double *r90)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nsh) return;
+    __shared__ double buf[CUM_CHUNK];
+    __shared__ double carry;
+    const int k = blockIdx.x;
     const int64_t f = soff[k], e = soff[k + 1];
-    double total = 0.0;
-    for (int64_t i = f; i < e; ++i) total += sm[i];
-    const double t50 = 0.5 * total, t90 = 0.9 * total;
-    int64_t k50 = e - 1, k90 = e - 1;
-    bool f50 = false;
-    double c = 0.0;
-    for (int64_t i = f; i < e; ++i) {
-        c += sm[i];
-        if (!f50 && c >= t50) {
-            k50 = i;
-            f50 = true;
+    if (threadIdx.x == 0) carry = 0.0;
+    for (int64_t base = f; base < e; base += CUM_CHUNK) {
+        const int len = (int)((e - base) < CUM_CHUNK ? (e - base) : CUM_CHUNK);
+        for (int j = threadIdx.x; j < len; j += BS) buf[j] = sm[base + j];
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double c = carry;
+#pragma unroll 8
+            for (int j = 0; j < len; ++j) {
+                c += buf[j];
+                buf[j] = c;
+            }
+            carry = c;
         }
-        if (c >= t90) {
-            k90 = i;
-            break;
-        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < len; j += BS) cum[base + j] = buf[j];
+        __syncthreads();
     }
-    r50[k] = sqrt(sd2[k50]);
-    r90[k] = sqrt(sd2[k90]);
+    if (threadIdx.x == 0) {
+        const double total = carry;
+        auto first_at_least = [&](double v) {
+            int64_t lo = f, hi = e;
+            while (lo < hi) {
+                const int64_t mid = lo + (hi - lo) / 2;
+                if (cum[mid] < v)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            return lo < e ? lo : e - 1;
+        };
+        r50[k] = sqrt(sd2[first_at_least(0.5 * total)]);
+        r90[k] = sqrt(sd2[first_at_least(0.9 * total)]);
+    }
 }
 
 // compute_stellar_1d's sums for one star halo per block
@@ -453,8 +473,10 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
         AHOP_HIP(rocprim::radix_sort_pairs(tmp, tb2, hids, hid, idxs, idx, (size_t)S, 0, hbits, s));
         hipLaunchKernelGGL(k_gather_sorted, dim3(gb), dim3(256), 0, s, idx, dr2, smass, (int64_t)S,
                            sd2, sms);
-        hipLaunchKernelGGL(k_star_r50, dim3((unsigned)((nsh + 63) / 64)), dim3(64), 0, s, nsh, d_sub,
-                           sd2, sms, r50, r90);
+        DevBuf bcum;
+        double *cum = (double *)bcum.ensure(sizeof(double) * S);
+        hipLaunchKernelGGL(k_star_r50, dim3((unsigned)nsh), dim3(BS), 0, s, d_sub, sd2, sms, cum,
+                           r50, r90);
         hipLaunchKernelGGL(k_stellar, dim3((unsigned)nsh), dim3(BS), 0, s, d_hl, d_sub, spos, svel,
                            smass, sage, smetal, sprops, r50, r90, P.lbox, out);
         AHOP_HIP(hipStreamSynchronize(s));
