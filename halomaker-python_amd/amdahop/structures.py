@@ -1,7 +1,8 @@
 """Structures from the drop-in's outputs: the MSM selection and the member grouping.
 
 The reference turns the node tables of ``neikdtree`` into halos and subhalos in Python,
-``select_with_MS_method_1319`` (src/compute_neiKDtree_mod.py:260-464), and then groups the
+``select_with_MS_method_1319`` (src/compute_neiKDtree_mod.py:260-464) or, for method HOP,
+``select_halos_1317`` (:228-252), and then groups the
 particles by structure (compute_adaptahop_131, :109-119).  Here the selection is restated on
 the node tables in NumPy (a few thousand nodes: host work, in the reference's order), and the
 per-particle part (relabelling every particle, then the argsort / np.unique grouping) runs on
@@ -161,6 +162,29 @@ def select_with_ms_method(node_0, strict=False):
                     first_sister[isub1 - 1] = istruct1
     return MSMStructures(node_to_struct, nb_halos, nb_sub, mother, first_sister,
                          first_daughter, level)
+
+
+def select_halos(node_0):
+    """HOP selection (select_halos_1317, compute_neiKDtree_mod.py:228-252): every node
+    belongs to its top-level ancestor (mother 0), found by walking mothers and reusing the
+    ancestors already found, as the reference does.  Returns (node_to_halo (nnodes,) int32,
+    nb_halos).  Top-level node ids must be 1..nb_halos, else ValueError as the reference's
+    relabelling raises (:246-249): the particle labels are these node ids."""
+    node_0 = node_0.view(NODE_DTYPE) if node_0.dtype != NODE_DTYPE else node_0
+    nnodes = len(node_0) - 1
+    mother = node_0["mother"]
+    node_to_halo = np.full(nnodes, -1, dtype=np.int32)
+    nb_halos = 0
+    for inode1 in range(1, nnodes + 1):
+        if mother[inode1] <= 0:
+            nb_halos += 1
+        ih1 = inode1
+        while mother[ih1] != 0:
+            ih1 = int(mother[ih1]) if node_to_halo[ih1 - 1] <= 0 else int(node_to_halo[ih1 - 1])
+        node_to_halo[inode1 - 1] = ih1
+    if np.any(node_to_halo > nb_halos) or np.any(node_to_halo <= 0):
+        raise ValueError("ihalo > nb_of_halos or nil")
+    return node_to_halo, nb_halos
 
 
 def group_members_device(node, labels, pids, offsets, node_to_struct=None, nnodes=None,

@@ -1,6 +1,6 @@
 """Particles grouped by structure on the GPU (ahop_group_members_device) against the
 reference's NumPy grouping (compute_neiKDtree_mod.py:109-119: argsort + np.unique), and the
-MSM selection + grouping chain on the drop-in's output."""
+HOP and MSM selection + grouping chains on the drop-in's output."""
 import numpy as np
 import pytest
 
@@ -83,7 +83,7 @@ def test_msm_and_grouping_on_the_dropin_output(gpu):
     structure's members number its final node mass (the reference's check,
     compute_neiKDtree_mod.py:381-392)."""
     from compute_adaptahop import neikdtree
-    from amdahop.structures import group_members, node_table, select_with_ms_method
+    from amdahop.structures import group_members, node_table, select_halos, select_with_ms_method
     ps = hosts_with_satellites()
     n = ps.n
     log(f"drop-in + MSM n={n}")
@@ -97,6 +97,11 @@ def test_msm_and_grouping_on_the_dropin_output(gpu):
     node_0 = node_table(neikdtree.real_table.copy(order="F"),
                         neikdtree.integer_table.copy(order="F"))
     neikdtree.close()
+    # method HOP: top-level ancestors; a halo's members number its top node's (inclusive) mass
+    n2h, nbh = select_halos(node_0.copy())
+    lab_h, _, unis_h, _, counts_h, _ = group_members(liste, node_to_struct=n2h)
+    assert np.array_equal(lab_h, np.where(liste > 0, n2h[np.maximum(liste, 1) - 1], 0))
+    assert np.array_equal(np.bincount(lab_h, minlength=nbh + 1)[1:], node_0["mass"][1:nbh + 1])
     r = select_with_ms_method(node_0, strict=True)
     log(f"{len(node_0) - 1} nodes -> {r.nb_halos} halos, {r.nb_sub} subhalos")
     assert r.nb_sub > 0                                   # the box has substructure

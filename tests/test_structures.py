@@ -156,3 +156,30 @@ def test_node_table_layout():
     assert len(n0) == 4 and n0["truemass"][2] == rt[4, 1] and n0["sister"][3] == it[5, 2]
     with pytest.raises(ValueError):
         node_table(rt[:7], it)
+
+
+def test_hop_selection_takes_the_top_level_ancestor():
+    from amdahop.structures import select_halos
+    # halos 1, 2; node 3 under 1, node 4 under 3, node 5 under 2
+    node_0 = make_nodes([0, 0, 1, 3, 2], [5, 5, 5, 5, 5])
+    n2h, nb = select_halos(node_0)
+    assert nb == 2 and list(n2h) == [1, 2, 1, 1, 2]
+    # a top-level node numbered after a subnode: its id exceeds the halo count
+    node_0 = make_nodes([0, 1, 0], [5, 5, 5])
+    with pytest.raises(ValueError, match="nb_of_halos"):
+        select_halos(node_0)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_hop_selection_on_random_forests(seed):
+    from amdahop.structures import select_halos
+    rng = np.random.default_rng(seed)
+    nn = int(rng.integers(5, 200))
+    mother, own, tm = random_forest(rng, nn)
+    n2h, nb = select_halos(make_nodes(mother, own))
+    for i in range(nn):                       # walk up independently
+        j = i + 1
+        while mother[j - 1] > 0:
+            j = mother[j - 1]
+        assert n2h[i] == j
+    assert nb == sum(1 for m in mother if m == 0)
