@@ -311,13 +311,19 @@ STELLAR_FIELDS = ["ndm", "nstar", "mdm", "m*", "mcontam", "sigma_dm", "sigma*",
                   "vrot_r90", "sig3d_r90", "sigcyl_r90"]
 
 
-def halo_stellar(offsets, pos, vel, mass, density, is_star, age, metal, props, *, lbox=1.0,
-                 gravconst=430.1, massp=0.0, sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0):
+CHEM_ELEMENTS = ("H", "O", "Fe", "Mg", "C", "N", "Si", "S", "D")
+
+
+def halo_stellar(offsets, pos, vel, mass, density, is_star, age, metal, props, *, chem=None,
+                 lbox=1.0, gravconst=430.1, massp=0.0, sc=False, dpmmc=False, dcell_min=0.0,
+                 eps_sc=0.0):
     """Stellar properties (mode 'full') of every halo of a CSR member layout: device tensors
     pos/vel (M, 3), mass/density/age/metal (M,) float64, is_star (M,) bool or uint8 (the
     reference's indexps >= ndm), props the rows of :func:`halo_props` (the mean velocity).
-    The SC / DPMMC options refine the stellar centre as the reference's do.  Returns
-    (tensor (H, 33) in the order of STELLAR_FIELDS, device ms)."""
+    The SC / DPMMC options refine the stellar centre as the reference's do.  ``chem`` (M, nchem)
+    float64, nchem <= 9: per-member abundances in CHEM_ELEMENTS order; their mass-weighted
+    means over each halo's stars are appended as columns "<element>_star".  Returns (tensor
+    (H, 33 + nchem) in the order of STELLAR_FIELDS then the chemistry, device ms)."""
     import numpy as np
     import torch
 
@@ -337,7 +343,14 @@ def halo_stellar(offsets, pos, vel, mass, density, is_star, age, metal, props, *
     if off.ndim != 1 or nh < 1 or off[0] != 0 or off[-1] != nm:
         raise ValueError("offsets must be a 1-D CSR array over all of mass, from 0")
     star = is_star.to(torch.uint8).contiguous()
+    nchem = 0
+    if chem is not None:
+        if (chem.dtype != torch.float64 or not chem.is_cuda or not chem.is_contiguous() or
+                chem.dim() != 2 or chem.shape[0] != nm or not 1 <= chem.shape[1] <= 9):
+            raise ValueError("chem must be a contiguous float64 device tensor (M, nchem <= 9)")
+        nchem = int(chem.shape[1])
     out = torch.empty((nh, len(STELLAR_FIELDS)), dtype=torch.float64, device=pos.device)
+    chem_out = torch.empty((nh, nchem), dtype=torch.float64, device=pos.device) if nchem else None
     p = HaloParams(float(lbox), 70.3, float(gravconst), 340.0, 1.0, 0, 1000, 1000, 0, 0,
                    1 if sc else 0, 1 if dpmmc else 0, float(dcell_min), float(eps_sc), 1)
     ms = ctypes.c_float(0.0)
@@ -346,6 +359,9 @@ def halo_stellar(offsets, pos, vel, mass, density, is_star, age, metal, props, *
     check(lib().ahop_halo_stellar_device(
         nh, ptr(off), vp(pos.data_ptr()), vp(vel.data_ptr()), vp(mass.data_ptr()),
         vp(density.data_ptr()), vp(star.data_ptr()), vp(age.data_ptr()), vp(metal.data_ptr()),
-        vp(props.data_ptr()), ctypes.byref(p), float(massp), vp(out.data_ptr()),
+        vp(chem.data_ptr()) if nchem else None, nchem, vp(props.data_ptr()), ctypes.byref(p),
+        float(massp), vp(out.data_ptr()), vp(chem_out.data_ptr()) if nchem else None,
         ctypes.byref(ms)))
+    if nchem:
+        out = torch.cat([out, chem_out], dim=1)
     return out, ms.value

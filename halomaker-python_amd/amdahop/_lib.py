@@ -77,8 +77,8 @@ SIGNATURES = {
                                          _i32, _i64, _i64, _i64, _dbl, _dbl, _vp, _vp, _vp, _vp,
                                          _vp, _vp, _vp]),
     "ahop_halo_stellar_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                                 ctypes.POINTER(HaloParams), _dbl, _vp,
-                                                 ctypes.POINTER(_f32)]),
+                                                 _i32, _vp, ctypes.POINTER(HaloParams), _dbl, _vp,
+                                                 _vp, ctypes.POINTER(_f32)]),
     "ahop_halo_profile_radii_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp,
                                                        _vp, _vp, _vp, ctypes.POINTER(_f32)]),
     "ahop_halo_profile_fits_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,

@@ -552,9 +552,9 @@ int ahop_ramses_read(const char *repository, const char *nchar, int32_t ncpu, in
 int ahop_halo_stellar_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
                              const double *d_vel, const double *d_mass, const double *d_density,
                              const uint8_t *d_is_star, const double *d_age,
-                             const double *d_metal, const double *d_props,
-                             const ahop_halo_params *params, double massp, double *d_out,
-                             float *ms)
+                             const double *d_metal, const double *d_chem, int32_t nchem,
+                             const double *d_props, const ahop_halo_params *params,
+                             double massp, double *d_out, double *d_chem_out, float *ms)
 {
     return guarded([&] {
         if (!offsets || !d_pos || !d_vel || !d_mass || !d_is_star || !d_age || !d_metal ||
@@ -571,7 +571,8 @@ int ahop_halo_stellar_device(int64_t nhalos, const int64_t *offsets, const doubl
         if (!(P.lbox > 0)) throw std::runtime_error("lbox must be > 0");
         Engine &e = engine();
         float t = halo_stellar_device(offsets, nhalos, d_pos, d_vel, d_mass, d_density, d_is_star,
-                                      d_age, d_metal, d_props, P, massp, d_out, e.stream());
+                                      d_age, d_metal, d_chem, nchem, d_props, P, massp, d_out,
+                                      d_chem_out, e.stream());
         if (ms) *ms = t;
     });
 }

@@ -35,12 +35,14 @@ struct HaloTuning {
     float ms = 0;   // device time of the final run, all passes
 };
 
-// Stellar properties (halo_stellar.hip): is_star / age / metal per member, props the
-// nhalos x HP_NFIELDS rows of halo_props_device (mean velocity), out nhalos x 33.
+// Stellar properties (halo_stellar.hip): is_star / age / metal per member, chem (M x nchem,
+// may be null with nchem 0), props the nhalos x HP_NFIELDS rows of halo_props_device (mean
+// velocity), out nhalos x 33, chem_out nhalos x nchem.
 float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, const double *vel,
                           const double *mass, const double *dens, const uint8_t *is_star,
-                          const double *age, const double *metal, const double *props,
-                          const HaloParamsDev &P, double massp, double *out, hipStream_t s);
+                          const double *age, const double *metal, const double *chem,
+                          int nchem, const double *props, const HaloParamsDev &P, double massp,
+                          double *out, double *chem_out, hipStream_t s);
 
 // pos/vel (M,3), mass (M), dens (M, may be null when cdm) are device pointers in CSR order;
 // h_off (nhalos+1) host offsets; d_out device (nhalos x HP_NFIELDS).  tile/bs <= 0 autotune.

@@ -10,7 +10,8 @@
 //   compute_stellar_1d :1635-1751: r50 / r90 from the cumulative mass in distance order (a
 //     sequential sum per halo, as np.cumsum, after two stable radix sorts), mass-weighted age
 //     and metallicity, the star formation rates, and vrot / sig3d / sigcyl over all stars,
-//     inside r50 and inside r90 (_stellar_kinematics_1d :1604-1633) (k_stellar).
+//     inside r50 and inside r90 (_stellar_kinematics_1d :1604-1633) (k_stellar), and the
+//     mass-weighted chemistry per element (k_stellar_chem).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -109,9 +110,10 @@ __global__ __launch_bounds__(BS) void k_star_gather(const int64_t *off, const in
                                                     const uint8_t *is_star, const double *pos,
                                                     const double *vel, const double *mass,
                                                     const double *dens, const double *age,
-                                                    const double *metal, double *spos,
-                                                    double *svel, double *smass, double *sdens,
-                                                    double *sage, double *smetal)
+                                                    const double *metal, const double *chem,
+                                                    int nchem, double *spos, double *svel,
+                                                    double *smass, double *sdens, double *sage,
+                                                    double *smetal, double *schem)
 {
     __shared__ int wcnt[NW];
     const int h = blockIdx.x;
@@ -139,6 +141,7 @@ __global__ __launch_bounds__(BS) void k_star_gather(const int64_t *off, const in
             sdens[j] = dens[i];
             sage[j] = age[i];
             smetal[j] = metal[i];
+            for (int c = 0; c < nchem; ++c) schem[j * nchem + c] = chem[i * nchem + c];
         }
         dst += tot;
     }
@@ -223,6 +226,34 @@ __global__ __launch_bounds__(BS) void k_star_r50(const int64_t *soff, const doub
         r50[k] = sqrt(sd2[first_at_least(0.5 * total)]);
         r90[k] = sqrt(sd2[first_at_least(0.9 * total)]);
     }
+}
+
+// stellar chemistry (:1671-1679): per element the mass-weighted mean over the stars with a
+// finite positive mass and a finite value (NaN when there are none)
+constexpr int MAX_CHEM = 9;
+
+__global__ __launch_bounds__(BS) void k_stellar_chem(const int32_t *hlist, const int64_t *soff,
+                                                     const double *smass, const double *schem,
+                                                     int nchem, double *chem_out)
+{
+    __shared__ double sh[2 * MAX_CHEM * NW];
+    const int k = blockIdx.x;
+    double v[2 * MAX_CHEM] = {0};
+    for (int64_t i = soff[k] + threadIdx.x; i < soff[k + 1]; i += BS) {
+        const double m = smass[i];
+        if (!(isfinite(m) && m > 0)) continue;
+        for (int c = 0; c < nchem; ++c) {
+            const double x = schem[i * nchem + c];
+            if (isfinite(x)) {
+                v[2 * c] += x * m;
+                v[2 * c + 1] += m;
+            }
+        }
+    }
+    block_sums<2 * MAX_CHEM>(v, sh);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < nchem; ++c)
+            chem_out[(int64_t)hlist[k] * nchem + c] = v[2 * c + 1] > 0 ? v[2 * c] / v[2 * c + 1] : NAN;
 }
 
 // compute_stellar_1d's sums for one star halo per block
@@ -390,9 +421,12 @@ __global__ __launch_bounds__(BS) void k_stellar(const int32_t *hlist, const int6
 
 float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, const double *vel,
                           const double *mass, const double *dens, const uint8_t *is_star,
-                          const double *age, const double *metal, const double *props,
-                          const HaloParamsDev &P, double massp, double *out, hipStream_t s)
+                          const double *age, const double *metal, const double *chem,
+                          int nchem, const double *props, const HaloParamsDev &P, double massp,
+                          double *out, double *chem_out, hipStream_t s)
 {
+    if (nchem < 0 || nchem > MAX_CHEM || (nchem > 0 && (!chem || !chem_out)))
+        throw std::runtime_error("nchem must be in [0, 9] with chemistry arrays");
     if (nh < 1 || nh > INT32_MAX / 2) throw std::runtime_error("need 1 <= nhalos < 2^30");
     if (h_off[0] != 0) throw std::runtime_error("offsets must start at 0");
     for (int64_t h = 0; h < nh; ++h)
@@ -407,6 +441,8 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
     int64_t *d_scount = (int64_t *)dsc.ensure(sizeof(int64_t) * nh);
     AHOP_HIP(hipMemcpyAsync(d_off, h_off, sizeof(int64_t) * (nh + 1), hipMemcpyHostToDevice, s));
     AHOP_HIP(hipEventRecord(e0, s));
+    if (nchem > 0)   // halos without stars keep NaN (all-ones bytes)
+        AHOP_HIP(hipMemsetAsync(chem_out, 0xFF, sizeof(double) * nchem * nh, s));
     hipLaunchKernelGGL(k_type_sums, dim3((unsigned)nh), dim3(BS), 0, s, d_off, vel, mass, is_star,
                        props, massp, d_scount, out);
     std::vector<int64_t> scount(nh);
@@ -426,7 +462,8 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
     float ms = 0;
     if (S > 0) {
         if (!dens) throw std::runtime_error("the stellar centre needs densities");
-        DevBuf bsoff, bsub, bhl, bpos, bvel, bm, bd, ba, bz, bprops, br50, br90;
+        DevBuf bsoff, bsub, bhl, bpos, bvel, bm, bd, ba, bz, bprops, br50, br90, bchem;
+        double *schem = nchem > 0 ? (double *)bchem.ensure(sizeof(double) * nchem * S) : nullptr;
         int64_t *d_soff = (int64_t *)bsoff.ensure(sizeof(int64_t) * (nh + 1));
         int64_t *d_sub = (int64_t *)bsub.ensure(sizeof(int64_t) * (nsh + 1));
         int32_t *d_hl = (int32_t *)bhl.ensure(sizeof(int32_t) * nsh);
@@ -443,7 +480,8 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
         AHOP_HIP(hipMemcpyAsync(d_sub, sub.data(), sizeof(int64_t) * (nsh + 1), hipMemcpyHostToDevice, s));
         AHOP_HIP(hipMemcpyAsync(d_hl, hlist.data(), sizeof(int32_t) * nsh, hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL(k_star_gather, dim3((unsigned)nh), dim3(BS), 0, s, d_off, d_soff, is_star,
-                           pos, vel, mass, dens, age, metal, spos, svel, smass, sdens, sage, smetal);
+                           pos, vel, mass, dens, age, metal, chem, nchem, spos, svel, smass, sdens,
+                           sage, smetal, schem);
         // stellar centre (densest star, then DPMMC / SC over the stars), m*, L*, r*
         HaloParamsDev Q = P;
         Q.cdm = 0;
@@ -479,6 +517,9 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
                            r50, r90);
         hipLaunchKernelGGL(k_stellar, dim3((unsigned)nsh), dim3(BS), 0, s, d_hl, d_sub, spos, svel,
                            smass, sage, smetal, sprops, r50, r90, P.lbox, out);
+        if (nchem > 0)
+            hipLaunchKernelGGL(k_stellar_chem, dim3((unsigned)nsh), dim3(BS), 0, s, d_hl, d_sub,
+                               smass, schem, nchem, chem_out);
         AHOP_HIP(hipStreamSynchronize(s));
     }
     // the span on the stream, host synchronisations for the star counts included

@@ -204,7 +204,9 @@ int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double 
 /* Stellar properties of every halo of a CSR member layout: det_mass_17 (:765-780),
  * det_center_18 (:963-966), the stellar parts of refine_centers_1b (:1569-1593),
  * compute_ang_mom_19 (:806-822) and r_halos_1a (:842-849), and compute_stellar_1d
- * (:1635-1751, without the optional chemistry), src/compute_halo_props.py.  d_is_star (M,
+ * (:1635-1751), src/compute_halo_props.py.  d_chem (M x nchem, nchem <= 9, may be NULL with
+ * nchem 0): per-member abundances in CHEM_ELEMENTS order, whose mass-weighted means over each
+ * halo's stars go to d_chem_out (nhalos x nchem, NaN without stars or valid values).  d_is_star (M,
  * uint8: the reference's indexps >= ndm), d_age (Gyr) and d_metal (M, read for stars), d_density
  * (M, needed when any member is a star), d_props the ahop_halo_props_device rows (the mean
  * velocity); params: lbox and the SC / DPMMC options (cdm and light ignored); massp for
@@ -212,9 +214,9 @@ int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double 
 int ahop_halo_stellar_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
                              const double *d_vel, const double *d_mass, const double *d_density,
                              const uint8_t *d_is_star, const double *d_age,
-                             const double *d_metal, const double *d_props,
-                             const ahop_halo_params *params, double massp, double *d_out,
-                             float *ms);
+                             const double *d_metal, const double *d_chem, int32_t nchem,
+                             const double *d_props, const ahop_halo_params *params,
+                             double massp, double *d_out, double *d_chem_out, float *ms);
 
 /* ---- extended profiles (compute_extended_profiles_1e, src/compute_halo_props.py:1819-1926),
  * halos whose members are all dark matter --------------------------------------------- */

@@ -12,11 +12,11 @@ metallicities, and the halo's total mean velocity:
   r_halos_1a :842-849         r*
   compute_stellar_1d :1635-1751  r50, r90, age, metal, SFR*, and vrot / sig3d / sigcyl over
                               all stars, inside r50 and inside r90 (_stellar_kinematics_1d
-                              :1604-1633); the optional chemistry fields are not covered.
+                              :1604-1633), and the optional chemistry (:1671-1679).
 
 Pinning (tests/test_halo_stellar_oracle.py): equal-mass stars at known radii (r50/r90 by
 counting), weighted averages and SFR sums by hand, a cold rotating ring (vrot = v, sigcyl =
-0), a cloud with no rotation, the early returns and defaults.
+0), a cloud with no rotation, the early returns and defaults, chemistry means by hand.
 """
 from __future__ import annotations
 
@@ -58,12 +58,15 @@ def stellar_kinematics(mass, dvx, dvy, dvz, vphi, vrad, vzyl, select):
 
 
 def stellar_props(pos, vel, mass, density, is_star, age, metal, vbar, *, lbox=1.0,
-                  massp=0.0, sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0):
+                  massp=0.0, sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0, chem=None):
     """All STELLAR_FIELDS of one halo (dict).  ``age`` / ``metal`` are per member (read for
-    stars only); ``vbar`` the halo's mean velocity (det_center_18)."""
+    stars only); ``vbar`` the halo's mean velocity (det_center_18); ``chem`` (n, nchem)
+    per-member abundances, whose stellar means go to out["chem"] (:1671-1679)."""
     L = lbox
     out = dict.fromkeys(STELLAR_FIELDS, np.nan)
     out.update(_DEFAULTS)
+    if chem is not None:
+        out["chem"] = np.full(chem.shape[1], np.nan)
     star = np.asarray(is_star, dtype=bool)
     dm = ~star
     idm = mass[dm]                                                 # det_mass_17 :765-780
@@ -108,6 +111,14 @@ def stellar_props(pos, vel, mass, density, is_star, age, metal, vbar, *, lbox=1.
     out["r90"] = np.sqrt(dr2[order[np.searchsorted(cum, 0.9 * total)]])
     out["age"] = np.average(sage, weights=smass)
     out["metal"] = np.average(smetal, weights=smass)
+    if chem is not None:                                           # :1671-1679
+        schem = chem[star]
+        valid_w = np.isfinite(smass) & (smass > 0)
+        for e in range(chem.shape[1]):
+            values = schem[:, e]
+            valid = valid_w & np.isfinite(values)
+            if np.any(valid):
+                out["chem"][e] = np.average(values[valid], weights=smass[valid])
     in50 = dr2 < out["r50"] ** 2
     in90 = dr2 < out["r90"] ** 2
     y100 = sage < 0.1
@@ -147,11 +158,12 @@ def stellar_props(pos, vel, mass, density, is_star, age, metal, vbar, *, lbox=1.
     return out
 
 
-def all_halos(offsets, pos, vel, mass, density, is_star, age, metal, vbars, **kw):
+def all_halos(offsets, pos, vel, mass, density, is_star, age, metal, vbars, chem=None, **kw):
+    """One row of STELLAR_FIELDS per halo, followed by the chemistry columns when given."""
     rows = []
     for h in range(len(offsets) - 1):
         s, e = int(offsets[h]), int(offsets[h + 1])
         o = stellar_props(pos[s:e], vel[s:e], mass[s:e], density[s:e], is_star[s:e], age[s:e],
-                          metal[s:e], vbars[h], **kw)
-        rows.append([o[k] for k in STELLAR_FIELDS])
+                          metal[s:e], vbars[h], chem=None if chem is None else chem[s:e], **kw)
+        rows.append([o[k] for k in STELLAR_FIELDS] + ([] if chem is None else list(o["chem"])))
     return np.array(rows, dtype=np.float64)

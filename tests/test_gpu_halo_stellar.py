@@ -31,13 +31,14 @@ def with_stars(hs, seed, frac=0.2, starless=(3, 7)):
     return star, age, metal, dens
 
 
-def run(hs, gpu, star, age, metal, dens, **kw):
+def run(hs, gpu, star, age, metal, dens, chem=None, **kw):
     import torch
     from amdahop import halo_props, halo_stellar
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(gpu)
     props, _ = halo_props(hs.offsets, t(hs.pos), t(hs.vel), t(hs.mass), hub_pt=70.3)
     out, ms = halo_stellar(hs.offsets, t(hs.pos), t(hs.vel), t(hs.mass), t(dens), t(star),
-                           t(age), t(metal), props, massp=float(np.median(hs.mass)), **kw)
+                           t(age), t(metal), props, chem=None if chem is None else t(chem),
+                           massp=float(np.median(hs.mass)), **kw)
     return props.cpu().numpy(), out.cpu().numpy(), ms
 
 
@@ -103,3 +104,24 @@ def test_all_dark_matter(gpu):
     z = np.zeros(n)
     props, out, _ = run(hs, gpu, np.zeros(n, dtype=bool), z, z, np.ones(n))
     compare(hs, props, out, np.zeros(n, dtype=bool), z, z, np.ones(n))
+
+
+def test_stellar_chemistry(gpu):
+    """Mass-weighted abundances over each halo's stars, NaN values and massless stars skipped
+    (:1671-1679); within 1e-12 of the oracle, NaN where no star has a valid value."""
+    from oracle.halo_stellar import STELLAR_FIELDS, all_halos
+    hs = G.nfw_halos(40, 16, alpha=0.9, nmin=30, nmax=5000, mass_sigma_dex=0.3)
+    star, age, metal, dens = with_stars(hs, 3)
+    rng = np.random.default_rng(4)
+    chem = rng.uniform(0, 1, (len(hs.mass), 9))
+    chem[rng.random(chem.shape) < 0.05] = np.nan
+    chem[:, 8] = np.nan                                 # an element with no valid value
+    props, out, _ = run(hs, gpu, star, age, metal, dens, chem=chem)
+    ref = all_halos(hs.offsets, hs.pos, hs.vel, hs.mass, dens, star, age, metal, props[:, 4:7],
+                    chem=chem, massp=float(np.median(hs.mass)))
+    n = len(STELLAR_FIELDS)
+    assert out.shape == (hs.nhalos, n + 9)
+    assert np.array_equal(np.isnan(out[:, n:]), np.isnan(ref[:, n:]))
+    ok = ~np.isnan(ref[:, n:])
+    assert np.all(np.abs(out[:, n:][ok] - ref[:, n:][ok]) <= 1e-12 * np.abs(ref[:, n:][ok]))
+    assert np.isnan(out[:, -1]).all() and np.isnan(out[3, n:]).all()

@@ -108,3 +108,14 @@ def test_stellar_centre_follows_sc_on_the_stars():
     o2 = props(stars, np.zeros((300, 3)), np.ones(300), dens[:300], np.ones(300), **kw)
     assert (o["px*"], o["py*"], o["pz*"]) == (o2["px*"], o2["py*"], o2["pz*"])
     assert not np.array_equal([o["px*"], o["py*"], o["pz*"]], stars[int(np.argmax(dens[:300]))])
+
+
+def test_chemistry_means_skip_invalid_values():
+    pos = np.array([[0, 0, 0], [0.001, 0, 0], [0.002, 0, 0], [0.003, 0, 0]], float)
+    m = np.array([1.0, 3.0, 0.0, 2.0])                 # a massless star is skipped
+    chem = np.array([[0.7, 1e-3], [0.8, np.nan], [5.0, 5.0], [0.6, 4e-3]])
+    o = props(pos, np.zeros((4, 3)), m, [9, 1, 1, 1], np.ones(4), chem=chem)
+    assert o["chem"][0] == pytest.approx((0.7 + 2.4 + 1.2) / 6, rel=1e-15)
+    assert o["chem"][1] == pytest.approx((1e-3 + 8e-3) / 3, rel=1e-15)
+    o = props(pos, np.zeros((4, 3)), m, [9, 1, 1, 1], np.zeros(4), chem=chem)
+    assert np.isnan(o["chem"]).all()                   # no stars
