@@ -275,6 +275,7 @@ def halo_profiles(offsets, pos, mass, props, *, lbox=1.0, gravconst=430.1):
     dev = pos.device
     sdist = torch.empty(nm, dtype=torch.float64, device=dev)
     smass = torch.empty(nm, dtype=torch.float64, device=dev)
+    scum = torch.empty(nm, dtype=torch.float64, device=dev)
     aux = torch.empty((nh, 4), dtype=torch.float64, device=dev)
     prof = torch.empty((nh, len(PROFILE_FIELDS)), dtype=torch.float64, device=dev)
     torch.cuda.synchronize(dev)
@@ -283,7 +284,7 @@ def halo_profiles(offsets, pos, mass, props, *, lbox=1.0, gravconst=430.1):
     check(lib().ahop_halo_profile_radii_device(
         nh, ptr(off), vp(pos.data_ptr()), vp(mass.data_ptr()), vp(props.data_ptr()),
         float(lbox), float(gravconst), vp(sdist.data_ptr()), vp(smass.data_ptr()),
-        vp(aux.data_ptr()), vp(prof.data_ptr()), ctypes.byref(ms1)))
+        vp(scum.data_ptr()), vp(aux.data_ptr()), vp(prof.data_ptr()), ctypes.byref(ms1)))
     a = aux.cpu().numpy()
     rvir = props[:, HALO_FIELDS.index("rvir")].cpu().numpy()
     nin = a[:, 1].astype(np.int64)
@@ -298,8 +299,8 @@ def halo_profiles(offsets, pos, mass, props, *, lbox=1.0, gravconst=430.1):
     d_edges = torch.from_numpy(edges).to(dev) if len(edges) else None
     torch.cuda.synchronize(dev)
     check(lib().ahop_halo_profile_fits_device(
-        nh, ptr(off), vp(sdist.data_ptr()), vp(smass.data_ptr()), vp(props.data_ptr()),
-        vp(aux.data_ptr()), ptr(eoff), vp(d_edges.data_ptr()) if d_edges is not None else None,
+        nh, ptr(off), vp(sdist.data_ptr()), vp(smass.data_ptr()), vp(scum.data_ptr()),
+        vp(props.data_ptr()), vp(aux.data_ptr()), ptr(eoff), vp(d_edges.data_ptr()) if d_edges is not None else None,
         vp(prof.data_ptr()), ctypes.byref(ms2)))
     return prof, ms1.value + ms2.value
 

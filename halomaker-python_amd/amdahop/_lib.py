@@ -80,9 +80,9 @@ SIGNATURES = {
                                                  _i32, _vp, ctypes.POINTER(HaloParams), _dbl, _vp,
                                                  _vp, ctypes.POINTER(_f32)]),
     "ahop_halo_profile_radii_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp,
-                                                       _vp, _vp, _vp, ctypes.POINTER(_f32)]),
+                                                       _vp, _vp, _vp, _vp, ctypes.POINTER(_f32)]),
     "ahop_halo_profile_fits_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
-                                                      _vp, ctypes.POINTER(_f32)]),
+                                                      _vp, _vp, ctypes.POINTER(_f32)]),
     "ahop_group_members_device": (ctypes.c_int, [_vp, _i64, _vp, _i32, _i32, _vp, _vp, _vp,
                                                  ctypes.POINTER(_f32)]),
     "ahop_halo_props_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp,

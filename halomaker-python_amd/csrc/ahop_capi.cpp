@@ -580,32 +580,35 @@ int ahop_halo_stellar_device(int64_t nhalos, const int64_t *offsets, const doubl
 int ahop_halo_profile_radii_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
                                    const double *d_mass, const double *d_props, double lbox,
                                    double gravconst, double *d_sdist, double *d_smass,
-                                   double *d_aux, double *d_prof, float *ms)
+                                   double *d_scum, double *d_aux, double *d_prof, float *ms)
 {
     return guarded([&] {
-        if (!offsets || !d_pos || !d_mass || !d_props || !d_sdist || !d_smass || !d_aux || !d_prof)
+        if (!offsets || !d_pos || !d_mass || !d_props || !d_sdist || !d_smass || !d_scum ||
+            !d_aux || !d_prof)
             throw std::runtime_error("null argument");
         if (!(lbox > 0)) throw std::runtime_error("lbox must be > 0");
         if (nhalos > 0 && offsets[nhalos] > (int64_t)UINT32_MAX)
             throw std::runtime_error("at most 2^32-1 members");
         Engine &e = engine();
         float t = halo_profile_radii(offsets, nhalos, d_pos, d_mass, d_props, lbox, gravconst,
-                                     d_sdist, d_smass, d_aux, d_prof, e.stream());
+                                     d_sdist, d_smass, d_scum, d_aux, d_prof, e.stream());
         if (ms) *ms = t;
     });
 }
 
 int ahop_halo_profile_fits_device(int64_t nhalos, const int64_t *offsets, const double *d_sdist,
-                                  const double *d_smass, const double *d_props,
-                                  const double *d_aux, const int64_t *edge_offsets,
-                                  const double *d_edges, double *d_prof, float *ms)
+                                  const double *d_smass, const double *d_scum,
+                                  const double *d_props, const double *d_aux,
+                                  const int64_t *edge_offsets, const double *d_edges,
+                                  double *d_prof, float *ms)
 {
     return guarded([&] {
-        if (!offsets || !d_sdist || !d_smass || !d_props || !d_aux || !edge_offsets || !d_prof)
+        if (!offsets || !d_sdist || !d_smass || !d_scum || !d_props || !d_aux || !edge_offsets ||
+            !d_prof)
             throw std::runtime_error("null argument");
         if (edge_offsets[nhalos] > 0 && !d_edges) throw std::runtime_error("null edges");
         Engine &e = engine();
-        float t = halo_profile_fits(offsets, nhalos, d_sdist, d_smass, d_props, d_aux,
+        float t = halo_profile_fits(offsets, nhalos, d_sdist, d_smass, d_scum, d_props, d_aux,
                                     edge_offsets, d_edges, d_prof, e.stream());
         if (ms) *ms = t;
     });

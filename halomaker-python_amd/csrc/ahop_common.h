@@ -124,16 +124,18 @@ float group_members(const int32_t *d_node, int64_t n, const int32_t *d_node_to_s
                     int64_t *d_offsets, hipStream_t s);
 
 // Extended profiles (halo_profiles.hip).  radii: distances from each halo's centre sorted per
-// halo (d_sdist, d_smass: M), per halo aux (first non-zero slot, members inside rvir, first
+// halo (d_sdist, d_smass: M; d_scum: M, the cumulative mass from each halo's first
+// non-zero distance), per halo aux (first non-zero slot, members inside rvir, first
 // and last such distance) and vmaxcir, rmaxcir in d_prof (nh x 6; the fit columns NaN).
 // fits: with the caller's bin edges (h_eoff: host nh+1 offsets into d_edges; 0 edges = no
 // fits) the NFW and inner-slope fits.  props: nh x 29 halo property rows.  Device ms.
 float halo_profile_radii(const int64_t *h_off, int64_t nh, const double *d_pos,
                          const double *d_mass, const double *d_props, double L, double G,
-                         double *d_sdist, double *d_smass, double *d_aux, double *d_prof,
-                         hipStream_t s);
+                         double *d_sdist, double *d_smass, double *d_scum, double *d_aux,
+                         double *d_prof, hipStream_t s);
 float halo_profile_fits(const int64_t *h_off, int64_t nh, const double *d_sdist,
-                        const double *d_smass, const double *d_props, const double *d_aux,
+                        const double *d_smass, const double *d_scum, const double *d_props,
+                        const double *d_aux,
                         const int64_t *h_eoff, const double *d_edges, double *d_prof,
                         hipStream_t s);
 

@@ -102,7 +102,8 @@ __global__ void k_prof_gather(const int32_t *idx, const double *dist, const doub
 template <int BS>
 __global__ __launch_bounds__(BS) void k_prof_vmax(const int64_t *off, const double *sd,
                                                   const double *sm, const double *props,
-                                                  double G, double *aux, double *prof)
+                                                  double G, double *cm, double *aux,
+                                                  double *prof)
 {
     constexpr int NW = BS / 64;
     __shared__ double wsum[NW];
@@ -132,6 +133,7 @@ __global__ __launch_bounds__(BS) void k_prof_vmax(const int64_t *off, const doub
         __syncthreads();
         carry = tot;
         if (i < e) {
+            cm[i] = before + m;                         // cumulative mass from slot z
             const double v = sqrt(G * (before + m) / sd[i]);
             if (v > best) {
                 best = v;
@@ -184,7 +186,8 @@ __device__ __forceinline__ void nfw_point(double r, double y, double w, double r
 
 // one wave per halo
 __global__ __launch_bounds__(64) void k_prof_fit(const int64_t *off, const double *sd,
-                                                 const double *sm, const double *props,
+                                                 const double *sm, const double *cm,
+                                                 const double *props,
                                                  const double *aux, const int64_t *eoff,
                                                  const double *edges, double *prof)
 {
@@ -203,14 +206,14 @@ __global__ __launch_bounds__(64) void k_prof_fit(const int64_t *off, const doubl
     const double rvir = props[(int64_t)h * PROPS_STRIDE + P_RVIR];
     double *o = prof + (int64_t)h * NPROF;
 
-    // np.histogram with explicit edges: counts below each edge (<= for the last edge)
+    // np.histogram with explicit edges: counts below each edge (<= for the last edge); the
+    // bin masses as differences of the cumulative mass, as np.histogram computes weighted bins
     for (int j = lane; j <= nb; j += 64) cum[j] = bound(sd, lo, hi, b[j], j == nb);
     __syncthreads();
-    for (int j = 0; j < nb; ++j) {
-        double s = 0.0;
-        for (int64_t i = cum[j] + lane; i < cum[j + 1]; i += 64) s += sm[i];
-        s = wave_sum(s);
-        if (lane == 0) bmass[j] = s;
+    for (int j = lane; j < nb; j += 64) {
+        const double c1 = cum[j + 1] > lo ? cm[cum[j + 1] - 1] : 0.0;
+        const double c0 = cum[j] > lo ? cm[cum[j] - 1] : 0.0;
+        bmass[j] = c1 - c0;
     }
     __syncthreads();
     // valid bins, compacted in order
@@ -279,6 +282,13 @@ __global__ __launch_bounds__(64) void k_prof_fit(const int64_t *off, const doubl
                 const double d0 = -(m11 * g0 - A01 * g1) / det;
                 const double d1 = -(m00 * g1 - A01 * g0) / det;
                 const double rs2 = fmin(fmax(rs + d0, rs_min), rvir), a2 = a + d1;
+                // a step below rounding level: converged (evaluating it would only be rejected
+                // by rounding, raising lam step after step)
+                if (fabs(rs2 - rs) <= 1e-14 * rs && fabs(d1) <= 1e-14 * fmax(1.0, fabs(a))) {
+                    converged = true;
+                    accepted = true;
+                    break;
+                }
                 const double c2 = cost_at(rs2, a2);
                 if (c2 <= cost) {
                     converged = fabs(rs2 - rs) <= 1e-15 * rs && fabs(a2 - a) <= 1e-15 * fmax(1.0, fabs(a));
@@ -379,8 +389,8 @@ __global__ __launch_bounds__(64) void k_prof_fit(const int64_t *off, const doubl
 
 float halo_profile_radii(const int64_t *h_off, int64_t nh, const double *d_pos,
                          const double *d_mass, const double *d_props, double L, double G,
-                         double *d_sdist, double *d_smass, double *d_aux, double *d_prof,
-                         hipStream_t s)
+                         double *d_sdist, double *d_smass, double *d_scum, double *d_aux,
+                         double *d_prof, hipStream_t s)
 {
     if (nh < 1 || nh > INT32_MAX / 2) throw std::runtime_error("need 1 <= nhalos < 2^30");
     if (h_off[0] != 0) throw std::runtime_error("offsets must start at 0");
@@ -418,7 +428,7 @@ float halo_profile_radii(const int64_t *h_off, int64_t nh, const double *d_pos,
     hipLaunchKernelGGL(k_prof_gather, dim3(gb), dim3(256), 0, s, idx, dist, d_mass, (int64_t)M,
                        d_sdist, d_smass);
     hipLaunchKernelGGL(k_prof_vmax<256>, dim3((unsigned)nh), dim3(256), 0, s, d_off, d_sdist,
-                       d_smass, d_props, G, d_aux, d_prof);
+                       d_smass, d_props, G, d_scum, d_aux, d_prof);
     AHOP_HIP(hipEventRecord(e1, s));
     AHOP_HIP(hipStreamSynchronize(s));
     float ms = 0;
@@ -429,7 +439,8 @@ float halo_profile_radii(const int64_t *h_off, int64_t nh, const double *d_pos,
 }
 
 float halo_profile_fits(const int64_t *h_off, int64_t nh, const double *d_sdist,
-                        const double *d_smass, const double *d_props, const double *d_aux,
+                        const double *d_smass, const double *d_scum, const double *d_props,
+                        const double *d_aux,
                         const int64_t *h_eoff, const double *d_edges, double *d_prof,
                         hipStream_t s)
 {
@@ -449,7 +460,7 @@ float halo_profile_fits(const int64_t *h_off, int64_t nh, const double *d_sdist,
     AHOP_HIP(hipMemcpyAsync(d_eoff, h_eoff, sizeof(int64_t) * (nh + 1), hipMemcpyHostToDevice, s));
     AHOP_HIP(hipEventRecord(e0, s));
     hipLaunchKernelGGL(k_prof_fit, dim3((unsigned)nh), dim3(64), 0, s, d_off, d_sdist, d_smass,
-                       d_props, d_aux, d_eoff, d_edges, d_prof);
+                       d_scum, d_props, d_aux, d_eoff, d_edges, d_prof);
     AHOP_HIP(hipEventRecord(e1, s));
     AHOP_HIP(hipStreamSynchronize(s));
     float ms = 0;

@@ -226,14 +226,15 @@ int ahop_halo_stellar_device(int64_t nhalos, const int64_t *offsets, const doubl
 
 /* Step 1: member distances from each halo's centre (d_props: the nhalos x AHOP_HALO_NFIELDS
  * rows of ahop_halo_props_device, whose centre and rvir are used), sorted per halo into
- * d_sdist / d_smass (M each), the circular-velocity peak (:1827-1840) into d_prof columns 0-1
+ * d_sdist / d_smass (M each), their cumulative mass from each halo's first non-zero distance
+ * into d_scum (M), the circular-velocity peak (:1827-1840) into d_prof columns 0-1
  * (columns 2-5 set to NaN), and per halo into d_aux (4 doubles): slot of the first non-zero
  * distance within the halo, members with 0 < d <= rvir, and the first and last such distance.
  * offsets: host, nhalos+1. */
 int ahop_halo_profile_radii_device(int64_t nhalos, const int64_t *offsets, const double *d_pos,
                                    const double *d_mass, const double *d_props, double lbox,
                                    double gravconst, double *d_sdist, double *d_smass,
-                                   double *d_aux, double *d_prof, float *ms);
+                                   double *d_scum, double *d_aux, double *d_prof, float *ms);
 
 /* Step 2: the NFW fit of the binned log-density (:1861-1891) and the inner slope
  * (:1893-1926) into d_prof columns 2-5, for the halos given bin edges: edge_offsets (host,
@@ -242,9 +243,10 @@ int ahop_halo_profile_radii_device(int64_t nhalos, const int64_t *offsets, const
  * has nbins+1 edges, nbins = max(8, min(80, int(sqrt(members inside rvir)))), computed by
  * the caller as np.logspace(log10(d_first), log10(d_last), nbins+1). */
 int ahop_halo_profile_fits_device(int64_t nhalos, const int64_t *offsets, const double *d_sdist,
-                                  const double *d_smass, const double *d_props,
-                                  const double *d_aux, const int64_t *edge_offsets,
-                                  const double *d_edges, double *d_prof, float *ms);
+                                  const double *d_smass, const double *d_scum,
+                                  const double *d_props, const double *d_aux,
+                                  const int64_t *edge_offsets, const double *d_edges,
+                                  double *d_prof, float *ms);
 
 /* ---- RAMSES particle files (host, one OpenMP thread per file) ------------------------ */
 
