@@ -122,8 +122,7 @@ struct DeviceArray {
 void free_outputs(ModuleState &s)
 {
     s.allocated = false;
-    s.liste_parts.clear();
-    s.liste_parts.shrink_to_fit();
+    // liste_parts keeps its buffer for the next call (refilled there; unreadable meanwhile)
     s.density.clear();
     s.real_table.clear();
     s.integer_table.clear();
@@ -305,6 +304,7 @@ int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_c
         tp.zlongs2 = s.zlongs2;
         tp.nthreads = s.nbpes;   // the reference runs this stage on nbPes OpenMP threads
         TreeOutputs to;
+        to.liste_parts = std::move(s.liste_parts);   // recycle the previous call's buffer
         create_group_tree(ti, tp, to);
         auto t2 = std::chrono::steady_clock::now();
         if ((int64_t)to.liste_parts.size() == (int64_t)s.nusedpart) {

@@ -38,8 +38,10 @@ struct Builder {
     const TreeParams &p;
     int64_t npart;
     int32_t ngroups;
-    // 1-based arrays mirroring the reference module state
-    std::vector<int32_t> firstpart, idpart_adapt, liste_parts;
+    // 1-based arrays mirroring the reference module state; the two per-particle ones are
+    // process-wide workspaces reused from call to call (first-touch page faults once)
+    std::vector<int32_t> firstpart;
+    std::vector<int32_t> &idpart_adapt, &liste_parts;
     std::vector<int32_t> nhnei, offset1;   // dev_group_nhnei (truncated while colouring), offsets
     std::vector<int32_t> idgroup, igroupid, idgroup_tmp, color;
     std::vector<Node> node;
@@ -52,6 +54,14 @@ struct Builder {
     bool failed = false;
     std::string error;
 
+    static void zero_fill(std::vector<int32_t> &v, int64_t n, int nt)
+    {
+        if ((int64_t)v.size() != n) v.resize(n);
+        int32_t *d = v.data();
+#pragma omp parallel for num_threads(nt) schedule(static, 65536)
+        for (int64_t i = 0; i < n; ++i) d[i] = 0;
+    }
+
     void record_error(const char *what)
     {
 #pragma omp critical(ahop_tree_error)
@@ -63,7 +73,9 @@ struct Builder {
         }
     }
 
-    Builder(const TreeInputs &i, const TreeParams &pp) : in(i), p(pp)
+    Builder(const TreeInputs &i, const TreeParams &pp, std::vector<int32_t> &ws_idpart,
+            std::vector<int32_t> &ws_liste)
+        : in(i), p(pp), idpart_adapt(ws_idpart), liste_parts(ws_liste)
     {
         npart = in.npart;
         ngroups = in.ngroups;
@@ -82,7 +94,7 @@ struct Builder {
     void link_members(int nt)
     {
         firstpart.assign(ngroups + 1, 0);
-        idpart_adapt.assign(npart + 1, 0);
+        zero_fill(idpart_adapt, npart + 1, nt);
         const int T = (int)std::max<int64_t>(1, std::min<int64_t>(nt, npart / 65536));
         std::vector<int32_t> head((size_t)T * (ngroups + 1), 0), tail((size_t)T * (ngroups + 1), 0);
 #pragma omp parallel for num_threads(T) schedule(static, 1)
@@ -493,7 +505,7 @@ struct Builder {
         igroupid.assign(ngroups + 1, 0);
         idgroup_tmp.assign(ngroups + 1, 0);
         color.assign(ngroups + 1, 0);
-        liste_parts.assign(npart + 1, 0);
+        zero_fill(liste_parts, npart + 1, nt);
         double rhot = p.rho_threshold;
         nnodes = 0;
         node[0].mother = 0;
@@ -528,7 +540,11 @@ struct Builder {
         }
         reorder();
         out.nnodes = nnodes;
-        out.liste_parts.assign(liste_parts.begin() + 1, liste_parts.end());
+        out.liste_parts.resize(npart);            // the caller's recycled buffer: no new pages
+        int32_t *dst = out.liste_parts.data();
+        const int32_t *src = liste_parts.data() + 1;
+#pragma omp parallel for num_threads(nt) schedule(static, 65536)
+        for (int64_t i = 0; i < npart; ++i) dst[i] = src[i];
         out.real_table.assign(8 * (size_t)nnodes, 0.0);
         out.integer_table.assign(6 * (size_t)nnodes, 0);
         for (int32_t i = 1; i <= nnodes; ++i) {   // node2table f90:550-573, column-major (8|6, nnodes)
@@ -557,7 +573,9 @@ struct Builder {
 
 void create_group_tree(const TreeInputs &in, const TreeParams &p, TreeOutputs &out)
 {
-    Builder b(in, p);
+    // per-particle workspaces kept for the next call (the module is not re-entrant)
+    static std::vector<int32_t> ws_idpart, ws_liste;
+    Builder b(in, p, ws_idpart, ws_liste);
     b.run(out);
 }
 

@@ -29,7 +29,8 @@ struct TreeInputs {
 
 struct TreeOutputs {
     int32_t nnodes = 0;
-    std::vector<int32_t> liste_parts;     // node id of each particle (0 = none)
+    std::vector<int32_t> liste_parts;     // node id of each particle (0 = none); may come in
+                                          // holding a previous buffer, which is reused
     std::vector<double> real_table;       // (8, nnodes) column-major
     std::vector<int32_t> integer_table;   // (6, nnodes) column-major
 };
