@@ -325,10 +325,13 @@ int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_c
                          "    [amdahop] sort %.3f ms, tree %.3f ms, kNN+density %.3f ms "
                          "(FP64 re-run of %lld queries %.3f ms), hop+groups %.3f ms, saddle %.3f ms\n"
                          "    [amdahop] active %lld, groups %d, saddle entries %lld, nodes %d; "
-                         "create nodes (host) %.3f ms, call total %.3f ms\n",
+                         "create nodes (host) %.3f ms, call total %.3f ms\n"
+                         "    [amdahop] create nodes: link %.3f, colour %.3f, nodes %.3f, "
+                         "reorder %.3f, tables %.3f ms\n",
                          t.sort, t.tree, t.knn, (long long)r.nfallback, t.fallback, t.hop,
                          t.saddle, (long long)r.nactive, r.ngroups, (long long)r.nsaddle, s.nnodes,
-                         host_ms, all_ms);
+                         host_ms, all_ms, to.ms_link, to.ms_colour, to.ms_nodes, to.ms_reorder,
+                         to.ms_output);
         }
     });
 }

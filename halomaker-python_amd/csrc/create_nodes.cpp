@@ -492,6 +492,8 @@ struct Builder {
     {
         const int nt = p.nthreads > 0 ? p.nthreads : omp_get_max_threads();
         nthreads_ = nt;
+        auto now = [] { return omp_get_wtime() * 1e3; };
+        double t0 = now();
         link_members(nt);
         nhnei.assign(ngroups + 1, 0);
         offset1.assign(ngroups + 2, 1);
@@ -509,8 +511,10 @@ struct Builder {
         double rhot = p.rho_threshold;
         nnodes = 0;
         node[0].mother = 0;
+        double t1 = now();
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 64)
         for (int32_t g = 1; g <= ngroups; ++g) treat_linkonly(g, rhot);
+        double t2 = now();
         node[0].mass = 0;
         node[0].truemass = 0.0;
         node[0].radius = 0.0;
@@ -538,7 +542,9 @@ struct Builder {
             }   // the end of the parallel region waits for every task
             if (failed) throw std::runtime_error(error);
         }
+        double t3 = now();
         reorder();
+        double t4 = now();
         out.nnodes = nnodes;
         out.liste_parts.resize(npart);            // the caller's recycled buffer: no new pages
         int32_t *dst = out.liste_parts.data();
@@ -566,6 +572,12 @@ struct Builder {
             t[4] = nd.nsisters;
             t[5] = nd.sister;
         }
+        double t5 = now();
+        out.ms_link = t1 - t0;
+        out.ms_colour = t2 - t1;
+        out.ms_nodes = t3 - t2;
+        out.ms_reorder = t4 - t3;
+        out.ms_output = t5 - t4;
     }
 };
 

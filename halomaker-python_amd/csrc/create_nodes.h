@@ -33,6 +33,7 @@ struct TreeOutputs {
                                           // holding a previous buffer, which is reused
     std::vector<double> real_table;       // (8, nnodes) column-major
     std::vector<int32_t> integer_table;   // (6, nnodes) column-major
+    double ms_link = 0, ms_colour = 0, ms_nodes = 0, ms_reorder = 0, ms_output = 0;   // stages
 };
 
 void create_group_tree(const TreeInputs &in, const TreeParams &p, TreeOutputs &out);
