@@ -203,3 +203,17 @@ def test_recentring_errors(gpu, halos):
     # the library is still usable after the errors
     out, _ = halo_props(halos.offsets, pos, vel, mass, hub_pt=70.3, **SC)
     assert torch.isfinite(out[:, 1]).all()
+
+
+def test_light_mode_skips_the_shell_stages(gpu, halos, halos_ref):
+    """light=True: the member passes only; their columns match the full run, and the columns
+    of the shell tables, energies and virial scan read NaN."""
+    from oracle.halo_props import F
+    out, _ = run(halos, gpu, hub_pt=70.3, light=True)
+    for k in ("ep", "et", "rvir", "mvir", "cvel", "tvir", "spin"):
+        assert np.isnan(out[:, F[k]]).all(), k
+    keep = [F[k] for k in ("m", "px", "py", "pz", "vx", "vy", "vz", "sigma", "Lx", "Ly", "Lz",
+                           "r", "Ixx", "Iyy", "Izz", "sha", "shb", "shc", "ek")]
+    full = halos_ref[:, keep]
+    rel = np.abs(out[:, keep] - full) / np.maximum(np.abs(full), 1e-300)
+    assert np.nanmax(rel) <= 1e-9

@@ -118,3 +118,20 @@ def test_msm_and_grouping_on_the_dropin_output(gpu):
         starters.setdefault(s, inode0 + 1)
     for s, inode1 in starters.items():
         assert node_0["mass"][inode1] == per_struct[s], s
+
+
+def test_device_grouping_checks_its_arguments(gpu):
+    import torch
+    from amdahop.structures import group_members_device
+    node = torch.zeros(10, dtype=torch.int32, device=gpu)
+    lab = torch.empty(10, dtype=torch.int32, device=gpu)
+    pids = torch.empty(10, dtype=torch.int32, device=gpu)
+    with pytest.raises(ValueError, match="offsets"):
+        group_members_device(node, lab, pids, torch.empty(3, dtype=torch.int64, device=gpu),
+                             nnodes=5)
+    with pytest.raises(ValueError, match="labels"):
+        group_members_device(node, lab[:5], pids, torch.empty(7, dtype=torch.int64, device=gpu),
+                             nnodes=5)
+    off = torch.empty(7, dtype=torch.int64, device=gpu)
+    group_members_device(node, lab, pids, off, nnodes=5)
+    assert off.cpu().numpy().tolist() == [0] + [10] * 6       # everything in label 0
