@@ -33,15 +33,24 @@ struct Node {
     double position[3] = {0, 0, 0};
 };
 
+// one particle of a group list: its index, density, mass and position, so that walking a list
+// reads memory in order
+struct PartRec {
+    double dens, mass, x[3];
+    int32_t ipar, pad;
+};
+
 struct Builder {
     const TreeInputs &in;
     const TreeParams &p;
     int64_t npart;
     int32_t ngroups;
-    // 1-based arrays mirroring the reference module state; the two per-particle ones are
-    // process-wide workspaces reused from call to call (first-touch page faults once)
-    std::vector<int32_t> firstpart;
-    std::vector<int32_t> &idpart_adapt, &liste_parts;
+    // each group's particle list (the reference's firstpart / idpart_adapt chains) held as
+    // a contiguous segment of packed records in list order, [seg_b[g], seg_e[g]); the
+    // record array and liste_parts are process-wide workspaces reused from call to call
+    std::vector<PartRec> &rec;
+    std::vector<int64_t> seg_b, seg_e;
+    std::vector<int32_t> &liste_parts;
     std::vector<int32_t> nhnei, offset1;   // dev_group_nhnei (truncated while colouring), offsets
     std::vector<int32_t> idgroup, igroupid, idgroup_tmp, color;
     std::vector<Node> node;
@@ -73,9 +82,9 @@ struct Builder {
         }
     }
 
-    Builder(const TreeInputs &i, const TreeParams &pp, std::vector<int32_t> &ws_idpart,
+    Builder(const TreeInputs &i, const TreeParams &pp, std::vector<PartRec> &ws_rec,
             std::vector<int32_t> &ws_liste)
-        : in(i), p(pp), idpart_adapt(ws_idpart), liste_parts(ws_liste)
+        : in(i), p(pp), rec(ws_rec), liste_parts(ws_liste)
     {
         npart = in.npart;
         ngroups = in.ngroups;
@@ -87,62 +96,62 @@ struct Builder {
     int32_t isad(int32_t idx) const { return in.sad_isad[idx - 1]; }         // 1-based idx
     double rho_sad(int32_t idx) const { return in.sad_rho[idx - 1]; }
 
-    // member linked lists: find_local_maxima f90:847-861 (prepend in index order)
-    // Each list is the group's members in decreasing index.  Built in parallel: every thread
-    // prepends the members of its contiguous index range to its own per-group lists, then
-    // each group's lists are chained from the highest range to the lowest.
+    // link_members: the particles above the threshold of each group, in the reference's list
+    // order (prepended in index order, so descending index, f90:853-861), packed into that
+    // group's segment.  Threads count their index ranges per group, then each particle goes
+    // to its slot from the end of its group's segment.
     void link_members(int nt)
     {
-        firstpart.assign(ngroups + 1, 0);
-        zero_fill(idpart_adapt, npart + 1, nt);
         const int T = (int)std::max<int64_t>(1, std::min<int64_t>(nt, npart / 65536));
-        std::vector<int32_t> head((size_t)T * (ngroups + 1), 0), tail((size_t)T * (ngroups + 1), 0);
+        std::vector<int64_t> cnt((size_t)T * (ngroups + 1), 0);
+        auto active = [&](int64_t ipar) {
+            return in.density[ipar - 1] > p.rho_threshold && in.gid[ipar - 1] > 0;
+        };
 #pragma omp parallel for num_threads(T) schedule(static, 1)
         for (int t = 0; t < T; ++t) {
             const int64_t lo = 1 + npart * t / T, hi = npart * (t + 1) / T;
-            int32_t *h = &head[(size_t)t * (ngroups + 1)], *tl = &tail[(size_t)t * (ngroups + 1)];
-            for (int64_t ipar = lo; ipar <= hi; ++ipar) {
-                if (in.density[ipar - 1] <= p.rho_threshold) continue;
-                int32_t g = in.gid[ipar - 1];
-                if (g > 0) {
-                    if (h[g] == 0) tl[g] = (int32_t)ipar;
-                    idpart_adapt[ipar] = h[g];
-                    h[g] = (int32_t)ipar;
-                }
-            }
+            int64_t *c = &cnt[(size_t)t * (ngroups + 1)];
+            for (int64_t ipar = lo; ipar <= hi; ++ipar)
+                if (active(ipar)) ++c[in.gid[ipar - 1]];
         }
-#pragma omp parallel for num_threads(nt) schedule(static, 1024)
+        seg_b.assign(ngroups + 2, 0);
+        seg_e.assign(ngroups + 2, 0);
         for (int32_t g = 1; g <= ngroups; ++g) {
-            int32_t first = 0, last = 0;
-            for (int t = T - 1; t >= 0; --t) {
-                const int32_t h = head[(size_t)t * (ngroups + 1) + g];
-                if (h == 0) continue;
-                if (first == 0) first = h;
-                else idpart_adapt[last] = h;
-                last = tail[(size_t)t * (ngroups + 1) + g];
+            int64_t sum = 0;
+            for (int t = 0; t < T; ++t) {
+                int64_t &c = cnt[(size_t)t * (ngroups + 1) + g];
+                const int64_t v = c;
+                c = sum;                                    // ascending rank base of thread t
+                sum += v;
             }
-            firstpart[g] = first;
+            seg_b[g] = g > 1 ? seg_e[g - 1] : 0;
+            seg_e[g] = seg_b[g] + sum;
+        }
+        const int64_t nact = ngroups > 0 ? seg_e[ngroups] : 0;
+        if ((int64_t)rec.size() < nact) rec.resize(nact);
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+        for (int t = 0; t < T; ++t) {
+            const int64_t lo = 1 + npart * t / T, hi = npart * (t + 1) / T;
+            int64_t *rank = &cnt[(size_t)t * (ngroups + 1)];
+            for (int64_t ipar = lo; ipar <= hi; ++ipar) {
+                if (!active(ipar)) continue;
+                const int32_t g = in.gid[ipar - 1];
+                PartRec &r = rec[seg_e[g] - 1 - rank[g]++];
+                r.dens = in.density[ipar - 1];
+                r.mass = in.mass[ipar - 1];
+                for (int d = 0; d < 3; ++d) r.x[d] = in.pos[3 * (ipar - 1) + d];
+                r.ipar = (int32_t)ipar;
+            }
         }
     }
 
-    // treat_particles_linkonly f90:2406-2429
+    // treat_particles_linkonly f90:2406-2429: drop the particles at or below rhot, in order
     void treat_linkonly(int32_t igroup, double rhot)
     {
-        int32_t ipar = firstpart[igroup], iparold = 0;
-        bool first_good = false;
-        while (ipar > 0) {
-            if (dens(ipar) > rhot) {
-                if (!first_good) {
-                    first_good = true;
-                    firstpart[igroup] = ipar;
-                } else {
-                    idpart_adapt[iparold] = ipar;
-                }
-                iparold = ipar;
-            }
-            ipar = idpart_adapt[ipar];
-        }
-        if (!first_good) firstpart[igroup] = 0;
+        int64_t w = seg_b[igroup];
+        for (int64_t r = seg_b[igroup]; r < seg_e[igroup]; ++r)
+            if (rec[r].dens > rhot) rec[w++] = rec[r];
+        seg_e[igroup] = w;
     }
 
     // treat_particles f90:2317-2401
@@ -154,33 +163,31 @@ struct Builder {
         rsquare = 0.0;
         densmoy = 0.0;
         posg[0] = posg[1] = posg[2] = 0.0;
-        int32_t ipar = firstpart[igroup], iparold = 0;
         bool first_good = false;
         double densmax = 0, densmin = 0;
         const double L[3] = {p.xlong, p.ylong, p.zlong};
         const double Ls2[3] = {p.xlongs2, p.ylongs2, p.zlongs2};
-        while (ipar > 0) {
-            double density_ipar = dens(ipar);
+        int64_t w = seg_b[igroup];
+        for (int64_t ri = seg_b[igroup]; ri < seg_e[igroup]; ++ri) {
+            const PartRec pr = rec[ri];
+            double density_ipar = pr.dens;
             if (density_ipar > rhot) {
                 if (!first_good) {
                     if (igroupref == 0) {
-                        for (int d = 0; d < 3; ++d) posref[d] = pos(ipar, d);
+                        for (int d = 0; d < 3; ++d) posref[d] = pr.x[d];
                         igroupref = igroup;
                     }
                     first_good = true;
-                    firstpart[igroup] = ipar;
                     densmin = density_ipar;
                     densmax = densmin;
-                } else {
-                    idpart_adapt[iparold] = ipar;
                 }
-                iparold = ipar;
+                rec[w++] = pr;                              // the list keeps it, in order
                 imass = imass + 1;
-                double xmasspart = in.mass[ipar - 1];
+                double xmasspart = pr.mass;
                 truemass = truemass + xmasspart;
                 double pd[3];
                 for (int d = 0; d < 3; ++d) {
-                    pd[d] = pos(ipar, d) - posref[d];
+                    pd[d] = pr.x[d] - posref[d];
                     if (pd[d] >= Ls2[d]) pd[d] = pd[d] - L[d];
                     else if (pd[d] < -Ls2[d]) pd[d] = pd[d] + L[d];
                 }
@@ -193,10 +200,8 @@ struct Builder {
                 densmax = std::max(densmax, density_ipar);
                 densmin = std::min(densmin, density_ipar);
             }
-            ipar = idpart_adapt[ipar];
         }
-        if (first_good) idpart_adapt[iparold] = 0;
-        else firstpart[igroup] = 0;
+        seg_e[igroup] = w;
         if (!first_good || densmin <= rhot || densmax != densityg(igroup))
             throw std::runtime_error("treat_particles: group " + std::to_string(igroup) +
                                      " is inconsistent with its density maximum");
@@ -318,11 +323,8 @@ struct Builder {
         int32_t inode = inode0, igr1 = igr1_0, igr2 = igr2_0;
         if (do_paint) {
             for (int32_t igr = igr1; igr <= igr2; ++igr) {
-                int32_t ipar = firstpart[idgroup[igr]];
-                while (ipar > 0) {
-                    liste_parts[ipar] = inode;
-                    ipar = idpart_adapt[ipar];
-                }
+                const int32_t g = idgroup[igr];
+                for (int64_t r = seg_b[g]; r < seg_e[g]; ++r) liste_parts[rec[r].ipar] = inode;
             }
         }
         const double L[3] = {p.xlong, p.ylong, p.zlong};
@@ -586,8 +588,9 @@ struct Builder {
 void create_group_tree(const TreeInputs &in, const TreeParams &p, TreeOutputs &out)
 {
     // per-particle workspaces kept for the next call (the module is not re-entrant)
-    static std::vector<int32_t> ws_idpart, ws_liste;
-    Builder b(in, p, ws_idpart, ws_liste);
+    static std::vector<PartRec> ws_rec;
+    static std::vector<int32_t> ws_liste;
+    Builder b(in, p, ws_rec, ws_liste);
     b.run(out);
 }
 
