@@ -77,7 +77,7 @@ float group_members(const int32_t *d_node, int64_t n, const int32_t *d_node_to_s
     if (nnodes < 0 || nlab < 0) throw std::runtime_error("nnodes and nstruct must be >= 0");
     if (!d_node_to_struct && nlab != nnodes)
         throw std::runtime_error("without a node-to-structure map the labels are the node ids");
-    DevBuf keys, keys_sorted, iota, temp, badbuf;
+    static DevBuf keys, keys_sorted, iota, temp, badbuf;
     uint32_t *k0 = (uint32_t *)keys.ensure(sizeof(uint32_t) * n);
     uint32_t *k1 = (uint32_t *)keys_sorted.ensure(sizeof(uint32_t) * n);
     int32_t *io = (int32_t *)iota.ensure(sizeof(int32_t) * n);

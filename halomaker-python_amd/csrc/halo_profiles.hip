@@ -400,7 +400,7 @@ float halo_profile_radii(const int64_t *h_off, int64_t nh, const double *d_pos,
     if (M > INT32_MAX) throw std::runtime_error("at most 2^31-1 members");
     // sorted per halo by two stable device-wide radix sorts: by distance, then by halo (the
     // second keeps the first's order within a halo), so no halo waits on one workgroup
-    DevBuf doff, ddist, dd2, didx, di2, dhid, dh2, temp;
+    static DevBuf doff, ddist, dd2, didx, di2, dhid, dh2, temp;
     int64_t *d_off = (int64_t *)doff.ensure(sizeof(int64_t) * (nh + 1));
     double *dist = (double *)ddist.ensure(sizeof(double) * M);
     double *dist_s = (double *)dd2.ensure(sizeof(double) * M);
@@ -450,7 +450,7 @@ float halo_profile_fits(const int64_t *h_off, int64_t nh, const double *d_sdist,
         if (ne != 0 && (ne < 9 || ne > MAXBINS + 1))
             throw std::runtime_error("a halo's bin edges must number 0 or 9 .. 81");
     }
-    DevBuf doff, deo;
+    static DevBuf doff, deo;
     int64_t *d_off = (int64_t *)doff.ensure(sizeof(int64_t) * (nh + 1));
     int64_t *d_eoff = (int64_t *)deo.ensure(sizeof(int64_t) * (nh + 1));
     hipEvent_t e0, e1;

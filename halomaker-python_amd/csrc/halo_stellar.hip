@@ -436,7 +436,7 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
     hipEvent_t e0, e1;
     AHOP_HIP(hipEventCreate(&e0));
     AHOP_HIP(hipEventCreate(&e1));
-    DevBuf doff, dsc, dsoff;
+    static DevBuf doff, dsc, dsoff;
     int64_t *d_off = (int64_t *)doff.ensure(sizeof(int64_t) * (nh + 1));
     int64_t *d_scount = (int64_t *)dsc.ensure(sizeof(int64_t) * nh);
     AHOP_HIP(hipMemcpyAsync(d_off, h_off, sizeof(int64_t) * (nh + 1), hipMemcpyHostToDevice, s));
@@ -462,7 +462,7 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
     float ms = 0;
     if (S > 0) {
         if (!dens) throw std::runtime_error("the stellar centre needs densities");
-        DevBuf bsoff, bsub, bhl, bpos, bvel, bm, bd, ba, bz, bprops, br50, br90, bchem;
+        static DevBuf bsoff, bsub, bhl, bpos, bvel, bm, bd, ba, bz, bprops, br50, br90, bchem;
         double *schem = nchem > 0 ? (double *)bchem.ensure(sizeof(double) * nchem * S) : nullptr;
         int64_t *d_soff = (int64_t *)bsoff.ensure(sizeof(int64_t) * (nh + 1));
         int64_t *d_sub = (int64_t *)bsub.ensure(sizeof(int64_t) * (nsh + 1));
@@ -488,7 +488,7 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
         Q.light = 1;
         (void)halo_props_device(sub.data(), nsh, spos, svel, smass, sdens, Q, sprops, 4096, 256, s);
         // stars of each halo in distance order: stable sorts by distance, then by star halo
-        DevBuf bdr, bdr2, bi, bi2, bh, bh2, bsd, bsm, temp;
+        static DevBuf bdr, bdr2, bi, bi2, bh, bh2, bsd, bsm, temp;
         double *dr2 = (double *)bdr.ensure(sizeof(double) * S);
         double *dr2s = (double *)bdr2.ensure(sizeof(double) * S);
         int32_t *idx = (int32_t *)bi.ensure(sizeof(int32_t) * S);
@@ -511,7 +511,7 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
         AHOP_HIP(rocprim::radix_sort_pairs(tmp, tb2, hids, hid, idxs, idx, (size_t)S, 0, hbits, s));
         hipLaunchKernelGGL(k_gather_sorted, dim3(gb), dim3(256), 0, s, idx, dr2, smass, (int64_t)S,
                            sd2, sms);
-        DevBuf bcum;
+        static DevBuf bcum;
         double *cum = (double *)bcum.ensure(sizeof(double) * S);
         hipLaunchKernelGGL(k_star_r50, dim3((unsigned)nsh), dim3(BS), 0, s, d_sub, sd2, sms, cum,
                            r50, r90);
