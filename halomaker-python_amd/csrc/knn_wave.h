@@ -356,7 +356,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         const float bd_ = box_dist2_lower((c), spos, nfbox, pxf, pyf, pzf, Lxf, Lyf, Lzf,     \
                                           mLx, mLy, mLz);                                      \
         if constexpr (MIXED) {                                                                 \
-            (out) = active && bd_ < tau + 2 * fp32_key_err(tau, hQ, Lmin);                     \
+            (out) = active && bd_ < tn;                                                        \
         } else {                                                                               \
             (out) = active && (double)bd_ < tau;                                               \
         }                                                                                      \
@@ -397,6 +397,9 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     stack[0] = 0;
     bool stack_overflow = false;
     const bool root_is_bucket = (A.n <= A.bucket);
+    // the node on top of the stack was pushed by the node just handled, and its visit test
+    // from then (top_need) still holds: tau changes only when a bucket is staged
+    bool top_fresh = false, top_need = false;
     while (sp > 0) {
         // the traversal is wave-uniform: say so, so that node data come through scalar loads
         const int node = __builtin_amdgcn_readfirstlane(stack[--sp]);
@@ -407,8 +410,12 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
             l = nlast[node];
         }
         ++st_nodes;
-        bool need;
-        KNN_NEED(node, need);
+        // the visit threshold on box distances, the same for this node and its children
+        float tn = 0.0f;
+        if constexpr (MIXED) tn = tau + 2 * fp32_key_err(tau, hQ, Lmin);
+        bool need = top_need;
+        if (!top_fresh) KNN_NEED(node, need);
+        top_fresh = false;
         if (!__any(need)) continue;
         if (l - f + 1 <= A.bucket || (root_is_bucket && node == 0)) {
             // a bucket inside the wave-wide seed window was seen by every lane already
@@ -424,28 +431,37 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         bool n1, n2;
         KNN_NEED(c1, n1);
         KNN_NEED(c2, n2);
+        top_fresh = true;
         const bool a1 = __any(n1), a2 = __any(n2);
         if (sp + 2 > STACK) {
             stack_overflow = true;   // reported after the loop (no global atomics inside it)
             break;
         }
         if (a1 && a2) {
-            // nearest first, in sorted-slot distance to the wave's queries
+            // first the child more lanes need; on a tie, the nearer in sorted-slot distance
+            // to the wave's queries
+            const int k1 = __popcll(__ballot(n1)), k2 = __popcll(__ballot(n2));
             const int f1_ = c1 < 0 ? ~c1 : nfirst[c1], l1_ = c1 < 0 ? ~c1 : nlast[c1];
             const int f2_ = c2 < 0 ? ~c2 : nfirst[c2], l2_ = c2 < 0 ? ~c2 : nlast[c2];
             const int d1 = max(0, max(f1_ - qmid, qmid - l1_));
             const int d2 = max(0, max(f2_ - qmid, qmid - l2_));
-            if (d1 <= d2) {
+            if (k1 > k2 || (k1 == k2 && d1 <= d2)) {
                 stack[sp++] = c2;
                 stack[sp++] = c1;
+                top_need = n1;
             } else {
                 stack[sp++] = c1;
                 stack[sp++] = c2;
+                top_need = n2;
             }
         } else if (a1) {
             stack[sp++] = c1;
+            top_need = n1;
         } else if (a2) {
             stack[sp++] = c2;
+            top_need = n2;
+        } else {
+            top_fresh = false;
         }
     }
     KNN_FLUSH(false);
