@@ -351,10 +351,12 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         ++nbuf;                                                                                \
     } while (0)
 
-#define KNN_NEED(c, out)                                                                       \
+    // visit test of node c for this lane (out), with the node's box distance (bdout)
+#define KNN_NEED(c, out, bdout)                                                                \
     do {                                                                                       \
         const float bd_ = box_dist2_lower((c), spos, nfbox, pxf, pyf, pzf, Lxf, Lyf, Lzf,     \
                                           mLx, mLy, mLz);                                      \
+        (bdout) = bd_;                                                                         \
         if constexpr (MIXED) {                                                                 \
             (out) = active && bd_ < tn;                                                        \
         } else {                                                                               \
@@ -369,6 +371,37 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         for (int f = s0; f <= s1; f += WAVE) {
             const int cnt = min(WAVE, s1 - f + 1);
             KNN_STAGE(f, cnt, true);
+            if constexpr (MIXED) {
+                // Two full buckets of the window are staged, and every lane with a
+                // search has accepted all of them but its own slot: their first KH slots go
+                // straight into its heap, its own slot (if among them) with key +inf, which
+                // the first insertion evicts, and the fill's picks of them are dropped.
+                if (nbuf == 2 && f == s0 + WAVE && cnt == WAVE && KH <= 2 * WAVE) {
+                    const int off = q - s0;
+                    const uint64_t self0 = off < WAVE ? 1ull << off : 0ull;
+                    const uint64_t self1 = (off >= WAVE && off < 2 * WAVE) ? 1ull << (off - WAVE) : 0ull;
+                    if (m0 == ~self0 && m1 == ~self1) {
+#pragma unroll
+                        for (int e = 0; e < RT; ++e) {
+                            KeyT d;
+                            KNN_KEY(make_float4(sx[e], sy[e], sz[e], 0.0f), d);
+                            if (e < KH) HSET(e, e == off ? (KeyT)INFINITY : d, s0 + e);
+                        }
+                        for (int e = RT; e < KH; ++e) {
+                            KeyT d;
+                            KNN_KEY(make_float4(sx[e], sy[e], sz[e], 0.0f), d);
+                            d = e == off ? (KeyT)INFINITY : d;
+                            hp[(e - RT) * WAVE + lane] = make_uint2(__float_as_uint(d), (uint32_t)(s0 + e));
+                        }
+                        filled = KH;
+                        st_ins += KH;
+                        KNN_HEAPIFY();
+                        tau = HK(0);
+                        m0 = KH >= WAVE ? 0ull : m0 & ~((1ull << KH) - 1ull);
+                        m1 = KH <= WAVE ? m1 : (KH >= 2 * WAVE ? 0ull : m1 & ~((1ull << (KH - WAVE)) - 1ull));
+                    }
+                }
+            }
         }
     } else {
         for (int j = seed_lo; j <= seed_hi; ++j) {
@@ -414,7 +447,8 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         float tn = 0.0f;
         if constexpr (MIXED) tn = tau + 2 * fp32_key_err(tau, hQ, Lmin);
         bool need = top_need;
-        if (!top_fresh) KNN_NEED(node, need);
+        float bdn;
+        if (!top_fresh) KNN_NEED(node, need, bdn);
         top_fresh = false;
         if (!__any(need)) continue;
         if (l - f + 1 <= A.bucket || (root_is_bucket && node == 0)) {
@@ -429,8 +463,9 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         const int c1 = __builtin_amdgcn_readfirstlane(nleft[node]);
         const int c2 = __builtin_amdgcn_readfirstlane(nright[node]);
         bool n1, n2;
-        KNN_NEED(c1, n1);
-        KNN_NEED(c2, n2);
+        float bd1, bd2;
+        KNN_NEED(c1, n1, bd1);
+        KNN_NEED(c2, n2, bd2);
         top_fresh = true;
         const bool a1 = __any(n1), a2 = __any(n2);
         if (sp + 2 > STACK) {
@@ -438,9 +473,15 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
             break;
         }
         if (a1 && a2) {
-            // first the child more lanes need; on a tie, the nearer in sorted-slot distance
-            // to the wave's queries
-            const int k1 = __popcll(__ballot(n1)), k2 = __popcll(__ballot(n2));
+            // first the child that more lanes need closely (box distance under half the
+            // threshold), then the one more lanes need at all; on a tie, the nearer in
+            // sorted-slot distance to the wave's queries
+            int k1 = __popcll(__ballot(n1 && bd1 < 0.5f * tn));
+            int k2 = __popcll(__ballot(n2 && bd2 < 0.5f * tn));
+            if (k1 == k2) {
+                k1 = __popcll(__ballot(n1));
+                k2 = __popcll(__ballot(n2));
+            }
             const int f1_ = c1 < 0 ? ~c1 : nfirst[c1], l1_ = c1 < 0 ? ~c1 : nlast[c1];
             const int f2_ = c2 < 0 ? ~c2 : nfirst[c2], l2_ = c2 < 0 ? ~c2 : nlast[c2];
             const int d1 = max(0, max(f1_ - qmid, qmid - l1_));
