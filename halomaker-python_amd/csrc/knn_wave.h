@@ -314,10 +314,12 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     } while (0)
 
     // Stage sorted slots [f, f+cnt) (cnt <= 64) and record each lane's accepted subset;
-    // seeding accepts only the lane's own window, the traversal everything outside it.
+    // seeding accepts only the lane's own window, the traversal everything outside it.  The
+    // seeding flushes a full stage before staging the next bucket, the traversal right after
+    // filling it.
 #define KNN_STAGE(fst, cnt, seeding)                                                           \
     do {                                                                                       \
-        if (nbuf == NBUF) KNN_FLUSH(seeding);                                                  \
+        if ((seeding) && nbuf == NBUF) KNN_FLUSH(true);                                        \
         __syncthreads();                                                                       \
         if (lane < (cnt)) {                                                                    \
             const double4 c_ = spos[(fst) + lane];                                             \
@@ -458,6 +460,9 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
             ++st_buckets;
             st_cand += cnt;
             KNN_STAGE(f, cnt, false);
+            // flushed as soon as the stage is full, so that the tests of the nodes in
+            // between see the new roots
+            if (nbuf == NBUF) KNN_FLUSH(false);
             continue;
         }
         const int c1 = __builtin_amdgcn_readfirstlane(nleft[node]);
