@@ -387,7 +387,8 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
                         for (int e = 0; e < RT; ++e) {
                             KeyT d;
                             KNN_KEY(make_float4(sx[e], sy[e], sz[e], 0.0f), d);
-                            // (with a capacity of 32 or more, KH > RT: a test fewer)
+                            // (the host launches capacities of 32 or more only for k > 16,
+                            // so there KH > RT and the size test can go)
                             if (KCAP >= 32 || e < KH) HSET(e, e == off ? (KeyT)INFINITY : d, s0 + e);
                         }
                         for (int e = RT; e < KH; ++e) {
