@@ -44,28 +44,66 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sample_n, nthreads):
-    """Reference OpenMP Fortran (oracle/_ref) on a bounded sample of the same workload:
-    the same generator and parameters at sample_n particles (clump count scaled)."""
+def host_cpus():
+    """The host's CPUs as this process sees them: threads for the CPU legs (OMP_NUM_THREADS
+    when set, which is the CPU share a GPU job is given, else every CPU of the affinity mask),
+    nproc, the affinity mask size and the CPU model."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": omp or aff, "nproc": os.cpu_count(), "affinity": aff,
+            "omp_num_threads": omp or None, "cpu_model": model}
+
+
+def cpu_baseline(ps, sample_n, nthreads, repeats, full_stages):
+    """The reference's OpenMP Fortran (oracle/_ref, compiled from the reference source) on
+    this host's cores: its tree + density + local-maxima stages (f90:2967, :682, :748), the
+    reference's kNN+density+hop path as shipped, on the headline box itself (sample_n = N) or
+    on the same generator at sample_n particles; best of `repeats`.  With full_stages, also
+    one run of the whole public entry with the reference's own stage prints, with and without
+    its optimize_nodes create-nodes path."""
     import oracle as O
     from amdahop import generate as G
-    ps = G.config("c3", n=sample_n)
+    cpus = host_cpus()
+    if sample_n != ps.n:
+        ps = G.config("c3", n=sample_n)
     if O.ref_available():
         R = O.Reference(ps.pos, ps.mass, ps.nvoisins, nthreads=nthreads)
-        t = min(R.time_knn_density_hop() for _ in range(2))
+        ts = [R.time_knn_density_hop() for _ in range(repeats)]
         kind = "reference"
     else:
-        o = O.Oracle(ps.pos, ps.mass, ps.nvoisins, nthreads=nthreads)
-        t0 = time.perf_counter()
-        rho, _ = o.density()
-        o.maxima(rho)
-        t = time.perf_counter() - t0
-        o.close()
+        ts = []
+        for _ in range(repeats):
+            o = O.Oracle(ps.pos, ps.mass, ps.nvoisins, nthreads=nthreads)
+            t0 = time.perf_counter()
+            rho, _ = o.density()
+            o.maxima(rho)
+            ts.append(time.perf_counter() - t0)
+            o.close()
         kind = "port"
-    return {"value": round(sample_n / t / 1e6, 4), "unit": "Mparticles/s", "cores": nthreads,
-            "kind": kind,
-            "sample": f"c3 generator at N={sample_n} (k=64, nhop=16, clumps scaled), "
-                      f"tree+density+local-maxima stages, best of 2, {t:.2f} s"}
+    t = min(ts)
+    what = ("the headline box itself" if ps.n == sample_n and sample_n == 16_777_216
+            else f"the same generator at N={sample_n} (clumps scaled)")
+    out = {"value": round(ps.n / t / 1e6, 4), "unit": "Mparticles/s", "cores": nthreads,
+           "kind": kind,
+           "sample": f"{what}, k=64, nhop=16: tree + density + local-maxima stages, best of "
+                     f"{repeats} ({', '.join(f'{x:.2f}' for x in ts)} s)",
+           "host": cpus}
+    if full_stages and kind == "reference":
+        out["reference_stages_s"] = {}
+        for opt in (False, True):
+            st = R.timed_full(optimize_nodes=opt)
+            out["reference_stages_s"]["optimize_nodes" if opt else "default"] = st
+            log(f"reference full run (optimize_nodes={opt}): {st}")
+    return out
 
 
 def halo_props_stage(steps, cpu_sample_halos, with_cpu):
@@ -338,6 +376,46 @@ def adjoint_stage(steps, with_cpu):
     return out
 
 
+def dropin_stage(ps):
+    """One call of the drop-in compute_adaptahop.neikdtree.compute_adaptahop on the headline
+    box, from the caller's host arrays to the module's outputs (node ids and node tables), the
+    way compute_neiKDtree_mod.py drives it: wall time of the call (the first, which allocates,
+    and a second), and the host structure tree's stage times and parallel branches."""
+    import ctypes
+
+    import numpy as np
+    from amdahop._lib import check, lib, ptr
+    from compute_adaptahop import neikdtree
+
+    n, k = ps.n, ps.nvoisins
+    nthreads = host_cpus()["threads"]
+    neikdtree.sync_from_init_adaptahop(n, 20, 20, 0.7, 0.7, 0.3, 0.3, 1.0, 1.0, 0.7, 70.0, 1.0,
+                                       1.0, 1.0, 1.0, 1.0, 0.5, 0.5, 0.5, 70.0, 1.0, [0, 0, 0])
+    neikdtree.sync_from_change_pos(n, n, 0.001 / n ** (1 / 3), 0.001, 1.0, 1.0)
+    neikdtree.sync_others(False, False, n, nthreads, 80.0, 1.0 / n, 0.0, 16, k, 4.0, 1.0, "MSM",
+                          30, False)
+    pos_f = np.asfortranarray(ps.pos.T)
+    walls = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        neikdtree.compute_adaptahop(pos_f, ps.mass)
+        walls.append((time.perf_counter() - t0) * 1e3)
+    st = np.zeros(4, dtype=np.int64)
+    tms = np.zeros(5)
+    check(lib().ahop_tree_stats(ptr(st), ptr(tms)))
+    neikdtree.close()
+    out = {"first_call_ms": round(walls[0], 2), "call_ms": round(walls[1], 2),
+           "nodes": int(st[3]), "host_threads": nthreads,
+           "host_tree_ms": round(float(tms.sum()), 3),
+           "host_tree_stages_ms": {k_: round(float(v), 3) for k_, v in
+                                   zip(("link", "colour", "create_nodes", "reorder", "tables"),
+                                       tms)},
+           "taskloop_levels": int(st[0]), "subtree_tasks": int(st[2])}
+    log(f"drop-in call: {walls[1]:.1f} ms (first {walls[0]:.1f} ms), host tree "
+        f"{tms.sum():.1f} ms, {int(st[3])} nodes")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -347,7 +425,13 @@ def main():
     ap.add_argument("--bucket", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--cpu-sample-n", type=int, default=2_097_152)
+    ap.add_argument("--cpu-sample-n", type=int, default=2_097_152,
+                    help="particles of the reference's CPU run (16777216: the headline box)")
+    ap.add_argument("--cpu-repeats", type=int, default=3)
+    ap.add_argument("--cpu-full-stages", action="store_true",
+                    help="also run the reference's whole entry point with its own stage "
+                         "prints, with and without optimize_nodes")
+    ap.add_argument("--no-dropin", action="store_true")
     ap.add_argument("--no-halo-props", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-grouping", action="store_true")
@@ -367,7 +451,7 @@ def main():
 
     import numpy as np
     import torch
-    from amdahop import generate as G, run_device
+    from amdahop import device_hop, generate as G, run_device
 
     dev = torch.device("cuda:0")
     ps = G.config(args.workload)
@@ -398,6 +482,10 @@ def main():
     infos = [step(True) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # the mixed step's outputs, kept for the comparison with the FP64 path
+    dens_mixed = dens.clone()
+    gid_mixed = gid.clone()
+    hop_mixed = device_hop(n, dev)
     ms = (t1 - t0) * 1e3 / args.steps
     value = n / (ms * 1e-3) / 1e6
     knn_ms = sum(i.t_knn for i in infos) / len(infos)
@@ -441,6 +529,21 @@ def main():
                               "knn_ms": round(sum(i.t_knn for i in finfo) / nf, 3)}
         extra["speedup_mixed_vs_fp64"] = round(fms / ms, 3)
         log(f"fp64: {fms:.2f} ms/step knn {extra['fp64_path']['knn_ms']:.2f}")
+        # the metric's own claim, checked on every particle of the run: density error of the
+        # mixed path against the FP64 path, and equal group ids and hop targets
+        hop64 = device_hop(n, dev)
+        rel = ((dens_mixed - dens).abs() / dens.abs()).max().item()
+        gid_eq = bool(torch.equal(gid_mixed, gid))
+        hop_eq = bool(torch.equal(hop_mixed, hop64))
+        extra["rho_max_rel_err_vs_fp64"] = rel
+        extra["ids_equal"] = gid_eq and hop_eq
+        extra["mixed_vs_fp64_checked"] = {"particles": n, "density_bit_equal":
+                                          bool(torch.equal(dens_mixed, dens)),
+                                          "group_ids_equal": gid_eq, "hop_targets_equal": hop_eq}
+        log(f"mixed vs fp64: max rel density error {rel:.3g}, group ids equal {gid_eq}, "
+            f"hop targets equal {hop_eq}")
+        del hop64
+    del dens_mixed, gid_mixed, hop_mixed
     sinfo = step(True, saddle=True)
     extra["saddle_stage"] = {"ms": round(sinfo.t_saddle, 3), "directed_edges_kept": int(sinfo.nsaddle)}
     extra["stages_ms"] = {"sort": round(last.t_sort, 3), "tree": round(last.t_tree, 3),
@@ -480,12 +583,15 @@ def main():
     if not args.no_ramses:
         extra["ramses_reader"] = ramses_stage(not args.no_cpu_baseline)
 
+    if not args.no_dropin:
+        extra["dropin_call"] = dropin_stage(ps)
+
     cpu = None
     if not args.no_cpu_baseline:
-        nthreads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
-        nthreads = min(nthreads, 16)
+        nthreads = host_cpus()["threads"]
         log(f"cpu baseline: {args.cpu_sample_n} particles on {nthreads} threads")
-        cpu = cpu_baseline(args.cpu_sample_n, nthreads)
+        cpu = cpu_baseline(ps, args.cpu_sample_n, nthreads, args.cpu_repeats,
+                           args.cpu_full_stages)
 
     out = {"metric": METRIC, "value": round(value, 3), "unit": "Mparticles/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),

@@ -13,7 +13,7 @@ import ctypes
 
 from ._lib import HotParams, RunInfo, check, lib
 
-__all__ = ["run_device", "device_saddle", "knn_device", "density_adjoint", "sph_density",
+__all__ = ["run_device", "device_saddle", "device_hop", "knn_device", "density_adjoint", "sph_density",
            "halo_props", "HALO_FIELDS", "halo_profiles", "PROFILE_FIELDS", "halo_stellar",
            "STELLAR_FIELDS", "HotParams", "RunInfo"]
 
@@ -180,6 +180,18 @@ def device_saddle(info):
     rho = np.empty(max(info.nsaddle, 1), dtype=np.float64)
     check(lib().ahop_device_saddle(ptr(off), ptr(isad), ptr(rho)))
     return off, isad[:info.nsaddle], rho[:info.nsaddle]
+
+
+def device_hop(n, device=None, out=None):
+    """Hop targets of the last :func:`run_device` call as an (n,) int32 device tensor: the
+    0-based original index each particle hops to (-1 where inactive)."""
+    import torch
+
+    dev = device if device is not None else torch.device("cuda:0")
+    hop = out if out is not None else torch.empty(int(n), dtype=torch.int32, device=dev)
+    _check_out(hop, (int(n),), torch.int32, hop.device, "out")
+    check(lib().ahop_device_hop(ctypes.c_void_p(hop.data_ptr()), int(n)))
+    return hop
 
 
 HALO_FIELDS = ["m", "px", "py", "pz", "vx", "vy", "vz", "sigma", "Lx", "Ly", "Lz", "r",

@@ -67,6 +67,8 @@ SIGNATURES = {
     "ahop_run_device": (ctypes.c_int, [_vp, _vp, _i64, ctypes.POINTER(HotParams), _vp, _vp,
                                        ctypes.POINTER(RunInfo)]),
     "ahop_device_saddle": (ctypes.c_int, [_vp, _vp, _vp]),
+    "ahop_device_hop": (ctypes.c_int, [_vp, _i64]),
+    "ahop_tree_stats": (ctypes.c_int, [_vp, _vp]),
     "ahop_knn_device": (ctypes.c_int, [_vp, _vp, _i64, ctypes.POINTER(HotParams), _vp, _vp, _vp,
                                        ctypes.POINTER(RunInfo)]),
     "ahop_density_adjoint_device": (ctypes.c_int, [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _dbl,

@@ -82,7 +82,24 @@ struct ModuleState {
     // last device run (ahop_run_device)
     std::vector<int32_t> dev_sad_offset, dev_sad_isad;
     std::vector<double> dev_sad_rho;
+    int64_t dev_n = 0;   // particle count of the last ahop_run_device (0: none)
+    // counters and stage times of the last host structure-tree stage
+    int64_t tree_stats[4] = {0, 0, 0, 0};
+    double tree_ms[5] = {0, 0, 0, 0, 0};
 };
+
+void keep_tree_stats(ModuleState &s, const TreeOutputs &to)
+{
+    s.tree_stats[0] = to.taskloop_levels;
+    s.tree_stats[1] = to.taskloop_components;
+    s.tree_stats[2] = to.subtree_tasks;
+    s.tree_stats[3] = to.nnodes;
+    s.tree_ms[0] = to.ms_link;
+    s.tree_ms[1] = to.ms_colour;
+    s.tree_ms[2] = to.ms_nodes;
+    s.tree_ms[3] = to.ms_reorder;
+    s.tree_ms[4] = to.ms_output;
+}
 
 ModuleState &S()
 {
@@ -249,6 +266,7 @@ int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_c
         if (s.nusedpart < n) throw std::runtime_error("nusedpart must be >= npart");
         if (!pos_in || !mass_in) throw std::runtime_error("null input array");
         auto t0 = std::chrono::steady_clock::now();
+        s.dev_n = 0;   // the engine's results now belong to this call
         Engine &e = engine();
         Params p = hot_params(s);
         DeviceArray dpos(sizeof(double) * 3 * n), dmass(sizeof(double) * n);
@@ -306,6 +324,7 @@ int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_c
         TreeOutputs to;
         to.liste_parts = std::move(s.liste_parts);   // recycle the previous call's buffer
         create_group_tree(ti, tp, to);
+        keep_tree_stats(s, to);
         auto t2 = std::chrono::steady_clock::now();
         if ((int64_t)to.liste_parts.size() == (int64_t)s.nusedpart) {
             s.liste_parts = std::move(to.liste_parts);
@@ -460,6 +479,7 @@ int ahop_knn_device(const double *d_pos, const double *d_mass, int64_t n,
     return guarded([&] {
         Params p = hot_params(params);
         if (!d_pos || !d_mass || !d_density || !d_nbr) throw std::runtime_error("null array");
+        S().dev_n = 0;   // the engine's buffers now belong to this call
         Engine &e = engine();
         e.knn_lists(d_pos, d_mass, n, p, d_density, d_nbr, d_order);
         fill_info(info, e, false);
@@ -625,7 +645,9 @@ int ahop_run_device(const double *d_pos, const double *d_mass, int64_t n,
         Params p = hot_params(params);
         if (!d_pos || !d_mass) throw std::runtime_error("null input array");
         Engine &e = engine();
+        S().dev_n = 0;
         e.run(d_pos, d_mass, n, p, params->saddle != 0);
+        S().dev_n = n;
         const DeviceResults &r = e.results();
         hipStream_t st = e.stream();
         if (d_density)
@@ -665,6 +687,28 @@ int ahop_device_saddle(int32_t *offset, int32_t *isad, double *rho)
             std::memcpy(isad, s.dev_sad_isad.data(), sizeof(int32_t) * s.dev_sad_isad.size());
         if (rho && !s.dev_sad_rho.empty())
             std::memcpy(rho, s.dev_sad_rho.data(), sizeof(double) * s.dev_sad_rho.size());
+    });
+}
+
+int ahop_device_hop(int32_t *d_hop, int64_t n)
+{
+    return guarded([&] {
+        if (!d_hop) throw std::runtime_error("null array");
+        if (S().dev_n == 0) throw std::runtime_error("no results: call ahop_run_device first");
+        if (n != S().dev_n) throw std::runtime_error("n differs from the last ahop_run_device");
+        Engine &e = engine();
+        AHOP_HIP(hipMemcpyAsync(d_hop, e.results().hop, sizeof(int32_t) * n,
+                                hipMemcpyDeviceToDevice, e.stream()));
+        AHOP_HIP(hipStreamSynchronize(e.stream()));
+    });
+}
+
+int ahop_tree_stats(int64_t *out, double *out_ms)
+{
+    return guarded([&] {
+        auto &s = S();
+        if (out) std::memcpy(out, s.tree_stats, sizeof(s.tree_stats));
+        if (out_ms) std::memcpy(out_ms, s.tree_ms, sizeof(s.tree_ms));
     });
 }
 
@@ -734,6 +778,7 @@ int ahop_create_group_tree(int64_t npart, const double *pos, const double *mass,
         tp.xlongs2 = tp.ylongs2 = tp.zlongs2 = xlong * 0.5;
         TreeOutputs to;
         create_group_tree(ti, tp, to);
+        keep_tree_stats(S(), to);
         if (liste_parts) std::memcpy(liste_parts, to.liste_parts.data(), sizeof(int32_t) * npart);
         if (nnodes) *nnodes = to.nnodes;
         if (real_table) std::memcpy(real_table, to.real_table.data(), sizeof(double) * to.real_table.size());

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <stdexcept>
 #include <string>
+#include <atomic>
 #include <vector>
 
 namespace ahop {
@@ -59,6 +60,9 @@ struct Builder {
     // levels with at least this many colour components treat them in parallel; child
     // subtrees with at least this many groups become tasks
     static constexpr int32_t PAR_COMPONENTS = 16, TASK_MIN_GROUPS = 2;
+    // how the parallel branches were taken: levels whose components went to a taskloop,
+    // components treated there, and sibling subtrees deferred as tasks
+    std::atomic<int64_t> n_taskloop_levels{0}, n_taskloop_components{0}, n_subtree_tasks{0};
     // first error raised inside a task (tasks cannot throw across the parallel region)
     bool failed = false;
     std::string error;
@@ -372,6 +376,8 @@ struct Builder {
                 ok_color(inc_color_tot + 1);
             std::vector<char> ifok(inc_color_tot + 1, 0);
             if (inc_color_tot >= PAR_COMPONENTS) {
+                n_taskloop_levels += 1;
+                n_taskloop_components += inc_color_tot;
                 // components touch disjoint groups and particles
 #pragma omp taskloop grainsize(4) shared(igrposnew, posgg, massg, truemassg, densmaxg, mass_compg, \
                                          rsquare, densmoy, ifok)
@@ -427,6 +433,7 @@ struct Builder {
                     const double trho = rhot * (1.0 + p.fudge / std::sqrt((double)mass_compg[ic]));
                     const int32_t child = inode1 + is - 1;
                     const int32_t a = igrposnew[ic - 1] + 1, b = igrposnew[ic];
+                    if (b - a + 1 >= TASK_MIN_GROUPS) n_subtree_tasks += 1;
                     // sibling subtrees are independent; small ones run inline
 #pragma omp task firstprivate(trho, child, a, b) if (b - a + 1 >= TASK_MIN_GROUPS)
                     create_nodes_task(trho, child, a, b);
@@ -580,6 +587,9 @@ struct Builder {
         out.ms_nodes = t3 - t2;
         out.ms_reorder = t4 - t3;
         out.ms_output = t5 - t4;
+        out.taskloop_levels = n_taskloop_levels;
+        out.taskloop_components = n_taskloop_components;
+        out.subtree_tasks = n_subtree_tasks;
     }
 };
 

@@ -34,6 +34,9 @@ struct TreeOutputs {
     std::vector<double> real_table;       // (8, nnodes) column-major
     std::vector<int32_t> integer_table;   // (6, nnodes) column-major
     double ms_link = 0, ms_colour = 0, ms_nodes = 0, ms_reorder = 0, ms_output = 0;   // stages
+    // parallel branches taken: levels whose colour components were treated in a taskloop,
+    // the components treated there, and child subtrees deferred as tasks
+    int64_t taskloop_levels = 0, taskloop_components = 0, subtree_tasks = 0;
 };
 
 void create_group_tree(const TreeInputs &in, const TreeParams &p, TreeOutputs &out);

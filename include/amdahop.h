@@ -105,6 +105,11 @@ int ahop_run_device(const double *d_pos, const double *d_mass, int64_t n,
                     ahop_run_info *info);
 /* Host copies of the saddle CSR of the last ahop_run_device (sizes from its info). */
 int ahop_device_saddle(int32_t *offset, int32_t *isad, double *rho);
+/* Hop targets of the last ahop_run_device, copied to d_hop (n int32, device pointer): the
+ * 0-based original index each particle hops to (find_local_maxima f90:774-795: the densest of
+ * itself and its first nhop neighbours, ties to the smaller index), -1 where inactive.  n must
+ * be that run's particle count.  Synchronous. */
+int ahop_device_hop(int32_t *d_hop, int64_t n);
 
 /* kNN + SPH density alone, keeping every particle's neighbour row: d_density (n) and
  * d_nbr (n x nvoisins int32, original 0-based ids in ascending distance, self excluded),
@@ -287,6 +292,14 @@ int ahop_create_group_tree(int64_t npart, const double *pos, const double *mass,
                            double fudge, double alphap, double epsilon, int32_t nmembthresh,
                            double xlong, int32_t *liste_parts, int32_t *nnodes,
                            double *real_table, int32_t *integer_table);
+
+/* Counters of the last host structure-tree stage (ahop_compute_adaptahop or
+ * ahop_create_group_tree; create_group_tree f90:885-983 and omp_dev_create_nodes_task
+ * :1245-1628 run it with OpenMP tasks): out[0] levels whose colour components were treated
+ * in parallel (an OpenMP taskloop), out[1] the components treated there, out[2] child subtrees
+ * deferred as tasks, out[3] nodes.  out_ms (may be NULL): milliseconds of linking the group
+ * members, the first colouring, node creation, the deterministic reordering and the tables. */
+int ahop_tree_stats(int64_t *out /* [4] */, double *out_ms /* [5] */);
 
 #ifdef __cplusplus
 }

@@ -157,7 +157,8 @@ def ref_lib():
     if _ref is None:
         L = ctypes.CDLL(REF_SO)
         L.ref_configure.argtypes = [_i64, ctypes.c_int, ctypes.c_int, _dbl, _dbl, _dbl, _dbl,
-                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, _dbl]
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, _dbl,
+                                    ctypes.c_int, ctypes.c_int]
         for f in ("ref_run_full", "ref_stage_begin"):
             getattr(L, f).argtypes = [_i64, _vp, _vp]
         L.ref_knn.argtypes = [ctypes.c_int, _vp, _vp]
@@ -190,7 +191,7 @@ class Reference:
     def staged(self, knn_ids=None, saddle=True):
         """tree -> density -> maxima [-> saddle]; returns a dict of stage outputs."""
         L = ref_lib()
-        L.ref_configure(*self.args)
+        L.ref_configure(*self.args, 0, 0)
         L.ref_stage_begin(self.n, _ptr(self.pos_f), _ptr(self.mass))
         out = {}
         L.ref_stage_tree()
@@ -229,7 +230,7 @@ class Reference:
         (f90:2967, :682, :748) -- its kNN+density+hop path as shipped."""
         import time
         L = ref_lib()
-        L.ref_configure(*self.args)
+        L.ref_configure(*self.args, 0, 0)
         L.ref_stage_begin(self.n, _ptr(self.pos_f), _ptr(self.mass))
         t0 = time.perf_counter()
         L.ref_stage_tree()
@@ -239,10 +240,47 @@ class Reference:
         L.ref_close()
         return t1 - t0
 
+    def timed_full(self, optimize_nodes=False):
+        """The public compute_adaptahop entry (f90:495-543) with the reference's verbose
+        prints on: its own per-stage seconds (system_clock, printed to stderr with 10 ms
+        resolution: tree :3021, density :742, local maxima :879, saddle list :902, create
+        nodes :961) and the wall seconds of the whole call.  optimize_nodes selects the
+        reference's blocked-prefix create-nodes path (opt_create_nodes_omp_root, f90:955-958)."""
+        import re
+        import sys
+        import tempfile
+        import time
+        L = ref_lib()
+        L.ref_configure(*self.args, 1, 1 if optimize_nodes else 0)
+        sys.stderr.flush()
+        with tempfile.TemporaryFile() as tf:
+            saved = os.dup(2)
+            os.dup2(tf.fileno(), 2)
+            try:
+                t0 = time.perf_counter()
+                L.ref_run_full(self.n, _ptr(self.pos_f), _ptr(self.mass))
+                wall = time.perf_counter() - t0
+            finally:
+                os.dup2(saved, 2)
+                os.close(saved)
+            tf.seek(0)
+            text = tf.read().decode(errors="replace")
+        nn = L.ref_nnodes()
+        L.ref_close()
+        names = {"create the tree structure": "tree", "compute mean density": "density",
+                 "find local maxima": "maxima", "saddle_list": "saddle",
+                 "create nodes": "create_nodes"}
+        out = {"wall": wall, "nnodes": nn}
+        for m in re.finditer(r"-->\s*([0-9.]+) seconds to (.+)", text):
+            key = names.get(m.group(2).strip())
+            if key:
+                out[key] = float(m.group(1))
+        return out
+
     def full(self):
         """The public compute_adaptahop entry (f90:495-543): final node ids and tables."""
         L = ref_lib()
-        L.ref_configure(*self.args)
+        L.ref_configure(*self.args, 0, 0)
         L.ref_run_full(self.n, _ptr(self.pos_f), _ptr(self.mass))
         lp = np.empty(self.n, dtype=np.int32)
         L.ref_get_liste_parts(_ptr(lp), self.n)

@@ -289,7 +289,7 @@ int64_t orc_density(const orc_ctx *c, double *density, int32_t *nbr_nhop)
     {
         double *dist2 = malloc(sizeof(double) * (k + 1));
         int32_t *nei = malloc(sizeof(int32_t) * (k + 1));
-#pragma omp for schedule(static)
+#pragma omp for schedule(dynamic, 256)
         for (int64_t i = 0; i < c->n; ++i) {
             int32_t ipar = (int32_t)(i + 1);
             find_nearest(c, ipar, dist2, nei);

@@ -40,10 +40,12 @@ contains
    ! Set every module parameter the way compute_neiKDtree_mod.py does
    ! (init_adaptahop_130 :155-197, change_pos_1310 :200-210,
    ! sync_fortran :738-747) for a unit-free box of side xl.
-   subroutine ref_configure(n, nvois, nhp, rho_thr, fdg, alph, fdgeps, nmemb, nlev, nthreads, xl) &
-         bind(C, name="ref_configure")
+   ! verb /= 0 turns on the reference's verbose stage prints (its own system_clock timings,
+   ! on stderr); optnodes /= 0 selects its optimize_nodes create-nodes path.
+   subroutine ref_configure(n, nvois, nhp, rho_thr, fdg, alph, fdgeps, nmemb, nlev, nthreads, xl, &
+         verb, optnodes) bind(C, name="ref_configure")
       integer(c_int64_t), value :: n
-      integer(c_int), value :: nvois, nhp, nmemb, nlev, nthreads
+      integer(c_int), value :: nvois, nhp, nmemb, nlev, nthreads, verb, optnodes
       real(c_double), value :: rho_thr, fdg, alph, fdgeps, xl
       real(c_double) :: shift(3), eps
       integer(c_int) :: n4
@@ -53,8 +55,8 @@ contains
            0.7d0, 70d0, xl, xl, xl, xl, xl, xl*0.5d0, xl*0.5d0, xl*0.5d0, 70d0, 1d0, shift)
       eps = fdgeps*xl/dble(n)**(1d0/3d0)
       call sync_from_change_pos(n, n4, eps, fdgeps, xl, xl)
-      call sync_others(.false., .false., n4, nthreads, rho_thr, 1d0/dble(n), 0d0, nhp, nvois, &
-           fdg, alph, 'MSM', nlev, .false.)
+      call sync_others(verb /= 0, .false., n4, nthreads, rho_thr, 1d0/dble(n), 0d0, nhp, nvois, &
+           fdg, alph, 'MSM', nlev, optnodes /= 0)
    end subroutine ref_configure
 
    ! Full public entry point, exactly as the Python bridge calls it.

@@ -7,7 +7,7 @@ and saddle CSR, and through its public compute_adaptahop entry for the final nod
 and node tables.  Inputs are regenerated from amdahop.generate; their SHA-256 is stored
 so a generator change is detected.  Run from the repository root:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [--tables-only]
 """
 import os
 import sys
@@ -19,6 +19,7 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "halomaker-python_amd"))
 
 import oracle as O  # noqa: E402
+from oracle.hop_saddle import csr_as_pairs  # noqa: E402
 from amdahop import generate as G  # noqa: E402
 
 CASES = {
@@ -30,8 +31,48 @@ PARAMS = dict(nhop=16, rho_threshold=80.0, fudge=4.0, alphap=1.0, fudgepsilon=0.
               nmembers=20, nlevelmax=30)
 
 
+# Node ids and node tables of the reference at the C1 and C2 sizes (262,144 and 2,097,152
+# particles, k=32).  The stage outputs the host structure tree starts from (densities, group
+# ids, saddle pairs) are not stored: a test regenerates them with the oracle and checks their
+# SHA-256 against the reference's, stored here (the saddle pairs as (g1, g2, s) sorted by
+# (g1, g2), the multiset the reference's unstable per-group sort leaves as the contract).
+TABLE_CASES = {
+    "c1_tables": ("c1", 262144, 150),
+    "c2_tables": ("c2", 2097152, 1200),
+}
+
+
+def _sha(*arrays):
+    import hashlib
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
 def main():
     out_dir = os.path.dirname(os.path.abspath(__file__))
+    for name, (cfg, n, ncl) in TABLE_CASES.items():
+        ps = G.config(cfg, n=n, nclumps=ncl)
+        R = O.Reference(ps.pos, ps.mass, ps.nvoisins, **PARAMS, nthreads=8)
+        st = R.staged()
+        full = R.full()
+        if not np.array_equal(st["density"], full["density"]):
+            raise SystemExit("staged and full reference runs disagree")
+        np.savez_compressed(
+            os.path.join(out_dir, f"{name}.npz"),
+            config=cfg, n=n, nclumps=ncl, nvoisins=ps.nvoisins,
+            input_digest=G.digest(ps.pos, ps.mass),
+            stage_digest=_sha(st["density"], st["gid"], st["densityg"]),
+            saddle_digest=_sha(*csr_as_pairs(st["saddle_offset"], st["saddle_isad"],
+                                             st["saddle_rho"])),
+            ngroups=st["ngroups"], nsaddle=len(st["saddle_isad"]),
+            liste_parts=full["liste_parts"], nnodes=full["nnodes"],
+            real_table=full["real_table"], integer_table=full["integer_table"])
+        print(name, "ngroups", st["ngroups"], "nnodes", full["nnodes"],
+              "saddle entries", len(st["saddle_isad"]))
+    if "--tables-only" in sys.argv:
+        return
     for name, (cfg, n, ncl) in CASES.items():
         ps = G.config(cfg, n=n, nclumps=ncl)
         R = O.Reference(ps.pos, ps.mass, ps.nvoisins, **PARAMS, nthreads=4)

@@ -125,3 +125,64 @@ def test_sync_calls_accept_the_bridge_argument_lists():
     neikdtree.sync_from_change_pos(np.int64(1000), 1000, 1e-4, 0.001, 1.0, 1.0)
     neikdtree.sync_others(False, False, 1000, 16, 80.0, 1e-3, 0.0, 16, 32, 4.0, 1.0, "MSM", 30, False)
     neikdtree.close()
+
+
+def _sha(*arrays):
+    import hashlib
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def table_case_stages(g):
+    """The stage outputs the host structure tree starts from, regenerated with the oracle for
+    a node-table golden and checked against the reference's digests: densities, group ids,
+    group peak densities and the saddle CSR (per group in decreasing saddle density)."""
+    import oracle as O
+    from oracle import hop_saddle as HS
+    ps = golden_input(g)
+    o = O.Oracle(ps.pos, ps.mass, int(g["nvoisins"]), nthreads=8)
+    rho, _, nb = o.density(with_nbrs=True)
+    o.close()
+    hop = HS.hop_targets(rho, nb, 80.0)
+    gid, ng = HS.group_ids(hop)
+    dg = np.zeros(ng)
+    roots = np.nonzero(hop == np.arange(ps.n))[0]
+    dg[gid[roots] - 1] = rho[roots]
+    assert _sha(rho, gid, dg) == str(g["stage_digest"])
+    g1, g2, s = HS.saddle_pairs(rho, gid, nb)
+    assert _sha(g1, g2, s) == str(g["saddle_digest"])
+    order = np.lexsort((g2, -s, g1))
+    off = np.zeros(ng + 1, dtype=np.int32)
+    np.add.at(off, g1.astype(np.int64), 1)
+    off = np.cumsum(off).astype(np.int32)
+    return ps, rho, gid, ng, dg, off, g2[order].astype(np.int32), s[order]
+
+
+@pytest.mark.parametrize("name", ["c1_tables", "c2_tables"])
+def test_host_structure_tree_at_size(name):
+    """create_group_tree (f90:885-1131, :1245-1628) at 262,144 and 2,097,152 particles, on
+    stage outputs equal to the reference's: node ids and node tables bit for bit with the
+    reference's full run, and the parallel branches (the component taskloop and the subtree
+    tasks) taken."""
+    g = load_golden(name)
+    ps, rho, gid, ng, dg, off, isad, srho = table_case_stages(g)
+    eps = 0.001 / ps.n ** (1.0 / 3.0)
+    lp = np.empty(ps.n, dtype=np.int32)
+    nn = ctypes.c_int32(0)
+    rt = np.zeros(8 * 2 * ng)
+    it = np.zeros(6 * 2 * ng, dtype=np.int32)
+    check(lib().ahop_create_group_tree(ps.n, ptr(ps.pos), ptr(ps.mass), ptr(rho), ptr(gid), ng,
+                                       ptr(dg), ptr(off), ptr(isad), ptr(srho), 80.0, 4.0, 1.0,
+                                       eps, 20, 1.0, ptr(lp), ctypes.byref(nn), ptr(rt), ptr(it)))
+    k = nn.value
+    assert k == int(g["nnodes"])
+    assert np.array_equal(lp, g["liste_parts"])
+    assert np.array_equal(rt[:8 * k].reshape(8, k, order="F"), g["real_table"])
+    assert np.array_equal(it[:6 * k].reshape(6, k, order="F"), g["integer_table"])
+    st = np.zeros(4, dtype=np.int64)
+    check(lib().ahop_tree_stats(ptr(st), None))
+    print(f"{name}: {k} nodes, taskloop levels {st[0]} ({st[1]} components), "
+          f"subtree tasks {st[2]}")
+    assert st[3] == k and st[0] >= 1 and st[1] >= 16 and st[2] >= 1

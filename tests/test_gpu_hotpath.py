@@ -146,29 +146,74 @@ def test_c2_mixed_equals_fp64_and_oracle_sample(gpu):
     assert np.array_equal(o.density_at(idx), dmx[idx])
 
 
-def test_c3_full_size_sample_and_invariants(gpu):
-    """Headline workload: 16.8M particles, k=64, mixed families.  Densities of a random
-    sample are bit-exact vs the oracle; hop/group invariants hold on every particle."""
+@pytest.fixture(scope="module")
+def c3_dropin():
+    """The headline box (16,777,216 particles, k=64, mixed families) through the drop-in in
+    both precision modes."""
     ps = G.config("c3")
-    log(f"c3 n={ps.n}")
-    dmx, gmx, imx = _device_run(ps, True, saddle=False)
-    log(f"c3 fallback {imx.nfallback} groups {imx.ngroups} knn {imx.t_knn:.1f} ms")
+    runs = {}
+    for mode in ("mixed", "fp64"):
+        log(f"c3 drop-in mode={mode} n={ps.n}")
+        runs[mode] = dropin_run(ps, mode)
+    return ps, runs
+
+
+def _assert_saddle_decreasing(off, rho):
+    off = np.asarray(off, dtype=np.int64)
+    d = np.diff(rho)
+    inner = np.ones(d.shape[0], dtype=bool)
+    inner[off[1:-1][(off[1:-1] > 0) & (off[1:-1] < rho.shape[0])] - 1] = False
+    assert np.all(d[inner] <= 0), "saddle list not decreasing within a group"
+
+
+@pytest.mark.timeout(900)
+def test_c3_mixed_equals_fp64_everywhere(gpu, c3_dropin):
+    """Every output of the mixed path equals the FP64 path's over all 16.8M particles:
+    densities (so the relative density error is 0), group ids, hop targets, node ids, node
+    tables and the saddle multisets."""
+    from oracle import hop_saddle as HS
+    ps, runs = c3_dropin
+    m, f = runs["mixed"], runs["fp64"]
+    assert np.array_equal(m["density"], f["density"])
+    assert np.array_equal(m["gid"], f["gid"])
+    assert np.array_equal(m["hop"], f["hop"])
+    assert m["nnodes"] == f["nnodes"] and m["nnodes"] > 1000
+    assert np.array_equal(m["liste_parts"], f["liste_parts"])
+    assert np.array_equal(m["real_table"], f["real_table"])
+    assert np.array_equal(m["integer_table"], f["integer_table"])
+    for r in (m, f):
+        _assert_saddle_decreasing(r["saddle"][0], r["saddle"][2])
+    pm, pf = HS.csr_as_pairs(*m["saddle"]), HS.csr_as_pairs(*f["saddle"])
+    for a, b in zip(pm, pf):
+        assert np.array_equal(a, b)
+    log(f"c3: {int(m['gid'].max())} groups, {pm[0].shape[0]} saddle entries, "
+        f"{m['nnodes']} nodes, all equal between modes")
+
+
+@pytest.mark.timeout(900)
+def test_c3_full_size_matches_oracle(gpu, c3_dropin):
+    """Over all 16.8M particles: densities bit-exact against the oracle (the C restatement
+    pinned to the compiled reference), and hop targets, group ids and saddle multisets equal
+    to the reference's rules (oracle.hop_saddle, pinned to the restatement) applied to the
+    oracle's own neighbour rows and densities."""
+    from oracle import hop_saddle as HS
+    ps, runs = c3_dropin
+    r = runs["mixed"]
+    log("c3 oracle: density and first-16 neighbour rows of every particle")
     o = O.Oracle(ps.pos, ps.mass, ps.nvoisins)
-    idx = np.random.default_rng(10).choice(ps.n, size=4000, replace=False)
-    assert np.array_equal(o.density_at(idx), dmx[idx])
+    rho, nact, nb = o.density(with_nbrs=True)
     o.close()
-    act = dmx > 80.0
-    assert int(act.sum()) == imx.nactive
-    assert np.all(gmx[~act] == 0) and np.all(gmx[act] >= 1) and gmx.max() == imx.ngroups
-    # every hop chain climbs in (density, -index), so a group's root is its densest member
-    # (lowest index on ties); group ids number the roots by ascending index (f90:797-803)
-    idx_act = np.nonzero(act)[0]
-    order = np.lexsort((idx_act, -dmx[idx_act], gmx[idx_act]))
-    g_sorted = gmx[idx_act][order]
-    root_pos = np.nonzero(np.r_[True, g_sorted[1:] != g_sorted[:-1]])[0]
-    roots = idx_act[order][root_pos]
-    assert np.array_equal(g_sorted[root_pos], np.arange(1, imx.ngroups + 1))
-    assert np.all(np.diff(roots) > 0)
+    assert np.array_equal(r["density"], rho)
+    log(f"c3 oracle: densities equal ({nact} active); hop, groups and saddles")
+    hop = HS.hop_targets(rho, nb, 80.0)
+    assert np.array_equal(r["hop"], hop)
+    gid, ng = HS.group_ids(hop)
+    assert np.array_equal(r["gid"], gid)
+    pairs = HS.saddle_pairs(rho, gid, nb)
+    got = HS.csr_as_pairs(*r["saddle"])
+    for a, b in zip(got, pairs):
+        assert np.array_equal(a, b)
+    log(f"c3 oracle: {ng} groups, {pairs[0].shape[0]} saddle entries equal")
 
 
 def test_saddle_symmetric_and_bounded_c2(gpu):
@@ -238,3 +283,25 @@ def test_out_of_box_position_is_an_error(gpu):
     ps.pos[7, 1] = np.nan
     with pytest.raises(RuntimeError, match="inside"):
         dropin_run(ps, "mixed")
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["c1_tables", "c2_tables"])
+def test_dropin_node_tables_at_size(gpu, name, mode):
+    """The whole drop-in call at 262,144 and 2,097,152 particles (150 and 1,084 nodes):
+    node ids and node tables bit for bit with the reference's own full run, the host tree's
+    parallel branches taken."""
+    from amdahop._lib import check, lib, ptr
+    g = load_golden(name)
+    ps = golden_input(g)
+    log(f"drop-in tables {name} mode={mode} n={ps.n}")
+    r = dropin_run(ps, mode)
+    assert r["nnodes"] == int(g["nnodes"])
+    assert np.array_equal(r["liste_parts"], g["liste_parts"])
+    assert np.array_equal(r["real_table"], g["real_table"])
+    assert np.array_equal(r["integer_table"], g["integer_table"])
+    st = np.zeros(4, dtype=np.int64)
+    check(lib().ahop_tree_stats(ptr(st), None))
+    log(f"{name}: taskloop levels {st[0]}, subtree tasks {st[2]}")
+    assert st[0] >= 1 and st[2] >= 1

@@ -144,3 +144,61 @@ def test_restatement_equals_live_reference_new_seed():
     assert np.array_equal(gid, st["gid"])
     assert saddle_as_sets(off, isad, rs) == saddle_as_sets(st["saddle_offset"], st["saddle_isad"],
                                                            st["saddle_rho"])
+
+
+def _numpy_rules(o, rho):
+    from oracle import hop_saddle as HS
+    _, _, nb = o.density(with_nbrs=True)
+    hop = HS.hop_targets(rho, nb, o.rho_thr)
+    gid, ng = HS.group_ids(hop)
+    return nb, hop, gid, ng, HS.saddle_pairs(rho, gid, nb)
+
+
+def test_numpy_hop_saddle_rules_equal_restatement(case):
+    """oracle.hop_saddle (the full-size checker of the hop, group and saddle stages) equals the
+    per-particle C restatement, which equals the compiled reference."""
+    from oracle import hop_saddle as HS
+    g, ps, o, rho, hop, gid, dg, ng = case
+    nb, hop2, gid2, ng2, pairs = _numpy_rules(o, rho)
+    assert np.array_equal(hop2, hop)
+    assert ng2 == ng and np.array_equal(gid2, gid)
+    ref = HS.csr_as_pairs(g["saddle_offset"], g["saddle_isad"], g["saddle_rho"])
+    for a, b in zip(pairs, ref):
+        assert np.array_equal(a, b)
+
+
+def test_numpy_hop_saddle_rules_on_ties():
+    """Equal densities (duplicated positions and masses) exercise the min-index tie rule."""
+    from oracle import hop_saddle as HS
+    ps = G.config("c1", n=6000, nclumps=4, seed=3)
+    ps.pos[100:140] = ps.pos[200:240]
+    o = O.Oracle(ps.pos, ps.mass, ps.nvoisins, nthreads=4)
+    rho, _ = o.density()
+    hop, gid, dg, ng = o.maxima(rho)
+    assert len(np.unique(rho[rho > 80])) < int((rho > 80).sum())   # ties are present
+    nb, hop2, gid2, ng2, pairs = _numpy_rules(o, rho)
+    assert np.array_equal(hop2, hop) and np.array_equal(gid2, gid)
+    ref = HS.csr_as_pairs(*o.saddle(rho, gid, ng))
+    for a, b in zip(pairs, ref):
+        assert np.array_equal(a, b)
+    # planted mistakes in the rules are caught: ties to the larger index, max instead of min
+    # in the symmetrisation
+    act = np.nonzero(rho > 80)[0]
+    cand = np.concatenate([act[:, None], nb[act]], axis=1)
+    rc = rho[cand]
+    wrong = np.where(rc == rc.max(axis=1)[:, None], cand, -1).max(axis=1)
+    assert not np.array_equal(wrong, hop[act])
+    # directed maxima before the symmetrisation: some pairs differ between the two
+    # directions, so symmetrising with max instead of min would change them
+    i = np.nonzero(gid > 0)[0]
+    j = nb[i]
+    ga, gb = np.broadcast_to(gid[i][:, None], j.shape), gid[j]
+    keep = (gb > 0) & (gb != ga)
+    sv = np.minimum(np.broadcast_to(rho[i][:, None], j.shape), rho[j])[keep]
+    directed = {}
+    for a, b, v in zip(ga[keep].tolist(), gb[keep].tolist(), sv.tolist()):
+        directed[(a, b)] = max(directed.get((a, b), -1.0), v)
+    g1, g2, s = pairs
+    with_max = np.array([max(directed[(a, b)], directed[(b, a)])
+                         for a, b in zip(g1.tolist(), g2.tolist())])
+    assert not np.array_equal(with_max, ref[2])
