@@ -77,7 +77,10 @@ def cpu_baseline(ps, sample_n, nthreads, repeats, full_stages):
         ps = G.config("c3", n=sample_n)
     if O.ref_available():
         R = O.Reference(ps.pos, ps.mass, ps.nvoisins, nthreads=nthreads)
-        ts = [R.time_knn_density_hop() for _ in range(repeats)]
+        ts = []
+        for _ in range(repeats):
+            ts.append(R.time_knn_density_hop())
+            log(f"reference tree + density + local maxima on {ps.n} particles: {ts[-1]:.2f} s")
         kind = "reference"
     else:
         ts = []

@@ -196,20 +196,44 @@ def device_hop(n, device=None, out=None):
 
 HALO_FIELDS = ["m", "px", "py", "pz", "vx", "vy", "vz", "sigma", "Lx", "Ly", "Lz", "r",
                "Ixx", "Ixy", "Ixz", "Iyy", "Iyz", "Izz", "sha", "shb", "shc", "ek", "ep", "et",
-               "rvir", "mvir", "cvel", "tvir", "spin"]
+               "rvir", "mvir", "cvel", "tvir", "spin", "rho_0", "r_c"]
 
 
 def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst=430.1,
                vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000,
                tile=0, workgroup=0, sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0,
-               light=False):
+               light=False, flag_period=1, virial_pe_profile="shell",
+               virial_pe_norm_policy="profile_outer_for_large", virial_shell_self_mode="count",
+               ang_mom_of_r=False):
     """Per-halo properties of a CSR member layout (device tensors), one row per halo in
     the order of HALO_FIELDS.  ``sc`` / ``dpmmc`` recentre each halo as the reference's
     SC / DPMMC options do (with ``dcell_min`` and ``eps_sc``; DPMMC needs ``density``).
     ``light`` skips the shell tables, energies and virial scan (those columns read NaN).
-    Returns (out tensor (H, 29) float64, HaloInfo)."""
+    The reference's switches keep its names and values: ``flag_period`` (FlagPeriod; 0 turns
+    the periodic wrap off), ``virial_pe_profile`` ('shell' or 'baseline'),
+    ``virial_pe_norm_policy`` ('profile_outer_for_large' or 'legacy') and
+    ``virial_shell_self_mode`` ('count' or 'half_shell') of potential_profile_1b40
+    (compute_halo_props.py:1065-1123); an unknown value is a ValueError as there.
+    ``ang_mom_of_r`` (the per-shell angular momentum file of det_ang_momentum_per_shell_1b5)
+    is not provided: NotImplementedError.
+    Returns (out tensor (H, 31) float64, HaloInfo)."""
     import numpy as np
     import torch
+
+    modes = {"virial_pe_profile": (virial_pe_profile, ("shell", "baseline")),
+             "virial_pe_norm_policy": (virial_pe_norm_policy,
+                                       ("profile_outer_for_large", "legacy")),
+             "virial_shell_self_mode": (virial_shell_self_mode, ("count", "half_shell"))}
+    sw = {}
+    for name, (val, allowed) in modes.items():
+        if val not in allowed:
+            raise ValueError(f"unknown {name}={val!r}")
+        sw[name] = allowed.index(val)
+    if ang_mom_of_r:
+        raise NotImplementedError("ANG_MOM_OF_R (per-shell angular momentum file) is not "
+                                  "provided")
+    if flag_period == 0:
+        lbox = float("inf")       # no periodic wrap anywhere (FlagPeriod = 0)
 
     from ._lib import HaloInfo, HaloParams, ptr
     off = np.ascontiguousarray(offsets, dtype=np.int64)
@@ -227,7 +251,8 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
     p = HaloParams(float(lbox), float(hub_pt), float(gravconst), float(vir_overdens),
                    float(rho_mean), 1 if cdm else 0, int(nshell), int(direct_pe_nmax), int(tile),
                    int(workgroup), 1 if sc else 0, 1 if dpmmc else 0, float(dcell_min),
-                   float(eps_sc), 1 if light else 0)
+                   float(eps_sc), 1 if light else 0, sw["virial_pe_profile"],
+                   sw["virial_pe_norm_policy"], sw["virial_shell_self_mode"])
     info = HaloInfo()
     torch.cuda.synchronize(pos.device)
     dptr = ctypes.c_void_p(density.data_ptr()) if density is not None else None

@@ -29,7 +29,9 @@ class HaloParams(ctypes.Structure):
     _fields_ = [("lbox", _dbl), ("hub_pt", _dbl), ("gravconst", _dbl), ("vir_overdens", _dbl),
                 ("rho_mean", _dbl), ("cdm", _i32), ("nshell", _i32), ("direct_pe_nmax", _i32),
                 ("tile", _i32), ("workgroup", _i32), ("sc", _i32), ("dpmmc", _i32),
-                ("dcell_min", _dbl), ("eps_sc", _dbl), ("light", _i32)]
+                ("dcell_min", _dbl), ("eps_sc", _dbl), ("light", _i32),
+                ("virial_pe_profile", _i32), ("virial_pe_norm_policy", _i32),
+                ("virial_shell_self_mode", _i32)]
 
 
 class HaloInfo(ctypes.Structure):
@@ -77,7 +79,8 @@ SIGNATURES = {
                                           _i32, _vp]),
     "ahop_ramses_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _i32, _i32, _i32, _i32,
                                          _i32, _i64, _i64, _i64, _dbl, _dbl, _vp, _vp, _vp, _vp,
-                                         _vp, _vp, _vp]),
+                                         _vp, _vp, _vp, _vp, _vp]),
+    "ahop_ramses_descriptor": (ctypes.c_int, [ctypes.c_char_p, _vp, _vp, _vp]),
     "ahop_halo_stellar_device": (ctypes.c_int, [_i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                                  _i32, _vp, ctypes.POINTER(HaloParams), _dbl, _vp,
                                                  _vp, ctypes.POINTER(_f32)]),

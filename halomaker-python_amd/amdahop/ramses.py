@@ -13,6 +13,19 @@ from ._lib import check, lib, ptr
 
 FAMILIES = {"all": 0, "dm": 1, "star": 2}
 VERSIONS = {"Ra3": 3, "Ra4": 4}
+# the elements a Ra4 field descriptor may list as chem_<element> (src/chem_species.py:1)
+CHEM_ELEMENTS = ("H", "O", "Fe", "Mg", "C", "N", "Si", "S", "D")
+
+
+def descriptor(repository):
+    """What a Ra4 output's part_file_descriptor.txt provides for stars: (present, has initial
+    masses, tuple of the chem_<element> elements it lists).  Raises the reference's errors
+    for an initial_mass field without or before metallicity."""
+    present, has_m0, mask = ctypes.c_int32(0), ctypes.c_int32(0), ctypes.c_int32(0)
+    check(lib().ahop_ramses_descriptor(repository.encode(), ctypes.byref(present),
+                                       ctypes.byref(has_m0), ctypes.byref(mask)))
+    chem = tuple(e for i, e in enumerate(CHEM_ELEMENTS) if mask.value >> i & 1)
+    return bool(present.value), bool(has_m0.value), chem
 
 
 def _nchar(repository):
@@ -49,9 +62,11 @@ def read_particles(repository, rver="Ra3", family="all", stellar=True, scale_l=N
     """Selected particles of a RAMSES output directory in dense-id order: dict with ``pos``
     (N, 3) shifted by -0.5, ``vel`` (N, 3) in km/s, ``mass`` (N,), ``source_pid`` (N,) int64
     (families 'dm' / 'star'), ``birth`` / ``metal`` (nstar,) for the stellar payload (raw birth
-    epochs: the age is snapshot_age - epoch_to_age(birth) with the caller's cosmology), and
-    the counts ``ndm``, ``nstar``, ``npart``.  scale_l / scale_t default to the output's
-    info file."""
+    epochs: the age is snapshot_age - epoch_to_age(birth) with the caller's cosmology), the
+    counts ``ndm``, ``nstar``, ``npart``, and for a Ra4 output whose field descriptor lists
+    them, ``m0`` (nstar,) initial masses and ``chem`` (nstar, 9) abundances in CHEM_ELEMENTS
+    order (NaN where the files give none, as the reference allocates them), else None.
+    scale_l / scale_t default to the output's info file."""
     if family not in FAMILIES or rver not in VERSIONS:
         raise ValueError("family must be all / dm / star and rver Ra3 / Ra4")
     nchar = _nchar(repository)
@@ -77,25 +92,46 @@ def read_particles(repository, rver="Ra3", family="all", stellar=True, scale_l=N
     vel = np.zeros((nselected, 3))
     mass = np.zeros(nselected)
     spid = np.full(nselected, -1, dtype=np.int64) if family != "all" else None
-    birth = metal = None
+    birth = metal = m0 = chem = None
     if stellar and nstar > 0:
         birth = np.full(nstar, np.nan)
         metal = np.full(nstar, np.nan)
+        if rver == "Ra4":
+            _, has_m0, elems = descriptor(repository)
+            m0 = np.full(nstar, np.nan) if has_m0 else None
+            chem = np.full((nstar, len(CHEM_ELEMENTS)), np.nan) if elems else None
+
+    def p(a):
+        return ptr(a) if a is not None else None
+
     check(lib().ahop_ramses_read(
         rep, nch, ncpu, rv, fam, 1 if stellar else 0, int(nthreads), ndm_out, nselected, nstar,
-        float(scale_l), float(scale_t), ptr(pos), ptr(vel), ptr(mass),
-        ptr(spid) if spid is not None else None, ptr(birth) if birth is not None else None,
-        ptr(metal) if metal is not None else None, None))
+        float(scale_l), float(scale_t), ptr(pos), ptr(vel), ptr(mass), p(spid), p(birth),
+        p(metal), p(m0), p(chem), None))
     return dict(pos=pos, vel=vel, mass=mass, source_pid=spid, birth=birth, metal=metal,
-                ndm=ndm_out, nstar=nstar, npart=nselected)
+                m0=m0, chem=chem, ndm=ndm_out, nstar=nstar, npart=nselected)
+
+
+def write_descriptor(repository, stellar_fields):
+    """Write a Ra4 part_file_descriptor.txt in RAMSES's layout: the fixed fields up to the tag,
+    then ``stellar_fields`` (names, in record order; each a float64 record per particle)."""
+    fixed = [("position_x", "d"), ("position_y", "d"), ("position_z", "d"),
+             ("velocity_x", "d"), ("velocity_y", "d"), ("velocity_z", "d"), ("mass", "d"),
+             ("identity", "i"), ("levelp", "i"), ("family", "b"), ("tag", "b")]
+    rows = fixed + [(f, "d") for f in stellar_fields]
+    with open(os.path.join(repository, "part_file_descriptor.txt"), "w") as f:
+        f.write("# version:  1\n# ivar, variable_name, variable_type\n")
+        for i, (name, t) in enumerate(rows, start=1):
+            f.write(f"{i:3d}, {name}, {t}\n")
 
 
 def write_part_file(path, ncpu, pos, vel, mass, ids, rver="Ra3", birth=None, metal=None,
-                    family=None):
+                    family=None, stellar_records=None):
     """Write one RAMSES part file in the record layout _read_ramses_new_1010 reads (header
     records, positions, velocities, masses, ids, levels, the Ra4 family and tag bytes, then
-    birth epochs and metallicities when any birth epoch is non-zero).  For tests and
-    benchmarks: positions in [0, 1), ids int32, family int8 (Ra4)."""
+    the stellar records when any birth epoch is non-zero: birth epochs and metallicities, or
+    with ``stellar_records`` that list of per-particle arrays in the descriptor's order).
+    For tests and benchmarks: positions in [0, 1), ids int32, family int8 (Ra4)."""
     from scipy.io import FortranFile
     n = len(mass)
     nstar = 0 if birth is None else int(np.count_nonzero(birth))
@@ -119,5 +155,6 @@ def write_part_file(path, ncpu, pos, vel, mass, ids, rver="Ra3", birth=None, met
             f.write_record(np.asarray(family).astype(np.int8))
             f.write_record(np.zeros(n, np.int8))                     # tag
         if nstar > 0:
-            f.write_record(np.ascontiguousarray(birth, dtype=np.float64))
-            f.write_record(np.ascontiguousarray(metal, dtype=np.float64))
+            recs = stellar_records if stellar_records is not None else [birth, metal]
+            for r in recs:
+                f.write_record(np.ascontiguousarray(r, dtype=np.float64))

@@ -3,10 +3,16 @@
 The reference turns the node tables of ``neikdtree`` into halos and subhalos in Python,
 ``select_with_MS_method_1319`` (src/compute_neiKDtree_mod.py:260-464) or, for method HOP,
 ``select_halos_1317`` (:228-252), and then groups the
-particles by structure (compute_adaptahop_131, :109-119).  Here the selection is restated on
-the node tables in NumPy (a few thousand nodes: host work, in the reference's order), and the
-per-particle part (relabelling every particle, then the argsort / np.unique grouping) runs on
-the device through ``ahop_group_members_device``.
+particles by structure (compute_adaptahop_131, :109-119).  Here the selection works on the
+node tables (a few thousand nodes: host work) as whole-array NumPy passes: the sister chains
+become (owner, subnode) pairs in chain order, own masses are a segmented left fold, the main
+branches are a segmented first maximum taken level by level from the deepest nodes up, the
+structures are propagated level by level from the top down, and the structure tree's links
+come from the sorted (mother structure, structure) pairs.  Only the outcomes the reference's
+loop orders decide are kept from it: the subtraction order of truemass, the first subnode in
+sister order on equal truemass, structures numbered by their first node, halos and daughters
+listed in structure order.  The per-particle part (relabelling every particle, then the
+argsort / np.unique grouping) runs on the device through ``ahop_group_members_device``.
 """
 from __future__ import annotations
 
@@ -63,125 +69,189 @@ class MSMStructures:
         return self.nb_halos + self.nb_sub
 
 
+def _chain_positions(nxt):
+    """Heads of the linked lists given by ``nxt`` (1-based successor of each entry, 0 = end;
+    index 0 unused) and every entry's distance from its head, by pointer doubling on the
+    predecessor links."""
+    n1 = nxt.shape[0]
+    ids = np.arange(n1)
+    pred = ids.copy()                   # a head points to itself
+    has = nxt > 0
+    pred[nxt[has]] = ids[has]
+    dist = (pred != ids).astype(np.int64)
+    while True:
+        far = pred[pred]
+        if np.array_equal(far, pred):
+            return pred, dist
+        dist = dist + dist[pred]
+        pred = far
+
+
+def _depths(up):
+    """Depth of every entry of the forest given by ``up`` (1-based parent, 0 = top; index 0
+    unused): 0 at the top, by relaxation from the top down."""
+    depth = np.zeros(up.shape[0], dtype=np.int64)
+    inner = up > 0
+    for _ in range(up.shape[0]):
+        nd = np.where(inner, depth[up] + 1, 0)
+        if np.array_equal(nd, depth):
+            return depth
+        depth = nd
+    raise ValueError("the node links contain a cycle")
+
+
+def _children(node_0, nnodes):
+    """Every node's subnodes in the order of its sister chain (firstchild, then sister
+    links, as the reference walks them): (owner, child) pairs sorted by owner and chain
+    position.  Top-level nodes, which hang off the unexported root, have no owner."""
+    fc = node_0["firstchild"].astype(np.int64)
+    sis = node_0["sister"].astype(np.int64)
+    if np.any(fc[1:] > nnodes) or np.any(sis[1:] > nnodes) or np.any(sis[1:] < 0):
+        raise ValueError("firstchild / sister links outside the node table")
+    head, pos = _chain_positions(np.where(np.arange(nnodes + 1) > 0, sis, 0))
+    owner_of_head = np.zeros(nnodes + 1, dtype=np.int64)
+    starts = np.nonzero((fc > 0) & (np.arange(nnodes + 1) > 0))[0]
+    owner_of_head[fc[starts]] = starts
+    owner = owner_of_head[head]
+    owner[0] = 0
+    kid = np.nonzero(owner > 0)[0]
+    order = np.lexsort((pos[kid], owner[kid]))
+    return owner[kid][order], kid[order], owner
+
+
 def select_with_ms_method(node_0, strict=False):
     """MSM selection (select_with_MS_method_1319, compute_neiKDtree_mod.py:260-464) on a
-    ``node_0`` table (see :func:`node_table`), updated in place as the reference does: node
-    masses first lose their subnodes' masses (:266-280), then every node absorbs its most
-    massive subnode (by truemass, first on ties) and takes its densmax (:302-319).  Each halo
-    (mother <= 0) and each subnode that is not its mother's most massive starts a structure;
-    a most massive subnode joins its mother's (:322-346).  Then the structure tree (:398-429).
-    ``strict`` adds the reference's count check (:349-353).  Raises ValueError where the
-    reference does (a node left with no particles of its own)."""
+    ``node_0`` table (see :func:`node_table`), updated in place with the reference's results:
+    each node first keeps only its own particles (its inclusive mass and truemass less its
+    subnodes', :266-280; truemass subtracted subnode by subnode in sister order, which fixes
+    the rounding), then, from the deepest nodes up, absorbs its most massive subnode's
+    main-branch mass and truemass and takes its densmax (:300-319; the first of the largest
+    truemass in sister order).  A structure starts at every halo (mother <= 0) and at every
+    subnode that is not its mother's most massive; the others continue their mother's
+    structure (:321-346).  Structures are numbered by the node id of their first node, and the
+    structure tree lists halos and each structure's daughters in that order (:398-429).
+    ``strict`` adds a count check (:349-353).  Raises ValueError where the reference does (a
+    node left with no particles of its own).  The table must list every mother before its
+    subnodes, as create_group_tree's deterministic reorder does (levels ascending)."""
     node_0 = node_0.view(NODE_DTYPE) if node_0.dtype != NODE_DTYPE else node_0
     nnodes = len(node_0) - 1
     if nnodes < 1:
         raise ValueError("no nodes")
-    # own masses: subtract the subnodes' (inclusive) masses, nodes in ascending order (:266-280)
-    where = np.nonzero(node_0["firstchild"] > 0)[0]
-    if len(where) and where[0] == 0:
-        where = where[1:]
-    for inode1 in where:
-        isub1 = int(node_0["firstchild"][inode1])
-        while isub1 > 0:
-            node_0["mass"][inode1] -= node_0["mass"][isub1]
-            node_0["truemass"][inode1] -= node_0["truemass"][isub1]
-            isub1 = int(node_0["sister"][isub1])
-        if node_0["mass"][inode1] <= 0 or node_0["truemass"][inode1] <= 0.0:
-            raise ValueError(f"node {inode1} has mass {node_0['mass'][inode1]}, truemass "
-                             f"{node_0['truemass'][inode1]} after removing its subnodes")
-    # most massive subnode of every node, nodes in descending order (:300-319)
-    mostmasssub = np.full(nnodes, -1, dtype=np.int32)
-    firstchild = node_0["firstchild"]
-    for inode1 in range(nnodes, 0, -1):
-        isub1 = int(firstchild[inode1])
-        if isub1 > 0:
-            maxmasssub = node_0["truemass"][isub1]
-            mostmasssub[inode1 - 1] = isub1
-            isub1 = int(node_0["sister"][isub1])
-        while isub1 > 0:
-            if node_0["truemass"][isub1] > maxmasssub:
-                maxmasssub = node_0["truemass"][isub1]
-                mostmasssub[inode1 - 1] = isub1
-            isub1 = int(node_0["sister"][isub1])
-        if mostmasssub[inode1 - 1] > 0:
-            m = int(mostmasssub[inode1 - 1])
-            node_0["mass"][inode1] += node_0["mass"][m]
-            node_0["truemass"][inode1] += node_0["truemass"][m]
-            node_0["densmax"][inode1] = node_0["densmax"][m]
-    # structure of every node (:321-346): starters are halos and non-most-massive subnodes;
-    # a most massive subnode points to its mother, and pointer jumping finds the starter
-    mothers = node_0["mother"][1:nnodes + 1].astype(np.int64)
-    indices = np.arange(1, nnodes + 1)
-    is_halo = mothers <= 0
-    valid_mother_idx = np.where(~is_halo, mothers - 1, 0)
-    is_most_mass = (~is_halo) & (indices == mostmasssub[valid_mother_idx])
-    is_new_sub = (~is_halo) & (~is_most_mass)
-    nb_halos = int(np.sum(is_halo))
-    nb_sub = int(np.sum(is_new_sub))
-    is_starter = is_halo | is_new_sub
-    starter_ids = np.zeros(nnodes, dtype=np.int64)
-    starter_ids[is_starter] = np.cumsum(is_starter)[is_starter]
-    parent = indices - 1
-    parent[is_most_mass] = mothers[is_most_mass] - 1
-    while True:
-        new_parent = parent[parent]
-        if np.array_equal(parent, new_parent):
-            break
-        parent = new_parent
-    node_to_struct = starter_ids[parent].astype(np.int32)
+    owner_k, kid, owner = _children(node_0, nnodes)
+    mother = node_0["mother"].astype(np.int64)
+    mother[0] = 0
+    ids = np.arange(nnodes + 1)
+    if np.any(owner_k >= kid) or np.any((mother > 0) & (mother >= ids)) or np.any(mother > nnodes):
+        raise ValueError("node tables must list every mother before its subnodes")
+
+    # own particles: [inclusive, subnode 1, subnode 2, ...] folded left by subtraction per
+    # owner (ufunc reduceat folds left, the order of the reference's in-place updates)
+    owners = np.unique(owner_k)
+    seg_val_m = np.concatenate([node_0["mass"][owners].astype(np.int64),
+                                node_0["mass"][kid].astype(np.int64)])
+    seg_val_t = np.concatenate([node_0["truemass"][owners], node_0["truemass"][kid]])
+    seg_key = np.concatenate([owners, owner_k])
+    seg_rank = np.concatenate([np.full(owners.shape[0], -1), np.arange(kid.shape[0])])
+    seg = np.lexsort((seg_rank, seg_key))
+    first = np.r_[0, np.nonzero(np.diff(seg_key[seg]))[0] + 1] if seg.size else np.zeros(0, int)
+    own_m = node_0["mass"].astype(np.int64)
+    own_t = node_0["truemass"].astype(np.float64)
+    if owners.size:
+        own_m[owners] = np.subtract.reduceat(seg_val_m[seg], first)
+        own_t[owners] = np.subtract.reduceat(seg_val_t[seg], first)
+        bad = owners[(own_m[owners] <= 0) | (own_t[owners] <= 0.0)]
+        if bad.size:
+            b = int(bad[0])
+            raise ValueError(f"node {b} has mass {own_m[b]}, truemass {own_t[b]} after "
+                             f"removing its subnodes")
+
+    # main branches, deepest owners first: a node's main-branch truemass is its own plus that
+    # of its most massive subnode (the first maximum in sister order), the same for mass,
+    # and densmax is taken from that subnode
+    tot_m, tot_t = own_m.copy(), own_t.copy()
+    dmax = node_0["densmax"].astype(np.float64)
+    best = np.zeros(nnodes + 1, dtype=np.int64)
+    depth = _depths(owner)
+    kid_depth = depth[kid]
+    for d in range(int(depth.max()) if nnodes else 0, 0, -1):
+        sel = np.nonzero(kid_depth == d)[0]          # sister-ordered subnodes at depth d
+        if sel.size == 0:
+            continue
+        o, c = owner_k[sel], kid[sel]
+        bounds = np.r_[0, np.nonzero(np.diff(o))[0] + 1]
+        seg_max = np.maximum.reduceat(tot_t[c], bounds)
+        lens = np.diff(np.r_[bounds, sel.size])
+        at_max = tot_t[c] == np.repeat(seg_max, lens)
+        pick = np.minimum.reduceat(np.where(at_max, np.arange(sel.size), sel.size), bounds)
+        mo, mc = o[bounds], c[pick]
+        best[mo] = mc
+        tot_m[mo] = own_m[mo] + tot_m[mc]
+        tot_t[mo] = own_t[mo] + tot_t[mc]
+        dmax[mo] = dmax[mc]
+    node_0["mass"][1:] = tot_m[1:]
+    node_0["truemass"][1:] = tot_t[1:]
+    node_0["densmax"][1:] = dmax[1:]
+
+    # structures: numbered by their first node; a node that is its mother's most massive
+    # subnode continues its mother's structure, every other node starts one
+    is_halo = mother <= 0
+    continues = (~is_halo) & (best[np.where(is_halo, 0, mother)] == ids)
+    continues[0] = False
+    starts_s = ~continues
+    starts_s[0] = False
+    sid = np.cumsum(starts_s) * starts_s
+    struct = np.where(starts_s, sid, 0)
+    mdepth = _depths(mother)
+    for d in range(1, int(mdepth.max()) + 1):
+        lvl = np.nonzero((mdepth == d) & continues)[0]
+        struct[lvl] = struct[mother[lvl]]
+    node_to_struct = struct[1:].astype(np.int32)
+    nb_halos = int(np.count_nonzero(is_halo[1:]))
+    nb_sub = int(np.count_nonzero(starts_s)) - nb_halos
     nstruct = nb_halos + nb_sub
-    if strict and nstruct != int(starter_ids.max()):
-        raise ValueError(f"structure count {nstruct} != {int(starter_ids.max())}")
-    # structure tree, nodes in ascending order (:398-429)
-    mother = np.full(nstruct, -1, dtype=np.int32)
+    if np.any(node_to_struct <= 0) or (strict and nstruct != int(node_to_struct.max())):
+        raise ValueError("error in the structure count")
+
+    # structure tree from the (mother structure, structure) pairs, in structure order:
+    # halos chained one after the other, daughters listed under their mother
+    first_node = np.nonzero(starts_s)[0]                 # ascending, so structure order
+    smother = np.where(is_halo[first_node], 0, struct[np.where(is_halo, 0, mother)][first_node])
+    smother = smother.astype(np.int64)
+    up = np.r_[0, smother]
+    level = (_depths(up)[1:] + 1).astype(np.int32)
     first_sister = np.full(nstruct, -1, dtype=np.int32)
     first_daughter = np.full(nstruct, -1, dtype=np.int32)
-    level = np.zeros(nstruct, dtype=np.int32)
-    ihalo1 = -1
-    for inode0 in range(nnodes):
-        istruct1 = int(node_to_struct[inode0])
-        if istruct1 <= 0:
-            raise ValueError("node without a structure")
-        if mother[istruct1 - 1] < 0:
-            if node_0["mother"][inode0 + 1] <= 0:      # a main halo
-                mother[istruct1 - 1] = 0
-                if ihalo1 > 0:
-                    first_sister[ihalo1 - 1] = istruct1
-                ihalo1 = istruct1
-                level[istruct1 - 1] = 1
-            else:                                        # a subhalo
-                imother1 = int(node_to_struct[node_0["mother"][inode0 + 1] - 1])
-                mother[istruct1 - 1] = imother1
-                level[istruct1 - 1] = level[imother1 - 1] + 1
-                if first_daughter[imother1 - 1] <= 0:
-                    first_daughter[imother1 - 1] = istruct1
-                else:                                    # last of its mother's daughters
-                    isub1 = int(first_daughter[imother1 - 1])
-                    while first_sister[isub1 - 1] > 0:
-                        isub1 = int(first_sister[isub1 - 1])
-                    first_sister[isub1 - 1] = istruct1
-    return MSMStructures(node_to_struct, nb_halos, nb_sub, mother, first_sister,
-                         first_daughter, level)
+    order = np.lexsort((np.arange(1, nstruct + 1), smother))
+    sm_sorted = smother[order]
+    same = sm_sorted[1:] == sm_sorted[:-1]
+    first_sister[order[:-1][same]] = (order[1:][same] + 1).astype(np.int32)
+    lead = np.r_[True, ~same]
+    heads = order[lead]
+    sub_heads = heads[smother[heads] > 0]
+    first_daughter[smother[sub_heads] - 1] = (sub_heads + 1).astype(np.int32)
+    return MSMStructures(node_to_struct, nb_halos, nb_sub, smother.astype(np.int32),
+                         first_sister, first_daughter, level)
 
 
 def select_halos(node_0):
     """HOP selection (select_halos_1317, compute_neiKDtree_mod.py:228-252): every node
-    belongs to its top-level ancestor (mother 0), found by walking mothers and reusing the
-    ancestors already found, as the reference does.  Returns (node_to_halo (nnodes,) int32,
-    nb_halos).  Top-level node ids must be 1..nb_halos, else ValueError as the reference's
-    relabelling raises (:246-249): the particle labels are these node ids."""
+    belongs to its top-level ancestor (the node reached by following mothers up to one with
+    mother 0), whose node id labels its particles.  Returns (node_to_halo (nnodes,) int32,
+    nb_halos), nb_halos counting the nodes with mother <= 0.  A label outside 1..nb_halos is
+    a ValueError, as in the reference's relabelling (:246-249)."""
     node_0 = node_0.view(NODE_DTYPE) if node_0.dtype != NODE_DTYPE else node_0
     nnodes = len(node_0) - 1
-    mother = node_0["mother"]
-    node_to_halo = np.full(nnodes, -1, dtype=np.int32)
-    nb_halos = 0
-    for inode1 in range(1, nnodes + 1):
-        if mother[inode1] <= 0:
-            nb_halos += 1
-        ih1 = inode1
-        while mother[ih1] != 0:
-            ih1 = int(mother[ih1]) if node_to_halo[ih1 - 1] <= 0 else int(node_to_halo[ih1 - 1])
-        node_to_halo[inode1 - 1] = ih1
+    mother = node_0["mother"].astype(np.int64)
+    mother[0] = 0
+    if np.any(mother < 0) or np.any(mother > nnodes):
+        raise ValueError("mother links outside the node table")
+    depth = _depths(mother)
+    top = np.arange(nnodes + 1)
+    for d in range(1, int(depth.max()) + 1 if nnodes else 1):
+        lvl = np.nonzero(depth == d)[0]
+        top[lvl] = top[mother[lvl]]
+    node_to_halo = top[1:].astype(np.int32)
+    nb_halos = int(np.count_nonzero(node_0["mother"][1:] <= 0))
     if np.any(node_to_halo > nb_halos) or np.any(node_to_halo <= 0):
         raise ValueError("ihalo > nb_of_halos or nil")
     return node_to_halo, nb_halos

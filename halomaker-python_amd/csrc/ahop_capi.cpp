@@ -549,8 +549,8 @@ int ahop_ramses_read(const char *repository, const char *nchar, int32_t ncpu, in
                      int32_t family, int32_t stellar, int32_t nthreads, int64_t ndm_total,
                      int64_t nselected_total, int64_t nstar_total, double scale_l,
                      double scale_t, double *pos, double *vel, double *mass,
-                     int64_t *source_pid, double *birth, double *metal,
-                     ahop_ramses_counts *per_file)
+                     int64_t *source_pid, double *birth, double *metal, double *m0,
+                     double *chem, ahop_ramses_counts *per_file)
 {
     return guarded([&] {
         RamsesArgs A = ramses_args(repository, nchar, ncpu, rver, family, stellar, nthreads);
@@ -565,10 +565,24 @@ int ahop_ramses_read(const char *repository, const char *nchar, int32_t ncpu, in
         A.source_pid = source_pid;
         A.birth = birth;
         A.metal = metal;
+        A.m0 = m0;
+        A.chem = chem;
         std::vector<RamsesFileCount> c(ncpu > 0 ? ncpu : 0);
         ramses_read(A, c.data());
         if (per_file)
             for (int i = 0; i < ncpu; ++i) per_file[i] = {c[i].ndm, c[i].nstar, c[i].nselected};
+    });
+}
+
+int ahop_ramses_descriptor(const char *repository, int32_t *present, int32_t *has_m0,
+                           int32_t *chem_mask)
+{
+    return guarded([&] {
+        if (!repository) throw std::runtime_error("null path");
+        const RamsesDescriptorInfo d = ramses_descriptor(repository);
+        if (present) *present = d.present ? 1 : 0;
+        if (has_m0) *has_m0 = d.has_m0 ? 1 : 0;
+        if (chem_mask) *chem_mask = d.chem_mask;
     });
 }
 
@@ -734,6 +748,11 @@ int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double 
         P.dcell_min = params->dcell_min;
         P.eps_sc = params->eps_sc;
         P.light = params->light;
+        P.pe_baseline = params->virial_pe_profile;
+        P.pe_norm_legacy = params->virial_pe_norm_policy;
+        P.shell_half = params->virial_shell_self_mode;
+        for (int v : {P.pe_baseline, P.pe_norm_legacy, P.shell_half})
+            if (v != 0 && v != 1) throw std::runtime_error("virial switches must be 0 or 1");
         if (!(P.lbox > 0)) throw std::runtime_error("lbox must be > 0");
         Engine &e = engine();
         HaloTuning t = halo_props_device(offsets, nhalos, d_pos, d_vel, d_mass, d_density, P,

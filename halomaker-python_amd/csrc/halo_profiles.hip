@@ -21,13 +21,14 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "ahop_common.h"
+#include "halo_props.h"
 
 namespace ahop {
 
 namespace {
 
-constexpr int PROPS_STRIDE = 29;     // halo property row (ahop_halo_props_device)
-constexpr int P_PX = 1, P_RVIR = 24;
+constexpr int PROPS_STRIDE = HP_NFIELDS;   // halo property row (ahop_halo_props_device)
+constexpr int P_PX = HP_PX, P_RVIR = HP_RVIR;
 constexpr int NPROF = 6;             // vmaxcir, rmaxcir, cNFW, cNFWerr, inslope, inslopeerr
 constexpr int NAUX = 4;              // first non-zero slot, members inside rvir, d0, d1
 constexpr int MAXBINS = 80;

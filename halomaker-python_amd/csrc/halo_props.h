@@ -10,7 +10,7 @@ namespace ahop {
 enum HaloField {
     HP_M = 0, HP_PX, HP_PY, HP_PZ, HP_VX, HP_VY, HP_VZ, HP_SIGMA, HP_LX, HP_LY, HP_LZ, HP_R,
     HP_IXX, HP_IXY, HP_IXZ, HP_IYY, HP_IYZ, HP_IZZ, HP_SHA, HP_SHB, HP_SHC, HP_EK, HP_EP, HP_ET,
-    HP_RVIR, HP_MVIR, HP_CVEL, HP_TVIR, HP_SPIN, HP_NFIELDS
+    HP_RVIR, HP_MVIR, HP_CVEL, HP_TVIR, HP_SPIN, HP_RHO0, HP_RC, HP_NFIELDS
 };
 
 struct HaloParamsDev {
@@ -27,7 +27,12 @@ struct HaloParamsDev {
     double dcell_min = 0.0;     // smallest cell / sphere radius of the recentring
     double eps_sc = 0.0;        // sphere radius shrink fraction of SC
     int light = 0;              // 1: mass, centre, mean velocity, sigma, L, r, I, axes, ek only
-                                // (ep, et, rvir, mvir, cvel, tvir, spin left NaN)
+                                // (ep, et, rvir, mvir, cvel, tvir, spin, rho_0, r_c left NaN)
+    // the virial potential profile's switches (potential_profile_1b40 :1065-1123); 0 is the
+    // reference's default of each
+    int pe_baseline = 0;        // virial_pe_profile: 0 'shell', 1 'baseline' (-3/5 G M(<r)^2 RF)
+    int pe_norm_legacy = 0;     // virial_pe_norm_policy: 0 'profile_outer_for_large', 1 'legacy'
+    int shell_half = 0;         // virial_shell_self_mode: 0 'count', 1 'half_shell'
 };
 
 struct HaloTuning {

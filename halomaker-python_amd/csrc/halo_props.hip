@@ -969,6 +969,31 @@ __device__ void seq_cumsum(double *a, int n)
 
 constexpr int WAVE_HP = 64;   // first thread of the second wave of a workgroup
 
+// Carlson's symmetric elliptic integral RF(x, y, z) by duplication (B. C. Carlson, "Numerical
+// computation of real or complex elliptic integrals", Numer. Algorithms 10 (1995) 13-26,
+// eqs. 2.2-2.4): the arguments are pulled together until the fifth-order series about their
+// mean holds to double precision.  The reference evaluates it with scipy.special.elliprf.
+__device__ double carlson_rf(double x, double y, double z)
+{
+    double a = (x + y + z) / 3.0;
+    const double a0 = a, x0 = x, y0 = y;
+    // (3 r)^(-1/6) with r = 2^-53 bounds the truncation of the series at r
+    double q = 4.4e2 * fmax(fabs(a - x), fmax(fabs(a - y), fabs(a - z)));
+    double f = 1.0;   // 4^-n
+    for (int n = 0; n < 64 && q * f >= fabs(a); ++n) {
+        const double sx = sqrt(x), sy = sqrt(y), sz = sqrt(z);
+        const double lam = sx * sy + sx * sz + sy * sz;
+        x = 0.25 * (x + lam);
+        y = 0.25 * (y + lam);
+        z = 0.25 * (z + lam);
+        a = 0.25 * (a + lam);
+        f *= 0.25;
+    }
+    const double X = (a0 - x0) * f / a, Y = (a0 - y0) * f / a, Z = -(X + Y);
+    const double e2 = X * Y - Z * Z, e3 = X * Y * Z;
+    return (1.0 - e2 / 10.0 + e3 / 14.0 + e2 * e2 / 24.0 - 3.0 * e2 * e3 / 44.0) / sqrt(a);
+}
+
 // ---- shell tables, potential profile and the virial scan, one workgroup per halo
 template <int BS>
 __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first,
@@ -1005,28 +1030,43 @@ __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first
     // shell potential terms in parallel (potential_profile_1b40 :1090-1103): dm from the
     // cumulative table (np.diff with prepend 0), mass before the shell = cumulative - dm
     const double reff0 = pow(S.amax * S.bmax * S.cmax, 1.0 / 3.0);
-    for (int b = threadIdx.x; b < nr; b += BS) {
-        const double cum = tm[b];
-        const double dm = cum - (b > 0 ? tm[b - 1] : 0.0);
-        const double before = cum - dm;
-        const double sf = cn[b] > 0 ? 0.5 * (cn[b] - 1.0) / cn[b] : 0.0;
-        const double scale = (b == 0) ? 0.5 / (double)(nr - 1) : (double)b / (double)(nr - 1);
-        tp[b] = (before + sf * dm) * dm / (scale * reff0);
+    double *o = out + (int64_t)h * HP_NFIELDS;
+    // RF of the axes (det_halo_energies_1b3 :1409-1412 and the 'baseline' profile :1085-1087)
+    const double rf = (P.pe_baseline || P.pe_norm_legacy)
+                          ? carlson_rf(o[HP_SHA] * o[HP_SHA], o[HP_SHB] * o[HP_SHB],
+                                       o[HP_SHC] * o[HP_SHC])
+                          : 0.0;
+    if (P.pe_baseline) {
+        // -3/5 G M(<r)^2 RF, the profile itself (no cumulative sum)
+        for (int b = threadIdx.x; b < nr; b += BS) tp[b] = -3.0 / 5.0 * P.gravconst * (tm[b] * tm[b]) * rf;
+    } else {
+        for (int b = threadIdx.x; b < nr; b += BS) {
+            const double cum = tm[b];
+            const double dm = cum - (b > 0 ? tm[b - 1] : 0.0);
+            const double before = cum - dm;
+            const double sf = P.shell_half ? 0.5 : (cn[b] > 0 ? 0.5 * (cn[b] - 1.0) / cn[b] : 0.0);
+            const double scale = (b == 0) ? 0.5 / (double)(nr - 1) : (double)b / (double)(nr - 1);
+            tp[b] = (before + sf * dm) * dm / (scale * reff0);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) seq_cumsum(tp, nr);
     }
     __syncthreads();
-    if (threadIdx.x == 0) seq_cumsum(tp, nr);
-    __syncthreads();
     const int64_t count = off[h + 1] - off[h];
-    const double outer = -P.gravconst * tp[nr - 1];
-    double *o = out + (int64_t)h * HP_NFIELDS;
+    // the profile's outer value: -G times the cumulative sum, or the baseline profile's last
+    const double gp = P.pe_baseline ? 1.0 : -P.gravconst;
+    const double outer = gp * tp[nr - 1];
     const bool small = count < P.direct_pe_nmax;
-    const double ep = small ? ep_direct[h] : outer;
-    const bool renorm = small && ep != outer && outer != 0.0;
+    // the halo's potential energy: the direct pair sum below the pair-sum limit; above it the
+    // profile's outer value ('profile_outer_for_large') or the RF estimate ('legacy')
+    const double ep = small ? ep_direct[h]
+                            : (P.pe_norm_legacy ? -3.0 / 5.0 * P.gravconst * (S.m * S.m) * rf : outer);
+    const bool renorm = (small || P.pe_norm_legacy) && ep != outer && outer != 0.0;
     __syncthreads();
-    // profile (-G * cumulative terms, rescaled to the direct energy for small halos) and the
+    // profile (rescaled to the halo's potential energy where the policy says so) and the
     // virialisation ratio |2K+W|/|K+W| of each shell; -1 marks an unbound shell (K+W >= 0)
     for (int b = threadIdx.x; b < nr; b += BS) {
-        double p = -P.gravconst * tp[b];
+        double p = gp * tp[b];
         if (renorm) p = p / outer * ep;
         tp[b] = p;
         const double kw = tk[b] + p;
@@ -1128,8 +1168,12 @@ __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first
         o[HP_MVIR] = mvir;
         o[HP_CVEL] = sqrt(P.gravconst * mvir / rv);
         o[HP_TVIR] = 35.9 * P.gravconst * mvir / rv;
+        // the truncated isothermal sphere profile of compute_halo_profile_1b41 (:1232-1238),
+        // defined at rvir
+        o[HP_RHO0] = mvir / (4.0 * PI * pow(rv, 3.0));
+        o[HP_RC] = rv;
     } else {
-        o[HP_RVIR] = o[HP_MVIR] = o[HP_CVEL] = o[HP_TVIR] = NAN;
+        o[HP_RVIR] = o[HP_MVIR] = o[HP_CVEL] = o[HP_TVIR] = o[HP_RHO0] = o[HP_RC] = NAN;
     }
     double hl = sqrt(o[HP_LX] * o[HP_LX] + o[HP_LY] * o[HP_LY] + o[HP_LZ] * o[HP_LZ]);
     o[HP_SPIN] = hl * sqrt(fabs(et)) / pow(S.m, 2.5) / P.gravconst;

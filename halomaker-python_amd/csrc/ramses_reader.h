@@ -21,7 +21,22 @@ struct RamsesArgs {
     double *pos = nullptr, *vel = nullptr, *mass = nullptr;   // nselected x 3, x 3, x 1
     int64_t *source_pid = nullptr;                            // family dm / star
     double *birth = nullptr, *metal = nullptr;                // nstar slots
+    // Ra4 with a field descriptor: initial masses (nstar slots) and abundances (nstar x
+    // RAMSES_NCHEM, in RAMSES_CHEM order); entries the files do not provide are left as they are
+    double *m0 = nullptr, *chem = nullptr;
 };
+
+// the chemical elements a Ra4 descriptor may list as chem_<element> (src/chem_species.py:1)
+constexpr int RAMSES_NCHEM = 9;
+extern const char *const RAMSES_CHEM[RAMSES_NCHEM];
+
+// What the Ra4 field descriptor (part_file_descriptor.txt) of an output provides for stars:
+// present (a descriptor was found), initial masses, and a bit per RAMSES_CHEM element.
+struct RamsesDescriptorInfo {
+    bool present = false, has_m0 = false;
+    int chem_mask = 0;
+};
+RamsesDescriptorInfo ramses_descriptor(const std::string &repository);
 
 void ramses_count(const RamsesArgs &A, RamsesFileCount *per_file);
 void ramses_read(const RamsesArgs &A, RamsesFileCount *per_file);

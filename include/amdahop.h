@@ -159,8 +159,10 @@ int ahop_group_members_device(const int32_t *d_node_of_part, int64_t n,
 
 /* Row layout of the output (one row of AHOP_HALO_NFIELDS doubles per halo):
  * m, px, py, pz, vx, vy, vz, sigma, Lx, Ly, Lz, r, Ixx, Ixy, Ixz, Iyy, Iyz, Izz,
- * sha, shb, shc, ek, ep, et, rvir, mvir, cvel, tvir, spin */
-#define AHOP_HALO_NFIELDS 29
+ * sha, shb, shc, ek, ep, et, rvir, mvir, cvel, tvir, spin, rho_0, r_c
+ * (rho_0, r_c: the truncated isothermal sphere of compute_halo_profile_1b41,
+ * src/compute_halo_props.py:1232-1238) */
+#define AHOP_HALO_NFIELDS 31
 
 typedef struct {
     double lbox;            /* Lbox_pt, periodic box side in the members' length unit */
@@ -178,8 +180,14 @@ typedef struct {
     double dcell_min;       /* dcell_min: smallest cell / sphere radius, > 0 when SC or DPMMC */
     double eps_sc;          /* eps_SC: sphere shrink fraction, in (0, 1) when SC */
     int32_t light;          /* 1: skip the shell tables, energies and virial scan (ep, et,
-                               rvir, mvir, cvel, tvir, spin read NaN) */
+                               rvir, mvir, cvel, tvir, spin, rho_0, r_c read NaN) */
+    /* switches of the virial potential profile (potential_profile_1b40,
+     * src/compute_halo_props.py:1065-1123), 0 = the reference's default */
+    int32_t virial_pe_profile;       /* 0 'shell', 1 'baseline' (-3/5 G M(<r)^2 RF(a^2,b^2,c^2)) */
+    int32_t virial_pe_norm_policy;   /* 0 'profile_outer_for_large', 1 'legacy' */
+    int32_t virial_shell_self_mode;  /* 0 'count', 1 'half_shell' */
 } ahop_halo_params;
+/* lbox = +inf turns the periodic wrap off everywhere (FlagPeriod = 0, :862 and :889). */
 
 typedef struct {
     int32_t tile, workgroup, trials;
@@ -278,8 +286,21 @@ int ahop_ramses_read(const char *repository, const char *nchar, int32_t ncpu, in
                      int32_t family, int32_t stellar, int32_t nthreads, int64_t ndm_total,
                      int64_t nselected_total, int64_t nstar_total, double scale_l,
                      double scale_t, double *pos, double *vel, double *mass,
-                     int64_t *source_pid, double *birth, double *metal,
-                     ahop_ramses_counts *per_file);
+                     int64_t *source_pid, double *birth, double *metal, double *m0,
+                     double *chem, ahop_ramses_counts *per_file);
+
+/* The Ra4 field descriptor of an output (repository/part_file_descriptor.txt, parsed as
+ * _parse_ra4_part_descriptor and _ra4_stellar_descriptor do, src/input_output.py:530-576):
+ * *present = 1 when the file exists, *has_m0 = 1 when it lists initial_mass (then
+ * ahop_ramses_read fills m0, nstar_total slots), and *chem_mask bit e when it lists
+ * chem_<element e> for the elements H, O, Fe, Mg, C, N, Si, S, D (src/chem_species.py:1; then
+ * ahop_ramses_read fills chem, nstar_total x 9 row-major, column e).  With a descriptor the
+ * stellar records after the tag are read where it places them and the other fields skipped
+ * (_read_ra4_stellar_record_block, :580-620); without one, birth times and metallicities are
+ * the two records after the tag.  The reference's errors: initial_mass without metallicity,
+ * or before it. */
+int ahop_ramses_descriptor(const char *repository, int32_t *present, int32_t *has_m0,
+                           int32_t *chem_mask);
 
 /* Host structure-tree stage alone (create_group_tree f90:885-983) on host inputs, for
  * callers that computed the hot-path stages elsewhere.  Outputs: liste_parts (npart),

@@ -1,11 +1,12 @@
 """NumPy restatement of the per-halo property reductions -- TEST INFRASTRUCTURE ONLY.
 
 Follows reference src/compute_halo_props.py per function (cited below) for one halo
-given its members (positions, velocities, masses, densities) in physical units, in the
-configuration the hot path covers: periodic box (FlagPeriod=1), no stellar payload, an
-AdaptaHOP selection method (not FOF, so refine_centers_1b :1555-1596 recentres with the
-DPMMC and SC options when they are on), virial potential profile 'shell' with self mode
-'count' and norm policy 'profile_outer_for_large' (the defaults at :1035-1047).
+given its members (positions, velocities, masses, densities) in physical units: no stellar
+payload, an AdaptaHOP selection method (not FOF, so refine_centers_1b :1555-1596 recentres
+with the DPMMC and SC options when they are on), FlagPeriod 1 (periodic, the default) or 0,
+and the virial profile switches of potential_profile_1b40 (:1036-1123): profile 'shell'
+(default) or 'baseline', self mode 'count' (default) or 'half_shell', norm policy
+'profile_outer_for_large' (default) or 'legacy'.
 
 Each reference function has its own function here so that each can be pinned on its
 own; :func:`halo_props` composes them in the reference's order and returns one float64
@@ -26,7 +27,7 @@ import numpy as np
 
 FIELDS = ["m", "px", "py", "pz", "vx", "vy", "vz", "sigma", "Lx", "Ly", "Lz", "r",
           "Ixx", "Ixy", "Ixz", "Iyy", "Iyz", "Izz", "sha", "shb", "shc", "ek", "ep", "et",
-          "rvir", "mvir", "cvel", "tvir", "spin"]
+          "rvir", "mvir", "cvel", "tvir", "spin", "rho_0", "r_c"]
 F = {k: i for i, k in enumerate(FIELDS)}
 
 
@@ -188,15 +189,24 @@ def main_axis(I, m):
     return np.sqrt(d / m), v
 
 
-def direct_pe(pos, mass, L, gravconst):
-    """det_halo_energies_1b3 :1417-1427: pair sum over i<j of -m_i m_j / (d_ij/L) * G/L."""
+def direct_pe(pos, mass, L, gravconst, periodic=True):
+    """det_halo_energies_1b3 :1417-1427: pair sum over i<j of -m_i m_j / (d_ij/L) * G/L
+    (the wrap only with FlagPeriod != 0)."""
     n = mass.shape[0]
     i, j = np.triu_indices(n, 1)
     drs = pos[j] - pos[i]
-    drs[drs > L / 2] -= L
-    drs[drs < -L / 2] += L
+    if periodic:
+        drs[drs > L / 2] -= L
+        drs[drs < -L / 2] += L
     dists = np.sqrt(np.sum(drs ** 2, axis=1)) / L
     return np.sum(-(mass[i] * mass[j]) / dists) * gravconst / L
+
+
+def rf_pe(m, axes, gravconst):
+    """det_halo_energies_1b3 :1407-1412: -3/5 G m^2 RF(a^2, b^2, c^2) (scipy's elliprf, as
+    num_rec.rf)."""
+    from scipy.special import elliprf
+    return (-3 / 5) * gravconst * m ** 2 * elliprf(axes[0] ** 2, axes[1] ** 2, axes[2] ** 2)
 
 
 def kinetic(pos, vel, mass, c, vbar, L, hub_pt):
@@ -237,15 +247,22 @@ def shell_tables(pos, mass, v2, c, axes, vecs, L, nr):
     return np.cumsum(tabm), np.cumsum(tabk), cnt, amax, bmax, cmax
 
 
-def potential_profile(tabm, cnt, amax, bmax, cmax, gravconst):
-    """potential_profile_1b40 :1069-1123, mode 'shell', self mode 'count' (before the
-    outer-value normalisation)."""
+def potential_profile(tabm, cnt, amax, bmax, cmax, gravconst, mode="shell", self_mode="count",
+                      axes=None):
+    """potential_profile_1b40 :1069-1107 (before the outer-value normalisation): mode 'shell'
+    with self mode 'count' or 'half_shell', or mode 'baseline'."""
     nr = tabm.shape[0]
+    if mode == "baseline":
+        from scipy.special import elliprf
+        return -3 / 5 * gravconst * tabm ** 2 * elliprf(axes[0] ** 2, axes[1] ** 2, axes[2] ** 2)
     dm = np.diff(tabm, prepend=0.0)
     mass_before = tabm - dm
-    self_factor = np.zeros(nr)
-    occ = cnt > 0
-    self_factor[occ] = 0.5 * (cnt[occ] - 1.0) / cnt[occ]
+    if self_mode == "half_shell":
+        self_factor = 0.5
+    else:
+        self_factor = np.zeros(nr)
+        occ = cnt > 0
+        self_factor[occ] = 0.5 * (cnt[occ] - 1.0) / cnt[occ]
     r_eff = shell_scale(nr) * (amax * bmax * cmax) ** (1.0 / 3.0)
     return -gravconst * np.cumsum((mass_before + self_factor * dm) * dm / r_eff)
 
@@ -294,13 +311,17 @@ def vir_props(tabm, tabk, tabp, amax, bmax, cmax, vir_overdens, rho_mean):
 
 def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst=430.1,
                vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000,
-               sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0):
+               sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0, flag_period=1,
+               virial_pe_profile="shell", virial_pe_norm_policy="profile_outer_for_large",
+               virial_shell_self_mode="count"):
     """All properties of one halo, composed as det_vir_1b :1241-1259 and the property loop
     (det_mass_17, det_center_18, refine_centers_1b, compute_ang_mom_19, r_halos_1a,
     det_main_axis_1b2, det_halo_energies_1b3, det_vir_props_1b4, compute_spin_parameter_1c
     :1599-1602; the order of _compute_halo_props :270-290)."""
     out = np.full(len(FIELDS), np.nan)
-    L = lbox
+    # FlagPeriod = 0: correct_for_periodicity returns its input (:862, :889), which the strict
+    # comparisons with L/2 reproduce for an infinite L; the pair sum drops its wrap
+    L = lbox if flag_period else np.inf
     n = mass.shape[0]
     m = np.sum(mass)                                          # det_mass_17 :754-780
     out[F["m"]] = m
@@ -320,12 +341,17 @@ def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst
     out[F["sha"]], out[F["shb"]], out[F["shc"]] = axes
     # det_halo_energies_1b3: pair sum below direct_pe_nmax; the RF estimate of larger halos
     # is overwritten by the profile's outer value under the default norm policy
-    ep = direct_pe(pos, mass, L, gravconst) if n < direct_pe_nmax else np.nan
+    ep = (direct_pe(pos, mass, lbox, gravconst, bool(flag_period)) if n < direct_pe_nmax
+          else rf_pe(m, axes, gravconst))
     ek, v2 = kinetic(pos, vel, mass, c, vbar, L, hub_pt)
     tabm, tabk, cnt, amax, bmax, cmax = shell_tables(pos, mass, v2, c, axes, vecs, L, nshell)
-    tabp = potential_profile(tabm, cnt, amax, bmax, cmax, gravconst)
+    tabp = potential_profile(tabm, cnt, amax, bmax, cmax, gravconst, virial_pe_profile,
+                             virial_shell_self_mode, axes)
     outer = tabp[-1]
-    if n < direct_pe_nmax:
+    if virial_pe_norm_policy == "legacy":
+        if ep != outer and outer != 0.0:
+            tabp = tabp / outer * ep
+    elif n < direct_pe_nmax:
         if ep != outer and outer != 0.0:
             tabp = tabp / outer * ep
     else:
@@ -338,6 +364,9 @@ def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst
         out[F["mvir"]] = mvir
         out[F["cvel"]] = np.sqrt(gravconst * mvir / out[F["rvir"]])
         out[F["tvir"]] = 35.9 * gravconst * mvir / out[F["rvir"]]
+        # compute_halo_profile_1b41 :1232-1238, profile 'TSIS'
+        out[F["rho_0"]] = out[F["mvir"]] / (4.0 * np.pi * out[F["rvir"]] ** 3)
+        out[F["r_c"]] = out[F["rvir"]]
     hl = np.sqrt(np.sum(Lv ** 2))
     out[F["spin"]] = hl * np.sqrt(np.abs(et)) / m ** 2.5 / gravconst
     return out

@@ -114,6 +114,37 @@ def test_non_default_constants(gpu):
     compare(out, ref)
 
 
+@pytest.mark.parametrize("switch", [dict(virial_shell_self_mode="half_shell"),
+                                    dict(virial_pe_norm_policy="legacy"),
+                                    dict(virial_pe_profile="baseline"),
+                                    dict(virial_pe_profile="baseline",
+                                         virial_pe_norm_policy="legacy"),
+                                    dict(flag_period=0)])
+def test_reference_switches(gpu, switch):
+    """The reference's non-default switches of the virial profile (potential_profile_1b40
+    :1065-1123: 'half_shell', 'legacy' with the RF estimate of det_halo_energies_1b3 for large
+    halos, 'baseline') and FlagPeriod 0 against the oracle."""
+    from oracle.halo_props import all_halos
+    hs = G.nfw_halos(40, 12, alpha=0.9, nmin=200, nmax=20000)
+    assert (np.diff(hs.offsets) >= 1000).any()
+    print(f"switch {switch}", flush=True)
+    ref = all_halos(hs.offsets, hs.pos, hs.vel, hs.mass, hub_pt=70.3, **switch)
+    out, _ = run(hs, gpu, hub_pt=70.3, **switch)
+    compare(out, ref)
+
+
+def test_unknown_switches_raise(gpu, halos):
+    import torch
+    from amdahop import halo_props
+    pos = torch.from_numpy(halos.pos).to(gpu)
+    vel = torch.from_numpy(halos.vel).to(gpu)
+    mass = torch.from_numpy(halos.mass).to(gpu)
+    with pytest.raises(ValueError, match="virial_pe_profile"):
+        halo_props(halos.offsets, pos, vel, mass, virial_pe_profile="exact")
+    with pytest.raises(NotImplementedError, match="ANG_MOM_OF_R"):
+        halo_props(halos.offsets, pos, vel, mass, ang_mom_of_r=True)
+
+
 def test_invalid_layouts_raise(gpu, halos):
     import torch
     from amdahop import halo_props
@@ -210,7 +241,7 @@ def test_light_mode_skips_the_shell_stages(gpu, halos, halos_ref):
     of the shell tables, energies and virial scan read NaN."""
     from oracle.halo_props import F
     out, _ = run(halos, gpu, hub_pt=70.3, light=True)
-    for k in ("ep", "et", "rvir", "mvir", "cvel", "tvir", "spin"):
+    for k in ("ep", "et", "rvir", "mvir", "cvel", "tvir", "spin", "rho_0", "r_c"):
         assert np.isnan(out[:, F[k]]).all(), k
     keep = [F[k] for k in ("m", "px", "py", "pz", "vx", "vy", "vz", "sigma", "Lx", "Ly", "Lz",
                            "r", "Ixx", "Iyy", "Izz", "sha", "shb", "shc", "ek")]

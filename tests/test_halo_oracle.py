@@ -357,3 +357,61 @@ def test_refine_centers_runs_dpmmc_then_sc():
     # off: the centre is left alone
     assert np.array_equal(HP.refine_centers(pos, m, dens, pos[3], 10.0, False, False, 0.06, 0.3),
                           pos[3])
+
+
+def test_rf_potential_of_a_sphere():
+    """det_halo_energies_1b3's estimate -3/5 G m^2 RF(a^2, b^2, c^2) (:1407-1412): for equal
+    axes RF(a^2, a^2, a^2) = 1/a, the potential energy -3/5 G m^2 / a of a uniform sphere."""
+    from oracle.halo_props import rf_pe
+    assert np.isclose(rf_pe(2.0, (0.5, 0.5, 0.5), 430.1), -0.6 * 430.1 * 4.0 / 0.5, rtol=1e-15)
+    # RF decreases in each argument, so axes (1, 0.5, 0.5) lie between the spheres of radius
+    # 0.5 and 1
+    e = rf_pe(1.0, (1.0, 0.5, 0.5), 1.0)
+    assert -0.6 / 0.5 < e < -0.6 / 1.0
+
+
+def test_virial_profile_switches():
+    """potential_profile_1b40's switches on a two-shell table (:1085-1105): 'half_shell' takes
+    half of each shell's own mass against itself where 'count' takes (n-1)/(2n); 'baseline'
+    is -3/5 G M(<r)^2 RF."""
+    from oracle.halo_props import potential_profile, shell_scale
+    from scipy.special import elliprf
+    tabm = np.array([2.0, 5.0])
+    cnt = np.array([2.0, 3.0])
+    a = b = c = 1.0
+    r = shell_scale(2) * 1.0                           # 0.5, 1.0
+    G = 2.0
+    count = potential_profile(tabm, cnt, a, b, c, G)
+    # shell 0: (0 + 1/4 * 2) * 2 / 0.5; shell 1: (2 + 1/3 * 3) * 3 / 1
+    assert np.allclose(count, -G * np.cumsum([0.25 * 2 * 2 / r[0], (2 + 3 / 3) * 3 / r[1]]))
+    half = potential_profile(tabm, cnt, a, b, c, G, self_mode="half_shell")
+    assert np.allclose(half, -G * np.cumsum([0.5 * 2 * 2 / r[0], (2 + 0.5 * 3) * 3 / r[1]]))
+    axes = (1.0, 0.8, 0.5)
+    base = potential_profile(tabm, cnt, a, b, c, G, mode="baseline", axes=axes)
+    assert np.allclose(base, -0.6 * G * tabm ** 2 * elliprf(1.0, 0.64, 0.25))
+
+
+def test_halo_switches_compose():
+    """In halo_props: 'legacy' rescales the profile of a large halo to the RF estimate, which
+    stays its ep; the default policy sets ep to the profile's outer value.  FlagPeriod 0 on a
+    halo across the box edge measures it unwrapped."""
+    from oracle.halo_props import F, halo_props, rf_pe
+    rng = np.random.default_rng(2)
+    n = 1200
+    pos = rng.normal(0, 0.01, (n, 3))
+    vel = rng.normal(0, 50, (n, 3))
+    mass = np.full(n, 1e-3)
+    d = halo_props(pos, vel, mass)
+    leg = halo_props(pos, vel, mass, virial_pe_norm_policy="legacy")
+    assert np.isclose(leg[F["ep"]], rf_pe(d[F["m"]], (d[F["sha"]], d[F["shb"]], d[F["shc"]]),
+                                          430.1), rtol=1e-14)
+    assert leg[F["ep"]] != d[F["ep"]]
+    for k in ("m", "px", "Lx", "r", "sha", "ek"):
+        assert leg[F[k]] == d[F[k]]
+    # across the edge: unwrapped, the members reach across the box
+    edge = pos + np.array([0.5, 0.0, 0.0])
+    edge[:, 0] = np.where(edge[:, 0] >= 0.5, edge[:, 0] - 1.0, edge[:, 0])
+    wrapped = halo_props(edge, vel, mass)
+    flat = halo_props(edge, vel, mass, flag_period=0)
+    assert np.isclose(wrapped[F["r"]], d[F["r"]], rtol=1e-9)
+    assert flat[F["r"]] > 0.4

@@ -108,3 +108,94 @@ def test_reference_errors(tmp_path):
     open(path, "wb").write(bytes(bad))
     with pytest.raises(RuntimeError, match="marker|past the end"):
         ramses.read_particles(rep, "Ra4", "all")
+
+
+def ra4_with_descriptor(tmp_path, fields, ncpu=3, ndm=500, nstar=200, seed=3, name="output_00042"):
+    """A Ra4 output whose stellar records follow ``fields`` (the descriptor's names after the
+    tag), each record holding a distinct value per particle so that a field read from the
+    wrong record is seen."""
+    rng = np.random.default_rng(seed)
+    n = ndm + nstar
+    pos = rng.uniform(0, 1, (n, 3))
+    vel = rng.normal(size=(n, 3)) * 1e-3
+    mass = rng.uniform(1e-9, 2e-9, n)
+    ids = np.concatenate([rng.permutation(ndm) + 1, rng.permutation(nstar) + 1])
+    fam = np.concatenate([np.ones(ndm), np.full(nstar, 2)]).astype(np.int8)
+    values = {f: rng.uniform(-3.0, 3.0, n) * (k + 1) for k, f in enumerate(fields)}
+    if "birth_time" in values:
+        values["birth_time"][ndm:] = -np.abs(values["birth_time"][ndm:]) - 0.1
+    d = tmp_path / name
+    d.mkdir()
+    ramses.write_descriptor(str(d), fields)
+    order = rng.permutation(n)
+    for icpu, part in enumerate(np.array_split(order, ncpu), start=1):
+        recs = [values[f][part] for f in fields]
+        write_part(str(d / f"part_{name[-5:]}.out{icpu:05d}"), ncpu, pos[part], vel[part],
+                   mass[part], ids[part], "Ra4", np.where(fam[part] == 2, -1.0, 0.0),
+                   None, fam[part], stellar_records=recs)
+    star_ids = ids[ndm:]
+    expect = {f: np.empty(nstar) for f in fields}
+    for f in fields:
+        expect[f][star_ids - 1] = values[f][ndm:]
+    return str(d), expect
+
+
+DESCRIPTOR_ORDERS = [
+    ["birth_time", "metallicity"],
+    ["metallicity", "birth_time"],
+    ["birth_time", "iord", "metallicity", "initial_mass", "chem_Fe", "chem_O", "ptype"],
+    ["metallicity", "chem_H", "birth_time", "chem_D", "initial_mass"],
+]
+
+
+@pytest.mark.parametrize("fields", DESCRIPTOR_ORDERS)
+def test_ra4_descriptor_orders(tmp_path, fields):
+    """Stellar records read where part_file_descriptor.txt places them (_ra4_stellar_descriptor
+    and _read_ra4_stellar_record_block, src/input_output.py:547-620): birth times,
+    metallicities, initial masses and chem abundances, other fields skipped; native reader and
+    restatement against the values written."""
+    rep, expect = ra4_with_descriptor(tmp_path, fields)
+    present, has_m0, elems = ramses.descriptor(rep)
+    assert present and has_m0 == ("initial_mass" in fields)
+    assert elems == tuple(e for e in ramses.CHEM_ELEMENTS if f"chem_{e}" in fields)
+    for family in ("all", "star"):
+        got = ramses.read_particles(rep, "Ra4", family, nthreads=2)
+        ref = ORR.read_particles(rep, "00042", 3, "Ra4", family, True)
+        same(got, ref)
+        for k in ("m0", "chem"):
+            if ref[k] is None:
+                assert got[k] is None
+            else:
+                assert np.array_equal(got[k], ref[k], equal_nan=True), k
+        assert np.array_equal(got["birth"], expect["birth_time"])
+        assert np.array_equal(got["metal"], expect["metallicity"])
+        if "initial_mass" in fields:
+            assert np.array_equal(got["m0"], expect["initial_mass"])
+        for i, e in enumerate(ramses.CHEM_ELEMENTS):
+            if f"chem_{e}" in fields:
+                assert np.array_equal(got["chem"][:, i], expect[f"chem_{e}"])
+            elif got["chem"] is not None:
+                assert np.all(np.isnan(got["chem"][:, i]))
+    # without the stellar payload nothing after the tag is read
+    assert ramses.read_particles(rep, "Ra4", "all", stellar=False)["birth"] is None
+
+
+@pytest.mark.parametrize("fields,msg", [
+    (["birth_time", "initial_mass"], "initial_mass but no metallicity"),
+    (["birth_time", "initial_mass", "metallicity"], "Unsupported Ra4 initial_mass record order"),
+])
+def test_ra4_descriptor_errors(tmp_path, fields, msg):
+    rep, _ = ra4_with_descriptor(tmp_path, fields)
+    with pytest.raises(RuntimeError, match=msg):
+        ramses.descriptor(rep)
+    with pytest.raises(RuntimeError, match=msg):
+        ramses.read_particles(rep, "Ra4", "all")
+    with pytest.raises(RuntimeError, match=msg):
+        ORR.read_particles(rep, "00042", 3, "Ra4", "all", True)
+
+
+def test_ra4_descriptor_without_birth_time_is_an_error(tmp_path):
+    """Stars need a birth time and a metallicity (the reference fails on them otherwise)."""
+    rep, _ = ra4_with_descriptor(tmp_path, ["metallicity", "chem_O"])
+    with pytest.raises(RuntimeError, match="birth_time or metallicity"):
+        ramses.read_particles(rep, "Ra4", "all")

@@ -183,3 +183,39 @@ def test_hop_selection_on_random_forests(seed):
             j = mother[j - 1]
         assert n2h[i] == j
     assert nb == sum(1 for m in mother if m == 0)
+
+
+def test_mothers_must_precede_their_subnodes():
+    node_0 = make_nodes([0, 1, 1], [5, 5, 5])
+    # renumber so that a subnode comes before its mother
+    bad = node_0.copy()
+    bad["mother"][1:] = [3, 3, 0]
+    bad["firstchild"][1:] = [0, 0, 1]
+    bad["sister"][1:] = [2, 0, 0]
+    with pytest.raises(ValueError, match="before its subnodes"):
+        select_with_ms_method(bad)
+
+
+@pytest.mark.parametrize("name", ["c1_tables", "c2_tables"])
+def test_msm_on_reference_tables(name):
+    """On the reference's own node tables and node ids (262,144 and 2,097,152 particles): the
+    reference's particle-count checks (compute_neiKDtree_mod.py:281-296 and :381-392) hold --
+    every node's own mass is the count of its particles, and every structure's top node
+    carries the particles relabelled to it -- and every structure's main branch follows the
+    heaviest subnode."""
+    from conftest import load_golden
+    g = load_golden(name)
+    node_0 = node_table(g["real_table"], g["integer_table"])
+    orig = node_0.copy()
+    lp = g["liste_parts"]
+    nn = len(node_0) - 1
+    own_count = np.bincount(lp, minlength=nn + 1)
+    r = select_with_ms_method(node_0)
+    assert r.nstruct == r.nb_halos + r.nb_sub and r.nb_halos == int(np.sum(orig["mother"][1:] <= 0))
+    labels = np.where(lp > 0, r.node_to_struct[np.maximum(lp, 1) - 1], 0)
+    per_struct = np.bincount(labels, minlength=r.nstruct + 1)
+    for s in range(1, r.nstruct + 1):
+        members = np.nonzero(r.node_to_struct == s)[0] + 1
+        top = members.min()
+        assert node_0["mass"][top] == per_struct[s]
+        assert np.sum(own_count[members]) == per_struct[s]
