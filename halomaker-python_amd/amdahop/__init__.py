@@ -96,12 +96,15 @@ def knn_device(pos, mass, nvoisins, box=1.0, mixed=True, bucket=64, return_order
     return (dens, nbr, info, order) if return_order else (dens, nbr, info)
 
 
-def density_adjoint(pos, mass, nbr, grad_rho, box=1.0, order=None):
+def density_adjoint(pos, mass, nbr, grad_rho, box=1.0, order=None, out=None, no_alloc=False):
     """dLoss/dpos (N,3) and dLoss/dmass (N,) of the SPH density for the upstream gradient
     ``grad_rho`` = dLoss/drho, with the neighbour rows ``nbr`` (N,k) frozen.  ``order`` (N,)
     int32, optional: a permutation of the particles that keeps neighbours close, the Morton
     order from :func:`knn_device` (``return_order=True``); with it the neighbours' terms are
-    summed per workgroup in LDS first, and far fewer global atomics are needed.  Returns
+    summed per workgroup in LDS first, and far fewer global atomics are needed.  ``out``:
+    optional (grad_pos (N,3), grad_mass (N,)) float64 device tensors to write into.
+    ``no_alloc``: call the allocation-free entry point (its scratch is the library's fixed
+    device array, N <= 4194304; no timing, so the milliseconds read 0).  Returns
     (grad_pos, grad_mass, device milliseconds)."""
     import torch
 
@@ -118,10 +121,23 @@ def density_adjoint(pos, mass, nbr, grad_rho, box=1.0, order=None):
         raise ValueError("grad_rho must have shape (N,) on the particles' device")
     nbr = nbr.contiguous()
     Lx, Ly, Lz = _box3(box)
-    gpos = torch.empty((n, 3), dtype=torch.float64, device=pos.device)
-    gmass = torch.empty(n, dtype=torch.float64, device=pos.device)
+    if out is not None:
+        gpos, gmass = out
+        _check_out(gpos, (n, 3), torch.float64, pos.device, "out[0]")
+        _check_out(gmass, (n,), torch.float64, pos.device, "out[1]")
+    else:
+        gpos = torch.empty((n, 3), dtype=torch.float64, device=pos.device)
+        gmass = torch.empty(n, dtype=torch.float64, device=pos.device)
     ms = ctypes.c_float(0.0)
     torch.cuda.synchronize(pos.device)
+    if no_alloc:
+        check(lib().ahop_density_adjoint_device_noalloc(
+            ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(mass.data_ptr()), n,
+            int(nbr.shape[1]), ctypes.c_void_p(nbr.data_ptr()),
+            ctypes.c_void_p(order.data_ptr() if order is not None else None),
+            ctypes.c_void_p(grad_rho.data_ptr()), float(Lx), float(Ly), float(Lz),
+            ctypes.c_void_p(gpos.data_ptr()), ctypes.c_void_p(gmass.data_ptr())))
+        return gpos, gmass, 0.0
     check(lib().ahop_density_adjoint_device(
         ctypes.c_void_p(pos.data_ptr()), ctypes.c_void_p(mass.data_ptr()), n, int(nbr.shape[1]),
         ctypes.c_void_p(nbr.data_ptr()),

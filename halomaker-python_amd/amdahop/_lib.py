@@ -75,6 +75,8 @@ SIGNATURES = {
                                        ctypes.POINTER(RunInfo)]),
     "ahop_density_adjoint_device": (ctypes.c_int, [_vp, _vp, _i64, _i32, _vp, _vp, _vp, _dbl,
                                                    _dbl, _dbl, _vp, _vp, ctypes.POINTER(_f32)]),
+    "ahop_density_adjoint_device_noalloc": (ctypes.c_int, [_vp, _vp, _i64, _i32, _vp, _vp, _vp,
+                                                           _dbl, _dbl, _dbl, _vp, _vp]),
     "ahop_ramses_count": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _i32, _i32, _i32, _i32,
                                           _i32, _vp]),
     "ahop_ramses_read": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _i32, _i32, _i32, _i32,

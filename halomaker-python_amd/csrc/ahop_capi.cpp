@@ -502,6 +502,21 @@ int ahop_density_adjoint_device(const double *d_pos, const double *d_mass, int64
     });
 }
 
+int ahop_density_adjoint_device_noalloc(const double *d_pos, const double *d_mass, int64_t n,
+                                        int32_t k, const int32_t *d_nbr, const int32_t *d_order,
+                                        const double *d_grad_rho, double xlong, double ylong,
+                                        double zlong, double *d_grad_pos, double *d_grad_mass)
+{
+    return guarded([&] {
+        if (!d_pos || !d_mass || !d_nbr || !d_grad_rho || !d_grad_pos || !d_grad_mass)
+            throw std::runtime_error("null array");
+        // the device's null stream: this entry point creates no stream (nor anything else);
+        // the call is synchronous like the others
+        sph_density_adjoint_noalloc(d_pos, d_mass, d_nbr, d_order, d_grad_rho, n, k, xlong, ylong,
+                                    zlong, d_grad_pos, d_grad_mass, nullptr);
+    });
+}
+
 int ahop_group_members_device(const int32_t *d_node_of_part, int64_t n,
                               const int32_t *d_node_to_struct, int32_t nnodes, int32_t nstruct,
                               int32_t *d_struct_of_part, int32_t *d_pids, int64_t *d_offsets,

@@ -143,6 +143,12 @@ float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32
                           const int32_t *d_order, const double *d_grad_rho, int64_t n, int K,
                           double Lx, double Ly, double Lz, double *d_gpos, double *d_gmass,
                           hipStream_t s);
+// The same without allocating: its scratch (an error counter and the order's inverse) is a
+// fixed device array of the library, n <= 4194304; calls overlapping on one GPU race.
+void sph_density_adjoint_noalloc(const double *d_pos, const double *d_mass,
+                                 const int32_t *d_nbr, const int32_t *d_order,
+                                 const double *d_grad_rho, int64_t n, int K, double Lx, double Ly,
+                                 double Lz, double *d_gpos, double *d_gmass, hipStream_t s);
 
 }  // namespace ahop
 

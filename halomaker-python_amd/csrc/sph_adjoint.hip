@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -207,60 +208,55 @@ __global__ void k_count_bad_ids(const int32_t *__restrict__ nbr, int64_t m, int6
 // single-rounded pi of the reference's density (f90:58)
 static constexpr double PI_REF_HOST = 3.1415927410125732421875;
 
-float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32_t *d_nbr,
-                          const int32_t *d_order, const double *d_grad_rho, int64_t n, int K,
-                          double Lx, double Ly, double Lz, double *d_gpos, double *d_gmass,
-                          hipStream_t s)
+namespace {
+
+__global__ void k_set_int(int *__restrict__ a, int v)
+{
+    if (blockIdx.x == 0 && threadIdx.x == 0) *a = v;
+}
+
+// Checks and launch of the adjoint with the caller's scratch: bad (one int) and, with an
+// order, slot (n int32, the order's inverse).  The neighbour ids must lie in [0, n) and the
+// order (if any) be a permutation of [0, n): both are counted on the device and read back
+// before the kernels that index with them run.  e0 / e1 (may be null) bracket the kernels.
+void check_args(int64_t n, int K, double Lx, double Ly, double Lz)
 {
     if (n < 2) throw std::runtime_error("need at least 2 particles");
     if (K < 1 || K >= n) throw std::runtime_error("need 1 <= k < npart");
     if (!(Lx > 0 && Ly > 0 && Lz > 0)) throw std::runtime_error("box sides must be > 0");
     if (n > INT32_MAX) throw std::runtime_error("npart must fit in int32");
+}
+
+void adjoint_launch(const double *d_pos, const double *d_mass, const int32_t *d_nbr,
+                    const int32_t *d_order, const double *d_grad_rho, int64_t n, int K,
+                    double Lx, double Ly, double Lz, double *d_gpos, double *d_gmass, int *d_bad,
+                    int32_t *d_slot, hipStream_t s, hipEvent_t e0, hipEvent_t e1)
+{
+    check_args(n, K, Lx, Ly, Lz);
     const double A = Lx * Ly * Lz / PI_REF_HOST;
     const unsigned b1 = (unsigned)((n + 255) / 256);
-    int32_t *d_slot = nullptr;
-    {
-        // neighbour ids in [0, n), and the order (if any) a permutation whose inverse is
-        // built here; the kernels index positions and LDS with them
-        int *d_bad = nullptr;
-        AHOP_HIP(hipMalloc(&d_bad, sizeof(int)));
-        hipError_t e = d_order ? hipMalloc(&d_slot, sizeof(int32_t) * n) : hipSuccess;
-        int bad[2] = {0, 0};
-        if (e == hipSuccess) e = hipMemcpyAsync(d_bad, &bad[0], sizeof(int), hipMemcpyHostToDevice, s);
-        if (e == hipSuccess) {
-            const int64_t m = n * (int64_t)K;
-            hipLaunchKernelGGL(k_count_bad_ids, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s,
-                               d_nbr, m, n, d_bad);
-            e = hipMemcpyAsync(&bad[0], d_bad, sizeof(int), hipMemcpyDeviceToHost, s);
-        }
-        if (e == hipSuccess && d_order) {
-            e = hipMemcpyAsync(d_bad, &bad[1], sizeof(int), hipMemcpyHostToDevice, s);
-            if (e == hipSuccess) {
-                hipLaunchKernelGGL(k_count_bad_ids, dim3(b1), dim3(256), 0, s, d_order, n, n, d_bad);
-                e = hipMemcpyAsync(&bad[1], d_bad, sizeof(int), hipMemcpyDeviceToHost, s);
-            }
-            if (e == hipSuccess) e = hipStreamSynchronize(s);
-            if (e == hipSuccess && bad[1] == 0) {
-                hipLaunchKernelGGL(k_fill_m1, dim3(b1), dim3(256), 0, s, d_slot, n);
-                hipLaunchKernelGGL(k_inverse_order, dim3(b1), dim3(256), 0, s, d_order, n, d_slot);
-                hipLaunchKernelGGL(k_count_negative, dim3(b1), dim3(256), 0, s, d_slot, n, d_bad);
-                e = hipMemcpyAsync(&bad[1], d_bad, sizeof(int), hipMemcpyDeviceToHost, s);
-            }
-        }
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        (void)hipFree(d_bad);
-        if (e != hipSuccess || bad[0] || bad[1]) {
-            if (d_slot) (void)hipFree(d_slot);
-            AHOP_HIP(e);
-            if (bad[0])
-                throw std::runtime_error(std::to_string(bad[0]) + " neighbour ids outside [0, npart)");
-            throw std::runtime_error("order is not a permutation of [0, npart)");
+    int bad[2] = {0, 0};
+    const int64_t m = n * (int64_t)K;
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, d_bad, 0);
+    hipLaunchKernelGGL(k_count_bad_ids, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, d_nbr,
+                       m, n, d_bad);
+    AHOP_HIP(hipMemcpyAsync(&bad[0], d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    if (d_order) {
+        hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s, d_bad, 0);
+        hipLaunchKernelGGL(k_count_bad_ids, dim3(b1), dim3(256), 0, s, d_order, n, n, d_bad);
+        AHOP_HIP(hipMemcpyAsync(&bad[1], d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+        AHOP_HIP(hipStreamSynchronize(s));
+        if (bad[1] == 0) {
+            hipLaunchKernelGGL(k_fill_m1, dim3(b1), dim3(256), 0, s, d_slot, n);
+            hipLaunchKernelGGL(k_inverse_order, dim3(b1), dim3(256), 0, s, d_order, n, d_slot);
+            hipLaunchKernelGGL(k_count_negative, dim3(b1), dim3(256), 0, s, d_slot, n, d_bad);
+            AHOP_HIP(hipMemcpyAsync(&bad[1], d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
         }
     }
-    hipEvent_t e0, e1;
-    AHOP_HIP(hipEventCreate(&e0));
-    AHOP_HIP(hipEventCreate(&e1));
-    AHOP_HIP(hipEventRecord(e0, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+    if (bad[0]) throw std::runtime_error(std::to_string(bad[0]) + " neighbour ids outside [0, npart)");
+    if (bad[1]) throw std::runtime_error("order is not a permutation of [0, npart)");
+    if (e0) AHOP_HIP(hipEventRecord(e0, s));
     const unsigned b3 = (unsigned)((3 * n + 255) / 256);
     hipLaunchKernelGGL(k_zero, dim3(b3), dim3(256), 0, s, d_gpos, 3 * n);
     hipLaunchKernelGGL(k_zero, dim3(b1), dim3(256), 0, s, d_gmass, n);
@@ -272,15 +268,82 @@ float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32
         hipLaunchKernelGGL(k_sph_adjoint<false>, dim3((unsigned)((n + ADJ_BS - 1) / ADJ_BS)),
                            dim3(ADJ_BS), 0, s, d_pos, d_mass, d_nbr, nullptr, nullptr, d_grad_rho,
                            n, K, Lx, Ly, Lz, A, d_gpos, d_gmass);
-    AHOP_HIP(hipEventRecord(e1, s));
-    hipError_t e = hipStreamSynchronize(s);
-    if (d_slot) (void)hipFree(d_slot);
-    AHOP_HIP(e);
+    if (e1) AHOP_HIP(hipEventRecord(e1, s));
+    AHOP_HIP(hipStreamSynchronize(s));
+}
+
+// Device memory of the allocation-free entry point: the error counter (first 256 bytes) and
+// the inverse of the order (ADJOINT_NOALLOC_MAX int32).  Every GPU holds its own copy for the
+// life of the process; host code reaches it through the address hipGetSymbolAddress gives on
+// that device, never through the symbol itself.
+constexpr int64_t ADJOINT_NOALLOC_MAX = 4194304;
+constexpr int64_t SCRATCH_HEAD = 64;   // int32 entries before the inverse order (256 bytes)
+__device__ __align__(256) int32_t scratch[SCRATCH_HEAD + ADJOINT_NOALLOC_MAX];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+int32_t *scratch_ptr[MAX_DEVICES] = {};
+
+// the current device's address of `scratch`, looked up once per device
+int32_t *device_scratch()
+{
+    int dev = 0;
+    AHOP_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= MAX_DEVICES) throw std::runtime_error("device index out of range");
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        AHOP_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) throw std::runtime_error("no device address for the adjoint's scratch");
+        scratch_ptr[dev] = static_cast<int32_t *>(p);
+    }
+    return scratch_ptr[dev];
+}
+
+}  // namespace
+
+float sph_density_adjoint(const double *d_pos, const double *d_mass, const int32_t *d_nbr,
+                          const int32_t *d_order, const double *d_grad_rho, int64_t n, int K,
+                          double Lx, double Ly, double Lz, double *d_gpos, double *d_gmass,
+                          hipStream_t s)
+{
+    struct Scratch {
+        int *bad = nullptr;
+        int32_t *slot = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Scratch()
+        {
+            if (bad) (void)hipFree(bad);
+            if (slot) (void)hipFree(slot);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } w;
+    check_args(n, K, Lx, Ly, Lz);
+    AHOP_HIP(hipMalloc(&w.bad, sizeof(int)));
+    if (d_order) AHOP_HIP(hipMalloc(&w.slot, sizeof(int32_t) * n));
+    AHOP_HIP(hipEventCreate(&w.e0));
+    AHOP_HIP(hipEventCreate(&w.e1));
+    adjoint_launch(d_pos, d_mass, d_nbr, d_order, d_grad_rho, n, K, Lx, Ly, Lz, d_gpos, d_gmass,
+                   w.bad, w.slot, s, w.e0, w.e1);
     float ms = 0;
-    AHOP_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    AHOP_HIP(hipEventElapsedTime(&ms, w.e0, w.e1));
     return ms;
+}
+
+void sph_density_adjoint_noalloc(const double *d_pos, const double *d_mass,
+                                 const int32_t *d_nbr, const int32_t *d_order,
+                                 const double *d_grad_rho, int64_t n, int K, double Lx, double Ly,
+                                 double Lz, double *d_gpos, double *d_gmass, hipStream_t s)
+{
+    check_args(n, K, Lx, Ly, Lz);
+    if (n > ADJOINT_NOALLOC_MAX)
+        throw std::runtime_error("the allocation-free adjoint takes at most " +
+                                 std::to_string(ADJOINT_NOALLOC_MAX) + " particles (got " +
+                                 std::to_string(n) + ")");
+    int32_t *base = device_scratch();
+    adjoint_launch(d_pos, d_mass, d_nbr, d_order, d_grad_rho, n, K, Lx, Ly, Lz, d_gpos, d_gmass,
+                   reinterpret_cast<int *>(base), base + SCRATCH_HEAD, s, nullptr, nullptr);
 }
 
 }  // namespace ahop

@@ -138,6 +138,21 @@ int ahop_density_adjoint_device(const double *d_pos, const double *d_mass, int64
                                 double ylong, double zlong, double *d_grad_pos,
                                 double *d_grad_mass, float *ms);
 
+/* Allocation-free variant of ahop_density_adjoint_device.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * or training loops, harnesses that forbid hipMalloc).  Same arguments, checks and outputs,
+ * without the timing; it allocates nothing (no device memory, stream or event) and runs on
+ * the device's null stream.  Its scratch -- an error counter and the inverse of d_order -- is a
+ * fixed array in the library's device memory (about 16 MiB, held by each GPU the library runs
+ * on until the process ends), so n is at most AHOP_ADJOINT_NOALLOC_MAX; larger n is an error.
+ * Every call on a GPU shares that array: calls that overlap on one GPU (from other host
+ * threads) race.  Synchronous. */
+#define AHOP_ADJOINT_NOALLOC_MAX 4194304
+int ahop_density_adjoint_device_noalloc(const double *d_pos, const double *d_mass, int64_t n,
+                                        int32_t k, const int32_t *d_nbr, const int32_t *d_order,
+                                        const double *d_grad_rho, double xlong, double ylong,
+                                        double zlong, double *d_grad_pos, double *d_grad_mass);
+
 /* ---- particles grouped by structure ------------------------------------------------- */
 
 /* Members of every structure in one call, on device arrays.  d_node_of_part (n): the node id
