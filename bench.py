@@ -24,6 +24,7 @@ for _p in (ROOT, PKG):
 
 METRIC = "Mparticles/s kNN+density+hop at 16.8M; mixed vs FP64 GPU at ≤1e-6 rel ρ err"
 HBM_PEAK_GBS = 8000.0
+FP64_VALU_PEAK_TFLOPS = 78.6
 # algorithmic HBM bytes per particle of the kNN+density kernel: read position 24 B and
 # mass 8 B, write density 8 B (the whole step adds the hop stage's density re-read 8 B and
 # group-id write 8 B: 56 B per particle)
@@ -128,12 +129,14 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
     mass = torch.from_numpy(hs.mass).to(dev)
     _, info = halo_props(hs.offsets, pos, vel, mass, hub_pt=70.3)      # autotune
     tile, wg = info.tile, info.workgroup
-    dev_ms = []
+    dev_ms, pe_ms, vir_ms = [], [], []
     torch.cuda.synchronize()
     w0 = time.perf_counter()
     for _ in range(steps):
         _, i2 = halo_props(hs.offsets, pos, vel, mass, hub_pt=70.3, tile=tile, workgroup=wg)
         dev_ms.append(i2.ms)
+        pe_ms.append(i2.ms_direct_pe)
+        vir_ms.append(i2.ms_virial)
     torch.cuda.synchronize()
     wall_ms = (time.perf_counter() - w0) * 1e3 / steps
     ms = float(np.median(dev_ms))
@@ -145,7 +148,33 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                         "bytes_per_member": HALO_BYTES_PER_MEMBER}}
-    log(f"halo props: {ms:.2f} ms device ({wall_ms:.2f} ms wall), tile {tile} wg {wg}")
+    # the pair-sum potential of the halos below the pair-sum limit (det_halo_energies_1b3) is
+    # FP64 arithmetic, O(n^2): its own roofline is the FP64 vector rate.  Pairs are the work
+    # unit; the FP64 operations per launch come from the hardware counters of the same
+    # workload (profiles/halo_fp64_c4.json, SQ_INSTS_VALU_FLOPS_FP64 summed over the launch)
+    cnt = np.diff(hs.offsets)
+    small = cnt[cnt < 1000].astype(np.int64)
+    pairs = int(np.sum(small * (small - 1) // 2))
+    pms = float(np.median(pe_ms))
+    out["direct_pe"] = {"ms": round(pms, 3), "halos": int(small.size), "pairs": pairs,
+                        "Gpairs_per_s": round(pairs / (pms * 1e-3) / 1e9, 2)}
+    out["virial_ms"] = round(float(np.median(vir_ms)), 3)
+    fpath = os.path.join(ROOT, "profiles", "halo_fp64_c4.json")
+    if os.path.exists(fpath):
+        with open(fpath) as f:
+            fc = json.load(f)
+        if fc.get("members") == M and fc.get("halos") == hs.nhalos:
+            for kname, key, kms in (("k_direct_pe", "direct_pe", pms),
+                                    ("k_virial", "virial", out["virial_ms"])):
+                fl = fc["fp64_flops"][kname]
+                ach = fl / (kms * 1e-3) / 1e12
+                out.setdefault("fp64_roofline", {})[key] = {
+                    "bound": "fp64 valu", "achieved": round(ach, 3),
+                    "peak": FP64_VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(ach / FP64_VALU_PEAK_TFLOPS, 4), "fp64_flops_per_launch": fl,
+                    "kernel_ms": round(kms, 3), "source": os.path.relpath(fpath, ROOT)}
+    log(f"halo props: {ms:.2f} ms device ({wall_ms:.2f} ms wall), tile {tile} wg {wg}; "
+        f"pair sums {pms:.2f} ms for {pairs} pairs, virial {out['virial_ms']:.2f} ms")
     # the same with the SC recentring of the reference's example parameter file (eps_SC 0.3,
     # dcell_min 0.001 Mpc: 1e-5 of a 100 Mpc box, the unit length here)
     sc = dict(sc=True, dcell_min=1e-5, eps_sc=0.3)

@@ -35,7 +35,8 @@ class HaloParams(ctypes.Structure):
 
 
 class HaloInfo(ctypes.Structure):
-    _fields_ = [("tile", _i32), ("workgroup", _i32), ("trials", _i32), ("ms", _f32)]
+    _fields_ = [("tile", _i32), ("workgroup", _i32), ("trials", _i32), ("ms", _f32),
+                ("ms_direct_pe", _f32), ("ms_virial", _f32)]
 
 
 class RunInfo(ctypes.Structure):

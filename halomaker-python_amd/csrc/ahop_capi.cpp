@@ -777,6 +777,8 @@ int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double 
             info->workgroup = t.bs;
             info->trials = t.trials;
             info->ms = t.ms;
+            info->ms_direct_pe = t.sub_ms[0];
+            info->ms_virial = t.sub_ms[1];
         }
     });
 }

@@ -656,6 +656,12 @@ __device__ __forceinline__ void heap_sift_any(RegTop &h, uint2 *hp, KeyT *hk, in
 // k = 64 the LDS heap (48 x 64 x 8 B), the stack (512 B) and 2 buckets of x, y, z (1536 B)
 // take 26624 B, 6 waves; a 3rd bucket fits only 5 and measured 25% slower (at C3,
 // 16.8 M particles: 138 ms against 172 ms).
+// Pending candidates each lane takes per step of the mixed kernel's traversal flushes: two,
+// whose stage reads are in flight together, inserted smallest first (C3 kernel 106.8 ->
+// 104.5 ms; four took 108.4 ms, and two in the seeding flushes too 106.0 ms: registers).
+#ifndef AHOP_PICKW
+#define AHOP_PICKW 2
+#endif
 #ifndef AHOP_MIXED_NBUF
 #define AHOP_MIXED_NBUF 2
 #endif

@@ -38,6 +38,9 @@ struct HaloParamsDev {
 struct HaloTuning {
     int tile = 0, bs = 0, trials = 0;
     float ms = 0;   // device time of the final run, all passes
+    // of which the pair-sum potential (k_direct_pe) and the shell tables / virial scan
+    // (k_virial); 0 in light mode
+    float sub_ms[2] = {0, 0};
 };
 
 // Stellar properties (halo_stellar.hip): is_star / age / metal per member, chem (M x nchem,

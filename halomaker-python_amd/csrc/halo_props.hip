@@ -908,8 +908,19 @@ __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *smal
     for (int i = threadIdx.x; i < n; i += BS)
         mem[i] = make_double4(D.pos[3 * (f + i)], D.pos[3 * (f + i) + 1], D.pos[3 * (f + i) + 2],
                               D.mass[f + i]);
-    __syncthreads();
     const double L = D.L;
+    // a halo whose members all lie within L/4 of its first member on every axis has every
+    // pair closer than L/2 per axis, where min(t, L - t) = t: its pairs skip the fold
+    bool far = false;
+    {
+        const double4 m0 = mem[0];
+        for (int i = threadIdx.x; i < n; i += BS) {
+            const double4 c = mem[i];
+            far = far || fabs(c.x - m0.x) >= 0.25 * L || fabs(c.y - m0.y) >= 0.25 * L ||
+                  fabs(c.z - m0.z) >= 0.25 * L;
+        }
+    }
+    const bool fold = __syncthreads_or(far);
     const int64_t P = (int64_t)n * (n - 1) / 2;
     const int64_t p0 = P * threadIdx.x / BS, p1 = P * (threadIdx.x + 1) / BS;
     double acc = 0.0;
@@ -918,21 +929,27 @@ __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *smal
         int j = (int)(p0 - pair_row_start(i, n)) + i + 1;
         double4 a = mem[i];
         double row = 0.0;   // sum of m_j / d_ij over this thread's part of row i
-        for (int64_t p = p0; p < p1; ++p) {
-            const double4 b = mem[j];
-            double tx = fabs(b.x - a.x), ty = fabs(b.y - a.y), tz = fabs(b.z - a.z);
-            tx = fmin(tx, L - tx);
-            ty = fmin(ty, L - ty);
-            tz = fmin(tz, L - tz);
-            row += b.w * rsqrt(tx * tx + ty * ty + tz * tz);
-            if (++j == n) {
-                acc += a.w * row;
-                row = 0.0;
-                ++i;
-                j = i + 1;
-                if (i < n) a = mem[i];
-            }
+#define PE_PAIRS(SEP)                                                                          \
+    for (int64_t p = p0; p < p1; ++p) {                                                        \
+        const double4 b = mem[j];                                                              \
+        double tx, ty, tz;                                                                     \
+        SEP;                                                                                   \
+        row += b.w * rsqrt(tx * tx + ty * ty + tz * tz);                                       \
+        if (++j == n) {                                                                        \
+            acc += a.w * row;                                                                  \
+            row = 0.0;                                                                         \
+            ++i;                                                                               \
+            j = i + 1;                                                                         \
+            if (i < n) a = mem[i];                                                             \
+        }                                                                                      \
+    }
+        if (fold) {
+            PE_PAIRS(tx = fabs(b.x - a.x); ty = fabs(b.y - a.y); tz = fabs(b.z - a.z);
+                     tx = fmin(tx, L - tx); ty = fmin(ty, L - ty); tz = fmin(tz, L - tz))
+        } else {
+            PE_PAIRS(tx = b.x - a.x; ty = b.y - a.y; tz = b.z - a.z)
         }
+#undef PE_PAIRS
         acc += a.w * row;
     }
     const int t = threadIdx.x;
@@ -1262,11 +1279,14 @@ static void sc_tiled(const std::vector<int32_t> &big, const Tiles &tl, const Hal
                        d_off, pos, sc, st, err);
 }
 
-// Runs every pass for one tile size / workgroup size; returns device milliseconds.
+// Runs every pass for one tile size / workgroup size; returns device milliseconds, and in
+// sub_ms (may be null) those of the pair-sum potential (k_direct_pe) and of the shell tables
+// and virial scan (k_virial).
 template <int BS>
 static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, const double *pos,
                         const double *vel, const double *mass, const double *dens, int64_t T,
-                        const HaloParamsDev &P, double *d_out, hipStream_t s, bool full)
+                        const HaloParamsDev &P, double *d_out, hipStream_t s, bool full,
+                        float *sub_ms = nullptr)
 {
     const int nh = (int)hoff.size() - 1;
     Tiles tl = make_tiles(hoff, T);
@@ -1294,9 +1314,12 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
         int *missed = dalloc<int>(keep, 2);            // [0]: shell misses, [1]: recentring errors
         AHOP_HIP(hipMemsetAsync(missed, 0, 2 * sizeof(int), s));
         const unsigned hb = (unsigned)((nh + 127) / 128);
-        hipEvent_t e0, e1;
+        hipEvent_t e0, e1, ep0, ep1;
         AHOP_HIP(hipEventCreate(&e0));
         AHOP_HIP(hipEventCreate(&e1));
+        AHOP_HIP(hipEventCreate(&ep0));
+        AHOP_HIP(hipEventCreate(&ep1));
+        bool timed_pe = false;
         AHOP_HIP(hipEventRecord(e0, s));
         if (P.light)   // the columns of the skipped stages read NaN (all-ones bytes)
             AHOP_HIP(hipMemsetAsync(d_out, 0xFF, sizeof(double) * HP_NFIELDS * nh, s));
@@ -1352,14 +1375,19 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
                     nsmall_max = std::max(nsmall_max, hoff[h + 1] - hoff[h]);
                 }
             double *ep = dalloc<double>(keep, nh);
+            int32_t *d_small = nullptr;
             if (!small.empty()) {
                 if (nsmall_max > PE_NMAX) throw std::runtime_error("direct_pe_nmax above the LDS limit");
-                int32_t *d_small = dalloc<int32_t>(keep, small.size());
+                d_small = dalloc<int32_t>(keep, small.size());
                 AHOP_HIP(hipMemcpyAsync(d_small, small.data(), sizeof(int32_t) * small.size(),
                                         hipMemcpyHostToDevice, s));
+            }
+            AHOP_HIP(hipEventRecord(ep0, s));
+            if (!small.empty())
                 hipLaunchKernelGGL(k_direct_pe<256>, dim3((unsigned)small.size()), dim3(256),
                                    sizeof(double4) * nsmall_max, s, D, d_small, ep);
-            }
+            AHOP_HIP(hipEventRecord(ep1, s));
+            timed_pe = true;
             hipLaunchKernelGGL(k_virial<256>, dim3(nh), dim3(256), sizeof(double) * 4 * nr, s, nh,
                                d_tf, d_off, hist, nr, st, ep, P, d_out);
         }
@@ -1367,8 +1395,17 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
         AHOP_HIP(hipStreamSynchronize(s));
         float ms = 0;
         AHOP_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (sub_ms) {
+            sub_ms[0] = sub_ms[1] = 0.0f;
+            if (timed_pe) {
+                AHOP_HIP(hipEventElapsedTime(&sub_ms[0], ep0, ep1));
+                AHOP_HIP(hipEventElapsedTime(&sub_ms[1], ep1, e1));
+            }
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(ep0);
+        (void)hipEventDestroy(ep1);
         int hmissed[2] = {0, 0};
         AHOP_HIP(hipMemcpy(hmissed, missed, 2 * sizeof(int), hipMemcpyDeviceToHost));
         cleanup();
@@ -1440,9 +1477,11 @@ HaloTuning halo_props_device(const int64_t *h_off, int64_t nhalos, const double 
             tune.bs = bs;
         }
         if (tune.bs == 128)
-            tune.ms = run_passes<128>(hoff, d_off, pos, vel, mass, dens, tune.tile, P, d_out, s, true);
+            tune.ms = run_passes<128>(hoff, d_off, pos, vel, mass, dens, tune.tile, P, d_out, s, true,
+                                      tune.sub_ms);
         else if (tune.bs == 256)
-            tune.ms = run_passes<256>(hoff, d_off, pos, vel, mass, dens, tune.tile, P, d_out, s, true);
+            tune.ms = run_passes<256>(hoff, d_off, pos, vel, mass, dens, tune.tile, P, d_out, s, true,
+                                      tune.sub_ms);
         else
             throw std::runtime_error("workgroup size must be 128 or 256");
     } catch (...) {

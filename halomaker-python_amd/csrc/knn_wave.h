@@ -275,6 +275,80 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         nbuf = 0;                                                                              \
     } while (0)
 
+    // One pending candidate of this lane as KNN_PICK takes it, or +inf when the lane has none
+    // left (the stage index then stays in range: bucket 1, slot 63).
+#define KNN_PICK_V(dout, idout)                                                                \
+    do {                                                                                       \
+        const bool any_ = (m0 | m1) != 0;                                                      \
+        const bool s0_ = m0 != 0;                                                              \
+        uint64_t mk_ = s0_ ? m0 : m1;                                                          \
+        const int fsel_ = s0_ ? f0 : f1;                                                       \
+        const int bsel_ = s0_ ? 0 : 1;                                                         \
+        const int j_ = __ffsll((long long)mk_) - 1;                                            \
+        mk_ &= mk_ - 1;                                                                        \
+        m0 = s0_ ? mk_ : m0;                                                                   \
+        m1 = s0_ ? m1 : mk_;                                                                   \
+        KNN_KEY(stage_at<MIXED>(sx, sy, sz, stage, bsel_ * WAVE + j_), dout);                  \
+        (dout) = any_ ? (dout) : (KeyT)INFINITY;                                               \
+        (idout) = fsel_ + j_;                                                                  \
+    } while (0)
+
+#define KNN_CSWAP(a, ia, b, ib)                                                                \
+    do {                                                                                       \
+        const bool sw_ = (b) < (a);                                                            \
+        const KeyT ta_ = sw_ ? (b) : (a), tb_ = sw_ ? (a) : (b);                               \
+        const int tia_ = sw_ ? (ib) : (ia), tib_ = sw_ ? (ia) : (ib);                          \
+        (a) = ta_;                                                                             \
+        (b) = tb_;                                                                             \
+        (ia) = tia_;                                                                           \
+        (ib) = tib_;                                                                           \
+    } while (0)
+
+    // Traversal flush of the mixed kernel: every step takes up to PICKW pending candidates of
+    // each lane at once (their stage reads in flight together), sorts them, and inserts them
+    // smallest first while they beat the lane's current k-th key.
+#define KNN_FLUSH_W()                                                                          \
+    do {                                                                                       \
+        while (__any((m0 | m1) != 0)) {                                                        \
+            ++st_steps;                                                                        \
+            KeyT a0_, a1_, a2_ = (KeyT)INFINITY, a3_ = (KeyT)INFINITY;                         \
+            int i0_, i1_, i2_ = 0, i3_ = 0;                                                    \
+            KNN_PICK_V(a0_, i0_);                                                              \
+            KNN_PICK_V(a1_, i1_);                                                              \
+            if constexpr (AHOP_PICKW > 2) {                                                    \
+                KNN_PICK_V(a2_, i2_);                                                          \
+                KNN_PICK_V(a3_, i3_);                                                          \
+                KNN_CSWAP(a0_, i0_, a1_, i1_);                                                 \
+                KNN_CSWAP(a2_, i2_, a3_, i3_);                                                 \
+                KNN_CSWAP(a0_, i0_, a2_, i2_);                                                 \
+                KNN_CSWAP(a1_, i1_, a3_, i3_);                                                 \
+                KNN_CSWAP(a1_, i1_, a2_, i2_);                                                 \
+            } else {                                                                           \
+                KNN_CSWAP(a0_, i0_, a1_, i1_);                                                 \
+            }                                                                                  \
+            while (__any(a0_ < tau)) {                                                         \
+                if (a0_ < tau) KNN_INSERT(a0_, i0_);                                           \
+                a0_ = a1_;                                                                     \
+                i0_ = i1_;                                                                     \
+                if constexpr (AHOP_PICKW > 2) {                                                \
+                    a1_ = a2_;                                                                 \
+                    i1_ = i2_;                                                                 \
+                    a2_ = a3_;                                                                 \
+                    i2_ = i3_;                                                                 \
+                    a3_ = (KeyT)INFINITY;                                                      \
+                } else {                                                                       \
+                    a1_ = (KeyT)INFINITY;                                                      \
+                }                                                                              \
+            }                                                                                  \
+        }                                                                                      \
+        nbuf = 0;                                                                              \
+    } while (0)
+#define KNN_FLUSH_T()                                                                          \
+    do {                                                                                       \
+        if constexpr (MIXED && NBUF == 2 && AHOP_PICKW > 1) KNN_FLUSH_W();                     \
+        else KNN_FLUSH(false);                                                                 \
+    } while (0)
+
     // Evaluate the staged candidates j = 0..cnt-1 of buffer b (slots fst + j) and set bit j of
     // mask_ where ACCEPT holds (ACCEPT sees the key d_ and the slot sj_).  The mixed kernel
     // reads the x, y, z arrays four candidates at a time; the bits of the stale entries past
@@ -464,7 +538,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
             KNN_STAGE(f, cnt, false);
             // flushed as soon as the stage is full, so that the tests of the nodes in
             // between see the new roots
-            if (nbuf == NBUF) KNN_FLUSH(false);
+            if (nbuf == NBUF) KNN_FLUSH_T();
             continue;
         }
         const int c1 = __builtin_amdgcn_readfirstlane(nleft[node]);
@@ -512,12 +586,16 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
             top_fresh = false;
         }
     }
-    KNN_FLUSH(false);
+    KNN_FLUSH_T();
     if (stack_overflow && lane == 0) atomicAdd(A.overflow, 1);
 #undef KNN_NEED
 #undef KNN_STAGE
 #undef KNN_SCAN
 #undef KNN_FLUSH
+#undef KNN_FLUSH_W
+#undef KNN_FLUSH_T
+#undef KNN_PICK_V
+#undef KNN_CSWAP
 #undef KNN_INSERT
 #undef KNN_KEY
 

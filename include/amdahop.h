@@ -207,6 +207,8 @@ typedef struct {
 typedef struct {
     int32_t tile, workgroup, trials;
     float ms;               /* device time of the final run, all passes */
+    float ms_direct_pe;     /* of which the pair-sum potential energies (det_halo_energies_1b3) */
+    float ms_virial;        /* and the shell tables, profile and virial scan (det_vir_props_1b4) */
 } ahop_halo_info;
 
 /* Properties of every halo of a CSR member layout in one call.  offsets: host array of

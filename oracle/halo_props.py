@@ -224,6 +224,11 @@ def shell_scale(nr):
     return s
 
 
+# the smallest distance of r_ell * nr from a shell boundary seen by the last shell_tables call
+# (how far the member nearest a boundary is from changing shell: i_ell = int(r_ell * nr))
+LAST_SHELL_MARGIN = [np.inf]
+
+
 def shell_tables(pos, mass, v2, c, axes, vecs, L, nr):
     """tab_props_inside_1b40 :969-1032 -> (cumulative mass, cumulative kinetic energy,
     shell counts, amax, bmax, cmax)."""
@@ -237,6 +242,9 @@ def shell_tables(pos, mass, v2, c, axes, vecs, L, nr):
     amax, bmax, cmax = rmax * sha * (1.0 + eps), rmax * shb * (1.0 + eps), rmax * shc * (1.0 + eps)
     r_ell = np.sqrt((dra / amax) ** 2 + (drb / bmax) ** 2 + (drc / cmax) ** 2)
     i_ell = (r_ell * nr).astype(int)
+    # (members at the centre, r_ell = 0, are in shell 0 in any arithmetic)
+    frac = (r_ell * nr - i_ell)[r_ell > 0]
+    LAST_SHELL_MARGIN[0] = float(np.min(np.minimum(frac, 1.0 - frac))) if frac.size else np.inf
     if np.any(i_ell >= nr):
         raise ValueError("tab_props_inside: a member falls outside the outer ellipsoid")
     tabm = np.zeros(nr)
