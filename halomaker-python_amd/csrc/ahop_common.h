@@ -33,6 +33,11 @@ struct DeviceTree {
     int32_t *flag = nullptr;
     double *box = nullptr;                      // 6 doubles per internal node: lo xyz, hi xyz
     float4 *fbox = nullptr;                     // FP32 centre / half-widths (2 per node)
+    // per internal node, what a visit of it reads, in one place: (left, right, first, last),
+    // and the FP32 boxes of its two children with their slot ranges: child k's
+    // (cx, cy, cz, hx) and (hy, hz, first, last) (ints as float bits), k = 1, 2
+    int4 *nrec = nullptr;
+    float4 *cbox = nullptr;
 };
 
 // Per-stage device timings of the last run (milliseconds, HIP events on the run's stream).
@@ -102,7 +107,8 @@ class Engine {
         b_leaf_parent, b_flag, b_box, b_temp, b_rho_s, b_nbr, b_hop_s, b_gid_s, b_flag_o,
         b_rank_o, b_density, b_gid, b_hop, b_counter, b_fallback, b_emit_cnt, b_emit_off,
         b_ekey, b_eval, b_ekey2, b_eval2, b_ukey, b_uval, b_keep, b_keep_off, b_skey, b_sval,
-        b_sad_off, b_sad_isad, b_sad_rho, b_sad_cnt, b_densityg, b_misc, b_roots, b_fbox, b_fb_sorted;
+        b_sad_off, b_sad_isad, b_sad_rho, b_sad_cnt, b_densityg, b_misc, b_roots, b_fbox, b_fb_sorted,
+        b_nrec, b_cbox;
     int64_t nmax_ = 0;
 };
 

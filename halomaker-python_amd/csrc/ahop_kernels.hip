@@ -379,6 +379,36 @@ __global__ void k_refit_upper(const int32_t *__restrict__ starts, const int *nst
     }
 }
 
+// Node records of the traversal: for internal node i its links and slot range, and the FP32
+// boxes of both children with their slot ranges, so that a visit reads one place.  A particle
+// child's box is its FP32-rounded position with zero half-widths (what the visit test computes
+// for a particle).
+__global__ void k_node_records(int64_t ni, const int32_t *__restrict__ first,
+                               const int32_t *__restrict__ last, const int32_t *__restrict__ left,
+                               const int32_t *__restrict__ right, const float4 *__restrict__ fbox,
+                               const double4 *__restrict__ spos, int4 *__restrict__ nrec,
+                               float4 *__restrict__ cbox)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ni) return;
+    const int c[2] = {left[i], right[i]};
+    nrec[i] = make_int4(c[0], c[1], first[i], last[i]);
+    for (int k = 0; k < 2; ++k) {
+        float4 a, b;
+        if (c[k] < 0) {
+            const double4 p = spos[~c[k]];
+            a = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
+            b = make_float4(0.0f, 0.0f, __int_as_float(~c[k]), __int_as_float(~c[k]));
+        } else {
+            a = fbox[2 * (int64_t)c[k]];
+            const float4 b0 = fbox[2 * (int64_t)c[k] + 1];
+            b = make_float4(b0.x, b0.y, __int_as_float(first[c[k]]), __int_as_float(last[c[k]]));
+        }
+        cbox[4 * i + 2 * k] = a;
+        cbox[4 * i + 2 * k + 1] = b;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // kNN + SPH density
 // ------------------------------------------------------------------------------------------
@@ -387,6 +417,8 @@ struct KnnArgs {
     const int32_t *first, *last, *left, *right;
     const double *box;
     const float4 *fbox;     // per internal node: (cx, cy, cz, hx), (hy, hz, 0, 0) in FP32
+    const int4 *nrec;       // per internal node: left, right, first, last
+    const float4 *cbox;     // per internal node: both children's boxes and slot ranges
     int n, K, nhop, bucket;
     double Lx, Ly, Lz, xyzl, rho_thr;
     const int32_t *qlist;   // optional explicit queries (sorted slots); null -> all slots
@@ -877,7 +909,7 @@ void Engine::release_all()
                      &b_counter, &b_fallback, &b_emit_cnt, &b_emit_off, &b_ekey, &b_eval,
                      &b_ekey2, &b_eval2, &b_ukey, &b_uval, &b_keep, &b_keep_off, &b_skey,
                      &b_sval, &b_sad_off, &b_sad_isad, &b_sad_rho, &b_sad_cnt, &b_densityg,
-                     &b_misc, &b_roots, &b_fbox, &b_fb_sorted};
+                     &b_misc, &b_roots, &b_fbox, &b_fb_sorted, &b_nrec, &b_cbox};
     for (auto *b : all) b->release();
     res_ = DeviceResults{};
 }
@@ -938,6 +970,10 @@ void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, co
     // a start is the lowest node crossing some block boundary: at most one per boundary
     hipLaunchKernelGGL(k_refit_upper, dim3(nblk(n / REFIT_BS + 1)), dim3(256), 0, s, starts, misc + 5,
                        t_.spos, t_.left, t_.right, t_.parent, t_.flag, t_.box, t_.fbox);
+    t_.nrec = as<int4>(b_nrec, ni);
+    t_.cbox = as<float4>(b_cbox, 4 * ni);
+    hipLaunchKernelGGL(k_node_records, dim3(nblk(ni)), dim3(256), 0, s, ni, t_.first, t_.last,
+                       t_.left, t_.right, t_.fbox, t_.spos, t_.nrec, t_.cbox);
     AHOP_HIP(hipEventRecord(ev_[2], s));
 }
 
@@ -945,11 +981,11 @@ template <int KCAP>
 static void launch_knn(bool mixed, const KnnArgs &a, unsigned nb, hipStream_t s)
 {
     if (mixed)
-        hipLaunchKernelGGL((k_knn<KCAP, true>), dim3(nb), dim3(64), 0, s, a, a.first, a.last, a.left,
-                           a.right, a.fbox, a.spos);
+        hipLaunchKernelGGL((k_knn<KCAP, true>), dim3(nb), dim3(64), 0, s, a, a.nrec, a.cbox,
+                           a.fbox, a.spos);
     else
-        hipLaunchKernelGGL((k_knn<KCAP, false>), dim3(nb), dim3(64), 0, s, a, a.first, a.last, a.left,
-                           a.right, a.fbox, a.spos);
+        hipLaunchKernelGGL((k_knn<KCAP, false>), dim3(nb), dim3(64), 0, s, a, a.nrec, a.cbox,
+                           a.fbox, a.spos);
 }
 
 static void dispatch_knn(int K, bool mixed, const KnnArgs &a, unsigned nb, hipStream_t s)
@@ -980,6 +1016,8 @@ void Engine::knn_density(const Params &p)
     a.right = t_.right;
     a.box = t_.box;
     a.fbox = t_.fbox;
+    a.nrec = t_.nrec;
+    a.cbox = t_.cbox;
     a.n = (int)n;
     a.K = p.nvoisins;
     a.nhop = p.nhop;

@@ -16,27 +16,10 @@
 // to nearest, half-widths rounded up, and the margin mL covers the rounding of the query,
 // the centre and the periodic fold, so the returned squared distance never exceeds the
 // true minimum distance from the query to the box
-__device__ __forceinline__ float box_dist2_lower(int c, const double4 *__restrict__ spos,
-                                                 const float4 *__restrict__ fbox, float px,
-                                                 float py, float pz, float Lx, float Ly,
-                                                 float Lz, float mLx, float mLy, float mLz)
+__device__ __forceinline__ float box_dist2_at(float bx, float by, float bz, float hx, float hy,
+                                              float hz, float px, float py, float pz, float Lx,
+                                              float Ly, float Lz, float mLx, float mLy, float mLz)
 {
-    float bx, by, bz, hx, hy, hz;
-    if (c < 0) {
-        const double4 p = spos[~c];
-        bx = (float)p.x;
-        by = (float)p.y;
-        bz = (float)p.z;
-        hx = hy = hz = 0.0f;
-    } else {
-        const float4 b0 = fbox[2 * c], b1 = fbox[2 * c + 1];
-        bx = b0.x;
-        by = b0.y;
-        bz = b0.z;
-        hx = b0.w;
-        hy = b1.x;
-        hz = b1.y;
-    }
     float tx = fabsf(px - bx), ty = fabsf(py - by), tz = fabsf(pz - bz);
     tx = fminf(tx, Lx - tx);
     ty = fminf(ty, Ly - ty);
@@ -45,6 +28,20 @@ __device__ __forceinline__ float box_dist2_lower(int c, const double4 *__restric
     const float dy = fmaxf(ty - hy - mLy, 0.0f);
     const float dz = fmaxf(tz - hz - mLz, 0.0f);
     return (dx * dx + dy * dy + dz * dz) * (1.0f - 4 * U32);
+}
+
+__device__ __forceinline__ float box_dist2_lower(int c, const double4 *__restrict__ spos,
+                                                 const float4 *__restrict__ fbox, float px,
+                                                 float py, float pz, float Lx, float Ly,
+                                                 float Lz, float mLx, float mLy, float mLz)
+{
+    if (c < 0) {
+        const double4 p = spos[~c];
+        return box_dist2_at((float)p.x, (float)p.y, (float)p.z, 0.0f, 0.0f, 0.0f, px, py, pz,
+                            Lx, Ly, Lz, mLx, mLy, mLz);
+    }
+    const float4 b0 = fbox[2 * c], b1 = fbox[2 * c + 1];
+    return box_dist2_at(b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, px, py, pz, Lx, Ly, Lz, mLx, mLy, mLz);
 }
 
 // Bound on |d2f - d2| for a candidate at true squared distance ~d from a query, when both
@@ -71,14 +68,15 @@ __device__ __forceinline__ auto stage_at(const float *sx, const float *sy, const
 
 // The tree arrays and the sorted particles are also passed as restrict-qualified kernel
 // arguments: the kernel never writes them, and saying so lets the wave-uniform node reads
-// go through scalar loads.  Two waves per SIMD caps the registers at 256 per lane, so that
+// go through scalar loads.  A node visit reads the node's record (links, slot range) and its
+// children's boxes and slot ranges from per-node arrays indexed by the node itself, so all of
+// it comes in one round of scalar loads (with the children's boxes indexed by the children,
+// the second round waited on the first: C3 kernel 104.3 -> 99.1 ms).  Two waves per SIMD caps the registers at 256 per lane, so that
 // the 6 waves per CU the mixed kernel's LDS allows can run (without the cap it takes 258
 // and runs 4 waves per CU).
 template <int KCAP, bool MIXED>
-__global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restrict__ nfirst,
-                                            const int32_t *__restrict__ nlast,
-                                            const int32_t *__restrict__ nleft,
-                                            const int32_t *__restrict__ nright,
+__global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int4 *__restrict__ nrec,
+                                            const float4 *__restrict__ ncbox,
                                             const float4 *__restrict__ nfbox,
                                             const double4 *__restrict__ spos)
 {
@@ -440,6 +438,20 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
         }                                                                                      \
     } while (0)
 
+    // visit test of a child whose box comes from its parent's record (a: cx, cy, cz, hx;
+    // b: hy, hz, ...)
+#define KNN_NEED_AT(a, b, out, bdout)                                                          \
+    do {                                                                                       \
+        const float bd_ = box_dist2_at((a).x, (a).y, (a).z, (a).w, (b).x, (b).y, pxf, pyf,      \
+                                       pzf, Lxf, Lyf, Lzf, mLx, mLy, mLz);                      \
+        (bdout) = bd_;                                                                         \
+        if constexpr (MIXED) {                                                                 \
+            (out) = active && bd_ < tn;                                                        \
+        } else {                                                                               \
+            (out) = active && (double)bd_ < tau;                                               \
+        }                                                                                      \
+    } while (0)
+
     // ---- seed: the wave's seed windows cover the contiguous slots [qb-Wd, qb+63+Wd]
     if (!qlist) {
         const int qb = wave * WAVE;
@@ -514,12 +526,21 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     while (sp > 0) {
         // the traversal is wave-uniform: say so, so that node data come through scalar loads
         const int node = __builtin_amdgcn_readfirstlane(stack[--sp]);
-        int f, l;
-        if (node < 0) f = l = ~node;
-        else {
-            f = nfirst[node];
-            l = nlast[node];
+        // everything the visit reads, in one round of scalar loads: the node's links and
+        // slot range, and its children's boxes and slot ranges
+        int4 rec;
+        float4 cb0, cb1, cb2, cb3;
+        if (node < 0) {
+            rec = make_int4(0, 0, ~node, ~node);
+            cb0 = cb1 = cb2 = cb3 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        } else {
+            rec = nrec[node];
+            cb0 = ncbox[4 * node];
+            cb1 = ncbox[4 * node + 1];
+            cb2 = ncbox[4 * node + 2];
+            cb3 = ncbox[4 * node + 3];
         }
+        const int f = rec.z, l = rec.w;
         ++st_nodes;
         // the visit threshold on box distances, the same for this node and its children
         float tn = 0.0f;
@@ -541,12 +562,12 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
             if (nbuf == NBUF) KNN_FLUSH_T();
             continue;
         }
-        const int c1 = __builtin_amdgcn_readfirstlane(nleft[node]);
-        const int c2 = __builtin_amdgcn_readfirstlane(nright[node]);
+        const int c1 = rec.x;
+        const int c2 = rec.y;
         bool n1, n2;
         float bd1, bd2;
-        KNN_NEED(c1, n1, bd1);
-        KNN_NEED(c2, n2, bd2);
+        KNN_NEED_AT(cb0, cb1, n1, bd1);
+        KNN_NEED_AT(cb2, cb3, n2, bd2);
         top_fresh = true;
         const bool a1 = __any(n1), a2 = __any(n2);
         if (sp + 2 > STACK) {
@@ -563,8 +584,8 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
                 k1 = __popcll(__ballot(n1));
                 k2 = __popcll(__ballot(n2));
             }
-            const int f1_ = c1 < 0 ? ~c1 : nfirst[c1], l1_ = c1 < 0 ? ~c1 : nlast[c1];
-            const int f2_ = c2 < 0 ? ~c2 : nfirst[c2], l2_ = c2 < 0 ? ~c2 : nlast[c2];
+            const int f1_ = __float_as_int(cb1.z), l1_ = __float_as_int(cb1.w);
+            const int f2_ = __float_as_int(cb3.z), l2_ = __float_as_int(cb3.w);
             const int d1 = max(0, max(f1_ - qmid, qmid - l1_));
             const int d2 = max(0, max(f2_ - qmid, qmid - l2_));
             if (k1 > k2 || (k1 == k2 && d1 <= d2)) {
@@ -589,6 +610,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int32_t *__restr
     KNN_FLUSH_T();
     if (stack_overflow && lane == 0) atomicAdd(A.overflow, 1);
 #undef KNN_NEED
+#undef KNN_NEED_AT
 #undef KNN_STAGE
 #undef KNN_SCAN
 #undef KNN_FLUSH
