@@ -277,10 +277,11 @@ class Reference:
                 out[key] = float(m.group(1))
         return out
 
-    def full(self):
-        """The public compute_adaptahop entry (f90:495-543): final node ids and tables."""
+    def full(self, optimize_nodes=False):
+        """The public compute_adaptahop entry (f90:495-543): final node ids and tables.
+        optimize_nodes selects the blocked-prefix create-nodes path (f90:955-958)."""
         L = ref_lib()
-        L.ref_configure(*self.args, 0, 0)
+        L.ref_configure(*self.args, 0, 1 if optimize_nodes else 0)
         L.ref_run_full(self.n, _ptr(self.pos_f), _ptr(self.mass))
         lp = np.empty(self.n, dtype=np.int32)
         L.ref_get_liste_parts(_ptr(lp), self.n)

@@ -202,3 +202,23 @@ def test_numpy_hop_saddle_rules_on_ties():
     with_max = np.array([max(directed[(a, b)], directed[(b, a)])
                          for a, b in zip(g1.tolist(), g2.tolist())])
     assert not np.array_equal(with_max, ref[2])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
+def test_reference_optimize_nodes_gives_the_default_tables():
+    """The reference's moment-prefix create-nodes path (optimize_nodes, f90:955-958,
+    :1632-2300) returns the node ids and integer tables of its default path (the committed C1
+    tables) bit for bit, and real tables that differ only by the rounding of its prefix sums
+    (relative 1e-11 at most; the radius, a square root of a difference of moments, moves
+    most).  The drop-in serves both values of the switch with the default path's arithmetic."""
+    g = load_golden("c1_tables")
+    ps = golden_input(g)
+    R = O.Reference(ps.pos, ps.mass, int(g["nvoisins"]), nthreads=8)
+    r = R.full(optimize_nodes=True)
+    assert r["nnodes"] == int(g["nnodes"])
+    assert np.array_equal(r["liste_parts"], g["liste_parts"])
+    assert np.array_equal(r["integer_table"], g["integer_table"])
+    rt, gt = r["real_table"], g["real_table"]
+    assert not np.array_equal(rt, gt)            # a different summation, not the same path
+    assert np.array_equal(rt[[0, 2, 4]], gt[[0, 2, 4]])     # saddle, densmax, truemass
+    assert np.allclose(rt, gt, rtol=1e-11, atol=0)
