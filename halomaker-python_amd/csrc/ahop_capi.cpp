@@ -321,6 +321,7 @@ int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_c
         tp.ylongs2 = s.ylongs2;
         tp.zlongs2 = s.zlongs2;
         tp.nthreads = s.nbpes;   // the reference runs this stage on nbPes OpenMP threads
+        tp.optimize_nodes = s.optimize_nodes;
         TreeOutputs to;
         to.liste_parts = std::move(s.liste_parts);   // recycle the previous call's buffer
         create_group_tree(ti, tp, to);
@@ -812,6 +813,7 @@ int ahop_create_group_tree(int64_t npart, const double *pos, const double *mass,
         tp.nmembthresh = nmembthresh;
         tp.xlong = tp.ylong = tp.zlong = xlong;
         tp.xlongs2 = tp.ylongs2 = tp.zlongs2 = xlong * 0.5;
+        tp.optimize_nodes = S().optimize_nodes;
         TreeOutputs to;
         create_group_tree(ti, tp, to);
         keep_tree_stats(S(), to);

@@ -7,6 +7,11 @@
 //   omp_dev_create_nodes_task     :1245-1628
 //   reorder_nodes_deterministic   :987-1131
 //   node2table                    :550-573
+// and, with optimize_nodes, the moment-prefix variant of the node creation:
+//   build_opt_prefix_moments      :1639-1745
+//   opt_count, treat_particles_opt :1748-1843
+//   omp_opt_create_nodes_task     :1845-2278 (its component sums)
+//   optimized_needs_exact_fallback :2283-2311
 // Child subtrees are OpenMP tasks, as in the reference: the groups, member lists, particles
 // and node entries of two sibling subtrees are disjoint, node entries are reserved with an
 // atomic counter, and the only scheduling-dependent effect, the pre-reorder node numbering,
@@ -41,6 +46,73 @@ struct PartRec {
     int32_t ipar, pad;
 };
 
+// one entry of a group's moment prefix: the k-th densest member's density and the sums over
+// the k densest members of mass, mass * position, mass * |position|^2 and density, positions
+// unwrapped about the group's densest member
+struct Moment {
+    double dens, m, s1[3], s2, sd;
+};
+
+// indexx f90:220-303 / opt_indexx :306-389 (median-of-three quicksort with an insertion sort
+// below 7 elements, whose insertion pass scans down to the first element): ascending order
+// of arr[0..n-1] as 1-based positions in indx[1..n].  The order among equal values is this
+// algorithm's, so it is restated rather than replaced by std::sort.
+void indexx_1based(int32_t n, const double *arr0, std::vector<int32_t> &indx)
+{
+    constexpr int M = 7, NSTACK = 128;
+    auto arr = [arr0](int32_t i) { return arr0[i - 1]; };
+    indx.assign((size_t)n + 1, 0);
+    int32_t istack[NSTACK + 1];
+    for (int32_t j = 1; j <= n; ++j) indx[j] = j;
+    int32_t jstack = 0, l = 1, ir = n;
+    for (;;) {
+        if (ir - l < M) {
+            for (int32_t j = l + 1; j <= ir; ++j) {
+                const int32_t indxt = indx[j];
+                const double a = arr(indxt);
+                int32_t i = j - 1;
+                for (; i >= 1; --i) {
+                    if (arr(indx[i]) <= a) break;
+                    indx[i + 1] = indx[i];
+                }
+                indx[i + 1] = indxt;
+            }
+            if (jstack == 0) return;
+            ir = istack[jstack];
+            l = istack[jstack - 1];
+            jstack -= 2;
+        } else {
+            const int32_t k = (l + ir) / 2;
+            std::swap(indx[k], indx[l + 1]);
+            if (arr(indx[l + 1]) > arr(indx[ir])) std::swap(indx[l + 1], indx[ir]);
+            if (arr(indx[l]) > arr(indx[ir])) std::swap(indx[l], indx[ir]);
+            if (arr(indx[l + 1]) > arr(indx[l])) std::swap(indx[l + 1], indx[l]);
+            int32_t i = l + 1, j = ir;
+            const int32_t indxt = indx[l];
+            const double a = arr(indxt);
+            for (;;) {
+                do ++i; while (arr(indx[i]) < a);
+                do --j; while (arr(indx[j]) > a);
+                if (j < i) break;
+                std::swap(indx[i], indx[j]);
+            }
+            indx[l] = indx[j];
+            indx[j] = indxt;
+            jstack += 2;
+            if (jstack > NSTACK) throw std::runtime_error("indexx: stack too small");
+            if (ir - i + 1 >= j - l) {
+                istack[jstack] = ir;
+                istack[jstack - 1] = i;
+                ir = j - 1;
+            } else {
+                istack[jstack] = j - 1;
+                istack[jstack - 1] = l;
+                l = i;
+            }
+        }
+    }
+}
+
 struct Builder {
     const TreeInputs &in;
     const TreeParams &p;
@@ -52,6 +124,10 @@ struct Builder {
     std::vector<PartRec> &rec;
     std::vector<int64_t> seg_b, seg_e;
     std::vector<int32_t> &liste_parts;
+    // optimize_nodes: group g's moment prefix is mom[mom_b[g] .. mom_b[g] + mom_n[g])
+    std::vector<Moment> &mom;
+    std::vector<int64_t> mom_b;
+    std::vector<int32_t> mom_n;
     std::vector<int32_t> nhnei, offset1;   // dev_group_nhnei (truncated while colouring), offsets
     std::vector<int32_t> idgroup, igroupid, idgroup_tmp, color;
     std::vector<Node> node;
@@ -87,8 +163,8 @@ struct Builder {
     }
 
     Builder(const TreeInputs &i, const TreeParams &pp, std::vector<PartRec> &ws_rec,
-            std::vector<int32_t> &ws_liste)
-        : in(i), p(pp), rec(ws_rec), liste_parts(ws_liste)
+            std::vector<int32_t> &ws_liste, std::vector<Moment> &ws_mom)
+        : in(i), p(pp), rec(ws_rec), liste_parts(ws_liste), mom(ws_mom)
     {
         npart = in.npart;
         ngroups = in.ngroups;
@@ -156,6 +232,119 @@ struct Builder {
         for (int64_t r = seg_b[igroup]; r < seg_e[igroup]; ++r)
             if (rec[r].dens > rhot) rec[w++] = rec[r];
         seg_e[igroup] = w;
+    }
+
+    // build_opt_prefix_moments f90:1639-1745, on the lists the first threshold pass left: each
+    // group's members in descending density (indexx's ascending order read backwards), the
+    // positions unwrapped about the densest member, the running sums in that order
+    void build_moments(int nt)
+    {
+        mom_b.assign(ngroups + 2, 0);
+        mom_n.assign(ngroups + 2, 0);
+        int64_t total = 0;
+        for (int32_t g = 1; g <= ngroups; ++g) {
+            mom_b[g] = seg_b[g];
+            mom_n[g] = (int32_t)(seg_e[g] - seg_b[g]);
+            total = std::max(total, seg_e[g]);
+        }
+        if ((int64_t)mom.size() < total) mom.resize(total);
+        const double L[3] = {p.xlong, p.ylong, p.zlong};
+        const double Ls2[3] = {p.xlongs2, p.ylongs2, p.zlongs2};
+#pragma omp parallel num_threads(nt)
+        {
+            std::vector<double> dens;
+            std::vector<int32_t> idx;
+#pragma omp for schedule(dynamic, 16)
+            for (int32_t g = 1; g <= ngroups; ++g) {
+                const int32_t n = mom_n[g];
+                if (n <= 0 || failed) continue;
+                const PartRec *r = rec.data() + seg_b[g];
+                dens.resize(n);
+                for (int32_t j = 0; j < n; ++j) dens[j] = r[j].dens;
+                try {
+                    indexx_1based(n, dens.data(), idx);
+                } catch (const std::exception &e) {
+                    record_error(e.what());
+                    continue;
+                }
+                double a[3];
+                for (int d = 0; d < 3; ++d) a[d] = r[idx[n] - 1].x[d];
+                double cm = 0.0, cs[3] = {0.0, 0.0, 0.0}, cq = 0.0, cd = 0.0;
+                Moment *mg = mom.data() + mom_b[g];
+                for (int32_t k = 1; k <= n; ++k) {
+                    const PartRec &q = r[idx[n - k + 1] - 1];
+                    double u[3];
+                    for (int d = 0; d < 3; ++d) {
+                        double dx = q.x[d] - a[d];
+                        if (dx >= Ls2[d]) dx = dx - L[d];
+                        else if (dx < -Ls2[d]) dx = dx + L[d];
+                        u[d] = a[d] + dx;
+                    }
+                    const double xm = q.mass;
+                    cm = cm + xm;
+                    cs[0] = cs[0] + xm * u[0];
+                    cs[1] = cs[1] + xm * u[1];
+                    cs[2] = cs[2] + xm * u[2];
+                    cq = cq + xm * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+                    cd = cd + q.dens;
+                    Moment &e = mg[k - 1];
+                    e.dens = q.dens;
+                    e.m = cm;
+                    for (int d = 0; d < 3; ++d) e.s1[d] = cs[d];
+                    e.s2 = cq;
+                    e.sd = cd;
+                }
+            }
+        }
+        if (failed) throw std::runtime_error(error);
+    }
+
+    // opt_count + treat_particles_opt f90:1748-1843: the members above rhot are a prefix of the
+    // descending order, found by bisection; the sums are the prefix's (the reference's blocks
+    // plus residual give the same running sums)
+    void treat_moments(int32_t g, double rhot, int32_t &cnt, double &tmass, double s1[3],
+                       double &s2, double &sd) const
+    {
+        const Moment *mg = mom.data() + mom_b[g];
+        int32_t lo = 0, hi = mom_n[g];
+        while (lo < hi) {
+            const int32_t mid = (lo + hi + 1) / 2;
+            if (mg[mid - 1].dens > rhot) lo = mid;
+            else hi = mid - 1;
+        }
+        cnt = lo;
+        if (lo <= 0) {
+            tmass = s2 = sd = 0.0;
+            s1[0] = s1[1] = s1[2] = 0.0;
+            return;
+        }
+        const Moment &e = mg[lo - 1];
+        tmass = e.m;
+        for (int d = 0; d < 3; ++d) s1[d] = e.s1[d];
+        s2 = e.s2;
+        sd = e.sd;
+    }
+
+    // optimized_needs_exact_fallback f90:2283-2311: components below 1000 members, or within a
+    // relative 1e-8 of one of the selection cuts, are recomputed from the member lists
+    bool needs_exact(int32_t mass_loc, double densmoy, double densmaxgroup,
+                     double densmoy_comp_max, double rsquare, double rhot) const
+    {
+        constexpr double rel_margin = 1e-8;
+        constexpr int32_t small_component_limit = 1000;
+        if (mass_loc <= 0) return false;
+        if (mass_loc < small_component_limit) return true;
+        const double density_cut = rhot * (1.0 + p.fudge / std::sqrt((double)mass_loc));
+        const double alpha_cut = p.alphap * densmoy;
+        const double m_density =
+            std::fabs(densmoy - density_cut) / std::max(std::fabs(density_cut), 1e-300);
+        const double m_comp = std::fabs(densmoy_comp_max - rhot) / std::max(std::fabs(rhot), 1e-300);
+        const double m_alpha =
+            std::fabs(densmaxgroup - alpha_cut) / std::max(std::fabs(alpha_cut), 1e-300);
+        const double m_radius =
+            std::fabs(rsquare - p.epsilon) / std::max(std::fabs(p.epsilon), 1e-300);
+        return m_density < rel_margin || m_comp < rel_margin || m_alpha < rel_margin ||
+               m_radius < rel_margin;
     }
 
     // treat_particles f90:2317-2401
@@ -274,6 +463,10 @@ struct Builder {
                          std::vector<int32_t> &mass_compg, std::vector<double> &rsquare,
                          std::vector<double> &densmoy, std::vector<char> &ifok)
     {
+        if (p.optimize_nodes && treat_component_moments(ic, rhot, igrposnew, posgg, massg,
+                                                        truemassg, densmaxg, mass_compg,
+                                                        rsquare, densmoy, ifok))
+            return;
         double posg[3] = {0, 0, 0};
         double posref[3] = {0, 0, 0};
         int32_t mass_loc = 0;
@@ -318,6 +511,63 @@ struct Builder {
                    (densmoy[ic] > rhot * (1.0 + p.fudge / std::sqrt((double)mass_loc)) ||
                     densmoy_comp_max > rhot) &&
                    densmaxg[ic] >= p.alphap * densmoy[ic] && rsquare[ic] >= p.epsilon;
+    }
+
+    // the component sums of omp_opt_create_nodes_task f90:1980-2036 from the moment prefixes;
+    // false (nothing written) when the component needs the exact recomputation
+    bool treat_component_moments(int32_t ic, double rhot, const std::vector<int32_t> &igrposnew,
+                                 std::vector<double> &posgg, std::vector<int32_t> &massg,
+                                 std::vector<double> &truemassg, std::vector<double> &densmaxg,
+                                 std::vector<int32_t> &mass_compg, std::vector<double> &rsquare,
+                                 std::vector<double> &densmoy, std::vector<char> &ifok)
+    {
+        double posg[3] = {0, 0, 0};
+        int32_t mass_loc = 0, mass_comp = 0;
+        double truemass = 0.0, rsquareg = 0.0, densmoyg = 0.0;
+        double densmaxgroup = -1.0, densmoy_comp_max = -1.0;
+        const int32_t igr1out = igrposnew[ic - 1] + 1, igr2out = igrposnew[ic];
+        for (int32_t igr = igr1out; igr <= igr2out; ++igr) {
+            const int32_t g = idgroup[igr];
+            densmaxgroup = std::max(densmaxgroup, densityg(g));
+            int32_t masstmp;
+            double truemasstmp, posgtmp[3], rsquaretmp, densmoytmp;
+            treat_moments(g, rhot, masstmp, truemasstmp, posgtmp, rsquaretmp, densmoytmp);
+            posg[0] = posg[0] + posgtmp[0];
+            posg[1] = posg[1] + posgtmp[1];
+            posg[2] = posg[2] + posgtmp[2];
+            rsquareg = rsquareg + rsquaretmp;
+            mass_loc = mass_loc + masstmp;
+            truemass = truemass + truemasstmp;
+            densmoyg = densmoyg + densmoytmp;
+            mass_comp = std::max(mass_comp, masstmp);
+            if (masstmp > 0) {
+                densmoytmp = densmoytmp / (double)masstmp;
+                densmoy_comp_max =
+                    std::max(densmoy_comp_max,
+                             densmoytmp / (1.0 + p.fudge / std::sqrt((double)masstmp)));
+            }
+        }
+        const double rsq = std::sqrt(std::fabs(
+            (truemass * rsquareg - (posg[0] * posg[0] + posg[1] * posg[1] + posg[2] * posg[2])) /
+            (truemass * truemass)));
+        const double dm = densmoyg / (double)mass_loc;
+        if (needs_exact(mass_loc, dm, densmaxgroup, densmoy_comp_max, rsq, rhot)) return false;
+        massg[ic] = mass_loc;
+        truemassg[ic] = truemass;
+        for (int d = 0; d < 3; ++d) posgg[3 * ic + d] = posg[d];
+        densmaxg[ic] = densmaxgroup;
+        mass_compg[ic] = mass_comp;
+        rsquare[ic] = rsq;
+        densmoy[ic] = dm;
+        ifok[ic] = mass_loc >= p.nmembthresh &&
+                   (dm > rhot * (1.0 + p.fudge / std::sqrt((double)mass_loc)) ||
+                    densmoy_comp_max > rhot) &&
+                   densmaxgroup >= p.alphap * dm && rsq >= p.epsilon;
+        // the member lists still drop the particles at or below rhot, so a child paints the
+        // members above its parent's threshold (the reference paints the same prefix,
+        // opt_count at that threshold)
+        for (int32_t igr = igr1out; igr <= igr2out; ++igr) treat_linkonly(idgroup[igr], rhot);
+        return true;
     }
 
     // omp_dev_create_nodes_task f90:1245-1628, run depth first
@@ -524,6 +774,7 @@ struct Builder {
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 64)
         for (int32_t g = 1; g <= ngroups; ++g) treat_linkonly(g, rhot);
         double t2 = now();
+        if (p.optimize_nodes) build_moments(nt);   // timed with node creation, as the reference
         node[0].mass = 0;
         node[0].truemass = 0.0;
         node[0].radius = 0.0;
@@ -600,7 +851,8 @@ void create_group_tree(const TreeInputs &in, const TreeParams &p, TreeOutputs &o
     // per-particle workspaces kept for the next call (the module is not re-entrant)
     static std::vector<PartRec> ws_rec;
     static std::vector<int32_t> ws_liste;
-    Builder b(in, p, ws_rec, ws_liste);
+    static std::vector<Moment> ws_mom;
+    Builder b(in, p, ws_rec, ws_liste, ws_mom);
     b.run(out);
 }
 

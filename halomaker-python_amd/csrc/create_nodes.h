@@ -11,6 +11,10 @@ struct TreeParams {
     double xlong = 1.0, ylong = 1.0, zlong = 1.0;
     double xlongs2 = 0.5, ylongs2 = 0.5, zlongs2 = 0.5;
     int32_t nthreads = 0;   // OpenMP threads of the stage (0: the OpenMP default)
+    // the reference's moment-prefix node creation (optimize_nodes, f90:954-958): a component
+    // of 1000 members or more, not within 1e-8 of a selection cut, takes its node properties
+    // from per-group prefix sums in descending density instead of the member lists
+    bool optimize_nodes = false;
 };
 
 // Inputs in original particle order (host memory).

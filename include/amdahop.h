@@ -322,7 +322,9 @@ int ahop_ramses_descriptor(const char *repository, int32_t *present, int32_t *ha
 /* Host structure-tree stage alone (create_group_tree f90:885-983) on host inputs, for
  * callers that computed the hot-path stages elsewhere.  Outputs: liste_parts (npart),
  * *nnodes, and the tables, which must have room for 2*ngroups nodes (nnodes <= 2*ngroups,
- * f90:906): real_table 8*2*ngroups doubles, integer_table 6*2*ngroups int32. */
+ * f90:906): real_table 8*2*ngroups doubles, integer_table 6*2*ngroups int32.  The node
+ * creation follows the optimize_nodes switch of the last ahop_sync_others call (default
+ * off), as the reference's module variable does (f90:954-958). */
 int ahop_create_group_tree(int64_t npart, const double *pos, const double *mass,
                            const double *density, const int32_t *gid, int32_t ngroups,
                            const double *densityg, const int32_t *sad_offset,

@@ -7,7 +7,10 @@ and saddle CSR, and through its public compute_adaptahop entry for the final nod
 and node tables.  Inputs are regenerated from amdahop.generate; their SHA-256 is stored
 so a generator change is detected.  Run from the repository root:
 
-    python tests/golden/make_golden.py [--tables-only]
+    python tests/golden/make_golden.py [--tables-only | --opt-only]
+
+``--opt-only`` writes only the node tables of the reference's moment-prefix create-nodes
+path (optimize_nodes) at the C1 and C2 sizes, ``<case>_opt.npz``.
 """
 import os
 import sys
@@ -50,8 +53,31 @@ def _sha(*arrays):
     return h.hexdigest()
 
 
+def opt_tables(out_dir):
+    """The same inputs through the public entry with optimize_nodes on (f90:954-958): node ids
+    and integer tables equal the default path's (checked here), the real tables are stored."""
+    for name, (cfg, n, ncl) in TABLE_CASES.items():
+        ps = G.config(cfg, n=n, nclumps=ncl)
+        R = O.Reference(ps.pos, ps.mass, ps.nvoisins, **PARAMS, nthreads=8)
+        full = R.full(optimize_nodes=True)
+        base = np.load(os.path.join(out_dir, f"{name}.npz"))
+        if str(base["input_digest"]) != G.digest(ps.pos, ps.mass):
+            raise SystemExit(f"{name}: inputs differ from the default-path golden")
+        if not (np.array_equal(full["liste_parts"], base["liste_parts"])
+                and np.array_equal(full["integer_table"], base["integer_table"])):
+            raise SystemExit(f"{name}: optimize_nodes changed the node ids or integer tables")
+        np.savez_compressed(os.path.join(out_dir, f"{name}_opt.npz"), nnodes=full["nnodes"],
+                            real_table=full["real_table"],
+                            integer_table=full["integer_table"])
+        ndiff = int(np.sum(full["real_table"] != base["real_table"]))
+        print(name, "optimize_nodes: nnodes", full["nnodes"], "real entries differing", ndiff)
+
+
 def main():
     out_dir = os.path.dirname(os.path.abspath(__file__))
+    if "--opt-only" in sys.argv:
+        opt_tables(out_dir)
+        return
     for name, (cfg, n, ncl) in TABLE_CASES.items():
         ps = G.config(cfg, n=n, nclumps=ncl)
         R = O.Reference(ps.pos, ps.mass, ps.nvoisins, **PARAMS, nthreads=8)
@@ -71,6 +97,7 @@ def main():
             real_table=full["real_table"], integer_table=full["integer_table"])
         print(name, "ngroups", st["ngroups"], "nnodes", full["nnodes"],
               "saddle entries", len(st["saddle_isad"]))
+    opt_tables(out_dir)
     if "--tables-only" in sys.argv:
         return
     for name, (cfg, n, ncl) in CASES.items():

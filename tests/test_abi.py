@@ -186,3 +186,46 @@ def test_host_structure_tree_at_size(name):
     print(f"{name}: {k} nodes, taskloop levels {st[0]} ({st[1]} components), "
           f"subtree tasks {st[2]}")
     assert st[3] == k and st[0] >= 1 and st[1] >= 16 and st[2] >= 1
+
+
+def _set_optimize_nodes(on, n):
+    from compute_adaptahop import neikdtree
+    neikdtree.sync_others(False, False, n, 16, 80.0, 1.0 / n, 0.0, 16, 32, 4.0, 1.0, "MSM", 30, on)
+
+
+@pytest.mark.parametrize("name", ["c1_tables", "c2_tables"])
+def test_host_structure_tree_optimize_nodes(name):
+    """The moment-prefix node creation (optimize_nodes, f90:954-958, :1632-2311) on stage
+    outputs equal to the reference's: node ids and both tables bit for bit with the
+    reference's own optimize_nodes run, whose real tables differ from the default path's
+    (position and radius of components spanning the periodic edge, rounding elsewhere)."""
+    g = load_golden(name)
+    opt = load_golden(name + "_opt")
+    ps, rho, gid, ng, dg, off, isad, srho = table_case_stages(g)
+    eps = 0.001 / ps.n ** (1.0 / 3.0)
+
+    def run():
+        lp = np.empty(ps.n, dtype=np.int32)
+        nn = ctypes.c_int32(0)
+        rt = np.zeros(8 * 2 * ng)
+        it = np.zeros(6 * 2 * ng, dtype=np.int32)
+        check(lib().ahop_create_group_tree(ps.n, ptr(ps.pos), ptr(ps.mass), ptr(rho), ptr(gid),
+                                           ng, ptr(dg), ptr(off), ptr(isad), ptr(srho), 80.0, 4.0,
+                                           1.0, eps, 20, 1.0, ptr(lp), ctypes.byref(nn), ptr(rt),
+                                           ptr(it)))
+        k = nn.value
+        return lp, k, rt[:8 * k].reshape(8, k, order="F"), it[:6 * k].reshape(6, k, order="F")
+
+    _set_optimize_nodes(True, ps.n)
+    try:
+        lp, k, rt, it = run()
+    finally:
+        _set_optimize_nodes(False, ps.n)
+    assert k == int(opt["nnodes"])
+    assert np.array_equal(lp, g["liste_parts"])
+    assert np.array_equal(it, opt["integer_table"])
+    assert np.array_equal(rt, opt["real_table"])
+    assert not np.array_equal(rt, g["real_table"])
+    # the switch is back off: the default path's tables again
+    lp, k, rt, it = run()
+    assert np.array_equal(rt, g["real_table"]) and np.array_equal(it, g["integer_table"])

@@ -24,7 +24,7 @@ def log(msg):
 
 
 def dropin_run(ps, mode="mixed", bucket=64, nhop=16, rho_threshold=80.0, xlong=1.0,
-               nvoisins=None):
+               nvoisins=None, optimize_nodes=False):
     """Drive the drop-in module exactly like compute_neiKDtree_mod.py does."""
     from compute_adaptahop import neikdtree
     n = ps.n
@@ -36,7 +36,7 @@ def dropin_run(ps, mode="mixed", bucket=64, nhop=16, rho_threshold=80.0, xlong=1
                                        70.0, 1.0, [0.0, 0.0, 0.0])
     neikdtree.sync_from_change_pos(n, n, 0.001 * xlong / n ** (1 / 3), 0.001, xlong, xlong)
     neikdtree.sync_others(False, False, n, 16, rho_threshold, 1.0 / n, 0.0, nhop, k, 4.0, 1.0,
-                          "MSM", 30, False)
+                          "MSM", 30, optimize_nodes)
     neikdtree.compute_adaptahop(np.asfortranarray(ps.pos.T), ps.mass)
     out = dict(density=neikdtree.density.copy(), liste_parts=neikdtree.liste_parts.copy(),
                nnodes=int(neikdtree.nnodes), real_table=neikdtree.real_table.copy(order="F"),
@@ -46,6 +46,9 @@ def dropin_run(ps, mode="mixed", bucket=64, nhop=16, rho_threshold=80.0, xlong=1
     neikdtree.close()
     neikdtree.set_precision("mixed")
     neikdtree.set_bucket(64)
+    if optimize_nodes:
+        neikdtree.sync_others(False, False, n, 16, rho_threshold, 1.0 / n, 0.0, nhop, k, 4.0, 1.0,
+                              "MSM", 30, False)
     return out
 
 
@@ -305,3 +308,19 @@ def test_dropin_node_tables_at_size(gpu, name, mode):
     check(lib().ahop_tree_stats(ptr(st), None))
     log(f"{name}: taskloop levels {st[0]}, subtree tasks {st[2]}")
     assert st[0] >= 1 and st[2] >= 1
+
+
+@pytest.mark.timeout(600)
+def test_dropin_optimize_nodes_tables(gpu):
+    """The drop-in call with optimize_nodes on at 2,097,152 particles: node ids and both
+    tables bit for bit with the reference's own optimize_nodes run (its moment-prefix node
+    creation, f90:1632-2311)."""
+    g = load_golden("c2_tables")
+    opt = load_golden("c2_tables_opt")
+    ps = golden_input(g)
+    log(f"drop-in optimize_nodes c2 n={ps.n}")
+    r = dropin_run(ps, "mixed", optimize_nodes=True)
+    assert r["nnodes"] == int(opt["nnodes"])
+    assert np.array_equal(r["liste_parts"], g["liste_parts"])
+    assert np.array_equal(r["integer_table"], opt["integer_table"])
+    assert np.array_equal(r["real_table"], opt["real_table"])

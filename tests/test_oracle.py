@@ -205,19 +205,23 @@ def test_numpy_hop_saddle_rules_on_ties():
 
 
 @pytest.mark.skipif(not O.ref_available(), reason="compiled reference not built here")
-def test_reference_optimize_nodes_gives_the_default_tables():
-    """The reference's moment-prefix create-nodes path (optimize_nodes, f90:955-958,
-    :1632-2300) returns the node ids and integer tables of its default path (the committed C1
-    tables) bit for bit, and real tables that differ only by the rounding of its prefix sums
-    (relative 1e-11 at most; the radius, a square root of a difference of moments, moves
-    most).  The drop-in serves both values of the switch with the default path's arithmetic."""
+def test_reference_optimize_nodes_tables():
+    """The reference's moment-prefix create-nodes path (optimize_nodes, f90:954-958,
+    :1632-2311), run live on C1: the stored optimize_nodes tables are its output; its node ids
+    and integer tables are the default path's, its real tables are not.  On C1 they differ by
+    the rounding of the prefix sums (relative 1e-11 at most: the radius, a square root of a
+    difference of moments, moves most); on C2 positions and radii of components that span the
+    periodic edge differ by far more (each group's moments are unwrapped about its own
+    densest member)."""
     g = load_golden("c1_tables")
+    opt = load_golden("c1_tables_opt")
     ps = golden_input(g)
     R = O.Reference(ps.pos, ps.mass, int(g["nvoisins"]), nthreads=8)
     r = R.full(optimize_nodes=True)
-    assert r["nnodes"] == int(g["nnodes"])
+    assert r["nnodes"] == int(g["nnodes"]) == int(opt["nnodes"])
     assert np.array_equal(r["liste_parts"], g["liste_parts"])
     assert np.array_equal(r["integer_table"], g["integer_table"])
+    assert np.array_equal(r["real_table"], opt["real_table"])
     rt, gt = r["real_table"], g["real_table"]
     assert not np.array_equal(rt, gt)            # a different summation, not the same path
     assert np.array_equal(rt[[0, 2, 4]], gt[[0, 2, 4]])     # saddle, densmax, truemass
