@@ -25,6 +25,7 @@ for _p in (ROOT, PKG):
 METRIC = "Mparticles/s kNN+density+hop at 16.8M; mixed vs FP64 GPU at ≤1e-6 rel ρ err"
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
+FP32_VALU_PEAK_TFLOPS = 157.3
 # algorithmic HBM bytes per particle of the kNN+density kernel: read position 24 B and
 # mass 8 B, write density 8 B (the whole step adds the hop stage's density re-read 8 B and
 # group-id write 8 B: 56 B per particle)
@@ -600,6 +601,14 @@ def main():
                 "kernel": "k_knn<64,mixed> (kNN + SPH density)", "kernel_ms": round(knn_ms, 3),
                 "bytes_per_particle": KNN_BYTES_PER_PARTICLE,
                 "step_achieved_GBs": round(STEP_BYTES_PER_PARTICLE * n / (ms * 1e-3) / 1e9, 2)}
+    # the kernel is bound by instruction issue, not HBM: its FP32 candidate arithmetic (every
+    # lane evaluates every staged candidate, 9 FP32 operations each: 3 differences, a square,
+    # two fused multiply-adds counted as 2 each, the comparison) against the FP32 vector peak
+    evals = int(last.knn_candidates) * 64
+    fl = 9.0 * evals / (knn_ms * 1e-3) / 1e12
+    roofline["fp32_candidates"] = {"evaluations": evals, "ops_per_evaluation": 9,
+                                   "achieved_TFLOPs": round(fl, 3), "peak_TFLOPs": FP32_VALU_PEAK_TFLOPS,
+                                   "frac": round(fl / FP32_VALU_PEAK_TFLOPS, 4)}
     tpath = args.traffic_json or os.path.join(ROOT, "profiles", f"knn_{args.workload}_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:

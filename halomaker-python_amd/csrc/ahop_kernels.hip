@@ -484,35 +484,46 @@ __device__ __forceinline__ void heap_sift(KeyT *hk, int32_t *hi, int lane, int s
 }
 
 // Heap of the mixed kernel: packed (FP32 key bits, id) entries, the top two levels in
-// registers.  The root (the largest key kept) is entry 0 (k0, i0); its RC = 16 children are
+// registers.  The root (the largest key kept) is entry 0 (k0, i0); its RC children are
 // entries 1..RC (rk, ri); every register child c has LC = 8 children, entries
-// LC * c + RT - LC + t (t < LC), which are leaves and live in LDS at
-// hp[(e - RT) * WAVE + lane].  Up to RT + LC * RC = 145 entries fit in these three levels.
+// LC * c + RT - LC + t (t < LC, RT = RC + 1), which are leaves and live in LDS at
+// hp[(e - RT) * WAVE + lane].  Up to RT + LC * RC entries fit in these three levels.
 // A replacement at the root finds the largest register child without LDS and, only when
-// that child has children, reads them from LDS in one round trip; with k + 1 = 65 entries
-// only 6 of the 16 register children have children, and the LDS part is 48 entries per
-// lane.  The register vectors are indexed with compile-time indices only (selects).
-constexpr int RC = 16;
-constexpr int RT = RC + 1;
+// that child has children, reads them from LDS in one round trip.  Up to k = 64 the root
+// has RC = 8 children: the k + 1 = 65 entries take 56 LDS entries per lane (5 waves per CU
+// by LDS) and 7 of the 8 register children have children.  With 16 register children
+// (48 LDS entries, 6 waves per CU) every replacement took a maximum over 16 entries and
+// their write-back, and the kernel needed 255 registers and spilled: at k = 64 it issued 145
+// thousand VALU instructions per wave against 112 thousand with 8, and ran 99.3 ms against
+// 98.6 ms at the 16.8M-particle box (the spare registers then carry two pending candidates
+// per lane through the seeding's flushes too: 96.8 ms).  Above k = 64 the root keeps 16
+// children (129 entries need them).  The register vectors are indexed with compile-time
+// indices only (selects).
+template <int KCAP>
+constexpr int heap_rc() { return KCAP > 64 ? 16 : 8; }
 constexpr int LC = 8;
+
+// first child of register entry c (1 <= c <= RC)
+template <int RC>
+__device__ __forceinline__ int rt_child0(int c) { return LC * c + RC + 1 - LC; }
+
 // vector types rather than arrays: element reads and writes are register inserts and
 // extracts, never loads from a stack slot, so the compiler cannot turn a select over
 // entries into an indexed memory access
-typedef float rtk_t __attribute__((ext_vector_type(RC)));
-typedef uint32_t rti_t __attribute__((ext_vector_type(RC)));
-
-// first child of register entry c (1 <= c <= RC)
-__device__ __forceinline__ int rt_child0(int c) { return LC * c + RT - LC; }
-
+template <int RC>
 struct RegTop {
+    typedef float rtk_t __attribute__((ext_vector_type(RC)));
+    typedef uint32_t rti_t __attribute__((ext_vector_type(RC)));
     float k0;
     uint32_t i0;
     rtk_t rk;
     rti_t ri;
 };
 
-__device__ __forceinline__ float rt_key(const RegTop &h, const uint2 *hp, int lane, int e)
+template <int RC>
+__device__ __forceinline__ float rt_key(const RegTop<RC> &h, const uint2 *hp, int lane, int e)
 {
+    constexpr int RT = RC + 1;
     if (e >= RT) return __uint_as_float(hp[(e - RT) * WAVE + lane].x);
     float v = h.k0;
 #pragma unroll
@@ -520,8 +531,10 @@ __device__ __forceinline__ float rt_key(const RegTop &h, const uint2 *hp, int la
     return v;
 }
 
-__device__ __forceinline__ uint32_t rt_id(const RegTop &h, const uint2 *hp, int lane, int e)
+template <int RC>
+__device__ __forceinline__ uint32_t rt_id(const RegTop<RC> &h, const uint2 *hp, int lane, int e)
 {
+    constexpr int RT = RC + 1;
     if (e >= RT) return hp[(e - RT) * WAVE + lane].y;
     uint32_t v = h.i0;
 #pragma unroll
@@ -529,8 +542,11 @@ __device__ __forceinline__ uint32_t rt_id(const RegTop &h, const uint2 *hp, int 
     return v;
 }
 
-__device__ __forceinline__ void rt_set(RegTop &h, uint2 *hp, int lane, int e, float k, uint32_t id)
+template <int RC>
+__device__ __forceinline__ void rt_set(RegTop<RC> &h, uint2 *hp, int lane, int e, float k,
+                                       uint32_t id)
 {
+    constexpr int RT = RC + 1;
     if (e >= RT) {
         hp[(e - RT) * WAVE + lane] = make_uint2(__float_as_uint(k), id);
     } else {
@@ -548,11 +564,12 @@ __device__ __forceinline__ void rt_set(RegTop &h, uint2 *hp, int lane, int e, fl
 // sift (d, id) down from register entry c, 1 <= c <= RC: the largest of its LDS children
 // (read all LC of them unchecked: the caller keeps entries past `size` at -inf) moves up
 // if it exceeds d, and d takes its place
-template <bool CHECKED>
-__device__ __forceinline__ void rt_sift_l1(RegTop &h, uint2 *hp, int lane, int size, float d,
+template <bool CHECKED, int RC>
+__device__ __forceinline__ void rt_sift_l1(RegTop<RC> &h, uint2 *hp, int lane, int size, float d,
                                            uint32_t id, int c)
 {
-    const int c0 = rt_child0(c);
+    constexpr int RT = RC + 1;
+    const int c0 = rt_child0<RC>(c);
     float vk = d;
     uint32_t vi = id;
     if (c0 < size) {
@@ -597,8 +614,8 @@ __device__ __forceinline__ void rt_sift_l1(RegTop &h, uint2 *hp, int lane, int s
 }
 
 // sift (d, id) down from the root
-template <bool CHECKED>
-__device__ __forceinline__ void rt_sift_root(RegTop &h, uint2 *hp, int lane, int size, float d,
+template <bool CHECKED, int RC>
+__device__ __forceinline__ void rt_sift_root(RegTop<RC> &h, uint2 *hp, int lane, int size, float d,
                                              uint32_t id)
 {
     float kc;
@@ -623,7 +640,7 @@ __device__ __forceinline__ void rt_sift_root(RegTop &h, uint2 *hp, int lane, int
     if (kc > d) {
         h.k0 = kc;
         h.i0 = idc;
-        rt_sift_l1<CHECKED>(h, hp, lane, size, d, id, ic);
+        rt_sift_l1<CHECKED, RC>(h, hp, lane, size, d, id, ic);
     } else {
         h.k0 = d;
         h.i0 = id;
@@ -632,22 +649,22 @@ __device__ __forceinline__ void rt_sift_root(RegTop &h, uint2 *hp, int lane, int
 
 // Heap entry access for both kernels: register-top packed (mixed) or key and id arrays
 // (FP64).
-template <bool PACKED, typename KeyT>
-__device__ __forceinline__ KeyT heap_key(const RegTop &h, const uint2 *hp, const KeyT *hk,
+template <bool PACKED, typename KeyT, int RC>
+__device__ __forceinline__ KeyT heap_key(const RegTop<RC> &h, const uint2 *hp, const KeyT *hk,
                                          int lane, int e)
 {
     if constexpr (PACKED) return rt_key(h, hp, lane, e);
     else return hk[e * WAVE + lane];
 }
-template <bool PACKED, typename KeyT>
-__device__ __forceinline__ int32_t heap_id(const RegTop &h, const uint2 *hp, const int32_t *hi,
+template <bool PACKED, typename KeyT, int RC>
+__device__ __forceinline__ int32_t heap_id(const RegTop<RC> &h, const uint2 *hp, const int32_t *hi,
                                            int lane, int e)
 {
     if constexpr (PACKED) return (int32_t)rt_id(h, hp, lane, e);
     else return hi[e * WAVE + lane];
 }
-template <bool PACKED, typename KeyT>
-__device__ __forceinline__ void heap_set(RegTop &h, uint2 *hp, KeyT *hk, int32_t *hi, int lane,
+template <bool PACKED, typename KeyT, int RC>
+__device__ __forceinline__ void heap_set(RegTop<RC> &h, uint2 *hp, KeyT *hk, int32_t *hi, int lane,
                                          int e, KeyT k, int32_t id)
 {
     if constexpr (PACKED) {
@@ -659,13 +676,13 @@ __device__ __forceinline__ void heap_set(RegTop &h, uint2 *hp, KeyT *hk, int32_t
 }
 // sift from entry e: the root, a register child, or (FP64) any entry; the mixed kernel's
 // LDS entries are leaves and are never sifted from
-template <bool PACKED, bool CHECKED, typename KeyT>
-__device__ __forceinline__ void heap_sift_any(RegTop &h, uint2 *hp, KeyT *hk, int32_t *hi,
+template <bool PACKED, bool CHECKED, typename KeyT, int RC>
+__device__ __forceinline__ void heap_sift_any(RegTop<RC> &h, uint2 *hp, KeyT *hk, int32_t *hi,
                                               int lane, int size, KeyT d, int32_t id, int e)
 {
     if constexpr (PACKED) {
-        if (e == 0) rt_sift_root<CHECKED>(h, hp, lane, size, (float)d, (uint32_t)id);
-        else rt_sift_l1<CHECKED>(h, hp, lane, size, (float)d, (uint32_t)id, e);
+        if (e == 0) rt_sift_root<CHECKED, RC>(h, hp, lane, size, (float)d, (uint32_t)id);
+        else rt_sift_l1<CHECKED, RC>(h, hp, lane, size, (float)d, (uint32_t)id, e);
     } else {
         heap_sift<KeyT, CHECKED>(hk, hi, lane, size, d, id, e);
     }

@@ -3,8 +3,8 @@
 // One wave handles 64 consecutive Morton-sorted query particles, one query per lane.
 // The wave walks the binary radix tree together (a node is entered when any lane needs
 // it), stages the particles of each leaf bucket in LDS and every lane keeps the k (mixed:
-// k+1) nearest candidates seen so far in a max-heap: mixed, the root and its 16 children
-// in registers and their children in LDS; FP64, an 8-ary heap in LDS.
+// k+1) nearest candidates seen so far in a max-heap: mixed, the root and its 8 children (16
+// above k = 64) in registers and their children in LDS; FP64, an 8-ary heap in LDS.
 //
 // The per-wave state is kept in plain local variables and the repeated steps are macros:
 // a state struct (or closures capturing by reference) made the compiler spill it to
@@ -89,6 +89,8 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int4 *__restrict
     // cost it a wave per CU of LDS) and checks child indices.
     constexpr int KX = 1;
     constexpr int KHCAP = MIXED ? KCAP + 1 : KCAP;
+    constexpr int RC = heap_rc<KCAP>();
+    constexpr int RT = RC + 1;
     constexpr bool SIFT_CHECKED = !MIXED;
     static_assert(KCAP % HEAP_D == 0, "heap capacity must be a multiple of the arity");
     // staged buckets whose insertions are deferred (the FP64 heap is twice as large, so its
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int4 *__restrict
     __shared__ uint2 hp[NLDS * WAVE];
     __shared__ KeyT hk[MIXED ? 1 : KHCAP * WAVE];
     __shared__ int32_t hi[MIXED ? 1 : KHCAP * WAVE];
-    RegTop rt;
+    RegTop<RC> rt;
     rt.k0 = -INFINITY;
     rt.i0 = 0u;
     rt.rk = -INFINITY;
@@ -238,7 +240,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int4 *__restrict
     do {                                                                                       \
         if constexpr (MIXED) {                                                                 \
             for (int e_ = RC; e_ >= 0; --e_)                                                   \
-                if (e_ == 0 || rt_child0(e_) < KH) HSIFT(KH, HK(e_), HI(e_), e_, SIFT_CHECKED); \
+                if (e_ == 0 || rt_child0<RC>(e_) < KH) HSIFT(KH, HK(e_), HI(e_), e_, SIFT_CHECKED); \
         } else {                                                                               \
             for (int e_ = (KH - 2) / HEAP_D; e_ >= 0; --e_)                                    \
                 HSIFT(KH, HK(e_), HI(e_), e_, SIFT_CHECKED);                                   \
@@ -341,6 +343,13 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int4 *__restrict
         }                                                                                      \
         nbuf = 0;                                                                              \
     } while (0)
+    // A seeding flush: once every lane's heap is full (the direct fill), two pending
+    // candidates per lane per step as in the traversal.
+#define KNN_FLUSH_S()                                                                          \
+    do {                                                                                       \
+        if (__all(filled >= KH)) KNN_FLUSH_T();                                                \
+        else KNN_FLUSH(true);                                                                  \
+    } while (0)
 #define KNN_FLUSH_T()                                                                          \
     do {                                                                                       \
         if constexpr (MIXED && NBUF == 2 && AHOP_PICKW > 1) KNN_FLUSH_W();                     \
@@ -391,7 +400,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int4 *__restrict
     // filling it.
 #define KNN_STAGE(fst, cnt, seeding)                                                           \
     do {                                                                                       \
-        if ((seeding) && nbuf == NBUF) KNN_FLUSH(true);                                        \
+        if ((seeding) && nbuf == NBUF) KNN_FLUSH_S();                                          \
         __syncthreads();                                                                       \
         if (lane < (cnt)) {                                                                    \
             const double4 c_ = spos[(fst) + lane];                                             \
@@ -507,7 +516,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int4 *__restrict
             if (d < tau) KNN_INSERT(d, j);
         }
     }
-    KNN_FLUSH(true);
+    KNN_FLUSH_S();
     if (filled < KH) {
         // a clipped window (array ends) or an idle lane: the unfilled entries are +inf
         KNN_HEAPIFY();
@@ -616,6 +625,7 @@ __global__ __launch_bounds__(64, 2) void k_knn(KnnArgs A, const int4 *__restrict
 #undef KNN_FLUSH
 #undef KNN_FLUSH_W
 #undef KNN_FLUSH_T
+#undef KNN_FLUSH_S
 #undef KNN_PICK_V
 #undef KNN_CSWAP
 #undef KNN_INSERT
