@@ -593,7 +593,10 @@ def main():
                                   "buckets": round(last.knn_buckets / w, 1),
                                   "candidates_per_lane": round(last.knn_candidates / w, 1),
                                   "insert_steps": round(last.knn_insert_steps / w, 1),
-                                  "inserts_per_lane": round(last.knn_inserts / w / 64, 1)}
+                                  "inserts_per_lane": round(last.knn_inserts / w / 64, 1),
+                                  "log_max": int(last.knn_log_max),
+                                  "log_overflow": int(last.knn_log_overflow),
+                                  "fp32_fp64_misorder": int(last.knn_misorder)}
 
     achieved = KNN_BYTES_PER_PARTICLE * n / (knn_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -44,7 +44,8 @@ class RunInfo(ctypes.Structure):
                 ("t_sort", _f32), ("t_tree", _f32), ("t_knn", _f32), ("t_fallback", _f32),
                 ("t_hop", _f32), ("t_saddle", _f32), ("t_total", _f32),
                 ("knn_nodes", _i64), ("knn_buckets", _i64), ("knn_candidates", _i64),
-                ("knn_insert_steps", _i64), ("knn_inserts", _i64), ("knn_waves", _i64)]
+                ("knn_insert_steps", _i64), ("knn_inserts", _i64), ("knn_waves", _i64),
+                ("knn_log_max", _i64), ("knn_log_overflow", _i64), ("knn_misorder", _i64)]
 
 
 # name -> (restype, argtypes); must list every function declared in include/amdahop.h

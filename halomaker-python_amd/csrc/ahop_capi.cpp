@@ -471,6 +471,9 @@ static void fill_info(ahop_run_info *info, const Engine &e, bool saddle)
     info->knn_insert_steps = (int64_t)r.knn_stats[3];
     info->knn_inserts = (int64_t)r.knn_stats[4];
     info->knn_waves = (int64_t)r.knn_stats[5];
+    info->knn_log_max = (int64_t)r.knn_stats[6];
+    info->knn_log_overflow = (int64_t)r.knn_stats[7];
+    info->knn_misorder = (int64_t)r.knn_stats[8];
 }
 
 int ahop_knn_device(const double *d_pos, const double *d_mass, int64_t n,

@@ -61,8 +61,10 @@ struct DeviceResults {
     double *densityg = nullptr;       // [ngroups]
     int64_t nsaddle = 0;
     // work counters of the main kNN launch: nodes visited, buckets scanned, candidate
-    // distances per lane, heap-insertion steps, heap insertions, waves
-    unsigned long long knn_stats[6] = {0, 0, 0, 0, 0, 0};
+    // distances per lane, heap-insertion steps, heap insertions, waves, and of the selection
+    // kernel the largest per-lane log, the lanes whose log overflowed and the lanes whose
+    // FP32 order of the k nearest differed from the FP64 order (both re-run in FP64)
+    unsigned long long knn_stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 // Growable device allocation owned by an Engine.
@@ -73,6 +75,8 @@ struct DevBuf {
     void release();
     ~DevBuf() { release(); }
 };
+
+struct KnnArgs;   // kNN kernel arguments (ahop_kernels.hip)
 
 class Engine {
   public:
@@ -94,6 +98,7 @@ class Engine {
   private:
     void build_tree(const double *d_pos, const double *d_mass, int64_t n, const Params &p);
     void knn_density(const Params &p);
+    void launch_knn_sel(const KnnArgs &a, unsigned nw, hipStream_t s);
     void hop_groups(const Params &p);
     void saddles(const Params &p);
 
@@ -108,7 +113,8 @@ class Engine {
         b_rank_o, b_density, b_gid, b_hop, b_counter, b_fallback, b_emit_cnt, b_emit_off,
         b_ekey, b_eval, b_ekey2, b_eval2, b_ukey, b_uval, b_keep, b_keep_off, b_skey, b_sval,
         b_sad_off, b_sad_isad, b_sad_rho, b_sad_cnt, b_densityg, b_misc, b_roots, b_fbox, b_fb_sorted,
-        b_nrec, b_cbox;
+        b_nrec, b_cbox, b_klog, b_wctr;
+    int sel_cus_ = 0;   // compute units of the device (persistent grid of the selection kernel)
     int64_t nmax_ = 0;
 };
 

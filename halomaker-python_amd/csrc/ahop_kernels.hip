@@ -436,7 +436,8 @@ struct KnnArgs {
 
 // work counters accumulated by k_knn when KnnArgs::stats is set
 enum { KNN_STAT_NODES = 0, KNN_STAT_BUCKETS, KNN_STAT_CANDIDATES, KNN_STAT_INSERT_STEPS,
-       KNN_STAT_INSERTS, KNN_STAT_WAVES, KNN_STAT_N };
+       KNN_STAT_INSERTS, KNN_STAT_WAVES, KNN_STAT_LOG_MAX, KNN_STAT_LOG_OVERFLOW,
+       KNN_STAT_MISORDER, KNN_STAT_N };
 
 // Heap of the KH smallest keys per lane, lane-interleaved in LDS (entry e of lane l at e*64+l,
 // so any per-lane entry index is bank-conflict free).
@@ -715,6 +716,7 @@ __device__ __forceinline__ void heap_sift_any(RegTop<RC> &h, uint2 *hp, KeyT *hk
 #define AHOP_MIXED_NBUF 2
 #endif
 #include "knn_wave.h"
+#include "knn_sel.h"
 
 // a double held by lane i, as a wave-uniform value
 __device__ __forceinline__ double lane_d(double v, int i)
@@ -1059,7 +1061,7 @@ void Engine::release_all()
                      &b_counter, &b_fallback, &b_emit_cnt, &b_emit_off, &b_ekey, &b_eval,
                      &b_ekey2, &b_eval2, &b_ukey, &b_uval, &b_keep, &b_keep_off, &b_skey,
                      &b_sval, &b_sad_off, &b_sad_isad, &b_sad_rho, &b_sad_cnt, &b_densityg,
-                     &b_misc, &b_roots, &b_fbox, &b_fb_sorted, &b_nrec, &b_cbox};
+                     &b_misc, &b_roots, &b_fbox, &b_fb_sorted, &b_nrec, &b_cbox, &b_klog, &b_wctr};
     for (auto *b : all) b->release();
     res_ = DeviceResults{};
 }
@@ -1148,6 +1150,35 @@ static void dispatch_knn(int K, bool mixed, const KnnArgs &a, unsigned nb, hipSt
     else launch_knn<128>(mixed, a, nb, s);
 }
 
+// The mixed kNN for k <= 64: the selection kernel on a persistent grid of as many waves as
+// the device holds at once (each with its log region), taking the nw waves of queries from
+// per-XCD counters.
+template <int KCAP>
+static void launch_sel(const KnnArgs &a, unsigned nw, int cus, DevBuf &klog, DevBuf &wctr,
+                       hipStream_t s)
+{
+    int per_cu = 0;
+    AHOP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_knn_sel<KCAP>, WAVE, 0));
+    const unsigned g = std::max(1u, std::min(nw, (unsigned)(std::max(per_cu, 1) * cus)));
+    uint2 *lg = static_cast<uint2 *>(klog.ensure((size_t)g * SEL_CAPL * WAVE * sizeof(uint2)));
+    int *ctr = static_cast<int *>(wctr.ensure(8 * sizeof(int)));
+    AHOP_HIP(hipMemsetAsync(ctr, 0, 8 * sizeof(int), s));
+    hipLaunchKernelGGL((k_knn_sel<KCAP>), dim3(g), dim3(WAVE), 0, s, a, a.nrec, a.cbox, a.fbox,
+                       a.spos, lg, ctr);
+}
+
+void Engine::launch_knn_sel(const KnnArgs &a, unsigned nw, hipStream_t s)
+{
+    if (sel_cus_ == 0) {
+        int dev = 0;
+        AHOP_HIP(hipGetDevice(&dev));
+        AHOP_HIP(hipDeviceGetAttribute(&sel_cus_, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    if (a.K <= 16) launch_sel<16>(a, nw, sel_cus_, b_klog, b_wctr, s);
+    else if (a.K <= 32) launch_sel<32>(a, nw, sel_cus_, b_klog, b_wctr, s);
+    else launch_sel<64>(a, nw, sel_cus_, b_klog, b_wctr, s);
+}
+
 void Engine::knn_density(const Params &p)
 {
     hipStream_t s = stream_;
@@ -1186,12 +1217,13 @@ void Engine::knn_density(const Params &p)
     a.fb_count = misc + 2;
     a.nact = cnt;
     a.overflow = misc + 1;
-    unsigned long long *stats = as<unsigned long long>(b_counter, 16) + 8;
-    AHOP_HIP(hipMemsetAsync(stats, 0, 8 * sizeof(unsigned long long), s));
+    unsigned long long *stats = as<unsigned long long>(b_counter, 24) + 8;
+    AHOP_HIP(hipMemsetAsync(stats, 0, 16 * sizeof(unsigned long long), s));
     a.stats = stats;
     unsigned nb = (unsigned)((n + 63) / 64);
     a.nblocks = (int)nb;
-    dispatch_knn(p.nvoisins, p.mixed != 0, a, nb, s);
+    if (p.mixed && p.nvoisins <= 64) launch_knn_sel(a, nb, s);
+    else dispatch_knn(p.nvoisins, p.mixed != 0, a, nb, s);
     AHOP_HIP(hipEventRecord(ev_[3], s));
     AHOP_HIP(hipMemcpyAsync(res_.knn_stats, stats, KNN_STAT_N * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, s));

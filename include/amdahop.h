@@ -93,6 +93,10 @@ typedef struct {
     /* work counters of the main kNN launch: tree nodes visited, leaf buckets scanned,
      * candidate distances (per lane), heap-insertion steps, heap insertions, waves */
     int64_t knn_nodes, knn_buckets, knn_candidates, knn_insert_steps, knn_inserts, knn_waves;
+    /* mixed path, k <= 64: the largest per-query candidate log, the queries whose log
+     * overflowed and those whose FP32 order of the k nearest differed from the FP64 order
+     * (both counted in nfallback) */
+    int64_t knn_log_max, knn_log_overflow, knn_misorder;
 } ahop_run_info;
 
 /* kNN + SPH density + hop + group ids (+ saddle edges) on device-resident inputs:
