@@ -10,11 +10,13 @@
 //    child) lives in LDS at 4 B per entry.  The wave's LDS is 18 KB instead of 30 KB, so 8
 //    waves fit per CU instead of 5 (the registers allow 8 too), and a replacement moves keys
 //    only;
-//  * every (key, sorted slot) it puts into the heap is appended to a log in global memory
-//    (one region per resident wave: the grid is persistent and takes waves of queries from
-//    per-XCD counters).  A key is evicted only as the heap's maximum, and the maximum never
-//    grows, so after the search the logged entries whose key is below the final root are
-//    exactly the heap's entries below it: the lane's k nearest when they number k.
+//  * every (key, sorted slot) it puts into the heap after the seeding's direct fill is
+//    appended to a per-lane log in global memory (one region per resident wave: the grid is
+//    persistent and takes waves of queries from per-XCD counters); the direct fill's entries
+//    are known from the seed window.  A key is evicted only as the heap's maximum, and the
+//    maximum never grows, so after the search the filled and logged entries whose key is
+//    below the final root are exactly the heap's entries below it: the lane's k nearest when
+//    they number k.
 // The k survivors are then put in FP32-key order in registers (Batcher's odd-even merge
 // network, ties by slot), their exact FP64 distances are checked to be in the same order
 // (FP32 near-ties can swap two; such a query is re-run on the FP64 path, as are those whose
@@ -23,10 +25,14 @@
 #pragma once
 #include <utility>
 
-// log entries per lane: the k + 1 of the seeding fill plus the replacements that follow
-// (about 104 per lane on average at the 16.8M-particle box, at most 2 380; 1 722 of its
-// 16.8M queries overflow 512 and are re-run on the FP64 path)
+// log entries per lane: the replacements after the seeding's direct fill (about 104 per lane
+// on average at the 16.8M-particle box, at most 2 315; 938 of its 16.8M queries overflow 512
+// and are re-run on the FP64 path)
 constexpr int SEL_CAPL = 512;
+// log entries the collection reads ahead per lane
+#ifndef SEL_P2U
+#define SEL_P2U 8
+#endif
 
 // keys of the heap's top two levels: the root and RC children
 template <int RC>
@@ -184,8 +190,10 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
     const float mLx = 4.0f * U32 * Lxf, mLy = 4.0f * U32 * Lyf, mLz = 4.0f * U32 * Lzf;
     const float Lmin = fminf(Lxf, fminf(Lyf, Lzf));
     const bool root_is_bucket = (A.n <= A.bucket);
-    // this block's log: entry e of this lane at lg[e * WAVE]
-    const int64_t lg_off = (int64_t)blockIdx.x * SEL_CAPL * WAVE + lane;
+    // this block's log: SEL_CAPL consecutive entries per lane, so that a lane's appends fill
+    // whole cache lines (lane-interleaved entries left lines partly written: 55 GB written
+    // per launch at the 16.8M-particle box, kernel 65.7 against 63.2 ms)
+    const int64_t lg_off = ((int64_t)blockIdx.x * WAVE + lane) * SEL_CAPL;
     // waves of queries: XCD x (blocks are dispatched round-robin over the 8 XCDs) first takes
     // the contiguous range x of them, so that neighbouring waves share an L2, then helps the
     // others
@@ -259,6 +267,8 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         // heap entries written by the seeding (lanes without a search count as full)
         int filled = tau < 0.0f ? KH : 0;
         int nlog = 0;
+        bool dfill = false;   // the lane's heap was filled from the two first staged buckets
+        int dfill_rs = 0;     // with their slots [dfill_rs, dfill_rs + KH) (relative)
 
 #define SEL_KEY(X, Y, Z, out)                                                                  \
     do {                                                                                       \
@@ -268,7 +278,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
 
 #define SEL_LOG(d, id)                                                                         \
     do {                                                                                       \
-        if (nlog < SEL_CAPL) lg[nlog * WAVE] = make_uint2(__float_as_uint(d), (uint32_t)(id)); \
+        if (nlog < SEL_CAPL) lg[nlog] = make_uint2(__float_as_uint(d), (uint32_t)(id)); \
         ++nlog;                                                                                \
     } while (0)
 
@@ -433,7 +443,8 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
             // Two full buckets of the window are staged, and every lane with a search has
             // accepted all of them but its own slot: KH of them, centred on the lane's own
             // slot as far as the two buckets allow, go straight into its heap (its own slot,
-            // if among them, with key +inf, which the first insertion evicts) and its log.
+            // if among them, with key +inf, which the first insertion evicts).  They are not
+            // logged: the collection stages the two buckets again (C3 kernel 62.2 -> 61.8 ms).
             if (nbuf == 2 && f == seed_lo + WAVE && cnt == WAVE) {
                 const int off = q - seed_lo;
                 const uint64_t self0 = off < WAVE ? 1ull << off : 0ull;
@@ -448,7 +459,6 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                             SEL_KEY(sx[r], sy[r], sz[r], d);
                             d = r == off ? INFINITY : d;
                             kt_set<RC>(h, hw, lane, e, d);
-                            lg[e * WAVE] = make_uint2(__float_as_uint(d), (uint32_t)(seed_lo + r));
                         }
                     }
                     for (int e = RT; e < KH; ++e) {
@@ -457,10 +467,10 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                         SEL_KEY(sx[r], sy[r], sz[r], d);
                         d = r == off ? INFINITY : d;
                         hw[(e - RT) * WAVE + lane] = d;
-                        lg[e * WAVE] = make_uint2(__float_as_uint(d), (uint32_t)(seed_lo + r));
                     }
                     filled = KH;
-                    nlog = KH;
+                    dfill = true;
+                    dfill_rs = rs;
                     st_ins += KH;
                     SEL_HEAPIFY();
                     tau = h.k0;
@@ -573,7 +583,6 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
 #undef SEL_HEAPIFY
 #undef SEL_INSERT
 #undef SEL_LOG
-#undef SEL_KEY
 
         // ---- collection: the logged entries below the final (k+1)-th key, compacted in
         // place to the front of the lane's log (entry e is read before any write reaches it)
@@ -583,21 +592,47 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         st_ovf += (nlog > SEL_CAPL) ? 1u : 0u;
         int cnt = 0;
         float kmax = -INFINITY;
-        for (int e = 0; __any(e < nl); e += 4) {
-            uint2 v[4];
+        for (int e = 0; __any(e < nl); e += SEL_P2U) {
+            uint2 v[SEL_P2U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u)
-                v[u] = e + u < nl ? lg[(e + u) * WAVE] : make_uint2(0x7f800000u, 0u);
+            for (int u = 0; u < SEL_P2U; ++u)
+                v[u] = e + u < nl ? lg[e + u] : make_uint2(0x7f800000u, 0u);
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < SEL_P2U; ++u) {
                 const float kk = __uint_as_float(v[u].x);
                 if (kk < T) {
-                    if (cnt < KCAP) lg[cnt * WAVE] = v[u];
+                    if (cnt < KCAP) lg[cnt] = v[u];
                     ++cnt;
                     kmax = fmaxf(kmax, kk);
                 }
             }
         }
+        if (__any(dfill)) {
+            // the entries of the direct fill, from the two first buckets of the seed window
+            // staged again (every log entry is read by now, so they may go anywhere)
+            __syncthreads();
+            for (int b = 0; b < 2; ++b) {
+                const double4 c_ = spos[seed_lo + b * WAVE + lane];
+                sx[b * WAVE + lane] = (float)wrapd(c_.x - cx, Lx);
+                sy[b * WAVE + lane] = (float)wrapd(c_.y - cy, Ly);
+                sz[b * WAVE + lane] = (float)wrapd(c_.z - cz, Lz);
+            }
+            __syncthreads();
+            if (dfill) {
+                const int off = q - seed_lo;
+                for (int e = 0; e < KH; ++e) {
+                    const int r = dfill_rs + e;
+                    float kk;
+                    SEL_KEY(sx[r], sy[r], sz[r], kk);
+                    if (r != off && kk < T) {
+                        if (cnt < KCAP) lg[cnt] = make_uint2(__float_as_uint(kk), (uint32_t)(seed_lo + r));
+                        ++cnt;
+                        kmax = fmaxf(kmax, kk);
+                    }
+                }
+            }
+        }
+#undef SEL_KEY
         // Certificate (as k_knn): the k kept are the true k nearest when the (k+1)-th key
         // clears the k-th by twice the error bound E of a key of that size and every particle
         // within that radius lies in the periodic image the wave-centred frame picks.  Ties
@@ -620,7 +655,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                 // every log has more than KCAP entries: all are loaded, and those past k are
                 // masked to all ones arithmetically (a select would become a branch around
                 // each load, serialising them)
-                const uint2 v = lg[e * WAVE];
+                const uint2 v = lg[e];
                 kv[e] = (((uint64_t)v.x << 32) | v.y) | (0ull - (uint64_t)(e >= K));
             }
             net_sort<KCAP>(kv, std::make_integer_sequence<int, SORT_NET<KCAP>.n>{});
