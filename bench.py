@@ -601,7 +601,7 @@ def main():
     achieved = KNN_BYTES_PER_PARTICLE * n / (knn_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "kernel": "k_knn<64,mixed> (kNN + SPH density)", "kernel_ms": round(knn_ms, 3),
+                "kernel": "k_knn_sel<64> (kNN + SPH density, mixed precision)", "kernel_ms": round(knn_ms, 3),
                 "bytes_per_particle": KNN_BYTES_PER_PARTICLE,
                 "step_achieved_GBs": round(STEP_BYTES_PER_PARTICLE * n / (ms * 1e-3) / 1e9, 2)}
     # the kernel is bound by instruction issue, not HBM: its FP32 candidate arithmetic (every
