@@ -10,12 +10,13 @@ Host-side entry points over libamdahop.so:
 from __future__ import annotations
 
 import ctypes
+from typing import NamedTuple
 
 from ._lib import HotParams, RunInfo, check, lib
 
 __all__ = ["run_device", "device_saddle", "device_hop", "knn_device", "density_adjoint", "sph_density",
            "halo_props", "HALO_FIELDS", "halo_profiles", "PROFILE_FIELDS", "halo_stellar",
-           "STELLAR_FIELDS", "HotParams", "RunInfo"]
+           "STELLAR_FIELDS", "AngMomShells", "write_ang_mom_of_r", "HotParams", "RunInfo"]
 
 
 def run_device(pos, mass, nvoisins, nhop=16, rho_threshold=80.0, box=1.0, mixed=True,
@@ -220,7 +221,7 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
                tile=0, workgroup=0, sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0,
                light=False, flag_period=1, virial_pe_profile="shell",
                virial_pe_norm_policy="profile_outer_for_large", virial_shell_self_mode="count",
-               ang_mom_of_r=False):
+               ang_mom_of_r=False, nshells=100):
     """Per-halo properties of a CSR member layout (device tensors), one row per halo in
     the order of HALO_FIELDS.  ``sc`` / ``dpmmc`` recentre each halo as the reference's
     SC / DPMMC options do (with ``dcell_min`` and ``eps_sc``; DPMMC needs ``density``).
@@ -230,9 +231,13 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
     ``virial_pe_norm_policy`` ('profile_outer_for_large' or 'legacy') and
     ``virial_shell_self_mode`` ('count' or 'half_shell') of potential_profile_1b40
     (compute_halo_props.py:1065-1123); an unknown value is a ValueError as there.
-    ``ang_mom_of_r`` (the per-shell angular momentum file of det_ang_momentum_per_shell_1b5)
-    is not provided: NotImplementedError.
-    Returns (out tensor (H, 31) float64, HaloInfo)."""
+    ``ang_mom_of_r`` (ANG_MOM_OF_R) also bins each halo's members into ``nshells`` (the
+    reference's 100) ellipsoidal shells of the all-member ellipsoid and sums mass and
+    angular momentum per shell, as det_ang_momentum_per_shell_1b5 (:1970-2013) does for the
+    halos whose rvir is finite (:1257); ``write_ang_mom_of_r`` writes its file.
+    Returns (out tensor (H, 31) float64, HaloInfo), and with ``ang_mom_of_r`` a third item,
+    AngMomShells(shells (H, 4, nshells): per shell m, Lx, Ly, Lz; axes (H, 12): amax, bmax,
+    cmax, then the principal axes v row-major), NaN rows for halos with no finite rvir."""
     import numpy as np
     import torch
 
@@ -245,9 +250,10 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
         if val not in allowed:
             raise ValueError(f"unknown {name}={val!r}")
         sw[name] = allowed.index(val)
-    if ang_mom_of_r:
-        raise NotImplementedError("ANG_MOM_OF_R (per-shell angular momentum file) is not "
-                                  "provided")
+    if ang_mom_of_r and light:
+        raise ValueError("ang_mom_of_r needs the full mode (light=False)")
+    if ang_mom_of_r and not 1 <= int(nshells) <= 2000:
+        raise ValueError("nshells must be in [1, 2000]")
     if flag_period == 0:
         lbox = float("inf")       # no periodic wrap anywhere (FlagPeriod = 0)
 
@@ -269,6 +275,14 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
                    int(workgroup), 1 if sc else 0, 1 if dpmmc else 0, float(dcell_min),
                    float(eps_sc), 1 if light else 0, sw["virial_pe_profile"],
                    sw["virial_pe_norm_policy"], sw["virial_shell_self_mode"])
+    ang = None
+    if ang_mom_of_r:
+        ang = AngMomShells(torch.empty((nh, 4, int(nshells)), dtype=torch.float64,
+                                       device=pos.device),
+                           torch.empty((nh, 12), dtype=torch.float64, device=pos.device))
+        p.ang_mom_nshells = int(nshells)
+        p.d_ang_mom = ang.shells.data_ptr()
+        p.d_ang_axes = ang.axes.data_ptr()
     info = HaloInfo()
     torch.cuda.synchronize(pos.device)
     dptr = ctypes.c_void_p(density.data_ptr()) if density is not None else None
@@ -276,7 +290,59 @@ def halo_props(offsets, pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, 
                                        ctypes.c_void_p(vel.data_ptr()),
                                        ctypes.c_void_p(mass.data_ptr()), dptr, ctypes.byref(p),
                                        ctypes.c_void_p(out.data_ptr()), ctypes.byref(info)))
+    if ang is not None:
+        return out, info, ang
     return out, info
+
+
+class AngMomShells(NamedTuple):
+    """Per-shell mass and angular momentum of halo_props(ang_mom_of_r=True)."""
+    shells: object   # (H, 4, nshells) device tensor: m, Lx, Ly, Lz per shell
+    axes: object     # (H, 12) device tensor: amax, bmax, cmax, v row-major
+
+
+def write_ang_mom_of_r(path, ids, out, ang, nb_of_halos, nb_of_subhalos):
+    """Write the ANG_MOM_OF_R file of compute_halo_props.py (:477-484, :1257-1258, :2014-2019)
+    as a Fortran sequential unformatted file (4-byte record markers): a record
+    (nb_of_halos, nb_of_subhalos) and a record (nshells), as int64 (the Python integers the
+    reference passes to FortranFile.write_record), then one record per halo with a finite
+    rvir, in the order of ``ids``: id (int32), px, py, pz, vx, vy, vz, rvir, mvir, m, r, spin,
+    amax, bmax, cmax, v[0, :3], v[1, :3], v[2, :3], m[:nshells], Lx[:nshells], Ly[:nshells],
+    Lz[:nshells] (float64).  The reference's own write of the last four (``L[0:nshells].x``
+    on a Python list of vectors) raises AttributeError; this writes the per-shell arrays it
+    names.  ``ids`` (H), ``out`` (H, 31) and ``ang`` (AngMomShells) as halo_props returns
+    them (tensors or arrays).  Returns the number of halo records written."""
+    import numpy as np
+
+    def host(t):
+        return t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+
+    ids = np.asarray(host(ids), dtype=np.int32)
+    o = host(out)
+    sh = host(ang.shells)
+    ax = host(ang.axes)
+    nsh = sh.shape[2]
+    cols = [HALO_FIELDS.index(f) for f in ("px", "py", "pz", "vx", "vy", "vz", "rvir", "mvir",
+                                            "m", "r", "spin")]
+
+    def record(f, payload):
+        n = np.array([len(payload)], dtype="<u4").tobytes()
+        f.write(n + payload + n)
+
+    nrec = 0
+    with open(path, "wb") as f:
+        record(f, np.array([nb_of_halos, nb_of_subhalos], dtype="<i8").tobytes())
+        record(f, np.array([nsh], dtype="<i8").tobytes())
+        for h in range(len(ids)):
+            if not np.isfinite(o[h, HALO_FIELDS.index("rvir")]):
+                continue
+            payload = (np.array([ids[h]], dtype="<i4").tobytes()
+                       + np.ascontiguousarray(o[h, cols], dtype="<f8").tobytes()
+                       + np.ascontiguousarray(ax[h], dtype="<f8").tobytes()
+                       + np.ascontiguousarray(sh[h], dtype="<f8").tobytes())
+            record(f, payload)
+            nrec += 1
+    return nrec
 
 
 PROFILE_FIELDS = ["vmaxcir", "rmaxcir", "cNFW", "cNFWerr", "inslope", "inslopeerr"]

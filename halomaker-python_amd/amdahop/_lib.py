@@ -31,7 +31,8 @@ class HaloParams(ctypes.Structure):
                 ("tile", _i32), ("workgroup", _i32), ("sc", _i32), ("dpmmc", _i32),
                 ("dcell_min", _dbl), ("eps_sc", _dbl), ("light", _i32),
                 ("virial_pe_profile", _i32), ("virial_pe_norm_policy", _i32),
-                ("virial_shell_self_mode", _i32)]
+                ("virial_shell_self_mode", _i32), ("ang_mom_nshells", _i32),
+                ("d_ang_mom", ctypes.c_void_p), ("d_ang_axes", ctypes.c_void_p)]
 
 
 class HaloInfo(ctypes.Structure):

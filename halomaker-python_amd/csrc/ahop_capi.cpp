@@ -770,6 +770,9 @@ int ahop_halo_props_device(int64_t nhalos, const int64_t *offsets, const double 
         P.pe_baseline = params->virial_pe_profile;
         P.pe_norm_legacy = params->virial_pe_norm_policy;
         P.shell_half = params->virial_shell_self_mode;
+        P.angmom_nshells = params->ang_mom_nshells;
+        P.angmom = params->d_ang_mom;
+        P.angmom_axes = params->d_ang_axes;
         for (int v : {P.pe_baseline, P.pe_norm_legacy, P.shell_half})
             if (v != 0 && v != 1) throw std::runtime_error("virial switches must be 0 or 1");
         if (!(P.lbox > 0)) throw std::runtime_error("lbox must be > 0");

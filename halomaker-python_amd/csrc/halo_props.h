@@ -33,6 +33,11 @@ struct HaloParamsDev {
     int pe_baseline = 0;        // virial_pe_profile: 0 'shell', 1 'baseline' (-3/5 G M(<r)^2 RF)
     int pe_norm_legacy = 0;     // virial_pe_norm_policy: 0 'profile_outer_for_large', 1 'legacy'
     int shell_half = 0;         // virial_shell_self_mode: 0 'count', 1 'half_shell'
+    // ANG_MOM_OF_R (det_ang_momentum_per_shell_1b5): shells of the per-shell angular momentum
+    // (0: off), and its device outputs (nhalos x 4 x shells; nhalos x 12)
+    int angmom_nshells = 0;
+    double *angmom = nullptr;
+    double *angmom_axes = nullptr;
 };
 
 struct HaloTuning {

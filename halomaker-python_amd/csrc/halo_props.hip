@@ -872,6 +872,75 @@ __global__ __launch_bounds__(BS) void k_pass4b(HaloDev D, const HaloState *st, i
     for (int b = threadIdx.x; b < 3 * nr; b += BS) out[b] = hs[b];
 }
 
+// ---- per-shell mass and angular momentum (det_ang_momentum_per_shell_1b5 :1970-2013), the
+// ANG_MOM_OF_R output: members binned into nsh ellipsoidal shells of the all-member ellipsoid
+// (amax, bmax, cmax of tab_props_inside_1b40), i_ell = int(r_ell * nsh); per shell the mass
+// and L = sum m (dr x dv), dr the wrapped offset from the centre, dv = v - the halo's mean
+// velocity (no Hubble term).  Per-tile partials in LDS (4*nsh doubles), combined per halo in
+// tile order by k_angmom_combine.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_angmom(HaloDev D, const HaloState *st, int nsh,
+                                               double *hist, int *missed)
+{
+    extern __shared__ double hs[];   // 4*nsh doubles: m, Lx, Ly, Lz
+    const int tile = blockIdx.x;
+    const int h = D.tile_halo[tile];
+    const int64_t s = D.tile_start[tile], e = D.tile_end[tile];
+    const HaloState &S = st[h];
+    for (int b = threadIdx.x; b < 4 * nsh; b += BS) hs[b] = 0.0;
+    __syncthreads();
+    for (int64_t i = s + threadIdx.x; i < e; i += BS) {
+        double ra, rb, rc;
+        ell_proj(D, S, i, ra, rb, rc);
+        // r_ell = sqrt((dra / amax)**2 + (drb / bmax)**2 + (drc / cmax)**2) (:1988)
+        const double qa = ra / S.amax, qb = rb / S.bmax, qc = rc / S.cmax;
+        const double fi = sqrt(qa * qa + qb * qb + qc * qc) * nsh;
+        const int ib = (int)fi;
+        if (!(fi < (double)nsh) || ib >= nsh) {
+            atomicAdd(missed, 1);
+            continue;
+        }
+        const double m = D.mass[i];
+        const double dx = wrap_strict(D.pos[3 * i] - S.px, D.L);
+        const double dy = wrap_strict(D.pos[3 * i + 1] - S.py, D.L);
+        const double dz = wrap_strict(D.pos[3 * i + 2] - S.pz, D.L);
+        const double vx = D.vel[3 * i] - S.vx, vy = D.vel[3 * i + 1] - S.vy,
+                     vz = D.vel[3 * i + 2] - S.vz;
+        atomicAdd(&hs[ib], m);
+        atomicAdd(&hs[nsh + ib], m * (dy * vz - dz * vy));
+        atomicAdd(&hs[2 * nsh + ib], m * (dz * vx - dx * vz));
+        atomicAdd(&hs[3 * nsh + ib], m * (dx * vy - dy * vx));
+    }
+    __syncthreads();
+    double *o = hist + (int64_t)tile * 4 * nsh;
+    for (int b = threadIdx.x; b < 4 * nsh; b += BS) o[b] = hs[b];
+}
+
+// per halo: its tiles' shell sums in tile order into shells (nh x 4 x nsh: m, Lx, Ly, Lz per
+// shell) and the ellipsoid (amax, bmax, cmax, then the eigenvectors v row-major) into axes
+// (nh x 12).  The reference writes a halo's record only when its rvir is finite (:1257);
+// the other halos' rows are NaN.
+__global__ void k_angmom_combine(int nh, const int32_t *tile_first, const double *hist, int nsh,
+                                 const HaloState *st, const double *props, double *shells,
+                                 double *axes)
+{
+    const int h = blockIdx.x;
+    if (h >= nh) return;
+    const bool keep = isfinite(props[(int64_t)h * HP_NFIELDS + HP_RVIR]);
+    const int t0 = tile_first[h], t1 = tile_first[h + 1];
+    for (int b = threadIdx.x; b < 4 * nsh; b += blockDim.x) {
+        double acc = 0.0;
+        for (int t = t0; t < t1; ++t) acc += hist[(int64_t)t * 4 * nsh + b];
+        shells[(int64_t)h * 4 * nsh + b] = keep ? acc : NAN;
+    }
+    if (threadIdx.x < 12) {
+        const HaloState &S = st[h];
+        const int q = threadIdx.x;
+        const double v = q == 0 ? S.amax : q == 1 ? S.bmax : q == 2 ? S.cmax : S.ev[q - 3];
+        axes[(int64_t)h * 12 + q] = keep ? v : NAN;
+    }
+}
+
 // first pair index of row i of the upper triangle of an n x n pair matrix (pairs (i, j > i)
 // numbered row by row)
 __device__ __forceinline__ int64_t pair_row_start(int i, int n)
@@ -1390,6 +1459,14 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
             timed_pe = true;
             hipLaunchKernelGGL(k_virial<256>, dim3(nh), dim3(256), sizeof(double) * 4 * nr, s, nh,
                                d_tf, d_off, hist, nr, st, ep, P, d_out);
+            if (P.angmom_nshells > 0) {
+                const int nsh = P.angmom_nshells;
+                double *ahist = dalloc<double>(keep, (size_t)nt * 4 * nsh);
+                hipLaunchKernelGGL(k_angmom<BS>, dim3(nt), dim3(BS), sizeof(double) * 4 * nsh, s,
+                                   D, st, nsh, ahist, missed);
+                hipLaunchKernelGGL(k_angmom_combine, dim3(nh), dim3(128), 0, s, nh, d_tf, ahist,
+                                   nsh, st, d_out, P.angmom, P.angmom_axes);
+            }
         }
         AHOP_HIP(hipEventRecord(e1, s));
         AHOP_HIP(hipStreamSynchronize(s));
@@ -1437,6 +1514,11 @@ HaloTuning halo_props_device(const int64_t *h_off, int64_t nhalos, const double 
     if (nhalos <= 0) throw std::runtime_error("need at least one halo");
     // the shell tables of one halo live in LDS (4 doubles per shell)
     if (P.nshell < 2 || P.nshell > 2000) throw std::runtime_error("nshell must be in [2, 2000]");
+    if (P.angmom_nshells < 0 || P.angmom_nshells > 2000)
+        throw std::runtime_error("the per-shell angular momentum needs 1 to 2000 shells");
+    if (P.angmom_nshells > 0 && (P.light || !P.angmom || !P.angmom_axes))
+        throw std::runtime_error("the per-shell angular momentum needs the full mode and its "
+                                 "two output arrays");
     if (!P.cdm && !dens) throw std::runtime_error("densest-member centre needs densities");
     if (P.dpmmc && !dens) throw std::runtime_error("DPMMC recentring needs densities");
     if ((P.sc || P.dpmmc) && !(P.dcell_min > 0))

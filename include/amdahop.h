@@ -205,6 +205,15 @@ typedef struct {
     int32_t virial_pe_profile;       /* 0 'shell', 1 'baseline' (-3/5 G M(<r)^2 RF(a^2,b^2,c^2)) */
     int32_t virial_pe_norm_policy;   /* 0 'profile_outer_for_large', 1 'legacy' */
     int32_t virial_shell_self_mode;  /* 0 'count', 1 'half_shell' */
+    /* ANG_MOM_OF_R: per-shell mass and angular momentum of det_ang_momentum_per_shell_1b5
+     * (src/compute_halo_props.py:1970-2013, called at :1257 for halos with a finite rvir).
+     * ang_mom_nshells 0 turns it off (the reference's nshells is 100, halo_defs.py:595);
+     * d_ang_mom (nhalos x 4 x ang_mom_nshells: per shell m, then Lx, Ly, Lz) and d_ang_axes
+     * (nhalos x 12: amax, bmax, cmax, then the principal axes v row-major) are device arrays;
+     * rows of halos whose rvir is not finite read NaN.  Needs light = 0. */
+    int32_t ang_mom_nshells;
+    double *d_ang_mom;
+    double *d_ang_axes;
 } ahop_halo_params;
 /* lbox = +inf turns the periodic wrap off everywhere (FlagPeriod = 0, :862 and :889). */
 

@@ -19,7 +19,9 @@ shell, the overdensity branch of a static halo), invariance under a periodic tra
 across the box edge and rotation covariance of L and I, and eigh ordering; for the
 recentring, the cases where it reduces to the centre of det_center_18, a core found
 through a decoy, the grid's cell choice, and the reference's indexing of the densest
-member (tests/test_halo_oracle.py).
+member (tests/test_halo_oracle.py); for the per-shell angular momentum of ANG_MOM_OF_R,
+members placed by hand on the principal axes of a rotated frame and the shells summing to the
+halo's mass and angular momentum.
 """
 from __future__ import annotations
 
@@ -317,15 +319,49 @@ def vir_props(tabm, tabk, tabp, amax, bmax, cmax, vir_overdens, rho_mean):
     return mvir, rvir
 
 
+def ang_momentum_per_shell(pos, vel, mass, c, vbar, vecs, amax, bmax, cmax, L, nshells=100):
+    """det_ang_momentum_per_shell_1b5 :1970-2013 -> (m, Lx, Ly, Lz) per shell, each (nshells,).
+    Members in the reference's order, one at a time: dr wrapped as correct_for_periodicity
+    (:854-887), projected on the principal axes (dra = dr.x v[0,0] + dr.y v[1,0] + dr.z v[2,0],
+    ...), i_ell = int(r_ell * nshells) with r_ell = sqrt((dra/amax)**2 + (drb/bmax)**2 +
+    (drc/cmax)**2), dv = v - the halo's mean velocity, and the shell gets m and
+    m * (dr x dv).  i_ell > nshells is the reference's ValueError, i_ell = nshells its
+    IndexError."""
+    m = np.zeros(nshells)
+    lx = np.zeros(nshells)
+    ly = np.zeros(nshells)
+    lz = np.zeros(nshells)
+    for i in range(mass.shape[0]):
+        dx = wrap_scalar(pos[i, 0] - c[0], L)
+        dy = wrap_scalar(pos[i, 1] - c[1], L)
+        dz = wrap_scalar(pos[i, 2] - c[2], L)
+        dra = dx * vecs[0, 0] + dy * vecs[1, 0] + dz * vecs[2, 0]
+        drb = dx * vecs[0, 1] + dy * vecs[1, 1] + dz * vecs[2, 1]
+        drc = dx * vecs[0, 2] + dy * vecs[1, 2] + dz * vecs[2, 2]
+        r_ell = np.sqrt((dra / amax) ** 2 + (drb / bmax) ** 2 + (drc / cmax) ** 2)
+        i_ell = int(r_ell * nshells)
+        if i_ell >= nshells:
+            raise ValueError(f"{i_ell},{nshells}")
+        vx, vy, vz = vel[i, 0] - vbar[0], vel[i, 1] - vbar[1], vel[i, 2] - vbar[2]
+        m[i_ell] += mass[i]
+        lx[i_ell] += mass[i] * (dy * vz - dz * vy)
+        ly[i_ell] += mass[i] * (dz * vx - dx * vz)
+        lz[i_ell] += mass[i] * (dx * vy - dy * vx)
+    return m, lx, ly, lz
+
+
 def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst=430.1,
                vir_overdens=340.0, rho_mean=1.0, cdm=True, nshell=1000, direct_pe_nmax=1000,
                sc=False, dpmmc=False, dcell_min=0.0, eps_sc=0.0, flag_period=1,
                virial_pe_profile="shell", virial_pe_norm_policy="profile_outer_for_large",
-               virial_shell_self_mode="count"):
+               virial_shell_self_mode="count", ang_mom_nshells=0, extras=None):
     """All properties of one halo, composed as det_vir_1b :1241-1259 and the property loop
     (det_mass_17, det_center_18, refine_centers_1b, compute_ang_mom_19, r_halos_1a,
     det_main_axis_1b2, det_halo_energies_1b3, det_vir_props_1b4, compute_spin_parameter_1c
-    :1599-1602; the order of _compute_halo_props :270-290)."""
+    :1599-1602; the order of _compute_halo_props :270-290).  With ``ang_mom_nshells`` > 0 and
+    a dict ``extras``, the per-shell angular momentum of det_ang_momentum_per_shell_1b5
+    (:1257-1258, for a finite rvir) goes into extras["shells"] (4, nshells) and the ellipsoid
+    into extras["axes"] (amax, bmax, cmax, v row-major); NaN otherwise."""
     out = np.full(len(FIELDS), np.nan)
     # FlagPeriod = 0: correct_for_periodicity returns its input (:862, :889), which the strict
     # comparisons with L/2 reproduce for an infinite L; the pair sum drops its wrap
@@ -377,6 +413,14 @@ def halo_props(pos, vel, mass, density=None, *, lbox=1.0, hub_pt=70.3, gravconst
         out[F["r_c"]] = out[F["rvir"]]
     hl = np.sqrt(np.sum(Lv ** 2))
     out[F["spin"]] = hl * np.sqrt(np.abs(et)) / m ** 2.5 / gravconst
+    if ang_mom_nshells > 0 and extras is not None:
+        extras["shells"] = np.full((4, ang_mom_nshells), np.nan)
+        extras["axes"] = np.full(12, np.nan)
+        if np.isfinite(out[F["rvir"]]):
+            extras["shells"] = np.array(ang_momentum_per_shell(pos, vel, mass, c, vbar, vecs,
+                                                               amax, bmax, cmax, L,
+                                                               ang_mom_nshells))
+            extras["axes"] = np.concatenate([[amax, bmax, cmax], vecs.ravel()])
     return out
 
 

@@ -141,8 +141,10 @@ def test_unknown_switches_raise(gpu, halos):
     mass = torch.from_numpy(halos.mass).to(gpu)
     with pytest.raises(ValueError, match="virial_pe_profile"):
         halo_props(halos.offsets, pos, vel, mass, virial_pe_profile="exact")
-    with pytest.raises(NotImplementedError, match="ANG_MOM_OF_R"):
-        halo_props(halos.offsets, pos, vel, mass, ang_mom_of_r=True)
+    with pytest.raises(ValueError, match="full mode"):
+        halo_props(halos.offsets, pos, vel, mass, ang_mom_of_r=True, light=True)
+    with pytest.raises(ValueError, match="nshells"):
+        halo_props(halos.offsets, pos, vel, mass, ang_mom_of_r=True, nshells=0)
 
 
 def test_invalid_layouts_raise(gpu, halos):
@@ -282,3 +284,56 @@ def test_full_workload_sample_matches_the_oracle(gpu):
          f"{int((margins < 1e-9).sum())} halos with a member within 1e-9 of a boundary")
     assert c[largest].max() == c.max() and (c[pick] >= 1000).sum() >= 10
     compare(out[pick], ref)
+
+
+def test_angular_momentum_per_shell_matches_the_oracle(gpu):
+    """ANG_MOM_OF_R: the device's per-shell masses and angular momenta (100 shells of the
+    all-member ellipsoid) against det_ang_momentum_per_shell_1b5's restatement, 1e-10 of the
+    halo's mass and of the sum of its shells' |L|, the ellipsoid to 1e-10; the same halos'
+    31 fields unchanged by the option; NaN rows exactly where rvir is not finite.  On the
+    100 largest halos of the 50 000-halo workload, the shells sum to each halo's m and L."""
+    import torch
+    from amdahop import halo_props
+    from oracle.halo_props import F, halo_props as oracle_props
+    hs = G.nfw_halos(40, 21, alpha=0.9, nmin=30, nmax=3000, mass_sigma_dex=0.3)
+    pos = torch.from_numpy(hs.pos).to(gpu)
+    vel = torch.from_numpy(hs.vel).to(gpu)
+    mass = torch.from_numpy(hs.mass).to(gpu)
+    out, _, ang = halo_props(hs.offsets, pos, vel, mass, ang_mom_of_r=True)
+    plain, _ = halo_props(hs.offsets, pos, vel, mass)
+    out, plain = out.cpu().numpy(), plain.cpu().numpy()
+    assert np.array_equal(out, plain, equal_nan=True)
+    sh, ax = ang.shells.cpu().numpy(), ang.axes.cpu().numpy()
+    worst_m = worst_l = worst_a = 0.0
+    for h in range(hs.nhalos):
+        s, e = int(hs.offsets[h]), int(hs.offsets[h + 1])
+        ex = {}
+        oracle_props(hs.pos[s:e], hs.vel[s:e], hs.mass[s:e], ang_mom_nshells=100, extras=ex)
+        assert np.array_equal(np.isnan(sh[h]), np.isnan(ex["shells"])), h
+        if np.isnan(ex["shells"]).all():
+            continue
+        worst_m = max(worst_m, np.max(np.abs(sh[h][0] - ex["shells"][0])) / ex["shells"][0].sum())
+        lscale = np.abs(ex["shells"][1:]).sum()
+        worst_l = max(worst_l, np.max(np.abs(sh[h][1:] - ex["shells"][1:])) / lscale)
+        worst_a = max(worst_a, np.max(np.abs(np.abs(ax[h]) - np.abs(ex["axes"])))
+                      / np.max(np.abs(ex["axes"][:3])))
+    print(f"worst relative errors: m {worst_m:.1e}, L {worst_l:.1e}, axes {worst_a:.1e}",
+          flush=True)
+    assert worst_m < TOL and worst_l < TOL and worst_a < TOL
+    big = G.halo_config()
+    top = np.argsort(np.diff(big.offsets))[-100:]
+    off = np.concatenate([[0], np.cumsum(np.diff(big.offsets)[top])])
+    idx = np.concatenate([np.arange(big.offsets[h], big.offsets[h + 1]) for h in top])
+    bp = torch.from_numpy(big.pos[idx]).to(gpu)
+    bv = torch.from_numpy(big.vel[idx]).to(gpu)
+    bm = torch.from_numpy(big.mass[idx]).to(gpu)
+    bo, _, bang = halo_props(off, bp, bv, bm, ang_mom_of_r=True)
+    bo, bsh = bo.cpu().numpy(), bang.shells.cpu().numpy()
+    fin = np.isfinite(bo[:, F["rvir"]])
+    print(f"{fin.sum()} of 100 large halos with a finite rvir, up to {np.diff(off).max()} "
+          "members", flush=True)
+    assert fin.sum() > 50
+    Lh = bo[fin][:, [F["Lx"], F["Ly"], F["Lz"]]]
+    assert np.allclose(bsh[fin, 0].sum(axis=1), bo[fin, F["m"]], rtol=1e-12)
+    assert np.all(np.abs(bsh[fin, 1:].sum(axis=2) - Lh) <=
+                  1e-10 * np.abs(bsh[fin, 1:]).sum(axis=(1, 2))[:, None])

@@ -415,3 +415,108 @@ def test_halo_switches_compose():
     flat = halo_props(edge, vel, mass, flag_period=0)
     assert np.isclose(wrapped[F["r"]], d[F["r"]], rtol=1e-9)
     assert flat[F["r"]] > 0.4
+
+
+def _rotation(seed):
+    q, r = np.linalg.qr(np.random.default_rng(seed).normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def test_angular_momentum_shells_by_hand():
+    """det_ang_momentum_per_shell_1b5 on members placed by hand along the principal axes of a
+    rotated frame (the axes are the columns of v, dra = dr . v[:, 0]): each lands in the
+    shell int(r_ell * nshells) of its fraction of amax / bmax / cmax, and the shell holds m
+    and m (dr x dv) with dv against the mean velocity.  A transposed v (rows as axes) puts
+    them elsewhere.  A member at r_ell >= 1 is the reference's error."""
+    from oracle.halo_props import ang_momentum_per_shell
+    v = _rotation(7)
+    amax, bmax, cmax = 0.3, 0.2, 0.1
+    c = np.array([0.49, -0.2, 0.1])
+    vbar = np.array([10.0, -5.0, 2.0])
+    # (axis, fraction of that semi-axis, velocity offset)
+    members = [(0, 0.055, (1.0, 2.0, 3.0)), (1, 0.505, (0.0, -4.0, 1.0)),
+               (2, 0.9995, (2.0, 0.0, -1.0)), (0, -0.3055, (0.5, 0.5, 0.5))]
+    semi = (amax, bmax, cmax)
+    raw = np.array([c + f * semi[a] * v[:, a] for a, f, _ in members])
+    assert (raw >= 0.5).any()                        # some member wraps across the edge
+    pos = np.where(raw >= 0.5, raw - 1.0, raw)
+    vel = np.array([vbar + np.array(dv) for _, _, dv in members])
+    mass = np.array([1.0, 2.0, 3.0, 4.0])
+    m, lx, ly, lz = ang_momentum_per_shell(pos, vel, mass, c, vbar, v, amax, bmax, cmax, 1.0)
+    want = {5: [0], 50: [1], 99: [2], 30: [3]}
+    assert set(np.nonzero(m)[0]) == set(want)
+    for shell, idx in want.items():
+        i = idx[0]
+        a, f, dv = members[i]
+        dr = f * semi[a] * v[:, a]
+        Lw = mass[i] * np.cross(dr, np.array(dv))
+        assert m[shell] == mass[i]
+        assert np.allclose([lx[shell], ly[shell], lz[shell]], Lw, rtol=1e-12, atol=1e-15)
+    mt, *_ = ang_momentum_per_shell(pos, vel, mass, c, vbar, v.T.copy(), amax, bmax, cmax, 1.0)
+    assert set(np.nonzero(mt)[0]) != set(want)
+    out = np.vstack([pos, c + 1.01 * amax * v[:, 0]])
+    with pytest.raises(ValueError):
+        ang_momentum_per_shell(out, np.vstack([vel, vbar]), np.append(mass, 1.0), c, vbar, v,
+                               amax, bmax, cmax, 1.0)
+
+
+def test_angular_momentum_shells_sum_to_the_halo():
+    """Summed over the shells, the per-shell masses give the halo's mass and the per-shell
+    angular momenta its L (compute_ang_mom_19, pinned by its own closed forms); the shells
+    of halo_props are those of the all-member ellipsoid, so the outermost occupied shell is
+    below nshells, and a halo without a finite rvir has NaN shells."""
+    from oracle.halo_props import F, halo_props
+    rng = np.random.default_rng(3)
+    n = 800
+    pos = rng.normal(0, 0.01, (n, 3)) * np.array([1.0, 0.6, 0.3])
+    pos[:, 0] += 0.495                                 # across the periodic edge
+    pos = np.where(pos >= 0.5, pos - 1.0, pos)
+    vel = rng.normal(0, 60, (n, 3)) + np.cross(pos, [0.0, 0.0, 3000.0])
+    mass = rng.uniform(0.5, 1.5, n) * 1e-3
+    ex = {}
+    out = halo_props(pos, vel, mass, ang_mom_nshells=100, extras=ex)
+    assert np.isfinite(out[F["rvir"]])
+    m, lx, ly, lz = ex["shells"]
+    assert np.isclose(m.sum(), out[F["m"]], rtol=1e-13)
+    Lh = out[[F["Lx"], F["Ly"], F["Lz"]]]
+    assert np.allclose([lx.sum(), ly.sum(), lz.sum()], Lh, rtol=1e-11,
+                       atol=1e-11 * np.linalg.norm(Lh))
+    assert m[-1] > 0 or m[-2] > 0                      # the ellipsoid is fitted to the members
+    ax = ex["axes"]
+    assert np.allclose(ax[:3] / ax[0], [1.0, out[F["shb"]] / out[F["sha"]],
+                                        out[F["shc"]] / out[F["sha"]]], rtol=1e-12)
+
+
+def test_ang_mom_of_r_file_layout(tmp_path):
+    """write_ang_mom_of_r: Fortran sequential records as FortranFile writes them, the
+    header (nb_of_halos, nb_of_subhalos), (nshells), then one record per halo with a finite
+    rvir: id, the 11 halo fields, amax, bmax, cmax, v, and the four per-shell arrays."""
+    from scipy.io import FortranFile
+    from amdahop import HALO_FIELDS, AngMomShells, write_ang_mom_of_r
+    rng = np.random.default_rng(5)
+    nh, nsh = 4, 7
+    out = rng.normal(size=(nh, len(HALO_FIELDS)))
+    out[2, HALO_FIELDS.index("rvir")] = np.nan
+    shells = rng.normal(size=(nh, 4, nsh))
+    axes = rng.normal(size=(nh, 12))
+    ids = np.array([3, 5, 8, 9])
+    path = tmp_path / "ang_mom_of_r.007"
+    assert write_ang_mom_of_r(str(path), ids, out, AngMomShells(shells, axes), 3, 1) == 3
+    f = FortranFile(str(path), "r")
+    assert list(f.read_record(np.int64)) == [3, 1]
+    assert list(f.read_record(np.int64)) == [nsh]
+    cols = [HALO_FIELDS.index(k) for k in ("px", "py", "pz", "vx", "vy", "vz", "rvir", "mvir",
+                                            "m", "r", "spin")]
+    for h in (0, 1, 3):
+        rec = f.read_record(np.dtype([("id", "<i4"), ("f", "<f8", 11), ("ax", "<f8", 12),
+                                      ("sh", "<f8", (4, nsh))]))[0]
+        assert rec["id"] == ids[h]
+        assert np.array_equal(rec["f"], out[h, cols])
+        assert np.array_equal(rec["ax"], axes[h])
+        assert np.array_equal(rec["sh"], shells[h])
+    with pytest.raises(Exception):
+        f.read_record(np.int64)
+    f.close()
