@@ -9,7 +9,8 @@
 //    8 children live in registers and whose third level (8 children under each register
 //    child) lives in LDS at 4 B per entry.  The wave's LDS is 18 KB instead of 30 KB, so 8
 //    waves fit per CU instead of 5 (the registers allow 8 too), and a replacement moves keys
-//    only;
+//    only.  Each key carries its place among its siblings in its low 3 mantissa bits, so one
+//    integer max over a group finds the largest and where it is;
 //  * every (key, sorted slot) it puts into the heap after the seeding's direct fill is
 //    appended to a per-lane log in global memory (one region per resident wave: the grid is
 //    persistent and takes waves of queries from per-XCD counters); the direct fill's entries
@@ -25,82 +26,101 @@
 #pragma once
 #include <utility>
 
-// log entries per lane: the replacements after the seeding's direct fill (about 104 per lane
-// on average at the 16.8M-particle box, at most 2 315; 938 of its 16.8M queries overflow 512
-// and are re-run on the FP64 path)
+// log entries per lane: the replacements after the seeding's direct fill (about 107 per lane
+// on average at the 16.8M-particle box, at most 640; 6 of its 16.8M queries overflow 512 and
+// are re-run on the FP64 path)
 constexpr int SEL_CAPL = 512;
+// slots of the wave's Morton seed window beyond the k/2 on either side of its queries (the
+// heap kernels take 16): 32 gives the lanes at a wave's ends better seeds (at the
+// 16.8M-particle box, 56.5 -> 55.9 ms, largest log 2 315 -> 640 entries, log overflows
+// 938 -> 6; 8 took 62.7 and 24 61.1 against 61.5 ms with the untagged heap)
+#ifndef SEL_SEED_PAD
+#define SEL_SEED_PAD 32
+#endif
 // log entries the collection reads ahead per lane
 #ifndef SEL_P2U
 #define SEL_P2U 8
 #endif
 
-// keys of the heap's top two levels: the root and RC children
+// The heap holds keys as int bit patterns of the FP32 keys (non-negative floats and +-inf
+// order as signed ints) with their low 3 mantissa bits replaced by a tag: an entry's position
+// in its group of LC = 8 siblings.  One max over a group then gives both the largest key and
+// where it sits (no compare-and-select chain for the index).  The heap orders the keys
+// quantised to 8 ulps (q(key) = bits & ~7), its root is a quantised key, and every
+// comparison with a candidate is on quantised values; the root still only decreases, so
+// every candidate ever dropped (rejected or evicted) has a full key at or above the final
+// root, and the certificate on full keys is unchanged.
+constexpr int KQ_MASK = ~7;
+
+__device__ __forceinline__ int kq(float k) { return __float_as_int(k) & KQ_MASK; }
+
+// keys of the heap's top two levels: the root (quantised, untagged) and RC children (tagged
+// with their index)
 template <int RC>
 struct KeyTop {
-    typedef float v_t __attribute__((ext_vector_type(RC)));
-    float k0;
+    typedef int v_t __attribute__((ext_vector_type(RC)));
+    int k0;
     v_t rk;
 };
 
-// Sift key d down from register child c (1 <= c <= RC): the largest of its LC children in LDS
-// moves up if it exceeds d, and d takes its place.  Register children whose group lies past
-// the heap's allocated entries (KHCAP) are leaves.  Equal keys are interchangeable here (no
-// ids ride along), so any largest child may move up.
+__device__ __forceinline__ int max8(int a, int b, int c, int d, int e, int f, int g, int h)
+{
+    return max(max(max(a, b), max(c, d)), max(max(e, f), max(g, h)));
+}
+
+// Sift quantised key dq down from register child c (1 <= c <= RC): the largest of its LC
+// children in LDS moves up if it exceeds dq, and dq takes its place (tagged with that place).
+// Register children whose group lies past the heap's allocated entries (KHCAP) are leaves.
 template <int KHCAP, int RC>
-__device__ __forceinline__ void kt_sift_l1(KeyTop<RC> &h, float *hw, int lane, float d, int c)
+__device__ __forceinline__ void kt_sift_l1(KeyTop<RC> &h, int *hw, int lane, int dq, int c)
 {
     constexpr int RT = RC + 1;
     const int c0 = rt_child0<RC>(c);
-    float vk = d;
+    int vk = dq;
     if (c0 < KHCAP) {
-        const float *base = hw + (c0 - RT) * WAVE + lane;
-        float kt[LC];
+        int *base = hw + (c0 - RT) * WAVE + lane;
+        int kt[LC];
 #pragma unroll
         for (int t = 0; t < LC; ++t) kt[t] = base[t * WAVE];
-        float kc = kt[0];
-#pragma unroll
-        for (int t = 1; t < LC; ++t) kc = fmaxf(kc, kt[t]);
-        int tc = LC - 1;
-#pragma unroll
-        for (int t = LC - 2; t >= 0; --t) tc = kt[t] == kc ? t : tc;
-        if (kc > d) {
-            hw[(c0 + tc - RT) * WAVE + lane] = d;
-            vk = kc;
+        const int m = max8(kt[0], kt[1], kt[2], kt[3], kt[4], kt[5], kt[6], kt[7]);
+        const int mq = m & KQ_MASK;
+        if (mq > dq) {
+            const int tc = m & 7;
+            base[tc * WAVE] = dq | tc;
+            vk = mq;
         }
     }
 #pragma unroll
-    for (int t = 0; t < RC; ++t) h.rk[t] = (c == t + 1) ? vk : h.rk[t];
+    for (int t = 0; t < RC; ++t) h.rk[t] = (c == t + 1) ? (vk | t) : h.rk[t];
 }
 
-// sift key d down from the root
+// sift quantised key dq down from the root
 template <int KHCAP, int RC>
-__device__ __forceinline__ void kt_sift_root(KeyTop<RC> &h, float *hw, int lane, float d)
+__device__ __forceinline__ void kt_sift_root(KeyTop<RC> &h, int *hw, int lane, int dq)
 {
-    float kc = h.rk[0];
-#pragma unroll
-    for (int t = 1; t < RC; ++t) kc = fmaxf(kc, h.rk[t]);
-    int ic = RC;
-#pragma unroll
-    for (int t = RC - 2; t >= 0; --t) ic = h.rk[t] == kc ? t + 1 : ic;
-    if (kc > d) {
-        h.k0 = kc;
-        kt_sift_l1<KHCAP, RC>(h, hw, lane, d, ic);
+    static_assert(RC == 8, "the register level is one tagged group of 8");
+    const int m = max8(h.rk[0], h.rk[1], h.rk[2], h.rk[3], h.rk[4], h.rk[5], h.rk[6], h.rk[7]);
+    const int mq = m & KQ_MASK;
+    if (mq > dq) {
+        h.k0 = mq;
+        kt_sift_l1<KHCAP, RC>(h, hw, lane, dq, (m & 7) + 1);
     } else {
-        h.k0 = d;
+        h.k0 = dq;
     }
 }
 
-// write key k at heap entry e (any entry; register entries by selects)
+// write quantised key q at heap entry e (any entry; register entries by selects), tagged with
+// its place among its siblings
 template <int RC>
-__device__ __forceinline__ void kt_set(KeyTop<RC> &h, float *hw, int lane, int e, float k)
+__device__ __forceinline__ void kt_set(KeyTop<RC> &h, int *hw, int lane, int e, int q)
 {
     constexpr int RT = RC + 1;
     if (e >= RT) {
-        hw[(e - RT) * WAVE + lane] = k;
+        hw[(e - RT) * WAVE + lane] = q | ((e - RT) % LC);
     } else {
-        h.k0 = e == 0 ? k : h.k0;
+        h.k0 = e == 0 ? q : h.k0;
 #pragma unroll
-        for (int t = 0; t < RC; ++t) h.rk[t] = (e == t + 1) ? k : h.rk[t];
+        for (int t = 0; t < RC; ++t) h.rk[t] = (e == t + 1) ? (q | t) : h.rk[t];
     }
 }
 
@@ -178,7 +198,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
     static_assert(KHCAP > RT && (KHCAP - RT) % LC == 0, "every register child has 0 or LC children");
     constexpr int NL = KHCAP - RT;
     // the heap's LDS entries; after the search, the lane's sorted neighbour slots
-    __shared__ float hw[(NL > KCAP ? NL : KCAP) * WAVE];
+    __shared__ int hw[(NL > KCAP ? NL : KCAP) * WAVE];
     __shared__ __align__(16) float sx[2 * WAVE];
     __shared__ __align__(16) float sy[2 * WAVE];
     __shared__ __align__(16) float sz[2 * WAVE];
@@ -224,7 +244,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         asm volatile("" : "+s"(K), "+v"(lgo));
         uint2 *const lg = klog + lgo;
         const int KH = K + 1;
-        const int Wd = K / 2 + AHOP_SEED_PAD;
+        const int Wd = K / 2 + SEL_SEED_PAD;
 
         const int q = wave * WAVE + lane;
         const bool active = q < A.n;
@@ -250,11 +270,12 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         const float pxf = (float)P.x, pyf = (float)P.y, pzf = (float)P.z;
 
         KeyTop<RC> h;
-        h.k0 = INFINITY;
+        h.k0 = kq(INFINITY);
 #pragma unroll
-        for (int t = 0; t < RC; ++t) h.rk[t] = t + 1 < KH ? INFINITY : -INFINITY;
+        for (int t = 0; t < RC; ++t) h.rk[t] = kq(t + 1 < KH ? INFINITY : -INFINITY) | t;
 #pragma unroll
-        for (int e = RT; e < KHCAP; ++e) hw[(e - RT) * WAVE + lane] = e < KH ? INFINITY : -INFINITY;
+        for (int e = RT; e < KHCAP; ++e)
+            hw[(e - RT) * WAVE + lane] = kq(e < KH ? INFINITY : -INFINITY) | ((e - RT) % LC);
         float tau = (active && !wave_unsafe) ? INFINITY : -1.0f;
         // the wave's seed window [qb - Wd, qb + 63 + Wd]: every lane seeds from all of it, and
         // the traversal skips it
@@ -284,16 +305,16 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
 
 #define SEL_INSERT(d, id)                                                                      \
     do {                                                                                       \
-        kt_sift_root<KHCAP, RC>(h, hw, lane, (d));                                             \
+        kt_sift_root<KHCAP, RC>(h, hw, lane, kq(d));                                           \
         SEL_LOG((d), (id));                                                                    \
         ++st_ins;                                                                              \
-        tau = h.k0;                                                                            \
+        tau = __int_as_float(h.k0);                                                            \
     } while (0)
 
 #define SEL_HEAPIFY()                                                                          \
     do {                                                                                       \
         _Pragma("unroll") for (int e_ = RC; e_ >= 1; --e_) if (rt_child0<RC>(e_) < KHCAP)      \
-            kt_sift_l1<KHCAP, RC>(h, hw, lane, h.rk[e_ - 1], e_);                              \
+            kt_sift_l1<KHCAP, RC>(h, hw, lane, h.rk[e_ - 1] & KQ_MASK, e_);                     \
         kt_sift_root<KHCAP, RC>(h, hw, lane, h.k0);                                            \
     } while (0)
 
@@ -327,12 +348,12 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                 int id_;                                                                       \
                 SEL_PICK(d_, id_);                                                             \
                 if (filled < KH) {                                                             \
-                    kt_set<RC>(h, hw, lane, filled, d_);                                       \
+                    kt_set<RC>(h, hw, lane, filled, kq(d_));                                   \
                     SEL_LOG(d_, id_);                                                          \
                     ++st_ins;                                                                  \
                     if (++filled == KH) {                                                      \
                         SEL_HEAPIFY();                                                         \
-                        tau = h.k0;                                                            \
+                        tau = __int_as_float(h.k0);                                            \
                     }                                                                          \
                 } else if (d_ < tau) {                                                         \
                     SEL_INSERT(d_, id_);                                                       \
@@ -458,7 +479,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                             float d;
                             SEL_KEY(sx[r], sy[r], sz[r], d);
                             d = r == off ? INFINITY : d;
-                            kt_set<RC>(h, hw, lane, e, d);
+                            kt_set<RC>(h, hw, lane, e, kq(d));
                         }
                     }
                     for (int e = RT; e < KH; ++e) {
@@ -466,14 +487,14 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                         float d;
                         SEL_KEY(sx[r], sy[r], sz[r], d);
                         d = r == off ? INFINITY : d;
-                        hw[(e - RT) * WAVE + lane] = d;
+                        hw[(e - RT) * WAVE + lane] = kq(d) | ((e - RT) % LC);
                     }
                     filled = KH;
                     dfill = true;
                     dfill_rs = rs;
                     st_ins += KH;
                     SEL_HEAPIFY();
-                    tau = h.k0;
+                    tau = __int_as_float(h.k0);
                     m0 &= ~bits_in(rs, rs + KH);
                     m1 &= ~bits_in(rs - WAVE, rs + KH - WAVE);
                 }
@@ -484,7 +505,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
             // a clipped window (array ends): the unfilled entries stay +inf
             SEL_HEAPIFY();
             filled = KH;
-            tau = h.k0;
+            tau = __int_as_float(h.k0);
         }
 
         // ---- wave-coherent traversal of the binary radix tree (as k_knn)
@@ -586,7 +607,9 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
 
         // ---- collection: the logged entries below the final (k+1)-th key, compacted in
         // place to the front of the lane's log (entry e is read before any write reaches it)
-        const float T = h.k0;
+        // the final root: the (k+1)-th smallest key, quantised (on the 8-ulp grid, so a key is
+        // below it exactly when its quantised value is)
+        const float T = __int_as_float(h.k0);
         const int nl = min(nlog, SEL_CAPL);
         st_logmax = max(st_logmax, (unsigned)nlog);
         st_ovf += (nlog > SEL_CAPL) ? 1u : 0u;
