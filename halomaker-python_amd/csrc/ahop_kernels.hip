@@ -147,6 +147,13 @@ __device__ __forceinline__ T wave_min(T v)
 // ------------------------------------------------------------------------------------------
 // tree build
 // ------------------------------------------------------------------------------------------
+// Morton keys keep the top MORTON_KEY_BITS of the 63-bit interleaving (16 bits per axis at
+// 48): the radix sort then takes 6 passes of 8 bits instead of 8.  Particles in one cell of
+// 2^-16 of the box share a key and keep their index order; the tree treats them as
+// duplicates, and the results do not depend on the tree's shape.
+#ifndef MORTON_KEY_BITS
+#define MORTON_KEY_BITS 48
+#endif
 __global__ void k_morton(const double *__restrict__ pos, int64_t n, double Lx, double Ly, double Lz,
                          uint64_t *__restrict__ key, int32_t *__restrict__ iota, int *__restrict__ bad)
 {
@@ -161,7 +168,7 @@ __global__ void k_morton(const double *__restrict__ pos, int64_t n, double Lx, d
     uint64_t ix = (uint64_t)fmin(fmax(fx, 0.0), S - 1.0);
     uint64_t iy = (uint64_t)fmin(fmax(fy, 0.0), S - 1.0);
     uint64_t iz = (uint64_t)fmin(fmax(fz, 0.0), S - 1.0);
-    key[i] = (spread21(ix) << 2) | (spread21(iy) << 1) | spread21(iz);
+    key[i] = ((spread21(ix) << 2) | (spread21(iy) << 1) | spread21(iz)) >> (63 - MORTON_KEY_BITS);
     iota[i] = (int32_t)i;
 }
 
@@ -186,7 +193,7 @@ __device__ __forceinline__ int delta(const uint64_t *__restrict__ k, int n, int 
 __global__ void k_radix_tree(const uint64_t *__restrict__ key, int n, int32_t *__restrict__ first,
                              int32_t *__restrict__ last, int32_t *__restrict__ left,
                              int32_t *__restrict__ right, int32_t *__restrict__ parent,
-                             int32_t *__restrict__ leaf_parent)
+                             int32_t *__restrict__ leaf_parent, int4 *__restrict__ nrec)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -212,6 +219,7 @@ __global__ void k_radix_tree(const uint64_t *__restrict__ key, int n, int32_t *_
     last[i] = la;
     left[i] = lc;
     right[i] = rc;
+    nrec[i] = make_int4(lc, rc, f, la);   // the links and range of the node's visit record
     if (lc >= 0) parent[lc] = i; else leaf_parent[~lc] = i;
     if (rc >= 0) parent[rc] = i; else leaf_parent[~rc] = i;
     if (i == 0) parent[0] = -1;
@@ -236,54 +244,117 @@ __device__ __forceinline__ void child_box(int c, const double4 *__restrict__ spo
     }
 }
 
-// FP32 copy of a node box for the traversal tests: centre rounded to nearest, half-widths
-// rounded up (the test's margin covers the centre rounding)
+// FP32 copy of a box for the traversal tests: centre rounded to nearest, half-widths rounded
+// up (the test's margin covers the centre rounding); a particle's box (lo = hi) is its
+// FP32-rounded position with zero half-widths
+__device__ __forceinline__ void fp32_box(const double lo[3], const double hi[3], float4 &a,
+                                         float4 &b)
+{
+    float c[3], h[3];
+    for (int q = 0; q < 3; ++q) {
+        c[q] = (float)(0.5 * (lo[q] + hi[q]));
+        h[q] = __double2float_ru(0.5 * (hi[q] - lo[q]));
+    }
+    a = make_float4(c[0], c[1], c[2], h[0]);
+    b = make_float4(h[1], h[2], 0.0f, 0.0f);
+}
+
 __device__ __forceinline__ void store_box(int node, const double lo[3], const double hi[3],
                                           double *box, float4 *fbox)
 {
-    float c[3], h[3];
     for (int a = 0; a < 3; ++a) {
         box[6 * (int64_t)node + a] = lo[a];
         box[6 * (int64_t)node + a + 3] = hi[a];
-        c[a] = (float)(0.5 * (lo[a] + hi[a]));
-        h[a] = __double2float_ru(0.5 * (hi[a] - lo[a]));
     }
-    fbox[2 * (int64_t)node] = make_float4(c[0], c[1], c[2], h[0]);
-    fbox[2 * (int64_t)node + 1] = make_float4(h[1], h[2], 0.0f, 0.0f);
+    float4 a, b;
+    fp32_box(lo, hi, a, b);
+    fbox[2 * (int64_t)node] = a;
+    fbox[2 * (int64_t)node + 1] = b;
 }
 
-// Bottom-up bounding boxes in two phases.  A fence per climbing thread at device scope
-// (needed for cross-XCD visibility) costs ~1e2 ms at 16.8M particles, so:
-//  phase 1 (k_refit_local): one workgroup per 256 consecutive leaves merges every node whose
+// the visit record's part for child k of a node: the child's FP32 box and its slot range
+// [cf, cl] (ints as float bits)
+__device__ __forceinline__ void store_child_rec(int node, int k, const double lo[3],
+                                                const double hi[3], int cf, int cl,
+                                                float4 *cbox)
+{
+    float4 a, b;
+    fp32_box(lo, hi, a, b);
+    cbox[4 * (int64_t)node + 2 * k] = a;
+    cbox[4 * (int64_t)node + 2 * k + 1] =
+        make_float4(b.x, b.y, __int_as_float(cf), __int_as_float(cl));
+}
+
+// Bottom-up bounding boxes in three passes, each merging a node once both children are done
+// and writing its FP64 box, its FP32 copy and, in the node's visit record, both children's
+// FP32 boxes and slot ranges.  A fence per climbing thread at device scope (needed for
+// cross-XCD visibility) costs ~1e2 ms at 16.8M particles, so no pass climbs across
+// workgroups:
+//  pass 1 (k_refit_local): one workgroup per 256 consecutive leaves merges every node whose
 //    sorted range lies inside its 256-leaf block.  Such a node's index lies in the block too
 //    (an internal node is one end of its own range), so the block preloads the links of its
 //    node indices and its leaves' positions into LDS and merges boxes there, communicating
 //    through LDS counters and workgroup-scope fences only;
-//  phase 2 (k_refit_upper): the few nodes whose range crosses a block boundary are merged by
-//    threads climbing from the lowest crossing nodes with device-scope fences.
-// Both write the FP64 box and its FP32 copy.
+//  pass 2 (k_refit_mid): one workgroup per 65 536 leaves lists the nodes whose range crosses a
+//    256-leaf block but lies inside its own block, and merges them in sweeps separated by
+//    barriers: a node merges in a sweep when both children were done before it (a leaf, a
+//    pass-1 node, or a node merged in an earlier sweep, whose writes the barrier made
+//    visible).  Nodes that cross its block go to a list for
+//  pass 3 (k_refit_top): one workgroup, the same sweeps over that list.
+// Pass 2 and 3 stamp a merged node's flag with the sweep's number (pass 3 from TOP_STAMP
+// on), so that a child merged in the current sweep is never taken as done.
 constexpr int REFIT_BS = 256;
+constexpr int MID_S = 65536;     // leaves per pass-2 workgroup
+constexpr int MID_BS = 1024;
+constexpr int MID_CAP = 4096;    // pass-2 list per workgroup (more go to pass 3)
+constexpr int TOP_STAMP = 1 << 20;
 
-__device__ __forceinline__ void merge_children(int node, const double4 *__restrict__ spos,
-                                               const int32_t *__restrict__ left,
-                                               const int32_t *__restrict__ right, double *box,
-                                               float4 *fbox)
+// the FP64 box and slot range of child c (a particle for c < 0)
+__device__ __forceinline__ void child_box_range(int c, const double4 *__restrict__ spos,
+                                                const int32_t *__restrict__ first,
+                                                const int32_t *__restrict__ last,
+                                                const double *box, double lo[3], double hi[3],
+                                                int &cf, int &cl)
 {
-    double a[6], b[6], lo[3], hi[3];
-    child_box(left[node], spos, box, a);
-    child_box(right[node], spos, box, b);
+    double b[6];
+    child_box(c, spos, box, b);
     for (int q = 0; q < 3; ++q) {
-        lo[q] = fmin(a[q], b[q]);
-        hi[q] = fmax(a[q + 3], b[q + 3]);
+        lo[q] = b[q];
+        hi[q] = b[q + 3];
+    }
+    if (c < 0) {
+        cf = cl = ~c;
+    } else {
+        cf = first[c];
+        cl = last[c];
+    }
+}
+
+__device__ __forceinline__ void merge_node(int node, const double4 *__restrict__ spos,
+                                           const int32_t *__restrict__ first,
+                                           const int32_t *__restrict__ last,
+                                           const int32_t *__restrict__ left,
+                                           const int32_t *__restrict__ right, double *box,
+                                           float4 *fbox, float4 *cbox)
+{
+    double alo[3], ahi[3], blo[3], bhi[3], lo[3], hi[3];
+    int af, al, bf, bl;
+    child_box_range(left[node], spos, first, last, box, alo, ahi, af, al);
+    child_box_range(right[node], spos, first, last, box, blo, bhi, bf, bl);
+    for (int q = 0; q < 3; ++q) {
+        lo[q] = fmin(alo[q], blo[q]);
+        hi[q] = fmax(ahi[q], bhi[q]);
     }
     store_box(node, lo, hi, box, fbox);
+    store_child_rec(node, 0, alo, ahi, af, al, cbox);
+    store_child_rec(node, 1, blo, bhi, bf, bl, cbox);
 }
 
 __global__ __launch_bounds__(REFIT_BS) void k_refit_local(
     int n, const double4 *__restrict__ spos, const int32_t *__restrict__ first,
     const int32_t *__restrict__ last, const int32_t *__restrict__ left,
     const int32_t *__restrict__ right, const int32_t *__restrict__ parent,
-    const int32_t *__restrict__ leaf_parent, double *box, float4 *fbox)
+    const int32_t *__restrict__ leaf_parent, double *box, float4 *fbox, float4 *cbox)
 {
     __shared__ int cnt[REFIT_BS];
     __shared__ int lf[REFIT_BS], ll[REFIT_BS], lc0[REFIT_BS], lc1[REFIT_BS], lpar[REFIT_BS];
@@ -316,97 +387,130 @@ __global__ __launch_bounds__(REFIT_BS) void k_refit_local(
         __threadfence_block();
         if (atomicAdd(&cnt[u], 1) == 0) return;   // sibling subtree not done yet
         __threadfence_block();
-        double lo[3], hi[3];
-        const int c0 = lc0[u], c1 = lc1[u];
         // children of a local node are the block's leaves or local nodes
+        const int c[2] = {lc0[u], lc1[u]};
+        double clo[2][3], chi[2][3], lo[3], hi[3];
+        for (int k = 0; k < 2; ++k)
+            for (int a = 0; a < 3; ++a) {
+                clo[k][a] = c[k] < 0 ? lpos[a][~c[k] - B0] : lbox[a][c[k] - B0];
+                chi[k][a] = c[k] < 0 ? lpos[a][~c[k] - B0] : lbox[a + 3][c[k] - B0];
+            }
         for (int a = 0; a < 3; ++a) {
-            const double a0 = c0 < 0 ? lpos[a][~c0 - B0] : lbox[a][c0 - B0];
-            const double a1 = c0 < 0 ? lpos[a][~c0 - B0] : lbox[a + 3][c0 - B0];
-            const double b0 = c1 < 0 ? lpos[a][~c1 - B0] : lbox[a][c1 - B0];
-            const double b1 = c1 < 0 ? lpos[a][~c1 - B0] : lbox[a + 3][c1 - B0];
-            lo[a] = fmin(a0, b0);
-            hi[a] = fmax(a1, b1);
+            lo[a] = fmin(clo[0][a], clo[1][a]);
+            hi[a] = fmax(chi[0][a], chi[1][a]);
             lbox[a][u] = lo[a];
             lbox[a + 3][u] = hi[a];
         }
         store_box(node, lo, hi, box, fbox);
+        for (int k = 0; k < 2; ++k) {
+            const int cf = c[k] < 0 ? ~c[k] : lf[c[k] - B0];
+            const int cl = c[k] < 0 ? ~c[k] : ll[c[k] - B0];
+            store_child_rec(node, k, clo[k], chi[k], cf, cl, cbox);
+        }
         node = lpar[u];
     }
 }
 
-// counter of a crossing node = number of its children already merged by phase 1 (or leaves)
-__global__ void k_refit_upper_init(int n, const int32_t *__restrict__ first,
-                                   const int32_t *__restrict__ last, const int32_t *__restrict__ left,
-                                   const int32_t *__restrict__ right, int32_t *flag, int32_t *starts,
-                                   int *nstarts)
+// a child is done for the sweep `stamp` when it is a particle, a pass-1 node, or was merged
+// with a smaller stamp
+__device__ __forceinline__ bool child_done(int c, const int32_t *__restrict__ first,
+                                           const int32_t *__restrict__ last,
+                                           const int32_t *flag, int stamp)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool start = false;
-    if (i < n - 1 && first[i] / REFIT_BS != last[i] / REFIT_BS) {
-        int done = 0;
-        int c[2] = {left[i], right[i]};
-        for (int k = 0; k < 2; ++k)
-            done += (c[k] < 0) || (first[c[k]] / REFIT_BS == last[c[k]] / REFIT_BS);
-        flag[i] = done;
-        start = done == 2;
-    }
-    // one atomic per wave reserves the wave's start slots
-    const unsigned long long b = __ballot(start);
-    if (!b) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)b) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(nstarts, __popcll(b));
-    base = __shfl(base, leader);
-    if (start) starts[base + __popcll(b & ((1ull << lane) - 1))] = i;
+    if (c < 0) return true;
+    if (first[c] / REFIT_BS == last[c] / REFIT_BS) return true;
+    const int f = flag[c];
+    return f != 0 && f < stamp;
 }
 
-__global__ void k_refit_upper(const int32_t *__restrict__ starts, const int *nstarts,
-                              const double4 *__restrict__ spos, const int32_t *__restrict__ left,
-                              const int32_t *__restrict__ right, const int32_t *__restrict__ parent,
-                              int32_t *flag, double *box, float4 *fbox)
+// Merges the listed nodes (list[k] < 0: already merged) in sweeps, stamps base + 1,
+// base + 2, ..., until all are merged or a sweep merges none (a child that this workgroup
+// cannot merge: the caller hands the rest on).  Run by every thread of the workgroup; each
+// list entry is read and written by one thread only.  Returns whether nodes are left.
+__device__ __forceinline__ bool refit_sweeps(int32_t *list, int m, int base, int *more,
+                                             int *progress, const double4 *__restrict__ spos,
+                                             const int32_t *__restrict__ first,
+                                             const int32_t *__restrict__ last,
+                                             const int32_t *__restrict__ left,
+                                             const int32_t *__restrict__ right, int32_t *flag,
+                                             double *box, float4 *fbox, float4 *cbox)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= *nstarts) return;
-    int node = starts[t];
-    while (true) {
-        merge_children(node, spos, left, right, box, fbox);
-        node = parent[node];
-        if (node < 0) return;
-        __threadfence();
-        if (atomicAdd(&flag[node], 1) == 0) return;        // the other child is still pending
-        __threadfence();
-    }
-}
-
-// Node records of the traversal: for internal node i its links and slot range, and the FP32
-// boxes of both children with their slot ranges, so that a visit reads one place.  A particle
-// child's box is its FP32-rounded position with zero half-widths (what the visit test computes
-// for a particle).
-__global__ void k_node_records(int64_t ni, const int32_t *__restrict__ first,
-                               const int32_t *__restrict__ last, const int32_t *__restrict__ left,
-                               const int32_t *__restrict__ right, const float4 *__restrict__ fbox,
-                               const double4 *__restrict__ spos, int4 *__restrict__ nrec,
-                               float4 *__restrict__ cbox)
-{
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= ni) return;
-    const int c[2] = {left[i], right[i]};
-    nrec[i] = make_int4(c[0], c[1], first[i], last[i]);
-    for (int k = 0; k < 2; ++k) {
-        float4 a, b;
-        if (c[k] < 0) {
-            const double4 p = spos[~c[k]];
-            a = make_float4((float)p.x, (float)p.y, (float)p.z, 0.0f);
-            b = make_float4(0.0f, 0.0f, __int_as_float(~c[k]), __int_as_float(~c[k]));
-        } else {
-            a = fbox[2 * (int64_t)c[k]];
-            const float4 b0 = fbox[2 * (int64_t)c[k] + 1];
-            b = make_float4(b0.x, b0.y, __int_as_float(first[c[k]]), __int_as_float(last[c[k]]));
+    for (int sweep = 1;; ++sweep) {
+        const int stamp = base + sweep;
+        if (threadIdx.x == 0) {
+            *more = 0;
+            *progress = 0;
         }
-        cbox[4 * i + 2 * k] = a;
-        cbox[4 * i + 2 * k + 1] = b;
+        __syncthreads();
+        bool pending = false, merged = false;
+        for (int k = threadIdx.x; k < m; k += blockDim.x) {
+            const int i = list[k];
+            if (i < 0) continue;
+            if (child_done(left[i], first, last, flag, stamp) &&
+                child_done(right[i], first, last, flag, stamp)) {
+                merge_node(i, spos, first, last, left, right, box, fbox, cbox);
+                flag[i] = stamp;
+                list[k] = -1;
+                merged = true;
+            } else {
+                pending = true;
+            }
+        }
+        if (pending) *more = 1;
+        if (merged) *progress = 1;
+        __syncthreads();
+        const int again = *more, moved = *progress;
+        __syncthreads();
+        if (!again) return false;
+        if (!moved) return true;
     }
+}
+
+__global__ __launch_bounds__(MID_BS) void k_refit_mid(
+    int n, const double4 *__restrict__ spos, const int32_t *__restrict__ first,
+    const int32_t *__restrict__ last, const int32_t *__restrict__ left,
+    const int32_t *__restrict__ right, int32_t *flag, double *box, float4 *fbox, float4 *cbox,
+    int32_t *top, int *ntop)
+{
+    __shared__ int32_t list[MID_CAP];
+    __shared__ int nlist, more, progress;
+    const int B0 = blockIdx.x * MID_S;
+    const int B1 = min(n, B0 + MID_S) - 1;
+    if (threadIdx.x == 0) nlist = 0;
+    __syncthreads();
+    for (int i = B0 + threadIdx.x; i <= min(B1, n - 2); i += MID_BS) {
+        const int f = first[i], l = last[i];
+        if (f / REFIT_BS == l / REFIT_BS) continue;   // merged by pass 1
+        if (f >= B0 && l <= B1) {
+            const int k = atomicAdd(&nlist, 1);
+            if (k < MID_CAP) {
+                list[k] = i;
+                continue;
+            }
+        }
+        top[atomicAdd(ntop, 1)] = i;                  // crosses this block (or the list is full)
+    }
+    __syncthreads();
+    const int m = min(nlist, MID_CAP);
+    if (refit_sweeps(list, m, 0, &more, &progress, spos, first, last, left, right, flag, box,
+                     fbox, cbox)) {
+        // nodes above one that went to pass 3 (a full list): pass 3 takes them as well
+        for (int k = threadIdx.x; k < m; k += MID_BS)
+            if (list[k] >= 0) top[atomicAdd(ntop, 1)] = list[k];
+    }
+}
+
+__global__ __launch_bounds__(MID_BS) void k_refit_top(
+    const double4 *__restrict__ spos, const int32_t *__restrict__ first,
+    const int32_t *__restrict__ last, const int32_t *__restrict__ left,
+    const int32_t *__restrict__ right, int32_t *flag, double *box, float4 *fbox, float4 *cbox,
+    int32_t *top, const int *ntop, int *stalled)
+{
+    __shared__ int more, progress;
+    if (refit_sweeps(top, *ntop, TOP_STAMP, &more, &progress, spos, first, last, left, right,
+                     flag, box, fbox, cbox) &&
+        threadIdx.x == 0)
+        *stalled = 1;   // a node whose children never came: reported by the host
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1093,10 +1197,10 @@ void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, co
                                  std::to_string(bad) + " particles are not)");
     size_t tb = 0;
     AHOP_HIP(rocprim::radix_sort_pairs(nullptr, tb, t_.key_tmp, t_.key, t_.iota, t_.perm,
-                                       (size_t)n, 0, 63, s));
+                                       (size_t)n, 0, MORTON_KEY_BITS, s));
     void *temp = b_temp.ensure(tb);
     AHOP_HIP(rocprim::radix_sort_pairs(temp, tb, t_.key_tmp, t_.key, t_.iota, t_.perm, (size_t)n,
-                                       0, 63, s));
+                                       0, MORTON_KEY_BITS, s));
     hipLaunchKernelGGL(k_gather, dim3(nblk(n)), dim3(256), 0, s, d_pos, d_mass, t_.perm, n,
                        t_.spos);
     AHOP_HIP(hipEventRecord(ev_[1], s));
@@ -1109,23 +1213,23 @@ void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, co
     t_.leaf_parent = as<int32_t>(b_leaf_parent, n);
     t_.flag = as<int32_t>(b_flag, ni);
     t_.box = as<double>(b_box, 6 * ni);
+    t_.nrec = as<int4>(b_nrec, ni);
+    t_.cbox = as<float4>(b_cbox, 4 * ni);
     hipLaunchKernelGGL(k_radix_tree, dim3(nblk(ni)), dim3(256), 0, s, t_.key, (int)n, t_.first,
-                       t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent);
+                       t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent, t_.nrec);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ni)), dim3(256), 0, s, t_.flag, ni, 0);
     t_.fbox = as<float4>(b_fbox, 2 * ni);
     hipLaunchKernelGGL(k_refit_local, dim3(nblk(n, REFIT_BS)), dim3(REFIT_BS), 0, s, (int)n,
                        t_.spos, t_.first, t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent,
-                       t_.box, t_.fbox);
-    int32_t *starts = as<int32_t>(b_roots, n);        // scratch: the roots buffer is free here
-    hipLaunchKernelGGL(k_refit_upper_init, dim3(nblk(ni)), dim3(256), 0, s, (int)n, t_.first,
-                       t_.last, t_.left, t_.right, t_.flag, starts, misc + 5);
-    // a start is the lowest node crossing some block boundary: at most one per boundary
-    hipLaunchKernelGGL(k_refit_upper, dim3(nblk(n / REFIT_BS + 1)), dim3(256), 0, s, starts, misc + 5,
-                       t_.spos, t_.left, t_.right, t_.parent, t_.flag, t_.box, t_.fbox);
-    t_.nrec = as<int4>(b_nrec, ni);
-    t_.cbox = as<float4>(b_cbox, 4 * ni);
-    hipLaunchKernelGGL(k_node_records, dim3(nblk(ni)), dim3(256), 0, s, ni, t_.first, t_.last,
-                       t_.left, t_.right, t_.fbox, t_.spos, t_.nrec, t_.cbox);
+                       t_.box, t_.fbox, t_.cbox);
+    // pass 3's list (misc[5] counts it): the roots buffer is free here
+    int32_t *top = as<int32_t>(b_roots, n);
+    hipLaunchKernelGGL(k_refit_mid, dim3((unsigned)((n + MID_S - 1) / MID_S)), dim3(MID_BS), 0, s,
+                       (int)n, t_.spos, t_.first, t_.last, t_.left, t_.right, t_.flag, t_.box,
+                       t_.fbox, t_.cbox, top, misc + 5);
+    hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(MID_BS), 0, s, t_.spos, t_.first, t_.last,
+                       t_.left, t_.right, t_.flag, t_.box, t_.fbox, t_.cbox, top, misc + 5,
+                       misc + 6);
     AHOP_HIP(hipEventRecord(ev_[2], s));
 }
 
@@ -1186,7 +1290,8 @@ void Engine::knn_density(const Params &p)
     double *rho_s = as<double>(b_rho_s, n);
     int32_t *nbr = as<int32_t>(b_nbr, n * p.nhop);
     int32_t *fb = as<int32_t>(b_fallback, n);
-    int *misc = (int *)b_misc.p;  // [0] bad, [1] overflow, [2] fb count, [3] changed
+    // [0] bad, [1] overflow, [2] fb count, [3] changed, [5] refit list, [6] refit stalled
+    int *misc = (int *)b_misc.p;
     unsigned long long *cnt = as<unsigned long long>(b_counter, 4);
     AHOP_HIP(hipMemsetAsync(cnt, 0, 4 * sizeof(unsigned long long), s));
     KnnArgs a{};
@@ -1227,9 +1332,10 @@ void Engine::knn_density(const Params &p)
     AHOP_HIP(hipEventRecord(ev_[3], s));
     AHOP_HIP(hipMemcpyAsync(res_.knn_stats, stats, KNN_STAT_N * sizeof(unsigned long long),
                             hipMemcpyDeviceToHost, s));
-    int host_misc[4];
-    AHOP_HIP(hipMemcpyAsync(host_misc, misc, 4 * sizeof(int), hipMemcpyDeviceToHost, s));
+    int host_misc[8];
+    AHOP_HIP(hipMemcpyAsync(host_misc, misc, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
     AHOP_HIP(hipStreamSynchronize(s));
+    if (host_misc[6] != 0) throw std::runtime_error("device tree: bounding-box refit stalled");
     if (host_misc[1] != 0) throw std::runtime_error("device tree deeper than the traversal stack");
     res_.nfallback = host_misc[2];
     if (host_misc[2] > 0) {

@@ -290,7 +290,8 @@ def test_angular_momentum_per_shell_matches_the_oracle(gpu):
     """ANG_MOM_OF_R: the device's per-shell masses and angular momenta (100 shells of the
     all-member ellipsoid) against det_ang_momentum_per_shell_1b5's restatement, 1e-10 of the
     halo's mass and of the sum of its shells' |L|, the ellipsoid to 1e-10; the same halos'
-    31 fields unchanged by the option; NaN rows exactly where rvir is not finite.  On the
+    31 fields unchanged by the option (to rounding); NaN rows exactly where rvir is not
+    finite.  On the
     100 largest halos of the 50 000-halo workload, the shells sum to each halo's m and L."""
     import torch
     from amdahop import halo_props
@@ -302,7 +303,9 @@ def test_angular_momentum_per_shell_matches_the_oracle(gpu):
     out, _, ang = halo_props(hs.offsets, pos, vel, mass, ang_mom_of_r=True)
     plain, _ = halo_props(hs.offsets, pos, vel, mass)
     out, plain = out.cpu().numpy(), plain.cpu().numpy()
-    assert np.array_equal(out, plain, equal_nan=True)
+    # (the shell tables are summed with LDS atomics, so two runs agree to rounding)
+    assert np.array_equal(np.isnan(out), np.isnan(plain))
+    assert np.allclose(out, plain, rtol=1e-12, atol=0.0, equal_nan=True)
     sh, ax = ang.shells.cpu().numpy(), ang.axes.cpu().numpy()
     worst_m = worst_l = worst_a = 0.0
     for h in range(hs.nhalos):
