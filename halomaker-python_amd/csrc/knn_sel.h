@@ -37,6 +37,10 @@ constexpr int SEL_CAPL = 512;
 #ifndef SEL_SEED_PAD
 #define SEL_SEED_PAD 32
 #endif
+// neighbours the density gathers at a time
+#ifndef SEL_DCH
+#define SEL_DCH 8
+#endif
 // log entries the collection reads ahead per lane
 #ifndef SEL_P2U
 #define SEL_P2U 8
@@ -297,9 +301,10 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         (out) = __builtin_fmaf(dz_, dz_, __builtin_fmaf(dy_, dy_, dx_ * dx_));                 \
     } while (0)
 
+    // (past the log's end the last entry is overwritten: such a lane is re-run in FP64)
 #define SEL_LOG(d, id)                                                                         \
     do {                                                                                       \
-        if (nlog < SEL_CAPL) lg[nlog] = make_uint2(__float_as_uint(d), (uint32_t)(id)); \
+        lg[min(nlog, SEL_CAPL - 1)] = make_uint2(__float_as_uint(d), (uint32_t)(id));         \
         ++nlog;                                                                                \
     } while (0)
 
@@ -614,19 +619,20 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         st_logmax = max(st_logmax, (unsigned)nlog);
         st_ovf += (nlog > SEL_CAPL) ? 1u : 0u;
         int cnt = 0;
-        float kmax = -INFINITY;
+        int kmaxb = 0;   // the largest collected key's bits (keys are non-negative)
         for (int e = 0; __any(e < nl); e += SEL_P2U) {
+            // a lane reads only its own entries (reading every lane up to the wave's longest
+            // log: C3 kernel 55.9 -> 56.8 ms)
             uint2 v[SEL_P2U];
 #pragma unroll
             for (int u = 0; u < SEL_P2U; ++u)
                 v[u] = e + u < nl ? lg[e + u] : make_uint2(0x7f800000u, 0u);
 #pragma unroll
             for (int u = 0; u < SEL_P2U; ++u) {
-                const float kk = __uint_as_float(v[u].x);
-                if (kk < T) {
+                if (__uint_as_float(v[u].x) < T) {
                     if (cnt < KCAP) lg[cnt] = v[u];
                     ++cnt;
-                    kmax = fmaxf(kmax, kk);
+                    kmaxb = max(kmaxb, (int)v[u].x);
                 }
             }
         }
@@ -650,7 +656,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                     if (r != off && kk < T) {
                         if (cnt < KCAP) lg[cnt] = make_uint2(__float_as_uint(kk), (uint32_t)(seed_lo + r));
                         ++cnt;
-                        kmax = fmaxf(kmax, kk);
+                        kmaxb = max(kmaxb, __float_as_int(kk));
                     }
                 }
             }
@@ -664,6 +670,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         bool ok = active && !wave_unsafe && nlog <= SEL_CAPL && cnt == K;
         if (ok && T != INFINITY) {
             const float E = fp32_key_err(T, hQ, Lmin);
+            const float kmax = __int_as_float(kmaxb);
             ok = T - kmax > 2 * E && E < 0.01f * T && sqrtf(T * 1.0001f) + hQ < 0.4999f * Lmin;
         }
         if (active && !ok) A.fb_list[atomicAdd(A.fb_count, 1)] = q;
@@ -698,18 +705,18 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
             double contrib = 0.0;
             double prev = -1.0;
             bool sorted = true;
-            // 8 neighbours gathered at a time; the terms past k-1 (on the query itself, a
-            // valid slot) add +0.0, which leaves the sum's bits unchanged
-            for (int i0 = 0; i0 < K - 1; i0 += 8) {
-                double4 c[8];
+            // SEL_DCH neighbours gathered at a time; the terms past k-1 (on the query itself,
+            // a valid slot) add +0.0, which leaves the sum's bits unchanged
+            for (int i0 = 0; i0 < K - 1; i0 += SEL_DCH) {
+                double4 c[SEL_DCH];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < SEL_DCH; ++u) {
                     const int i = i0 + u;
                     const uint32_t gi = hid[min(i, KCAP - 1) * WAVE];
                     c[u] = spos[i < K - 1 ? gi : (uint32_t)q];
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < SEL_DCH; ++u) {
                     const bool use = i0 + u < K - 1;
                     const double d = d2_ref(Q.x, Q.y, Q.z, c[u].x, c[u].y, c[u].z, Lx, Ly, Lz);
                     sorted = sorted && (d >= prev || !use);
