@@ -300,9 +300,10 @@ __device__ __forceinline__ void store_child_rec(int node, int k, const double lo
 //    barriers: a node merges in a sweep when both children were done before it (a leaf, a
 //    pass-1 node, or a node merged in an earlier sweep, whose writes the barrier made
 //    visible).  Nodes that cross its block go to a list for
-//  pass 3 (k_refit_top): one workgroup, the same sweeps over that list.
-// Pass 2 and 3 stamp a merged node's flag with the sweep's number (pass 3 from TOP_STAMP
-// on), so that a child merged in the current sweep is never taken as done.
+//  pass 3 (k_refit_top): one workgroup climbing that list bottom-up with counters (sweeps
+//    over it took 1.0 ms at 16.8M particles: the top of the tree is deep).
+// Pass 2 stamps a merged node's flag with the sweep's number, so that a child merged in the
+// current sweep is never taken as done.
 constexpr int REFIT_BS = 256;
 constexpr int MID_S = 65536;     // leaves per pass-2 workgroup
 constexpr int MID_BS = 1024;
@@ -500,17 +501,60 @@ __global__ __launch_bounds__(MID_BS) void k_refit_mid(
     }
 }
 
+// Pass 3: one workgroup.  Each listed node's flag is first set to -(1 + its children not yet
+// merged) (a child is merged when it is a particle, a pass-1 node or has a positive pass-2
+// stamp), or to a start mark when none is pending; then threads climb from the starts,
+// merging, and a parent is
+// merged by the thread whose child arrives last (workgroup-scope fences: the whole pass is on
+// one CU).  The parent of a listed node is listed too (its range is wider), or is none.
 __global__ __launch_bounds__(MID_BS) void k_refit_top(
     const double4 *__restrict__ spos, const int32_t *__restrict__ first,
     const int32_t *__restrict__ last, const int32_t *__restrict__ left,
-    const int32_t *__restrict__ right, int32_t *flag, double *box, float4 *fbox, float4 *cbox,
-    int32_t *top, const int *ntop, int *stalled)
+    const int32_t *__restrict__ right, const int32_t *__restrict__ parent, int32_t *flag,
+    double *box, float4 *fbox, float4 *cbox, const int32_t *top, const int *ntop, int *stalled)
 {
-    __shared__ int more, progress;
-    if (refit_sweeps(top, *ntop, TOP_STAMP, &more, &progress, spos, first, last, left, right,
-                     flag, box, fbox, cbox) &&
-        threadIdx.x == 0)
-        *stalled = 1;   // a node whose children never came: reported by the host
+    const int m = *ntop;
+    auto merged = [&](int c) {
+        return c < 0 || first[c] / REFIT_BS == last[c] / REFIT_BS ||
+               __hip_atomic_load(&flag[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > 0;
+    };
+    // a start (no child pending) is marked TOP_START, which arrivals never produce: a node with
+    // children pending counts -1 - pending up to -1 as they arrive
+    constexpr int TOP_START = -1000;
+    for (int k = threadIdx.x; k < m; k += MID_BS) {
+        const int i = top[k];
+        const int pending = (merged(left[i]) ? 0 : 1) + (merged(right[i]) ? 0 : 1);
+        __hip_atomic_store(&flag[i], pending == 0 ? TOP_START : -1 - pending, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    __syncthreads();
+    int climbed = 0;
+    for (int k = threadIdx.x; k < m; k += MID_BS) {
+        int node = top[k];
+        if (__hip_atomic_load(&flag[node], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) !=
+            TOP_START)
+            continue;   // not a start: a child is still pending
+        while (true) {
+            merge_node(node, spos, first, last, left, right, box, fbox, cbox);
+            ++climbed;
+            __hip_atomic_store(&flag[node], TOP_STAMP, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+            node = parent[node];
+            if (node < 0) break;
+            __threadfence_block();
+            const int old = __hip_atomic_fetch_add(&flag[node], 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (old != -2) break;   // the parent's other child is still pending
+            __threadfence_block();
+        }
+    }
+    // every listed node is merged exactly once; fewer merges mean a node never became ready
+    __shared__ int total;
+    if (threadIdx.x == 0) total = 0;
+    __syncthreads();
+    atomicAdd(&total, climbed);
+    __syncthreads();
+    if (threadIdx.x == 0 && total != m) *stalled = 1;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1229,8 +1273,8 @@ void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, co
                        (int)n, t_.spos, t_.first, t_.last, t_.left, t_.right, t_.flag, t_.box,
                        t_.fbox, t_.cbox, top, misc + 5);
     hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(MID_BS), 0, s, t_.spos, t_.first, t_.last,
-                       t_.left, t_.right, t_.flag, t_.box, t_.fbox, t_.cbox, top, misc + 5,
-                       misc + 6);
+                       t_.left, t_.right, t_.parent, t_.flag, t_.box, t_.fbox, t_.cbox, top,
+                       misc + 5, misc + 6);
     AHOP_HIP(hipEventRecord(ev_[2], s));
 }
 
