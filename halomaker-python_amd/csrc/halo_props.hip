@@ -941,34 +941,61 @@ __global__ void k_angmom_combine(int nh, const int32_t *tile_first, const double
     }
 }
 
-// first pair index of row i of the upper triangle of an n x n pair matrix (pairs (i, j > i)
-// numbered row by row)
-__device__ __forceinline__ int64_t pair_row_start(int i, int n)
+// sum over members j0 <= j < je of m_j / d_ij for member a = mem[i] (for a tile on the
+// diagonal only the j > i): j is the same on every lane of the wave (the member is read
+// once for all 64 lanes), four partial sums carry independent rsqrt chains.  FOLD takes the
+// minimum image min(|t|, L - |t|) per axis.  The distance is formed with fused multiply-adds:
+// the sum is held to 1e-10 of the energies, not bit for bit.
+template <bool FOLD, bool DIAG>
+__device__ __forceinline__ double pe_tile(const double4 *mem, const double4 a, int i, int j0,
+                                          int je, double L)
 {
-    return (int64_t)i * (2 * n - i - 1) / 2;
-}
-
-// row of pair p: the largest i with pair_row_start(i) <= p
-__device__ __forceinline__ int pair_row(int64_t p, int n)
-{
-    const double b = 2.0 * n - 1.0;
-    int i = (int)floor(0.5 * (b - sqrt(fmax(b * b - 8.0 * (double)p, 0.0))));
-    i = max(0, min(i, n - 2));
-    while (i > 0 && pair_row_start(i, n) > p) --i;
-    while (i < n - 2 && pair_row_start(i + 1, n) <= p) ++i;
-    return i;
+    double r0 = 0.0, r1 = 0.0, r2 = 0.0, r3 = 0.0;
+    auto term = [&](int j, double &racc) {
+        const double4 b = mem[j];
+        double tx = b.x - a.x, ty = b.y - a.y, tz = b.z - a.z;
+        if (FOLD) {
+            tx = fabs(tx);
+            ty = fabs(ty);
+            tz = fabs(tz);
+            tx = fmin(tx, L - tx);
+            ty = fmin(ty, L - ty);
+            tz = fmin(tz, L - tz);
+        }
+        double d2 = fma(tz, tz, fma(ty, ty, tx * tx));
+        double w = b.w;
+        if (DIAG) {
+            const bool on = j > i;
+            d2 = on ? d2 : 1.0;
+            w = on ? w : 0.0;
+        }
+        racc = fma(w, rsqrt(d2), racc);
+    };
+    int j = j0;
+    for (; j + 4 <= je; j += 4) {
+        term(j, r0);
+        term(j + 1, r1);
+        term(j + 2, r2);
+        term(j + 3, r3);
+    }
+    for (; j < je; ++j) term(j, r0);
+    return (r0 + r1) + (r2 + r3);
 }
 
 // ---- direct potential energy of halos below the pair-sum limit (det_halo_energies_1b3
 // :1417-1427): ep = sum_{i<j} -(m_i m_j) / (d_ij / L) * G / L = -G sum m_i m_j / d_ij, with
 // d_ij the minimum-image distance (the strict wrap at L/2 gives the same distance as
-// min(|d|, L - |d|)).  The halo's members are staged in LDS (at most PE_NMAX of them) and the
-// n(n-1)/2 pairs are cut into equal contiguous runs, one per thread, walked row by row.
+// min(|d|, L - |d|)).  The halo's members are staged in LDS (at most PE_NMAX of them) and cut
+// into tiles of 64; the T(T+1)/2 tile pairs (A <= B) go round the block's waves.  In a tile
+// pair lane l holds member 64A + l and runs over the members of tile B in step with the
+// other lanes (on the diagonal A = B only the pairs j > i count).
 constexpr int PE_NMAX = 2048;   // 64 KiB of LDS
+constexpr int PE_TILE = 64;     // members per tile: one per lane of a wave
 template <int BS>
 __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *small_halos,
                                                   double *ep_out)
 {
+    static_assert(BS % PE_TILE == 0, "whole waves");
     extern __shared__ double4 mem[];
     __shared__ double sh[BS];
     const int h = small_halos[blockIdx.x];
@@ -990,36 +1017,31 @@ __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *smal
         }
     }
     const bool fold = __syncthreads_or(far);
-    const int64_t P = (int64_t)n * (n - 1) / 2;
-    const int64_t p0 = P * threadIdx.x / BS, p1 = P * (threadIdx.x + 1) / BS;
+    const int lane = threadIdx.x % PE_TILE;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / PE_TILE);
+    const int T = (n + PE_TILE - 1) / PE_TILE;
+    const int NT = T * (T + 1) / 2;
     double acc = 0.0;
-    if (p0 < p1) {
-        int i = pair_row(p0, n);
-        int j = (int)(p0 - pair_row_start(i, n)) + i + 1;
-        double4 a = mem[i];
-        double row = 0.0;   // sum of m_j / d_ij over this thread's part of row i
-#define PE_PAIRS(SEP)                                                                          \
-    for (int64_t p = p0; p < p1; ++p) {                                                        \
-        const double4 b = mem[j];                                                              \
-        double tx, ty, tz;                                                                     \
-        SEP;                                                                                   \
-        row += b.w * rsqrt(tx * tx + ty * ty + tz * tz);                                       \
-        if (++j == n) {                                                                        \
-            acc += a.w * row;                                                                  \
-            row = 0.0;                                                                         \
-            ++i;                                                                               \
-            j = i + 1;                                                                         \
-            if (i < n) a = mem[i];                                                             \
-        }                                                                                      \
-    }
-        if (fold) {
-            PE_PAIRS(tx = fabs(b.x - a.x); ty = fabs(b.y - a.y); tz = fabs(b.z - a.z);
-                     tx = fmin(tx, L - tx); ty = fmin(ty, L - ty); tz = fmin(tz, L - tz))
-        } else {
-            PE_PAIRS(tx = b.x - a.x; ty = b.y - a.y; tz = b.z - a.z)
+    for (int q = wave; q < NT; q += BS / PE_TILE) {
+        int A = 0, rem = q;   // tile pair q, numbered row by row: (A, A + rem)
+        while (rem >= T - A) {
+            rem -= T - A;
+            ++A;
         }
-#undef PE_PAIRS
-        acc += a.w * row;
+        const int B = A + rem;
+        const int i = A * PE_TILE + lane;
+        const int j0 = B * PE_TILE, je = min(j0 + PE_TILE, n);
+        if (i < n) {
+            const double4 a = mem[i];
+            double row;
+            if (fold)
+                row = A == B ? pe_tile<true, true>(mem, a, i, j0, je, L)
+                             : pe_tile<true, false>(mem, a, i, j0, je, L);
+            else
+                row = A == B ? pe_tile<false, true>(mem, a, i, j0, je, L)
+                             : pe_tile<false, false>(mem, a, i, j0, je, L);
+            acc = fma(a.w, row, acc);
+        }
     }
     const int t = threadIdx.x;
     sh[t] = acc;
