@@ -67,6 +67,31 @@ struct KeyTop {
     v_t rk;
 };
 
+// The bucket scan only picks which staged candidates a lane will look at again: a picked
+// candidate's key is recomputed as above (SEL_KEY) and compared with the lane's (k+1)-th key
+// tau before it can enter the heap.  The scan therefore tests the expanded form
+//   e = fma(cx, -2qx, fma(cy, -2qy, fma(cz, -2qz, w))),  w = |c|^2 staged with the bucket,
+// against the threshold below, which is never smaller than needed: whenever the key d_s
+// (SEL_KEY, on the same FP32 values c and q) is below tau, e is below it.  With u = 2^-24 and D
+// = |c - q|^2 = |c|^2 - 2 c.q + |q|^2 exactly:
+//  * d_s >= (1 - u)^5 D (two roundings of each difference squared, three of the sums), so
+//    d_s < tau gives D < tau (1 + 5.01u) and |c| <= |q| + sqrt(tau)(1 + 2.6u) =: S + r;
+//  * |w - |c|^2| <= 3.01u |c|^2, and the three fmas add at most 3u (|w| + 2|c||q|), so
+//    e <= D - |q|^2 + 6.02u (S + r)(S + 2r);
+//  * the threshold (tau - Qn) + M, with Qn = |q|^2 (1 +- 3.01u) computed like w, loses at
+//    most 3.01u |q|^2 + 2.01u (tau + |q|^2) to roundings.
+// Together, e < threshold holds when M (1 - u) >= 14.3u (S + 2r)^2; M = 32u (sqrt(tau) +
+// 2 sqrt(Qn))^2 clears that (the tiny absolute term guards against underflow).  Lanes
+// without a search (tau < 0) get -inf: they accept nothing; tau = +inf accepts everything.
+// The scan costs 5 VALU per candidate (3 fmas, the subtraction, and v_alignbit shifting the
+// difference's sign bit into the lane's mask) against 10 for the key, a compare and a select.
+__device__ __forceinline__ float sel_thr(float tau, float Qn, float rqn)
+{
+    if (!(tau >= 0.0f)) return -INFINITY;
+    const float s = sqrtf(tau) + 2.0f * rqn;
+    return (tau - Qn) + (32.0f * U32 * (s * s) + 1e-30f);
+}
+
 __device__ __forceinline__ int max8(int a, int b, int c, int d, int e, int f, int g, int h)
 {
     return max(max(max(a, b), max(c, d)), max(max(e, f), max(g, h)));
@@ -206,6 +231,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
     __shared__ __align__(16) float sx[2 * WAVE];
     __shared__ __align__(16) float sy[2 * WAVE];
     __shared__ __align__(16) float sz[2 * WAVE];
+    __shared__ __align__(16) float sw[2 * WAVE];   // |c|^2 of the staged candidates
     __shared__ int32_t stack[STACK];
 
     const int lane = threadIdx.x;
@@ -266,6 +292,10 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         }
         const double rx = wrapd(P.x - cx, Lx), ry = wrapd(P.y - cy, Ly), rz = wrapd(P.z - cz, Lz);
         const float qx = (float)rx, qy = (float)ry, qz = (float)rz;
+        // the scan's expanded form: -2q (exact) and |q|^2, sqrt(|q|^2) (sel_thr)
+        const float m2qx = -2.0f * qx, m2qy = -2.0f * qy, m2qz = -2.0f * qz;
+        const float Qn = __builtin_fmaf(qz, qz, __builtin_fmaf(qy, qy, qx * qx));
+        const float rqn = sqrtf(Qn);
         const double hql = active ? fmax(fabs(rx), fmax(fabs(ry), fabs(rz))) : 0.0;
         // the error bounds of this lane's keys need only its own offset from the origin
         const float hQ = (float)hql * (1.0f + 4 * U32);
@@ -403,26 +433,30 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         else SEL_FLUSH_FILL();                                                                 \
     } while (0)
 
-    // evaluate the staged candidates j = 0..cnt-1 of buffer b (slots fst + j) and set bit j
-    // of mask_ where ACCEPT holds (ACCEPT sees the key d_ and the slot sj_); three 16-byte
-    // reads serve four candidates, and each 32-candidate half builds a 32-bit mask
-#define SEL_SCAN(b, fst, cnt, ACCEPT)                                                          \
+    // scan the staged candidates j = 0..cnt-1 of buffer b and set bit j of mask_ where the
+    // expanded-form test against thr passes (sel_thr).  Each 32-candidate half is walked
+    // from its top group of 4 down, every candidate's sign bit shifted in at the bottom, so
+    // that candidate j ends at bit j; bits of the top group past cnt are cleared at the end
+#define SEL_SCAN(b, cnt, thr)                                                                  \
     do {                                                                                       \
         uint32_t hm_[2] = {0u, 0u};                                                            \
         _Pragma("unroll") for (int h_ = 0; h_ < 2; ++h_) {                                     \
-            for (int j4_ = 32 * h_; j4_ < min((cnt), 32 * h_ + 32); j4_ += 4) {                \
+            const int nh_ = min((cnt), 32 * h_ + 32) - 32 * h_;                                \
+            for (int j4_ = 32 * h_ + ((nh_ + 3) & ~3) - 4; j4_ >= 32 * h_; j4_ -= 4) {         \
                 const float4 X_ = *reinterpret_cast<const float4 *>(sx + (b) * WAVE + j4_);    \
                 const float4 Y_ = *reinterpret_cast<const float4 *>(sy + (b) * WAVE + j4_);    \
                 const float4 Z_ = *reinterpret_cast<const float4 *>(sz + (b) * WAVE + j4_);    \
+                const float4 W_ = *reinterpret_cast<const float4 *>(sw + (b) * WAVE + j4_);    \
                 const float xs_[4] = {X_.x, X_.y, X_.z, X_.w};                                 \
                 const float ys_[4] = {Y_.x, Y_.y, Y_.z, Y_.w};                                 \
                 const float zs_[4] = {Z_.x, Z_.y, Z_.z, Z_.w};                                 \
-                _Pragma("unroll") for (int t_ = 0; t_ < 4; ++t_) {                             \
-                    const int j_ = j4_ + t_;                                                   \
-                    const int sj_ = (fst) + j_;                                                \
-                    float d_;                                                                  \
-                    SEL_KEY(xs_[t_], ys_[t_], zs_[t_], d_);                                    \
-                    if (ACCEPT) hm_[h_] |= 1u << (j_ - 32 * h_);                               \
+                const float ws_[4] = {W_.x, W_.y, W_.z, W_.w};                                 \
+                _Pragma("unroll") for (int t_ = 3; t_ >= 0; --t_) {                            \
+                    const float e_ = __builtin_fmaf(                                           \
+                        xs_[t_], m2qx,                                                         \
+                        __builtin_fmaf(ys_[t_], m2qy, __builtin_fmaf(zs_[t_], m2qz, ws_[t_]))); \
+                    hm_[h_] = __builtin_amdgcn_alignbit(hm_[h_], __float_as_uint(e_ - (thr)),  \
+                                                        31u);                                  \
                 }                                                                              \
             }                                                                                  \
         }                                                                                      \
@@ -440,20 +474,25 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         __syncthreads();                                                                       \
         if (lane < (cnt)) {                                                                    \
             const double4 c_ = spos[(fst) + lane];                                             \
-            sx[nbuf * WAVE + lane] = (float)wrapd(c_.x - cx, Lx);                              \
-            sy[nbuf * WAVE + lane] = (float)wrapd(c_.y - cy, Ly);                              \
-            sz[nbuf * WAVE + lane] = (float)wrapd(c_.z - cz, Lz);                              \
+            const float x_ = (float)wrapd(c_.x - cx, Lx);                                      \
+            const float y_ = (float)wrapd(c_.y - cy, Ly);                                      \
+            const float z_ = (float)wrapd(c_.z - cz, Lz);                                      \
+            sx[nbuf * WAVE + lane] = x_;                                                       \
+            sy[nbuf * WAVE + lane] = y_;                                                       \
+            sz[nbuf * WAVE + lane] = z_;                                                       \
+            sw[nbuf * WAVE + lane] = __builtin_fmaf(z_, z_, __builtin_fmaf(y_, y_, x_ * x_));  \
         }                                                                                      \
         __syncthreads();                                                                       \
         uint64_t mask_ = 0;                                                                    \
+        const float thr_ = sel_thr(tau, Qn, rqn);                                              \
+        SEL_SCAN(nbuf, cnt, thr_);                                                             \
         if ((seeding) || ((fst) <= seed_hi && (fst) + (cnt) - 1 >= seed_lo)) {                 \
-            SEL_SCAN(nbuf, fst, cnt,                                                           \
-                     d_ < tau && sj_ != q &&                                                   \
-                         ((seeding) ? (sj_ >= seed_lo && sj_ <= seed_hi)                       \
-                                    : !(sj_ >= seed_lo && sj_ <= seed_hi)));                   \
-        } else {                                                                               \
-            /* the bucket lies outside the seed window (and holds no query of the wave) */    \
-            SEL_SCAN(nbuf, fst, cnt, d_ < tau);                                                \
+            /* seeding keeps the seed window's slots, the traversal the others; neither  */    \
+            /* the lane's own slot (buckets outside the window hold no query of the wave) */   \
+            const uint64_t win_ = bits_in(seed_lo - (fst), seed_hi - (fst) + 1);               \
+            mask_ &= (seeding) ? win_ : ~win_;                                                 \
+            const int so_ = q - (fst);                                                         \
+            mask_ &= (so_ >= 0 && so_ < WAVE) ? ~(1ull << so_) : ~0ull;                        \
         }                                                                                      \
         m0 = nbuf == 0 ? mask_ : m0;                                                           \
         m1 = nbuf == 1 ? mask_ : m1;                                                           \
