@@ -119,8 +119,9 @@ __device__ __forceinline__ void kt_sift_l1(KeyTop<RC> &h, int *hw, int lane, int
             vk = mq;
         }
     }
+    const int nv = vk | (c - 1);   // tagged with its place
 #pragma unroll
-    for (int t = 0; t < RC; ++t) h.rk[t] = (c == t + 1) ? (vk | t) : h.rk[t];
+    for (int t = 0; t < RC; ++t) h.rk[t] = (c == t + 1) ? nv : h.rk[t];
 }
 
 // sift quantised key dq down from the root
@@ -148,8 +149,9 @@ __device__ __forceinline__ void kt_set(KeyTop<RC> &h, int *hw, int lane, int e, 
         hw[(e - RT) * WAVE + lane] = q | ((e - RT) % LC);
     } else {
         h.k0 = e == 0 ? q : h.k0;
+        const int nv = q | ((e - 1) & 7);   // tagged with its place (for e >= 1)
 #pragma unroll
-        for (int t = 0; t < RC; ++t) h.rk[t] = (e == t + 1) ? (q | t) : h.rk[t];
+        for (int t = 0; t < RC; ++t) h.rk[t] = (e == t + 1) ? nv : h.rk[t];
     }
 }
 
