@@ -324,8 +324,9 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         // heap entries written by the seeding (lanes without a search count as full)
         int filled = tau < 0.0f ? KH : 0;
         int nlog = 0;
-        bool dfill = false;   // the lane's heap was filled from the two first staged buckets
-        int dfill_rs = 0;     // with their slots [dfill_rs, dfill_rs + KH) (relative)
+        // The seeding logs nothing: all its candidates lie in the seed window, whose keys the
+        // collection computes again, and the traversal skips the window's slots
+        bool logon = false;
 
 #define SEL_KEY(X, Y, Z, out)                                                                  \
     do {                                                                                       \
@@ -336,8 +337,10 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
     // (past the log's end the last entry is overwritten: such a lane is re-run in FP64)
 #define SEL_LOG(d, id)                                                                         \
     do {                                                                                       \
-        lg[min(nlog, SEL_CAPL - 1)] = make_uint2(__float_as_uint(d), (uint32_t)(id));         \
-        ++nlog;                                                                                \
+        if (logon) { /* wave-uniform */                                                        \
+            lg[min(nlog, SEL_CAPL - 1)] = make_uint2(__float_as_uint(d), (uint32_t)(id));     \
+            ++nlog;                                                                            \
+        }                                                                                      \
     } while (0)
 
 #define SEL_INSERT(d, id)                                                                      \
@@ -359,16 +362,15 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
     // first), or +inf when it has none left (the stage index then stays in range)
 #define SEL_PICK(dout, idout)                                                                  \
     do {                                                                                       \
-        const bool any_ = (m0 | m1) != 0;                                                      \
         const bool s0_ = m0 != 0;                                                              \
         uint64_t mk_ = s0_ ? m0 : m1;                                                          \
+        const bool any_ = mk_ != 0;                                                            \
         const int fsel_ = s0_ ? f0 : f1;                                                       \
-        const int bsel_ = s0_ ? 0 : 1;                                                         \
-        const int j_ = __ffsll((long long)mk_) - 1;                                            \
+        const int j_ = (int)__builtin_ctzll(mk_ | (1ull << 63)); /* 63 when empty */          \
         mk_ &= mk_ - 1;                                                                        \
         m0 = s0_ ? mk_ : m0;                                                                   \
         m1 = s0_ ? m1 : mk_;                                                                   \
-        const int sidx_ = bsel_ * WAVE + (j_ & (WAVE - 1));                                    \
+        const int sidx_ = (s0_ ? 0 : WAVE) + j_;                                               \
         SEL_KEY(sx[sidx_], sy[sidx_], sz[sidx_], dout);                                        \
         (dout) = any_ ? (dout) : INFINITY;                                                     \
         (idout) = fsel_ + j_;                                                                  \
@@ -510,8 +512,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
             // Two full buckets of the window are staged, and every lane with a search has
             // accepted all of them but its own slot: KH of them, centred on the lane's own
             // slot as far as the two buckets allow, go straight into its heap (its own slot,
-            // if among them, with key +inf, which the first insertion evicts).  They are not
-            // logged: the collection stages the two buckets again (C3 kernel 62.2 -> 61.8 ms).
+            // if among them, with key +inf, which the first insertion evicts).
             if (nbuf == 2 && f == seed_lo + WAVE && cnt == WAVE) {
                 const int off = q - seed_lo;
                 const uint64_t self0 = off < WAVE ? 1ull << off : 0ull;
@@ -536,8 +537,6 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                         hw[(e - RT) * WAVE + lane] = kq(d) | ((e - RT) % LC);
                     }
                     filled = KH;
-                    dfill = true;
-                    dfill_rs = rs;
                     st_ins += KH;
                     SEL_HEAPIFY();
                     tau = __int_as_float(h.k0);
@@ -555,6 +554,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         }
 
         // ---- wave-coherent traversal of the binary radix tree (as k_knn)
+        logon = true;
         int sp = 1;
         stack[0] = 0;
         bool stack_overflow = false;
@@ -677,28 +677,30 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                 }
             }
         }
-        if (__any(dfill)) {
-            // the entries of the direct fill, from the two first buckets of the seed window
-            // staged again (every log entry is read by now, so they may go anywhere)
+        // the seed window's entries below T, from its buckets staged again (every log entry
+        // is read by now, so they may go anywhere): every window slot with a key below T went
+        // into the heap during the seeding, since the (k+1)-th key never fell below T
+        for (int wf = seed_lo; wf <= seed_hi; wf += 2 * WAVE) {
+            const int wn = min(2 * WAVE, seed_hi - wf + 1);
             __syncthreads();
             for (int b = 0; b < 2; ++b) {
-                const double4 c_ = spos[seed_lo + b * WAVE + lane];
-                sx[b * WAVE + lane] = (float)wrapd(c_.x - cx, Lx);
-                sy[b * WAVE + lane] = (float)wrapd(c_.y - cy, Ly);
-                sz[b * WAVE + lane] = (float)wrapd(c_.z - cz, Lz);
+                const int r = b * WAVE + lane;
+                if (r < wn) {
+                    const double4 c_ = spos[wf + r];
+                    sx[r] = (float)wrapd(c_.x - cx, Lx);
+                    sy[r] = (float)wrapd(c_.y - cy, Ly);
+                    sz[r] = (float)wrapd(c_.z - cz, Lz);
+                }
             }
             __syncthreads();
-            if (dfill) {
-                const int off = q - seed_lo;
-                for (int e = 0; e < KH; ++e) {
-                    const int r = dfill_rs + e;
-                    float kk;
-                    SEL_KEY(sx[r], sy[r], sz[r], kk);
-                    if (r != off && kk < T) {
-                        if (cnt < KCAP) lg[cnt] = make_uint2(__float_as_uint(kk), (uint32_t)(seed_lo + r));
-                        ++cnt;
-                        kmaxb = max(kmaxb, __float_as_int(kk));
-                    }
+            const int off = q - wf;
+            for (int r = 0; r < wn; ++r) {
+                float kk;
+                SEL_KEY(sx[r], sy[r], sz[r], kk);
+                if (r != off && kk < T) {
+                    if (cnt < KCAP) lg[cnt] = make_uint2(__float_as_uint(kk), (uint32_t)(wf + r));
+                    ++cnt;
+                    kmaxb = max(kmaxb, __float_as_int(kk));
                 }
             }
         }
