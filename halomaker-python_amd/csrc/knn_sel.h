@@ -5,12 +5,14 @@
 // As in k_knn, one wave serves 64 consecutive Morton-sorted queries, one per lane, walks the
 // radix tree with them together, stages leaf buckets in LDS and defers insertions to
 // lane-serial flushes.  What differs is what a lane keeps while it searches:
-//  * only the FP32 keys of its k + 1 nearest candidates so far, in a max-heap whose root and
-//    8 children live in registers and whose third level (8 children under each register
-//    child) lives in LDS at 4 B per entry.  The wave's LDS is 18 KB instead of 30 KB, so 8
-//    waves fit per CU instead of 5 (the registers allow 8 too), and a replacement moves keys
-//    only.  Each key carries its place among its siblings in its low 3 mantissa bits, so one
-//    integer max over a group finds the largest and where it is;
+//  * only the FP32 keys of its k + 1 nearest candidates so far, in a max-heap whose root is a
+//    register and whose two levels below (8 entries, then 8 under each) live in LDS at 4 B
+//    per entry, in the 16 KB that later hold the sorted neighbour slots.  The wave's LDS is
+//    18 KB instead of 30 KB, so 8 waves fit per CU instead of 5 (the registers allow 8
+//    too), and a replacement moves keys only, with one LDS store per level (the first
+//    level in registers needed 8 selects to write one entry back: C3 kernel 50.2 ->
+//    48.4 ms).  Each key carries its place among its siblings in its low 3 mantissa bits,
+//    so one integer max over a group finds the largest and where it is;
 //  * every (key, sorted slot) it puts into the heap after the seeding's direct fill is
 //    appended to a per-lane log in global memory (one region per resident wave: the grid is
 //    persistent and takes waves of queries from per-XCD counters); the direct fill's entries
@@ -58,15 +60,6 @@ constexpr int KQ_MASK = ~7;
 
 __device__ __forceinline__ int kq(float k) { return __float_as_int(k) & KQ_MASK; }
 
-// keys of the heap's top two levels: the root (quantised, untagged) and RC children (tagged
-// with their index)
-template <int RC>
-struct KeyTop {
-    typedef int v_t __attribute__((ext_vector_type(RC)));
-    int k0;
-    v_t rk;
-};
-
 // The bucket scan only picks which staged candidates a lane will look at again: a picked
 // candidate's key is recomputed as above (SEL_KEY) and compared with the lane's (k+1)-th key
 // tau before it can enter the heap.  The scan therefore tests the expanded form
@@ -97,17 +90,21 @@ __device__ __forceinline__ int max8(int a, int b, int c, int d, int e, int f, in
     return max(max(max(a, b), max(c, d)), max(max(e, f), max(g, h)));
 }
 
-// Sift quantised key dq down from register child c (1 <= c <= RC): the largest of its LC
-// children in LDS moves up if it exceeds dq, and dq takes its place (tagged with that place).
-// Register children whose group lies past the heap's allocated entries (KHCAP) are leaves.
-template <int KHCAP, int RC>
-__device__ __forceinline__ void kt_sift_l1(KeyTop<RC> &h, int *hw, int lane, int dq, int c)
+// The heap below its root (a register) lives in LDS, entry e >= 1 at slot e - 1 of the lane's
+// column: its LC first-level entries (slots 0..LC-1), then LC children under each of them
+// (first-level entry c, slot c, has its children at slots LC(c+1) ..  LC(c+1) + LC-1; those
+// past the heap's KHCAP entries are absent).  Keys are quantised and tagged with their place
+// among their siblings, so one integer max over a group gives the largest and its place.
+
+// Sift quantised key dq down from first-level entry c (its old key is being replaced): the
+// largest of its children moves up if it exceeds dq, and dq takes its place.
+template <int KHCAP>
+__device__ __forceinline__ void kt_sift_l1(int *hw, int lane, int dq, int c)
 {
-    constexpr int RT = RC + 1;
-    const int c0 = rt_child0<RC>(c);
+    const int c0 = LC * (c + 1);
     int vk = dq;
-    if (c0 < KHCAP) {
-        int *base = hw + (c0 - RT) * WAVE + lane;
+    if (c0 + 1 < KHCAP) {
+        int *base = hw + c0 * WAVE + lane;
         int kt[LC];
 #pragma unroll
         for (int t = 0; t < LC; ++t) kt[t] = base[t * WAVE];
@@ -119,40 +116,33 @@ __device__ __forceinline__ void kt_sift_l1(KeyTop<RC> &h, int *hw, int lane, int
             vk = mq;
         }
     }
-    const int nv = vk | (c - 1);   // tagged with its place
-#pragma unroll
-    for (int t = 0; t < RC; ++t) h.rk[t] = (c == t + 1) ? nv : h.rk[t];
+    hw[c * WAVE + lane] = vk | c;
 }
 
-// sift quantised key dq down from the root
-template <int KHCAP, int RC>
-__device__ __forceinline__ void kt_sift_root(KeyTop<RC> &h, int *hw, int lane, int dq)
+// sift quantised key dq down from the root k0
+template <int KHCAP>
+__device__ __forceinline__ void kt_sift_root(int &k0, int *hw, int lane, int dq)
 {
-    static_assert(RC == 8, "the register level is one tagged group of 8");
-    const int m = max8(h.rk[0], h.rk[1], h.rk[2], h.rk[3], h.rk[4], h.rk[5], h.rk[6], h.rk[7]);
+    static_assert(LC == 8, "groups of 8 siblings");
+    int kt[LC];
+#pragma unroll
+    for (int t = 0; t < LC; ++t) kt[t] = hw[t * WAVE + lane];
+    const int m = max8(kt[0], kt[1], kt[2], kt[3], kt[4], kt[5], kt[6], kt[7]);
     const int mq = m & KQ_MASK;
     if (mq > dq) {
-        h.k0 = mq;
-        kt_sift_l1<KHCAP, RC>(h, hw, lane, dq, (m & 7) + 1);
+        k0 = mq;
+        kt_sift_l1<KHCAP>(hw, lane, dq, m & 7);
     } else {
-        h.k0 = dq;
+        k0 = dq;
     }
 }
 
-// write quantised key q at heap entry e (any entry; register entries by selects), tagged with
-// its place among its siblings
-template <int RC>
-__device__ __forceinline__ void kt_set(KeyTop<RC> &h, int *hw, int lane, int e, int q)
+// write quantised key q at heap entry e (e = 0: the root), tagged with its place among its
+// siblings
+__device__ __forceinline__ void kt_set(int &k0, int *hw, int lane, int e, int q)
 {
-    constexpr int RT = RC + 1;
-    if (e >= RT) {
-        hw[(e - RT) * WAVE + lane] = q | ((e - RT) % LC);
-    } else {
-        h.k0 = e == 0 ? q : h.k0;
-        const int nv = q | ((e - 1) & 7);   // tagged with its place (for e >= 1)
-#pragma unroll
-        for (int t = 0; t < RC; ++t) h.rk[t] = (e == t + 1) ? nv : h.rk[t];
-    }
+    if (e == 0) k0 = q;
+    else hw[(e - 1) * WAVE + lane] = q | ((e - 1) & (LC - 1));
 }
 
 // Batcher's odd-even merge sorting network on N (a power of two) entries, as a list of
@@ -223,11 +213,10 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                                                 uint2 *__restrict__ klog, int *__restrict__ wctr)
 {
     constexpr int KHCAP = KCAP + 1;
-    constexpr int RC = 8;
-    constexpr int RT = RC + 1;
     static_assert(KCAP <= 64, "the selection kernel serves k <= 64");
-    static_assert(KHCAP > RT && (KHCAP - RT) % LC == 0, "every register child has 0 or LC children");
-    constexpr int NL = KHCAP - RT;
+    static_assert(KHCAP > LC + 1 && (KHCAP - 1 - LC) % LC == 0,
+                  "every first-level entry has 0 or LC children");
+    constexpr int NL = KHCAP - 1;   // the heap's LDS entries
     // the heap's LDS entries; after the search, the lane's sorted neighbour slots
     __shared__ int hw[(NL > KCAP ? NL : KCAP) * WAVE];
     __shared__ __align__(16) float sx[2 * WAVE];
@@ -305,13 +294,10 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         const bool wave_unsafe = wave_max(hql) >= 0.25 * fmin(Lx, fmin(Ly, Lz));
         const float pxf = (float)P.x, pyf = (float)P.y, pzf = (float)P.z;
 
-        KeyTop<RC> h;
-        h.k0 = kq(INFINITY);
+        int k0 = kq(INFINITY);   // the heap's root
 #pragma unroll
-        for (int t = 0; t < RC; ++t) h.rk[t] = kq(t + 1 < KH ? INFINITY : -INFINITY) | t;
-#pragma unroll
-        for (int e = RT; e < KHCAP; ++e)
-            hw[(e - RT) * WAVE + lane] = kq(e < KH ? INFINITY : -INFINITY) | ((e - RT) % LC);
+        for (int e = 1; e < KHCAP; ++e)
+            hw[(e - 1) * WAVE + lane] = kq(e < KH ? INFINITY : -INFINITY) | ((e - 1) % LC);
         float tau = (active && !wave_unsafe) ? INFINITY : -1.0f;
         // the wave's seed window [qb - Wd, qb + 63 + Wd]: every lane seeds from all of it, and
         // the traversal skips it
@@ -345,17 +331,17 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
 
 #define SEL_INSERT(d, id)                                                                      \
     do {                                                                                       \
-        kt_sift_root<KHCAP, RC>(h, hw, lane, kq(d));                                           \
+        kt_sift_root<KHCAP>(k0, hw, lane, kq(d));                                              \
         SEL_LOG((d), (id));                                                                    \
         ++st_ins;                                                                              \
-        tau = __int_as_float(h.k0);                                                            \
+        tau = __int_as_float(k0);                                                            \
     } while (0)
 
 #define SEL_HEAPIFY()                                                                          \
     do {                                                                                       \
-        _Pragma("unroll") for (int e_ = RC; e_ >= 1; --e_) if (rt_child0<RC>(e_) < KHCAP)      \
-            kt_sift_l1<KHCAP, RC>(h, hw, lane, h.rk[e_ - 1] & KQ_MASK, e_);                     \
-        kt_sift_root<KHCAP, RC>(h, hw, lane, h.k0);                                            \
+        _Pragma("unroll") for (int c_ = LC - 1; c_ >= 0; --c_) if (LC * (c_ + 1) + 1 < KHCAP)  \
+            kt_sift_l1<KHCAP>(hw, lane, hw[c_ * WAVE + lane] & KQ_MASK, c_);                   \
+        kt_sift_root<KHCAP>(k0, hw, lane, k0);                                                 \
     } while (0)
 
     // this lane's next pending candidate (first staged bucket with a set bit, lowest bit
@@ -387,12 +373,12 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                 int id_;                                                                       \
                 SEL_PICK(d_, id_);                                                             \
                 if (filled < KH) {                                                             \
-                    kt_set<RC>(h, hw, lane, filled, kq(d_));                                   \
+                    kt_set(k0, hw, lane, filled, kq(d_));                                      \
                     SEL_LOG(d_, id_);                                                          \
                     ++st_ins;                                                                  \
                     if (++filled == KH) {                                                      \
                         SEL_HEAPIFY();                                                         \
-                        tau = __int_as_float(h.k0);                                            \
+                        tau = __int_as_float(k0);                                            \
                     }                                                                          \
                 } else if (d_ < tau) {                                                         \
                     SEL_INSERT(d_, id_);                                                       \
@@ -519,27 +505,18 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
                 const uint64_t self1 = (off >= WAVE && off < 2 * WAVE) ? 1ull << (off - WAVE) : 0ull;
                 if (m0 == ~self0 && m1 == ~self1) {
                     const int rs = min(max(off - KH / 2, 0), 2 * WAVE - KH);
-#pragma unroll
-                    for (int e = 0; e < RT; ++e) {
-                        if (e < KH) {
-                            const int r = rs + e;
-                            float d;
-                            SEL_KEY(sx[r], sy[r], sz[r], d);
-                            d = r == off ? INFINITY : d;
-                            kt_set<RC>(h, hw, lane, e, kq(d));
-                        }
-                    }
-                    for (int e = RT; e < KH; ++e) {
+                    for (int e = 0; e < KH; ++e) {
                         const int r = rs + e;
                         float d;
                         SEL_KEY(sx[r], sy[r], sz[r], d);
                         d = r == off ? INFINITY : d;
-                        hw[(e - RT) * WAVE + lane] = kq(d) | ((e - RT) % LC);
+                        if (e == 0) k0 = kq(d);   /* wave-uniform */
+                        else hw[(e - 1) * WAVE + lane] = kq(d) | ((e - 1) % LC);
                     }
                     filled = KH;
                     st_ins += KH;
                     SEL_HEAPIFY();
-                    tau = __int_as_float(h.k0);
+                    tau = __int_as_float(k0);
                     m0 &= ~bits_in(rs, rs + KH);
                     m1 &= ~bits_in(rs - WAVE, rs + KH - WAVE);
                 }
@@ -550,7 +527,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
             // a clipped window (array ends): the unfilled entries stay +inf
             SEL_HEAPIFY();
             filled = KH;
-            tau = __int_as_float(h.k0);
+            tau = __int_as_float(k0);
         }
 
         // ---- wave-coherent traversal of the binary radix tree (as k_knn)
@@ -655,7 +632,7 @@ __global__ __launch_bounds__(64, 2) void k_knn_sel(KnnArgs A, const int4 *__rest
         // place to the front of the lane's log (entry e is read before any write reaches it)
         // the final root: the (k+1)-th smallest key, quantised (on the 8-ulp grid, so a key is
         // below it exactly when its quantised value is)
-        const float T = __int_as_float(h.k0);
+        const float T = __int_as_float(k0);
         const int nl = min(nlog, SEL_CAPL);
         st_logmax = max(st_logmax, (unsigned)nlog);
         st_ovf += (nlog > SEL_CAPL) ? 1u : 0u;
