@@ -173,7 +173,7 @@ def halo_props_stage(steps, cpu_sample_halos, with_cpu):
             fc = json.load(f)
         if fc.get("members") == M and fc.get("halos") == hs.nhalos:
             for kname, key, kms in (("k_direct_pe", "direct_pe", pms),
-                                    ("k_virial", "virial", out["virial_ms"])):
+                                    ("k_virial_lanes", "virial", out["virial_ms"])):
                 fl = fc["fp64_flops"][kname]
                 alg = pairs * PE_FLOPS_PER_PAIR if kname == "k_direct_pe" else fl
                 ach = alg / (kms * 1e-3) / 1e12

@@ -1102,77 +1102,349 @@ __device__ double carlson_rf(double x, double y, double z)
     return (1.0 - e2 / 10.0 + e3 / 14.0 + e2 * e2 / 24.0 - 3.0 * e2 * e3 / 44.0) / sqrt(a);
 }
 
-// ---- shell tables, potential profile and the virial scan, one workgroup per halo
-template <int BS>
-__global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first,
-                                               const int64_t *off, const double *hist, int nr,
-                                               const HaloState *st, const double *ep_direct,
-                                               HaloParamsDev P, double *out)
+// ---- the virial radius and mass from the scans over the shell tables (det_vir_props_1b4
+// :1127-1229) and the output fields.  The scans pick the outermost shell meeting a condition:
+//   i1r -- outermost shell holding mass (tm[i-1] < tm[i]), 0 if none;
+//   brk -- outermost bound shell with ratio <= 0.2 (-1 if none): the virial scan stops there;
+//          that shell is always a new best, so it sets mvir (tm_brk) and (the axes being a
+//          running minimum of increasing functions of the index) the axes; without such a
+//          shell the last ratio seen exceeds 0.2 and the overdensity branch decides;
+//   fb  -- outermost shell with tm >= vir_overdens rho_mean volmin ii1^3 whose inner neighbour
+//          holds less than the total, 0 if none (tm_ii1 = tm[max(fb, 1)]).
+// A non-finite ratio (from non-finite input) takes the reference's serial scan over the
+// tables tm (cumulative mass) and cn (ratios) instead (serial = true).
+__device__ void virial_finish(double *o, const HaloState &S, const HaloParamsDev &P, int ttab,
+                              double ep, double mtot, int i1r, int brk, double tm_brk, int fb,
+                              double tm_ii1, bool serial, const double *tm, const double *cn)
 {
-    extern __shared__ double work[];          // 4*nr doubles
-    const int h = blockIdx.x;
-    if (h >= nh) return;
-    double *tm = work;                         // shell mass -> cumulative mass
-    double *tk = tm + nr;                      // shell kinetic energy -> cumulative
-    double *cn = tk + nr;                      // shell counts -> virialisation ratio per shell
-    double *tp = cn + nr;                      // potential profile
-    const HaloState &S = st[h];
-    const int t0 = tile_first[h], t1 = tile_first[h + 1];
-    for (int b = threadIdx.x; b < nr; b += BS) {
-        double m = 0, k = 0, c = 0;
-        for (int t = t0; t < t1; ++t) {
-            const double *hh = hist + (int64_t)t * 3 * nr;
-            m += hh[b];
-            k += hh[nr + b];
-            c += hh[2 * nr + b];
+    const double amax = S.amax, bmax = S.bmax, cmax = S.cmax;
+    double ek = o[HP_EK];
+    double et = ek + ep;
+    o[HP_EP] = ep;
+    o[HP_ET] = et;
+    double mvir = mtot, avir, bvir, cvir, rvir;
+    int i1 = max(i1r, 1);
+    avir = (double)i1 / (ttab - 1) * amax;
+    bvir = (double)i1 / (ttab - 1) * bmax;
+    cvir = (double)i1 / (ttab - 1) * cmax;
+    rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
+    bool overdensity;
+    if (!serial) {
+        overdensity = true;
+        if (brk >= 0) {
+            mvir = tm_brk;
+            avir = fmin(avir, (double)brk / (ttab - 1) * amax);
+            bvir = fmin(bvir, (double)brk / (ttab - 1) * bmax);
+            cvir = fmin(cvir, (double)brk / (ttab - 1) * cmax);
+            rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
+            overdensity = mvir < P.vir_overdens * P.rho_mean * 4. / 3. * PI * avir * bvir * cvir;
         }
-        tm[b] = m;
-        tk[b] = k;
-        cn[b] = c;
+    } else {
+        double virth_old = 1.0, virth = 100.0;
+        for (int i0 = ttab - 1; i0 >= 0; --i0) {
+            if (cn[i0] == -1.0) continue;
+            virth = cn[i0];
+            if (virth < virth_old) {
+                mvir = tm[i0];
+                avir = fmin(avir, (double)i0 / (ttab - 1) * amax);
+                bvir = fmin(bvir, (double)i0 / (ttab - 1) * bmax);
+                cvir = fmin(cvir, (double)i0 / (ttab - 1) * cmax);
+                rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
+                virth_old = virth;
+                if (virth <= 0.20) break;
+            }
+        }
+        overdensity = (virth > 0.20) ||
+                      (mvir < P.vir_overdens * P.rho_mean * 4. / 3. * PI * avir * bvir * cvir);
     }
+    if (overdensity) {
+        const int ii1 = max(fb, 1);
+        mvir = tm_ii1;
+        avir = (double)ii1 / (ttab - 1) * amax;
+        bvir = (double)ii1 / (ttab - 1) * bmax;
+        cvir = (double)ii1 / (ttab - 1) * cmax;
+        rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
+    }
+    if (mvir > 0.0 && rvir > 0.0) {
+        double rv = fmin(rvir, o[HP_R]);
+        o[HP_RVIR] = rv;
+        o[HP_MVIR] = mvir;
+        o[HP_CVEL] = sqrt(P.gravconst * mvir / rv);
+        o[HP_TVIR] = 35.9 * P.gravconst * mvir / rv;
+        // the truncated isothermal sphere profile of compute_halo_profile_1b41 (:1232-1238),
+        // defined at rvir
+        o[HP_RHO0] = mvir / (4.0 * PI * pow(rv, 3.0));
+        o[HP_RC] = rv;
+    } else {
+        o[HP_RVIR] = o[HP_MVIR] = o[HP_CVEL] = o[HP_TVIR] = o[HP_RHO0] = o[HP_RC] = NAN;
+    }
+    double hl = sqrt(o[HP_LX] * o[HP_LX] + o[HP_LY] * o[HP_LY] + o[HP_LZ] * o[HP_LZ]);
+    o[HP_SPIN] = hl * sqrt(fabs(et)) / pow(S.m, 2.5) / P.gravconst;
+}
+
+// The shell tables' running sums are sequential (np.cumsum's order, :1024-1025, and the
+// potential profile's cumsum, :1090-1103): a chain of nr dependent FP64 additions per table.
+// One halo per lane runs its chains side by side with 63 other halos' (a workgroup per halo,
+// where one thread added while the others waited, took 2.3 ms of chains at the 50 000-halo
+// workload).  The per-tile histograms of a halo of several tiles are first folded, in tile
+// order, into the row of its first tile (k_hist_fold), so that a lane reads one row.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_hist_fold(const int32_t *halos, const int32_t *tile_first,
+                                                  int nr, double *hist)
+{
+    // grid: (halo, chunk of BS entries of its 3 nr); the tiles' values loaded 8 at a time,
+    // added in tile order
+    const int h = halos[blockIdx.x];
+    const int b = blockIdx.y * BS + threadIdx.x;
+    if (b >= 3 * nr) return;
+    const int t0 = tile_first[h], t1 = tile_first[h + 1];
+    const int64_t ld = 3 * (int64_t)nr;
+    double *row = hist + t0 * ld;
+    double v = row[b];
+    int t = t0 + 1;
+    for (; t + 8 <= t1; t += 8) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] = hist[(t + u) * ld + b];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v += x[u];
+    }
+    for (; t < t1; ++t) v += hist[t * ld + b];
+    row[b] = v;
+}
+
+// Potential-profile term of shell b (potential_profile_1b40 :1090-1103): dm from the cumulative
+// table (np.diff with prepend 0), mass before the shell = cumulative - dm.
+__device__ __forceinline__ double shell_scale(int b, int nr)
+{
+    return (b == 0) ? 0.5 / (double)(nr - 1) : (double)b / (double)(nr - 1);
+}
+
+// self-interaction fraction of a shell of cnt members ('count'; 'half_shell' takes 0.5)
+__device__ __forceinline__ double shell_self(const HaloParamsDev &P, double cnt)
+{
+    return P.shell_half ? 0.5 : (cnt > 0 ? 0.5 * (cnt - 1.0) / cnt : 0.0);
+}
+
+__device__ __forceinline__ double shell_pe_term(double cum, double prev, double sf, double scale,
+                                                double reff0)
+{
+    const double dm = cum - prev;
+    const double before = cum - dm;
+    return (before + sf * dm) * dm / (scale * reff0);
+}
+
+// ---- shell tables, potential profile and virial scan, one halo per lane.  Pass 1 runs the
+// running sums to their ends (the total mass and the profile's outer value, which scale the
+// profile and bound the scans); pass 2 runs the same sums again, shell by shell, for each
+// shell's virialisation ratio |2K+W|/|K+W| (-1 for an unbound shell, K+W >= 0) and the scans.
+// A halo whose ratios hold a NaN is flagged for k_virial_serial.
+constexpr int VIR_CH = 8;      // shells loaded at a time per table
+constexpr int VIR_HPW = 64;    // halos per wave (one per lane)
+constexpr int VIR_SFT = 1024;  // shells of fewer members take their self fraction from a table
+template <int BS>
+__global__ __launch_bounds__(BS) void k_virial_lanes(int nh, const int32_t *tile_first,
+                                                     const int64_t *off, const double *hist,
+                                                     int nr, const HaloState *st,
+                                                     const double *ep_direct, HaloParamsDev P,
+                                                     double *out, int32_t *serial)
+{
+    // the shells' scales r_b / r_max, the same for every halo, and the self-interaction
+    // fractions of shells of fewer than VIR_SFT members (each value the same expression)
+    extern __shared__ double sc_[];   // nr + VIR_SFT doubles
+    double *const sft_ = sc_ + nr;
+    for (int b = threadIdx.x; b < nr; b += BS) sc_[b] = shell_scale(b, nr);
+    for (int c = threadIdx.x; c < VIR_SFT; c += BS) sft_[c] = shell_self(P, (double)c);
     __syncthreads();
-    // cumulative tables: sequential left-to-right sums, as np.cumsum adds (:1024-1025)
-    if (threadIdx.x == 0) seq_cumsum(tm, nr);
-    else if (threadIdx.x == WAVE_HP) seq_cumsum(tk, nr);
-    __syncthreads();
-    // shell potential terms in parallel (potential_profile_1b40 :1090-1103): dm from the
-    // cumulative table (np.diff with prepend 0), mass before the shell = cumulative - dm
-    const double reff0 = pow(S.amax * S.bmax * S.cmax, 1.0 / 3.0);
+    const int h = blockIdx.x * VIR_HPW + threadIdx.x;
+    if (threadIdx.x >= VIR_HPW || h >= nh) return;
+    const double *hm = hist + (int64_t)tile_first[h] * 3 * nr;
+    const double *hk = hm + nr;
+    const double *hc = hk + nr;
+    const HaloState &S = st[h];
     double *o = out + (int64_t)h * HP_NFIELDS;
+    const double reff0 = pow(S.amax * S.bmax * S.cmax, 1.0 / 3.0);
     // RF of the axes (det_halo_energies_1b3 :1409-1412 and the 'baseline' profile :1085-1087)
     const double rf = (P.pe_baseline || P.pe_norm_legacy)
                           ? carlson_rf(o[HP_SHA] * o[HP_SHA], o[HP_SHB] * o[HP_SHB],
                                        o[HP_SHC] * o[HP_SHC])
                           : 0.0;
-    if (P.pe_baseline) {
-        // -3/5 G M(<r)^2 RF, the profile itself (no cumulative sum)
-        for (int b = threadIdx.x; b < nr; b += BS) tp[b] = -3.0 / 5.0 * P.gravconst * (tm[b] * tm[b]) * rf;
-    } else {
-        for (int b = threadIdx.x; b < nr; b += BS) {
-            const double cum = tm[b];
-            const double dm = cum - (b > 0 ? tm[b - 1] : 0.0);
-            const double before = cum - dm;
-            const double sf = P.shell_half ? 0.5 : (cn[b] > 0 ? 0.5 * (cn[b] - 1.0) / cn[b] : 0.0);
-            const double scale = (b == 0) ? 0.5 / (double)(nr - 1) : (double)b / (double)(nr - 1);
-            tp[b] = (before + sf * dm) * dm / (scale * reff0);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) seq_cumsum(tp, nr);
+    // one step of the three running sums: cumulative mass cm, kinetic energy ck, and the
+    // profile: -3/5 G M(<r)^2 RF itself for 'baseline' (no sum), else the cumulative sum cp of
+    // the shell terms
+#define VIR_STEP(u, b)                                                                         \
+    do {                                                                                       \
+        prev = cm;                                                                             \
+        cm += m_[u];                                                                           \
+        ck += k_[u];                                                                           \
+        if (P.pe_baseline) {                                                                   \
+            cp = -3.0 / 5.0 * P.gravconst * (cm * cm) * rf;                                    \
+        } else {                                                                               \
+            const double cnt_ = c_[u];                                                         \
+            const double sf_ = cnt_ < (double)VIR_SFT ? sft_[(int)cnt_] : shell_self(P, cnt_);  \
+            cp += shell_pe_term(cm, prev, sf_, sc_[b], reff0);                                 \
+        }                                                                                      \
+    } while (0)
+    // a chunk of VIR_CH shells per table (16-byte loads: rows and chunks are 16-byte aligned);
+    // the last, partial chunk element by element
+#define VIR_LOAD(b0)                                                                           \
+    do {                                                                                       \
+        if ((b0) + VIR_CH <= nr) {                                                             \
+            _Pragma("unroll") for (int u = 0; u < VIR_CH; u += 2) {                            \
+                const double2 mm_ = *reinterpret_cast<const double2 *>(hm + (b0) + u);         \
+                const double2 kk_ = *reinterpret_cast<const double2 *>(hk + (b0) + u);         \
+                const double2 cc_ = *reinterpret_cast<const double2 *>(hc + (b0) + u);         \
+                m_[u] = mm_.x;                                                                 \
+                m_[u + 1] = mm_.y;                                                             \
+                k_[u] = kk_.x;                                                                 \
+                k_[u + 1] = kk_.y;                                                             \
+                c_[u] = cc_.x;                                                                 \
+                c_[u + 1] = cc_.y;                                                             \
+            }                                                                                  \
+        } else {                                                                               \
+            _Pragma("unroll") for (int u = 0; u < VIR_CH; ++u) {                               \
+                const int bb_ = min((b0) + u, nr - 1);                                         \
+                m_[u] = hm[bb_];                                                               \
+                k_[u] = hk[bb_];                                                               \
+                c_[u] = hc[bb_];                                                               \
+            }                                                                                  \
+        }                                                                                      \
+    } while (0)
+    double m_[VIR_CH], k_[VIR_CH], c_[VIR_CH];
+    double cm = 0.0, ck = 0.0, cp = 0.0, prev = 0.0;
+    // full chunks without per-shell tests, then the rest
+    const int nfull = nr - nr % VIR_CH;
+    int b0 = 0;
+    for (; b0 < nfull; b0 += VIR_CH) {
+        VIR_LOAD(b0);
+#pragma unroll
+        for (int u = 0; u < VIR_CH; ++u) VIR_STEP(u, b0 + u);
     }
-    __syncthreads();
+    if (b0 < nr) {
+        VIR_LOAD(b0);
+#pragma unroll
+        for (int u = 0; u < VIR_CH; ++u)
+            if (b0 + u < nr) VIR_STEP(u, b0 + u);
+    }
+    const double mtot = cm;
     const int64_t count = off[h + 1] - off[h];
     // the profile's outer value: -G times the cumulative sum, or the baseline profile's last
     const double gp = P.pe_baseline ? 1.0 : -P.gravconst;
-    const double outer = gp * tp[nr - 1];
+    const double outer = gp * cp;
     const bool small = count < P.direct_pe_nmax;
     // the halo's potential energy: the direct pair sum below the pair-sum limit; above it the
     // profile's outer value ('profile_outer_for_large') or the RF estimate ('legacy')
     const double ep = small ? ep_direct[h]
                             : (P.pe_norm_legacy ? -3.0 / 5.0 * P.gravconst * (S.m * S.m) * rf : outer);
     const bool renorm = (small || P.pe_norm_legacy) && ep != outer && outer != 0.0;
+    const int ttab = nr;
+    const double volmin = 4. / 3. * PI * (S.amax / (ttab - 1)) * (S.bmax / (ttab - 1)) * (S.cmax / (ttab - 1));
+    // vir_overdens * rho_mean * volmin * ii1**3, the cube an exact integer
+    const double mv0 = P.vir_overdens * P.rho_mean * volmin;
+    int i1r = 0, brk = -1, fb = 0;
+    bool nan = false;
+    double tm_brk = 0.0, tm_fb = 0.0, tm1 = 0.0;
+    cm = ck = cp = prev = 0.0;
+    // one shell of pass 2: its ratio and the scans
+#define VIR_SCAN(u, b)                                                                         \
+    do {                                                                                       \
+        VIR_STEP(u, b);                                                                        \
+        /* profile (rescaled to the halo's potential energy where the policy says so) */       \
+        double pr = gp * cp;                                                                   \
+        if (renorm) pr = pr / outer * ep;                                                      \
+        const double kw = ck + pr;                                                             \
+        const double v = (kw >= 0.0) ? -1.0 : fabs((2.0 * ck + pr) / kw);                     \
+        if (v != v) nan = true;                                                                \
+        else if (v != -1.0 && v <= 0.20) {                                                     \
+            brk = (b);                                                                         \
+            tm_brk = cm;                                                                       \
+        }                                                                                      \
+        if ((b) >= 1) {                                                                        \
+            if (prev < cm) i1r = (b);                                                          \
+            if (cm >= mv0 * (double)((int64_t)(b) * (b) * (b)) && prev < mtot) {               \
+                fb = (b);                                                                      \
+                tm_fb = cm;                                                                    \
+            }                                                                                  \
+        }                                                                                      \
+        if ((b) == 1) tm1 = cm;                                                                \
+    } while (0)
+    b0 = 0;
+    for (; b0 < nfull; b0 += VIR_CH) {
+        VIR_LOAD(b0);
+#pragma unroll
+        for (int u = 0; u < VIR_CH; ++u) VIR_SCAN(u, b0 + u);
+    }
+    if (b0 < nr) {
+        VIR_LOAD(b0);
+#pragma unroll
+        for (int u = 0; u < VIR_CH; ++u)
+            if (b0 + u < nr) VIR_SCAN(u, b0 + u);
+    }
+#undef VIR_SCAN
+#undef VIR_LOAD
+#undef VIR_STEP
+    if (nan) {
+        serial[h] = 1;
+        return;
+    }
+    virial_finish(o, S, P, ttab, ep, mtot, i1r, brk, tm_brk, fb, fb >= 1 ? tm_fb : tm1, false,
+                  nullptr, nullptr);
+}
+
+// ---- the same, one workgroup per halo, for the halos k_virial_lanes flagged (a NaN ratio
+// takes the reference's serial scan over the tables): the tables in LDS, the running sums by
+// one thread each, the scans as block-wide maximum-index searches.  The other workgroups
+// return at once.
+template <int BS>
+__global__ __launch_bounds__(BS) void k_virial_serial(int nh, const int32_t *tile_first,
+                                                      const int64_t *off, const double *hist,
+                                                      int nr, const HaloState *st,
+                                                      const double *ep_direct, HaloParamsDev P,
+                                                      double *out, const int32_t *serial)
+{
+    extern __shared__ double work[];          // 4*nr doubles
+    const int h = blockIdx.x;
+    if (h >= nh || !serial[h]) return;
+    double *tm = work;                         // shell mass -> cumulative mass
+    double *tk = tm + nr;                      // shell kinetic energy -> cumulative
+    double *cn = tk + nr;                      // shell counts -> virialisation ratio per shell
+    double *tp = cn + nr;                      // potential profile
+    const HaloState &S = st[h];
+    // the halo's histograms, folded into its first tile's row
+    const double *row = hist + (int64_t)tile_first[h] * 3 * nr;
+    for (int b = threadIdx.x; b < nr; b += BS) {
+        tm[b] = row[b];
+        tk[b] = row[nr + b];
+        cn[b] = row[2 * nr + b];
+    }
     __syncthreads();
-    // profile (rescaled to the halo's potential energy where the policy says so) and the
-    // virialisation ratio |2K+W|/|K+W| of each shell; -1 marks an unbound shell (K+W >= 0)
+    if (threadIdx.x == 0) seq_cumsum(tm, nr);
+    else if (threadIdx.x == WAVE_HP) seq_cumsum(tk, nr);
+    __syncthreads();
+    const double reff0 = pow(S.amax * S.bmax * S.cmax, 1.0 / 3.0);
+    double *o = out + (int64_t)h * HP_NFIELDS;
+    const double rf = (P.pe_baseline || P.pe_norm_legacy)
+                          ? carlson_rf(o[HP_SHA] * o[HP_SHA], o[HP_SHB] * o[HP_SHB],
+                                       o[HP_SHC] * o[HP_SHC])
+                          : 0.0;
+    if (P.pe_baseline) {
+        for (int b = threadIdx.x; b < nr; b += BS) tp[b] = -3.0 / 5.0 * P.gravconst * (tm[b] * tm[b]) * rf;
+    } else {
+        for (int b = threadIdx.x; b < nr; b += BS)
+            tp[b] = shell_pe_term(tm[b], b > 0 ? tm[b - 1] : 0.0, shell_self(P, cn[b]),
+                                  shell_scale(b, nr), reff0);
+        __syncthreads();
+        if (threadIdx.x == 0) seq_cumsum(tp, nr);
+    }
+    __syncthreads();
+    const int64_t count = off[h + 1] - off[h];
+    const double gp = P.pe_baseline ? 1.0 : -P.gravconst;
+    const double outer = gp * tp[nr - 1];
+    const bool small = count < P.direct_pe_nmax;
+    const double ep = small ? ep_direct[h]
+                            : (P.pe_norm_legacy ? -3.0 / 5.0 * P.gravconst * (S.m * S.m) * rf : outer);
+    const bool renorm = (small || P.pe_norm_legacy) && ep != outer && outer != 0.0;
+    __syncthreads();
     for (int b = threadIdx.x; b < nr; b += BS) {
         double p = gp * tp[b];
         if (renorm) p = p / outer * ep;
@@ -1181,20 +1453,8 @@ __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first
         cn[b] = (kw >= 0.0) ? -1.0 : fabs((2.0 * tk[b] + p) / kw);
     }
     __syncthreads();
-    // det_vir_props_1b4 :1127-1229.  Its scans pick the outermost shell meeting a condition;
-    // on finite tables they are block-wide maximum-index searches:
-    //   i1  -- outermost shell holding mass (tm[i-1] < tm[i]), 1 if none;
-    //   brk -- the virial scan stops at the outermost bound shell with ratio <= 0.2; that
-    //          shell is always a new best, so it sets mvir and (the axes being a running
-    //          minimum of increasing functions of the index) the axes; without such a shell
-    //          the last ratio seen exceeds 0.2 and the overdensity branch decides;
-    //   ii1 -- outermost shell with tm >= vir_overdens rho_mean volmin ii1^3 whose inner
-    //          neighbour holds less than the total, 1 if none.
-    // A non-finite ratio (from non-finite input) takes the serial scans instead.
     const int ttab = nr;
-    const double amax = S.amax, bmax = S.bmax, cmax = S.cmax;
-    const double volmin = 4. / 3. * PI * (amax / (ttab - 1)) * (bmax / (ttab - 1)) * (cmax / (ttab - 1));
-    // vir_overdens * rho_mean * volmin * ii1**3, the cube an exact integer
+    const double volmin = 4. / 3. * PI * (S.amax / (ttab - 1)) * (S.bmax / (ttab - 1)) * (S.cmax / (ttab - 1));
     const double mv0 = P.vir_overdens * P.rho_mean * volmin;
     const double mtot = tm[ttab - 1];
     __shared__ int red[4][BS];
@@ -1222,69 +1482,9 @@ __global__ __launch_bounds__(BS) void k_virial(int nh, const int32_t *tile_first
         }
     }
     if (threadIdx.x != 0) return;
-    double ek = o[HP_EK];
-    double et = ek + ep;
-    o[HP_EP] = ep;
-    o[HP_ET] = et;
-    double mvir = mtot, avir, bvir, cvir, rvir;
-    int i1 = max(red[0][0], 1);
-    avir = (double)i1 / (ttab - 1) * amax;
-    bvir = (double)i1 / (ttab - 1) * bmax;
-    cvir = (double)i1 / (ttab - 1) * cmax;
-    rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
-    bool overdensity;
-    if (!red[3][0]) {
-        const int brk = red[1][0];
-        overdensity = true;
-        if (brk >= 0) {
-            mvir = tm[brk];
-            avir = fmin(avir, (double)brk / (ttab - 1) * amax);
-            bvir = fmin(bvir, (double)brk / (ttab - 1) * bmax);
-            cvir = fmin(cvir, (double)brk / (ttab - 1) * cmax);
-            rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
-            overdensity = mvir < P.vir_overdens * P.rho_mean * 4. / 3. * PI * avir * bvir * cvir;
-        }
-    } else {
-        double virth_old = 1.0, virth = 100.0;
-        for (int i0 = ttab - 1; i0 >= 0; --i0) {
-            if (cn[i0] == -1.0) continue;
-            virth = cn[i0];
-            if (virth < virth_old) {
-                mvir = tm[i0];
-                avir = fmin(avir, (double)i0 / (ttab - 1) * amax);
-                bvir = fmin(bvir, (double)i0 / (ttab - 1) * bmax);
-                cvir = fmin(cvir, (double)i0 / (ttab - 1) * cmax);
-                rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
-                virth_old = virth;
-                if (virth <= 0.20) break;
-            }
-        }
-        overdensity = (virth > 0.20) ||
-                      (mvir < P.vir_overdens * P.rho_mean * 4. / 3. * PI * avir * bvir * cvir);
-    }
-    if (overdensity) {
-        const int ii1 = max(red[2][0], 1);
-        mvir = tm[ii1];
-        avir = (double)ii1 / (ttab - 1) * amax;
-        bvir = (double)ii1 / (ttab - 1) * bmax;
-        cvir = (double)ii1 / (ttab - 1) * cmax;
-        rvir = pow(avir * bvir * cvir, 1.0 / 3.0);
-    }
-    if (mvir > 0.0 && rvir > 0.0) {
-        double rv = fmin(rvir, o[HP_R]);
-        o[HP_RVIR] = rv;
-        o[HP_MVIR] = mvir;
-        o[HP_CVEL] = sqrt(P.gravconst * mvir / rv);
-        o[HP_TVIR] = 35.9 * P.gravconst * mvir / rv;
-        // the truncated isothermal sphere profile of compute_halo_profile_1b41 (:1232-1238),
-        // defined at rvir
-        o[HP_RHO0] = mvir / (4.0 * PI * pow(rv, 3.0));
-        o[HP_RC] = rv;
-    } else {
-        o[HP_RVIR] = o[HP_MVIR] = o[HP_CVEL] = o[HP_TVIR] = o[HP_RHO0] = o[HP_RC] = NAN;
-    }
-    double hl = sqrt(o[HP_LX] * o[HP_LX] + o[HP_LY] * o[HP_LY] + o[HP_LZ] * o[HP_LZ]);
-    o[HP_SPIN] = hl * sqrt(fabs(et)) / pow(S.m, 2.5) / P.gravconst;
+    const int brk = red[1][0], fb = red[2][0];
+    virial_finish(o, S, P, ttab, ep, mtot, red[0][0], brk, brk >= 0 ? tm[brk] : 0.0, fb,
+                  tm[max(fb, 1)], red[3][0] != 0, tm, cn);
 }
 
 template <typename T>
@@ -1383,6 +1583,7 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
     Tiles tl = make_tiles(hoff, T);
     const int nt = (int)tl.halo.size();
     std::vector<int32_t> small;   // halos of the pair-sum branch (host copy kept until the sync)
+    std::vector<int32_t> multi;   // halos of several tiles (likewise)
     std::vector<int32_t> cls[4];  // recentring size classes (likewise)
     std::vector<int32_t> sc_tiles;
     std::vector<void *> keep;
@@ -1479,8 +1680,24 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
                                    sizeof(double4) * nsmall_max, s, D, d_small, ep);
             AHOP_HIP(hipEventRecord(ep1, s));
             timed_pe = true;
-            hipLaunchKernelGGL(k_virial<256>, dim3(nh), dim3(256), sizeof(double) * 4 * nr, s, nh,
-                               d_tf, d_off, hist, nr, st, ep, P, d_out);
+            for (int h = 0; h < nh; ++h)
+                if (tl.first[h + 1] - tl.first[h] > 1) multi.push_back(h);
+            int32_t *d_multi = nullptr;
+            if (!multi.empty()) {
+                d_multi = dalloc<int32_t>(keep, multi.size());
+                AHOP_HIP(hipMemcpyAsync(d_multi, multi.data(), sizeof(int32_t) * multi.size(),
+                                        hipMemcpyHostToDevice, s));
+                hipLaunchKernelGGL(k_hist_fold<256>,
+                                   dim3((unsigned)multi.size(), (unsigned)((3 * nr + 255) / 256)),
+                                   dim3(256), 0, s, d_multi, d_tf, nr, hist);
+            }
+            int32_t *d_serial = dalloc<int32_t>(keep, nh);
+            AHOP_HIP(hipMemsetAsync(d_serial, 0, sizeof(int32_t) * nh, s));
+            hipLaunchKernelGGL(k_virial_lanes<64>, dim3((nh + VIR_HPW - 1) / VIR_HPW), dim3(64),
+                               sizeof(double) * (nr + VIR_SFT), s, nh, d_tf, d_off, hist, nr, st,
+                               ep, P, d_out, d_serial);
+            hipLaunchKernelGGL(k_virial_serial<256>, dim3(nh), dim3(256), sizeof(double) * 4 * nr, s,
+                               nh, d_tf, d_off, hist, nr, st, ep, P, d_out, d_serial);
             if (P.angmom_nshells > 0) {
                 const int nsh = P.angmom_nshells;
                 double *ahist = dalloc<double>(keep, (size_t)nt * 4 * nsh);

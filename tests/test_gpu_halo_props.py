@@ -93,6 +93,29 @@ def test_tile_schedule_does_not_change_results(gpu, halos, halos_ref, tile, wg):
     compare(out, halos_ref)
 
 
+def test_non_finite_velocity_takes_the_serial_virial_scan(gpu):
+    """A NaN member velocity makes its halo's kinetic shell energies, and from its shell on
+    the virialisation ratios, NaN: the virial scan is then the reference's serial scan over
+    the ratios (det_vir_props_1b4 :1161-1180), which the device runs for exactly those halos
+    (one workgroup each; every other halo one lane).  Small tiles put the 3000-member halo on
+    several tiles, so its histograms are folded first."""
+    from oracle.halo_props import F, all_halos
+    import warnings
+    hs = G.nfw_halos(12, 21, alpha=1.0, nmin=40, nmax=3000)
+    vel = hs.vel.copy()
+    for h in (2, 3, 8):      # 3000, 47 and 1269 members
+        s0, s1 = hs.offsets[h], hs.offsets[h + 1]
+        vel[s0 + (s1 - s0) // 2, 1] = np.nan
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ref = all_halos(hs.offsets, hs.pos, vel, hs.mass, hub_pt=70.3)
+    for h in (2, 3, 8):
+        assert np.isnan(ref[h, F["ek"]]) and np.isfinite(ref[h, F["rvir"]]), h
+    hs_nan = G.HaloSet(hs.offsets, hs.pos, vel, hs.mass, hs.name)
+    out, _ = run(hs_nan, gpu, hub_pt=70.3, tile=1024, workgroup=256)
+    compare(out, ref)
+
+
 def test_densest_member_centre(gpu):
     from oracle.halo_props import all_halos
     hs = G.nfw_halos(60, 8, alpha=1.0, nmin=30, nmax=5000)
