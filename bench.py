@@ -26,9 +26,9 @@ METRIC = "Mparticles/s kNN+density+hop at 16.8M; mixed vs FP64 GPU at ≤1e-6 re
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_PEAK_TFLOPS = 78.6
 # FP64 operations (FMA as 2) of one pair of the direct potential: 3 differences, the squared
-# distance (a multiply and 2 FMAs), rsqrt (the transcendental and a Newton refinement of a
-# multiply and 4 FMA-or-multiply steps: 8), the mass-weighted accumulation (an FMA)
-PE_FLOPS_PER_PAIR = 19
+# distance (a multiply and 2 FMAs), 1/d (v_rsq_f64 and one Newton step of 2 multiplies and
+# 2 FMAs: 7), the mass-weighted accumulation (an FMA)
+PE_FLOPS_PER_PAIR = 17
 FP32_VALU_PEAK_TFLOPS = 157.3
 # algorithmic HBM bytes per particle of the kNN+density kernel: read position 24 B and
 # mass 8 B, write density 8 B (the whole step adds the hop stage's density re-read 8 B and

@@ -941,12 +941,24 @@ __global__ void k_angmom_combine(int nh, const int32_t *tile_first, const double
     }
 }
 
+// 1/sqrt(d) from v_rsq_f64 (relative error below 2^-24) and one Newton step: below 5e-15
+// relative over 2^-40..2^5 (measured on gfx950 against 1/sqrt in long double), far inside the
+// pair sum's 1e-10.  d = 0 (members at one position) gives NaN, not +inf: such a halo's sum is
+// redone with the library's rsqrt.
+__device__ __forceinline__ double rsqrt_nr1(double d)
+{
+    const double y0 = __builtin_amdgcn_rsq(d);
+    const double e = fma(-(d * y0), y0, 1.0);
+    return fma(0.5 * y0, e, y0);
+}
+
 // sum over members j0 <= j < je of m_j / d_ij for member a = mem[i] (for a tile on the
 // diagonal only the j > i): j is the same on every lane of the wave (the member is read
 // once for all 64 lanes), four partial sums carry independent rsqrt chains.  FOLD takes the
-// minimum image min(|t|, L - |t|) per axis.  The distance is formed with fused multiply-adds:
-// the sum is held to 1e-10 of the energies, not bit for bit.
-template <bool FOLD, bool DIAG>
+// minimum image min(|t|, L - |t|) per axis.  The distance is formed with fused multiply-adds
+// and 1/d with rsqrt_nr1 (EXACT: the library's rsqrt): the sum is held to 1e-10 of the
+// energies, not bit for bit.
+template <bool FOLD, bool DIAG, bool EXACT>
 __device__ __forceinline__ double pe_tile(const double4 *mem, const double4 a, int i, int j0,
                                           int je, double L)
 {
@@ -969,7 +981,7 @@ __device__ __forceinline__ double pe_tile(const double4 *mem, const double4 a, i
             d2 = on ? d2 : 1.0;
             w = on ? w : 0.0;
         }
-        racc = fma(w, rsqrt(d2), racc);
+        racc = fma(w, EXACT ? rsqrt(d2) : rsqrt_nr1(d2), racc);
     };
     int j = j0;
     for (; j + 4 <= je; j += 4) {
@@ -991,7 +1003,7 @@ __device__ __forceinline__ double pe_tile(const double4 *mem, const double4 a, i
 // other lanes (on the diagonal A = B only the pairs j > i count).
 constexpr int PE_NMAX = 2048;   // 64 KiB of LDS
 constexpr int PE_TILE = 64;     // members per tile: one per lane of a wave
-template <int BS>
+template <int BS, bool EXACT>
 __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *small_halos,
                                                   double *ep_out)
 {
@@ -999,6 +1011,8 @@ __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *smal
     extern __shared__ double4 mem[];
     __shared__ double sh[BS];
     const int h = small_halos[blockIdx.x];
+    // the exact pass redoes only the halos whose fast sum is NaN
+    if (EXACT && !isnan(ep_out[h])) return;
     const int64_t f = D.off[h];
     const int n = (int)(D.off[h + 1] - f);
     for (int i = threadIdx.x; i < n; i += BS)
@@ -1035,11 +1049,11 @@ __global__ __launch_bounds__(BS) void k_direct_pe(HaloDev D, const int32_t *smal
             const double4 a = mem[i];
             double row;
             if (fold)
-                row = A == B ? pe_tile<true, true>(mem, a, i, j0, je, L)
-                             : pe_tile<true, false>(mem, a, i, j0, je, L);
+                row = A == B ? pe_tile<true, true, EXACT>(mem, a, i, j0, je, L)
+                             : pe_tile<true, false, EXACT>(mem, a, i, j0, je, L);
             else
-                row = A == B ? pe_tile<false, true>(mem, a, i, j0, je, L)
-                             : pe_tile<false, false>(mem, a, i, j0, je, L);
+                row = A == B ? pe_tile<false, true, EXACT>(mem, a, i, j0, je, L)
+                             : pe_tile<false, false, EXACT>(mem, a, i, j0, je, L);
             acc = fma(a.w, row, acc);
         }
     }
@@ -1675,9 +1689,12 @@ static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, 
                                         hipMemcpyHostToDevice, s));
             }
             AHOP_HIP(hipEventRecord(ep0, s));
-            if (!small.empty())
-                hipLaunchKernelGGL(k_direct_pe<256>, dim3((unsigned)small.size()), dim3(256),
-                                   sizeof(double4) * nsmall_max, s, D, d_small, ep);
+            if (!small.empty()) {
+                hipLaunchKernelGGL((k_direct_pe<256, false>), dim3((unsigned)small.size()),
+                                   dim3(256), sizeof(double4) * nsmall_max, s, D, d_small, ep);
+                hipLaunchKernelGGL((k_direct_pe<256, true>), dim3((unsigned)small.size()),
+                                   dim3(256), sizeof(double4) * nsmall_max, s, D, d_small, ep);
+            }
             AHOP_HIP(hipEventRecord(ep1, s));
             timed_pe = true;
             for (int h = 0; h < nh; ++h)

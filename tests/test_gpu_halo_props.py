@@ -116,6 +116,30 @@ def test_non_finite_velocity_takes_the_serial_virial_scan(gpu):
     compare(out, ref)
 
 
+def test_members_at_one_position(gpu):
+    """Two members of a halo below the pair-sum limit at the same position: the reference's
+    pair sum divides by a zero distance, ep = -inf (det_halo_energies_1b3 :1417-1427).  The
+    device's fast 1/d (v_rsq_f64 and one Newton step) gives NaN there, and that halo's sum
+    is redone with the exact rsqrt; the rescaled profile then makes the virial ratios NaN,
+    so the halo also takes the serial virial scan."""
+    from oracle.halo_props import F, all_halos
+    import warnings
+    hs = G.nfw_halos(12, 22, alpha=1.0, nmin=40, nmax=3000)
+    pos = hs.pos.copy()
+    h = 4                                     # 186 members
+    s0 = hs.offsets[h]
+    pos[s0 + 5] = pos[s0 + 9]
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        ref = all_halos(hs.offsets, pos, hs.vel, hs.mass, hub_pt=70.3)
+    assert ref[h, F["ep"]] == -np.inf and np.isfinite(ref[h, F["rvir"]])
+    out, _ = run(G.HaloSet(hs.offsets, pos, hs.vel, hs.mass, hs.name), gpu, hub_pt=70.3)
+    assert out[h, F["ep"]] == -np.inf and out[h, F["spin"]] == np.inf
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        compare(out, ref)
+
+
 def test_densest_member_centre(gpu):
     from oracle.halo_props import all_halos
     hs = G.nfw_halos(60, 8, alpha=1.0, nmin=30, nmax=5000)
