@@ -44,7 +44,7 @@ struct HaloTuning {
     int tile = 0, bs = 0, trials = 0;
     float ms = 0;   // device time of the final run, all passes
     // of which the pair-sum potential (k_direct_pe) and the shell tables / virial scan
-    // (k_virial); 0 in light mode
+    // (k_hist_fold, k_virial_lanes, k_virial_serial); 0 in light mode
     float sub_ms[2] = {0, 0};
 };
 

@@ -1586,7 +1586,7 @@ static void sc_tiled(const std::vector<int32_t> &big, const Tiles &tl, const Hal
 
 // Runs every pass for one tile size / workgroup size; returns device milliseconds, and in
 // sub_ms (may be null) those of the pair-sum potential (k_direct_pe) and of the shell tables
-// and virial scan (k_virial).
+// and virial scan (k_hist_fold, k_virial_lanes, k_virial_serial).
 template <int BS>
 static float run_passes(const std::vector<int64_t> &hoff, const int64_t *d_off, const double *pos,
                         const double *vel, const double *mass, const double *dens, int64_t T,

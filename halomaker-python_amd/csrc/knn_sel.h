@@ -13,13 +13,13 @@
 //    level in registers needed 8 selects to write one entry back: C3 kernel 50.2 ->
 //    48.4 ms).  Each key carries its place among its siblings in its low 3 mantissa bits,
 //    so one integer max over a group finds the largest and where it is;
-//  * every (key, sorted slot) it puts into the heap after the seeding's direct fill is
-//    appended to a per-lane log in global memory (one region per resident wave: the grid is
-//    persistent and takes waves of queries from per-XCD counters); the direct fill's entries
-//    are known from the seed window.  A key is evicted only as the heap's maximum, and the
-//    maximum never grows, so after the search the filled and logged entries whose key is
-//    below the final root are exactly the heap's entries below it: the lane's k nearest when
-//    they number k.
+//  * every (key, sorted slot) it puts into the heap during the traversal is appended to a
+//    per-lane log in global memory (one region per resident wave: the grid is persistent and
+//    takes waves of queries from per-XCD counters); the seeding's candidates all lie in the
+//    seed window, whose keys the collection computes again.  A key is evicted only as the
+//    heap's maximum, and the maximum never grows, so after the search the window's and the
+//    logged entries whose key is below the final root are exactly the heap's entries below
+//    it: the lane's k nearest when they number k.
 // The k survivors are then put in FP32-key order in registers (Batcher's odd-even merge
 // network, ties by slot), their exact FP64 distances are checked to be in the same order
 // (FP32 near-ties can swap two; such a query is re-run on the FP64 path, as are those whose
@@ -28,9 +28,8 @@
 #pragma once
 #include <utility>
 
-// log entries per lane: the replacements after the seeding's direct fill (about 107 per lane
-// on average at the 16.8M-particle box, at most 640; 6 of its 16.8M queries overflow 512 and
-// are re-run on the FP64 path)
+// log entries per lane: the traversal's insertions (at the 16.8M-particle box at most 575;
+// 2 of its 16.8M queries overflow 512 and are re-run on the FP64 path)
 constexpr int SEL_CAPL = 512;
 // slots of the wave's Morton seed window beyond the k/2 on either side of its queries (the
 // heap kernels take 16): 32 gives the lanes at a wave's ends better seeds (at the
