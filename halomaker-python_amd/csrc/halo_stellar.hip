@@ -65,17 +65,23 @@ __device__ void block_sums(double (&v)[NQ], double *sh)
 
 // counts and masses by type, contamination, dispersions about the halo's mean velocity;
 // the row's other fields get the values clear_halo leaves without stars
-__global__ __launch_bounds__(BS) void k_type_sums(const int64_t *off, const double *vel,
+// Per-tile sums of a halo's members by type (a halo's members cut into tiles of at most
+// TYPE_TILE, so that the largest halo is not one workgroup's walk), combined per halo in tile
+// order by k_type_combine.
+constexpr int64_t TYPE_TILE = 8192;
+__global__ __launch_bounds__(BS) void k_type_sums(const int32_t *tile_halo,
+                                                  const int64_t *tile_start,
+                                                  const int64_t *tile_end, const double *vel,
                                                   const double *mass, const uint8_t *is_star,
-                                                  const double *props, double massp,
-                                                  int64_t *scount, double *out)
+                                                  const double *props, double massp, double *part)
 {
     __shared__ double sh[7 * NW];
-    const int h = blockIdx.x;
+    const int t = blockIdx.x;
+    const int h = tile_halo[t];
     const double *vb = props + (int64_t)h * PROPS_STRIDE + HP_VX;
     const double vx = vb[0], vy = vb[1], vz = vb[2];
     double v[7] = {0, 0, 0, 0, 0, 0, 0};   // ndm, nstar, mdm, m*, mcontam, dv2 dm, dv2 star
-    for (int64_t i = off[h] + threadIdx.x; i < off[h + 1]; i += BS) {
+    for (int64_t i = tile_start[t] + threadIdx.x; i < tile_end[t]; i += BS) {
         const double m = mass[i];
         const double dvx = vel[3 * i] - vx, dvy = vel[3 * i + 1] - vy, dvz = vel[3 * i + 2] - vz;
         const double dv2 = m * (dvx * dvx + dvy * dvy + dvz * dvz);
@@ -91,19 +97,29 @@ __global__ __launch_bounds__(BS) void k_type_sums(const int64_t *off, const doub
         }
     }
     block_sums<7>(v, sh);
-    if (threadIdx.x == 0) {
-        double *o = out + (int64_t)h * S_NFIELDS;
-        for (int q = 0; q < S_NFIELDS; ++q) o[q] = NAN;
-        for (int q = S_SFR; q <= S_SFR10_90; ++q) o[q] = 0.0;
-        o[S_NDM] = v[0];
-        o[S_NSTAR] = v[1];
-        o[S_MDM] = v[2];
-        o[S_MSTAR] = v[3];
-        o[S_MCONTAM] = v[4];
-        o[S_SIGDM] = v[2] > 0 ? sqrt(v[5] / v[2]) : 0.0;
-        o[S_SIGST] = v[3] > 0 ? sqrt(v[6] / v[3]) : 0.0;
-        scount[h] = (int64_t)v[1];
-    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 7; ++q) part[(int64_t)t * 7 + q] = v[q];
+}
+
+__global__ void k_type_combine(int64_t nh, const int32_t *tile_first, const double *part,
+                               int64_t *scount, double *out)
+{
+    const int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int t = tile_first[h]; t < tile_first[h + 1]; ++t)
+        for (int q = 0; q < 7; ++q) v[q] += part[(int64_t)t * 7 + q];
+    double *o = out + h * S_NFIELDS;
+    for (int q = 0; q < S_NFIELDS; ++q) o[q] = NAN;
+    for (int q = S_SFR; q <= S_SFR10_90; ++q) o[q] = 0.0;
+    o[S_NDM] = v[0];
+    o[S_NSTAR] = v[1];
+    o[S_MDM] = v[2];
+    o[S_MSTAR] = v[3];
+    o[S_MCONTAM] = v[4];
+    o[S_SIGDM] = v[2] > 0 ? sqrt(v[5] / v[2]) : 0.0;
+    o[S_SIGST] = v[3] > 0 ? sqrt(v[6] / v[3]) : 0.0;
+    scount[h] = (int64_t)v[1];
 }
 
 // Each halo's stars, in member order, go into the star arrays from soff[h].  The halos' member
@@ -433,8 +449,35 @@ float halo_stellar_device(const int64_t *h_off, int64_t nh, const double *pos, c
     AHOP_HIP(hipEventRecord(e0, s));
     if (nchem > 0)   // halos without stars keep NaN (all-ones bytes)
         AHOP_HIP(hipMemsetAsync(chem_out, 0xFF, sizeof(double) * nchem * nh, s));
-    hipLaunchKernelGGL(k_type_sums, dim3((unsigned)nh), dim3(BS), 0, s, d_off, vel, mass, is_star,
-                       props, massp, d_scount, out);
+    // tiles of the type sums (the host lists live until the synchronisation below)
+    std::vector<int32_t> th, tf(nh + 1);
+    std::vector<int64_t> ts, te;
+    {
+        for (int64_t h = 0; h < nh; ++h) {
+            tf[h] = (int32_t)th.size();
+            for (int64_t a = h_off[h]; a < h_off[h + 1]; a += TYPE_TILE) {
+                th.push_back((int32_t)h);
+                ts.push_back(a);
+                te.push_back(std::min(a + TYPE_TILE, h_off[h + 1]));
+            }
+        }
+        tf[nh] = (int32_t)th.size();
+        const int64_t nt = (int64_t)th.size();
+        static DevBuf bth, btf, bts, bte, bpart;
+        int32_t *d_th = (int32_t *)bth.ensure(sizeof(int32_t) * nt);
+        int32_t *d_tf = (int32_t *)btf.ensure(sizeof(int32_t) * (nh + 1));
+        int64_t *d_ts = (int64_t *)bts.ensure(sizeof(int64_t) * nt);
+        int64_t *d_te = (int64_t *)bte.ensure(sizeof(int64_t) * nt);
+        double *d_part = (double *)bpart.ensure(sizeof(double) * 7 * nt);
+        AHOP_HIP(hipMemcpyAsync(d_th, th.data(), sizeof(int32_t) * nt, hipMemcpyHostToDevice, s));
+        AHOP_HIP(hipMemcpyAsync(d_tf, tf.data(), sizeof(int32_t) * (nh + 1), hipMemcpyHostToDevice, s));
+        AHOP_HIP(hipMemcpyAsync(d_ts, ts.data(), sizeof(int64_t) * nt, hipMemcpyHostToDevice, s));
+        AHOP_HIP(hipMemcpyAsync(d_te, te.data(), sizeof(int64_t) * nt, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_type_sums, dim3((unsigned)nt), dim3(BS), 0, s, d_th, d_ts, d_te, vel,
+                           mass, is_star, props, massp, d_part);
+        hipLaunchKernelGGL(k_type_combine, dim3((unsigned)((nh + 127) / 128)), dim3(128), 0, s, nh,
+                           d_tf, d_part, d_scount, out);
+    }
     std::vector<int64_t> scount(nh);
     AHOP_HIP(hipMemcpyAsync(scount.data(), d_scount, sizeof(int64_t) * nh, hipMemcpyDeviceToHost, s));
     AHOP_HIP(hipStreamSynchronize(s));
