@@ -19,6 +19,7 @@
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_segmented_scan.hpp>
 
 #include "ahop_common.h"
 #include "halo_props.h"
@@ -100,49 +101,52 @@ __global__ void k_prof_gather(const int32_t *idx, const double *dist, const doub
 
 // per halo: leading zero distances, the circular-velocity peak over the cumulative mass in
 // distance order (first maximum), the members inside rvir and the first / last such distance
+// Cumulative mass from a halo's first member at a positive distance, over the members sorted
+// by distance: the masses of members at distance 0 masked to 0 and one device-wide segmented
+// scan with a head flag at every halo's first member (a workgroup per halo carrying its scan
+// chunk by chunk left the largest halo's 271 807 members to one workgroup: 1.75 ms).
+__global__ void k_prof_mask(const double *sd, const double *sm, int64_t M, double *mm,
+                            uint8_t *head)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    mm[i] = sd[i] > 0.0 ? sm[i] : 0.0;
+    head[i] = 0;
+}
+
+__global__ void k_prof_heads(const int64_t *off, int64_t nh, uint8_t *head)
+{
+    const int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (h < nh) head[off[h]] = 1;
+}
+
+// Peak circular velocity sqrt(G M(<r) / r) over the members at a positive distance: the first
+// maximum of each tile of VMAX_TILE sorted members (k_prof_vmax_tiles), then of each halo's
+// tiles in order with the outputs (k_prof_vmax_combine)
+constexpr int64_t VMAX_TILE = 8192;
 template <int BS>
-__global__ __launch_bounds__(BS) void k_prof_vmax(const int64_t *off, const double *sd,
-                                                  const double *sm, const double *props,
-                                                  double G, double *cm, double *aux,
-                                                  double *prof)
+__global__ __launch_bounds__(BS) void k_prof_vmax_tiles(const int64_t *tile_start,
+                                                        const int64_t *tile_end, const double *sd,
+                                                        const double *cm, double G,
+                                                        double *tbest, int64_t *tbi)
 {
     constexpr int NW = BS / 64;
-    __shared__ double wsum[NW];
     __shared__ double wbest[NW];
     __shared__ int64_t wbi[NW];
-    const int h = blockIdx.x;
+    const int tl = blockIdx.x;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int64_t f = off[h], e = off[h + 1];
-    const int64_t z = bound(sd, f, e, 0.0, true);      // first distance > 0
-    const double rvir = props[(int64_t)h * PROPS_STRIDE + P_RVIR];
-    double carry = 0.0, best = -INFINITY;
+    double best = -INFINITY;
     int64_t bi = INT64_MAX;
-    for (int64_t base = z; base < e; base += BS) {
-        const int64_t i = base + t;
-        double m = i < e ? sm[i] : 0.0;
-        // inclusive scan: within the wave, then the waves in order
-        for (int o = 1; o < 64; o <<= 1) {
-            const double u = __shfl_up(m, o);
-            if (lane >= o) m += u;
-        }
-        if (lane == 63) wsum[w] = m;
-        __syncthreads();
-        double before = carry;
-        for (int k = 0; k < w; ++k) before += wsum[k];
-        double tot = carry;
-        for (int k = 0; k < NW; ++k) tot += wsum[k];
-        __syncthreads();
-        carry = tot;
-        if (i < e) {
-            cm[i] = before + m;                         // cumulative mass from slot z
-            const double v = sqrt(G * (before + m) / sd[i]);
+    for (int64_t i = tile_start[tl] + t; i < tile_end[tl]; i += BS) {
+        const double d = sd[i];
+        if (d > 0.0) {
+            const double v = sqrt(G * cm[i] / d);
             if (v > best) {
                 best = v;
                 bi = i;
             }
         }
     }
-    // first maximum over the block
     for (int o = 32; o > 0; o >>= 1) {
         const double b2 = __shfl_xor(best, o);
         const int64_t i2 = __shfl_xor(bi, o);
@@ -162,18 +166,38 @@ __global__ __launch_bounds__(BS) void k_prof_vmax(const int64_t *off, const doub
                 best = wbest[k];
                 bi = wbi[k];
             }
-        double *o = prof + (int64_t)h * NPROF;
-        const bool any = z < e;
-        o[0] = any && bi != INT64_MAX ? best : NAN;
-        o[1] = any && bi != INT64_MAX ? sd[bi] : NAN;
-        for (int q = 2; q < NPROF; ++q) o[q] = NAN;
-        const int64_t nin = rvir > 0 ? bound(sd, z, e, rvir, true) - z : 0;
-        double *a = aux + (int64_t)h * NAUX;
-        a[0] = (double)(z - f);
-        a[1] = (double)nin;
-        a[2] = nin > 0 ? sd[z] : NAN;
-        a[3] = nin > 0 ? sd[z + nin - 1] : NAN;
+        tbest[tl] = best;
+        tbi[tl] = bi;
     }
+}
+
+__global__ void k_prof_vmax_combine(int64_t nh, const int64_t *off, const int32_t *tile_first,
+                                    const double *tbest, const int64_t *tbi, const double *sd,
+                                    const double *props, double *aux, double *prof)
+{
+    const int64_t h = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (h >= nh) return;
+    double best = -INFINITY;
+    int64_t bi = INT64_MAX;
+    for (int t = tile_first[h]; t < tile_first[h + 1]; ++t)
+        if (tbest[t] > best || (tbest[t] == best && tbi[t] < bi)) {
+            best = tbest[t];
+            bi = tbi[t];
+        }
+    const int64_t f = off[h], e = off[h + 1];
+    const int64_t z = bound(sd, f, e, 0.0, true);      // first distance > 0
+    const double rvir = props[h * PROPS_STRIDE + P_RVIR];
+    double *o = prof + h * NPROF;
+    const bool any = z < e;
+    o[0] = any && bi != INT64_MAX ? best : NAN;
+    o[1] = any && bi != INT64_MAX ? sd[bi] : NAN;
+    for (int q = 2; q < NPROF; ++q) o[q] = NAN;
+    const int64_t nin = rvir > 0 ? bound(sd, z, e, rvir, true) - z : 0;
+    double *a = aux + h * NAUX;
+    a[0] = (double)(z - f);
+    a[1] = (double)nin;
+    a[2] = nin > 0 ? sd[z] : NAN;
+    a[3] = nin > 0 ? sd[z + nin - 1] : NAN;
 }
 
 // the NFW model residual (log_nfw - y) * w and its derivative in rs (:1864-1866)
@@ -402,6 +426,9 @@ float halo_profile_radii(const int64_t *h_off, int64_t nh, const double *d_pos,
     // sorted per halo by two stable device-wide radix sorts: by distance, then by halo (the
     // second keeps the first's order within a halo), so no halo waits on one workgroup
     static DevBuf doff, ddist, dd2, didx, di2, dhid, dh2, temp;
+    // tiles of the peak-velocity search (the host lists live until the synchronisation below)
+    std::vector<int32_t> vtf(nh + 1);
+    std::vector<int64_t> vts, vte;
     int64_t *d_off = (int64_t *)doff.ensure(sizeof(int64_t) * (nh + 1));
     double *dist = (double *)ddist.ensure(sizeof(double) * M);
     double *dist_s = (double *)dd2.ensure(sizeof(double) * M);
@@ -428,8 +455,41 @@ float halo_profile_radii(const int64_t *h_off, int64_t nh, const double *d_pos,
     AHOP_HIP(rocprim::radix_sort_pairs(tmp, tb2, hid_s, hid, idx_s, idx, (size_t)M, 0, hbits, s));
     hipLaunchKernelGGL(k_prof_gather, dim3(gb), dim3(256), 0, s, idx, dist, d_mass, (int64_t)M,
                        d_sdist, d_smass);
-    hipLaunchKernelGGL(k_prof_vmax<256>, dim3((unsigned)nh), dim3(256), 0, s, d_off, d_sdist,
-                       d_smass, d_props, G, d_scum, d_aux, d_prof);
+    {
+        static DevBuf bmm, bhead, bscan, btf, bts, bte, bbest, bbi;
+        double *mm = (double *)bmm.ensure(sizeof(double) * M);
+        uint8_t *head = (uint8_t *)bhead.ensure(M);
+        hipLaunchKernelGGL(k_prof_mask, dim3(gb), dim3(256), 0, s, d_sdist, d_smass, (int64_t)M,
+                           mm, head);
+        hipLaunchKernelGGL(k_prof_heads, dim3((unsigned)((nh + 255) / 256)), dim3(256), 0, s,
+                           d_off, nh, head);
+        size_t tb = 0;
+        AHOP_HIP(rocprim::segmented_inclusive_scan(nullptr, tb, mm, d_scum, head, (size_t)M,
+                                                   rocprim::plus<double>(), s));
+        AHOP_HIP(rocprim::segmented_inclusive_scan(bscan.ensure(tb), tb, mm, d_scum, head,
+                                                   (size_t)M, rocprim::plus<double>(), s));
+        for (int64_t h = 0; h < nh; ++h) {
+            vtf[h] = (int32_t)vts.size();
+            for (int64_t a = h_off[h]; a < h_off[h + 1]; a += VMAX_TILE) {
+                vts.push_back(a);
+                vte.push_back(std::min(a + VMAX_TILE, h_off[h + 1]));
+            }
+        }
+        vtf[nh] = (int32_t)vts.size();
+        const int64_t nt = (int64_t)vts.size();
+        int32_t *d_tf = (int32_t *)btf.ensure(sizeof(int32_t) * (nh + 1));
+        int64_t *d_ts = (int64_t *)bts.ensure(sizeof(int64_t) * nt);
+        int64_t *d_te = (int64_t *)bte.ensure(sizeof(int64_t) * nt);
+        double *d_best = (double *)bbest.ensure(sizeof(double) * nt);
+        int64_t *d_bi = (int64_t *)bbi.ensure(sizeof(int64_t) * nt);
+        AHOP_HIP(hipMemcpyAsync(d_tf, vtf.data(), sizeof(int32_t) * (nh + 1), hipMemcpyHostToDevice, s));
+        AHOP_HIP(hipMemcpyAsync(d_ts, vts.data(), sizeof(int64_t) * nt, hipMemcpyHostToDevice, s));
+        AHOP_HIP(hipMemcpyAsync(d_te, vte.data(), sizeof(int64_t) * nt, hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(k_prof_vmax_tiles<256>, dim3((unsigned)nt), dim3(256), 0, s, d_ts, d_te,
+                           d_sdist, d_scum, G, d_best, d_bi);
+        hipLaunchKernelGGL(k_prof_vmax_combine, dim3((unsigned)((nh + 127) / 128)), dim3(128), 0, s,
+                           nh, d_off, d_tf, d_best, d_bi, d_sdist, d_props, d_aux, d_prof);
+    }
     AHOP_HIP(hipEventRecord(e1, s));
     AHOP_HIP(hipStreamSynchronize(s));
     float ms = 0;
