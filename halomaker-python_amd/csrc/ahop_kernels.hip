@@ -193,7 +193,7 @@ __device__ __forceinline__ int delta(const uint64_t *__restrict__ k, int n, int 
 __global__ void k_radix_tree(const uint64_t *__restrict__ key, int n, int32_t *__restrict__ first,
                              int32_t *__restrict__ last, int32_t *__restrict__ left,
                              int32_t *__restrict__ right, int32_t *__restrict__ parent,
-                             int32_t *__restrict__ leaf_parent, int4 *__restrict__ nrec)
+                             int32_t *__restrict__ leaf_parent)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -219,7 +219,6 @@ __global__ void k_radix_tree(const uint64_t *__restrict__ key, int n, int32_t *_
     last[i] = la;
     left[i] = lc;
     right[i] = rc;
-    nrec[i] = make_int4(lc, rc, f, la);   // the links and range of the node's visit record
     if (lc >= 0) parent[lc] = i; else leaf_parent[~lc] = i;
     if (rc >= 0) parent[rc] = i; else leaf_parent[~rc] = i;
     if (i == 0) parent[0] = -1;
@@ -336,8 +335,10 @@ __device__ __forceinline__ void merge_node(int node, const double4 *__restrict__
                                            const int32_t *__restrict__ last,
                                            const int32_t *__restrict__ left,
                                            const int32_t *__restrict__ right, double *box,
-                                           float4 *fbox, float4 *cbox)
+                                           float4 *fbox, float4 *cbox, int4 *nrec)
 {
+    // the links and range of the node's visit record
+    nrec[node] = make_int4(left[node], right[node], first[node], last[node]);
     double alo[3], ahi[3], blo[3], bhi[3], lo[3], hi[3];
     int af, al, bf, bl;
     child_box_range(left[node], spos, first, last, box, alo, ahi, af, al);
@@ -361,7 +362,8 @@ __global__ __launch_bounds__(REFIT_BS) void k_refit_local(
     int n, int bucket, const double4 *__restrict__ spos, const int32_t *__restrict__ first,
     const int32_t *__restrict__ last, const int32_t *__restrict__ left,
     const int32_t *__restrict__ right, const int32_t *__restrict__ parent,
-    const int32_t *__restrict__ leaf_parent, double *box, float4 *fbox, float4 *cbox)
+    const int32_t *__restrict__ leaf_parent, double *box, float4 *fbox, float4 *cbox,
+    int4 *nrec)
 {
     __shared__ int cnt[REFIT_BS];
     __shared__ int lf[REFIT_BS], ll[REFIT_BS], lc0[REFIT_BS], lc1[REFIT_BS], lpar[REFIT_BS];
@@ -418,6 +420,7 @@ __global__ __launch_bounds__(REFIT_BS) void k_refit_local(
                 box[6 * (int64_t)node + a + 3] = hi[a];
             }
         if (par_big) {
+            nrec[node] = make_int4(lc0[u], lc1[u], lf[u], ll[u]);
             float4 fa, fb;
             fp32_box(lo, hi, fa, fb);
             fbox[2 * (int64_t)node] = fa;
@@ -455,7 +458,7 @@ __device__ __forceinline__ bool refit_sweeps(int32_t *list, int m, int base, int
                                              const int32_t *__restrict__ last,
                                              const int32_t *__restrict__ left,
                                              const int32_t *__restrict__ right, int32_t *flag,
-                                             double *box, float4 *fbox, float4 *cbox)
+                                             double *box, float4 *fbox, float4 *cbox, int4 *nrec)
 {
     for (int sweep = 1;; ++sweep) {
         const int stamp = base + sweep;
@@ -470,7 +473,7 @@ __device__ __forceinline__ bool refit_sweeps(int32_t *list, int m, int base, int
             if (i < 0) continue;
             if (child_done(left[i], first, last, flag, stamp) &&
                 child_done(right[i], first, last, flag, stamp)) {
-                merge_node(i, spos, first, last, left, right, box, fbox, cbox);
+                merge_node(i, spos, first, last, left, right, box, fbox, cbox, nrec);
                 flag[i] = stamp;
                 list[k] = -1;
                 merged = true;
@@ -492,7 +495,7 @@ __global__ __launch_bounds__(MID_BS) void k_refit_mid(
     int n, const double4 *__restrict__ spos, const int32_t *__restrict__ first,
     const int32_t *__restrict__ last, const int32_t *__restrict__ left,
     const int32_t *__restrict__ right, int32_t *flag, double *box, float4 *fbox, float4 *cbox,
-    int32_t *top, int *ntop)
+    int4 *nrec, int32_t *top, int *ntop)
 {
     __shared__ int32_t list[MID_CAP];
     __shared__ int nlist, more, progress;
@@ -515,7 +518,7 @@ __global__ __launch_bounds__(MID_BS) void k_refit_mid(
     __syncthreads();
     const int m = min(nlist, MID_CAP);
     if (refit_sweeps(list, m, 0, &more, &progress, spos, first, last, left, right, flag, box,
-                     fbox, cbox)) {
+                     fbox, cbox, nrec)) {
         // nodes above one that went to pass 3 (a full list): pass 3 takes them as well
         for (int k = threadIdx.x; k < m; k += MID_BS)
             if (list[k] >= 0) top[atomicAdd(ntop, 1)] = list[k];
@@ -532,7 +535,8 @@ __global__ __launch_bounds__(MID_BS) void k_refit_top(
     const double4 *__restrict__ spos, const int32_t *__restrict__ first,
     const int32_t *__restrict__ last, const int32_t *__restrict__ left,
     const int32_t *__restrict__ right, const int32_t *__restrict__ parent, int32_t *flag,
-    double *box, float4 *fbox, float4 *cbox, const int32_t *top, const int *ntop, int *stalled)
+    double *box, float4 *fbox, float4 *cbox, int4 *nrec, const int32_t *top, const int *ntop,
+    int *stalled)
 {
     const int m = *ntop;
     auto merged = [&](int c) {
@@ -556,7 +560,7 @@ __global__ __launch_bounds__(MID_BS) void k_refit_top(
             TOP_START)
             continue;   // not a start: a child is still pending
         while (true) {
-            merge_node(node, spos, first, last, left, right, box, fbox, cbox);
+            merge_node(node, spos, first, last, left, right, box, fbox, cbox, nrec);
             ++climbed;
             __hip_atomic_store(&flag[node], TOP_STAMP, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -1282,20 +1286,20 @@ void Engine::build_tree(const double *d_pos, const double *d_mass, int64_t n, co
     t_.nrec = as<int4>(b_nrec, ni);
     t_.cbox = as<float4>(b_cbox, 4 * ni);
     hipLaunchKernelGGL(k_radix_tree, dim3(nblk(ni)), dim3(256), 0, s, t_.key, (int)n, t_.first,
-                       t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent, t_.nrec);
+                       t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent);
     hipLaunchKernelGGL(k_fill_i32, dim3(nblk(ni)), dim3(256), 0, s, t_.flag, ni, 0);
     t_.fbox = as<float4>(b_fbox, 2 * ni);
     hipLaunchKernelGGL(k_refit_local, dim3(nblk(n, REFIT_BS)), dim3(REFIT_BS), 0, s, (int)n,
                        p.bucket, t_.spos, t_.first, t_.last, t_.left, t_.right, t_.parent, t_.leaf_parent,
-                       t_.box, t_.fbox, t_.cbox);
+                       t_.box, t_.fbox, t_.cbox, t_.nrec);
     // pass 3's list (misc[5] counts it): the roots buffer is free here
     int32_t *top = as<int32_t>(b_roots, n);
     hipLaunchKernelGGL(k_refit_mid, dim3((unsigned)((n + MID_S - 1) / MID_S)), dim3(MID_BS), 0, s,
                        (int)n, t_.spos, t_.first, t_.last, t_.left, t_.right, t_.flag, t_.box,
-                       t_.fbox, t_.cbox, top, misc + 5);
+                       t_.fbox, t_.cbox, t_.nrec, top, misc + 5);
     hipLaunchKernelGGL(k_refit_top, dim3(1), dim3(MID_BS), 0, s, t_.spos, t_.first, t_.last,
-                       t_.left, t_.right, t_.parent, t_.flag, t_.box, t_.fbox, t_.cbox, top,
-                       misc + 5, misc + 6);
+                       t_.left, t_.right, t_.parent, t_.flag, t_.box, t_.fbox, t_.cbox, t_.nrec,
+                       top, misc + 5, misc + 6);
     AHOP_HIP(hipEventRecord(ev_[2], s));
 }
 
