@@ -7,6 +7,9 @@
 #include "halo_props.h"
 #include "ramses_reader.h"
 
+#include <omp.h>
+
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -86,6 +89,12 @@ struct ModuleState {
     // counters and stage times of the last host structure-tree stage
     int64_t tree_stats[4] = {0, 0, 0, 0};
     double tree_ms[5] = {0, 0, 0, 0, 0};
+    // compute_adaptahop's device copy of its inputs (positions, then masses) and the pinned
+    // staging buffers of their upload: kept across calls, released at process end
+    void *dev_in = nullptr;
+    size_t dev_in_cap = 0;
+    PinnedVec<char> stage[2];
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
 };
 
 void keep_tree_stats(ModuleState &s, const TreeOutputs &to)
@@ -105,6 +114,38 @@ ModuleState &S()
 {
     static ModuleState s;
     return s;
+}
+
+// Host-to-device copy of a caller's pageable array: host threads copy 32 MiB chunks into two
+// pinned staging buffers in turn while the DMA engine moves the previous chunk (a plain
+// hipMemcpy from pageable memory moved the 0.54 GB of a 16.8M-particle box at about 27 GB/s).
+void upload(ModuleState &s, void *dst, const void *src, size_t bytes, hipStream_t st)
+{
+    constexpr size_t CH = size_t(32) << 20;
+    for (int b = 0; b < 2; ++b) {
+        s.stage[b].resize(CH);
+        if (!s.stage_ev[b])
+            AHOP_HIP(hipEventCreateWithFlags(&s.stage_ev[b], hipEventDisableTiming));
+    }
+    const int nt = std::max(1, std::min(omp_get_max_threads(), 16));
+    bool pending[2] = {false, false};
+    int b = 0;
+    for (size_t off = 0; off < bytes; off += CH, b ^= 1) {
+        const size_t m = std::min(CH, bytes - off);
+        if (pending[b]) AHOP_HIP(hipEventSynchronize(s.stage_ev[b]));   // buffer b is free
+        char *stg = s.stage[b].data();
+        const char *from = static_cast<const char *>(src) + off;
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+        for (int t = 0; t < nt; ++t) {
+            const size_t a = m * t / nt, e = m * (t + 1) / nt;
+            std::memcpy(stg + a, from + a, e - a);
+        }
+        AHOP_HIP(hipMemcpyAsync(static_cast<char *>(dst) + off, stg, m, hipMemcpyHostToDevice,
+                                st));
+        AHOP_HIP(hipEventRecord(s.stage_ev[b], st));
+        pending[b] = true;
+    }
+    AHOP_HIP(hipStreamSynchronize(st));
 }
 
 Engine &engine()
@@ -129,12 +170,6 @@ int guarded(F &&f)
         return 1;
     }
 }
-
-struct DeviceArray {
-    void *p = nullptr;
-    explicit DeviceArray(size_t bytes) { AHOP_HIP(hipMalloc(&p, bytes < 8 ? 8 : bytes)); }
-    ~DeviceArray() { if (p) (void)hipFree(p); }
-};
 
 void free_outputs(ModuleState &s)
 {
@@ -269,10 +304,18 @@ int ahop_compute_adaptahop(const double *pos_in, int64_t pos_rows, int64_t pos_c
         s.dev_n = 0;   // the engine's results now belong to this call
         Engine &e = engine();
         Params p = hot_params(s);
-        DeviceArray dpos(sizeof(double) * 3 * n), dmass(sizeof(double) * n);
-        AHOP_HIP(hipMemcpy(dpos.p, pos_in, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-        AHOP_HIP(hipMemcpy(dmass.p, mass_in, sizeof(double) * n, hipMemcpyHostToDevice));
-        e.run((const double *)dpos.p, (const double *)dmass.p, n, p, true);
+        const size_t need = sizeof(double) * 4 * n;
+        if (need > s.dev_in_cap) {
+            if (s.dev_in) AHOP_HIP(hipFree(s.dev_in));
+            s.dev_in = nullptr;
+            s.dev_in_cap = 0;
+            AHOP_HIP(hipMalloc(&s.dev_in, need));
+            s.dev_in_cap = need;
+        }
+        double *dpos = static_cast<double *>(s.dev_in), *dmass = dpos + 3 * n;
+        upload(s, dpos, pos_in, sizeof(double) * 3 * n, e.stream());
+        upload(s, dmass, mass_in, sizeof(double) * n, e.stream());
+        e.run(dpos, dmass, n, p, true);
         const DeviceResults &r = e.results();
         s.density.resize(n);
         s.gid.resize(n);
