@@ -627,6 +627,21 @@ def main():
         if tr.get("n") == n and tr.get("bucket") == args.bucket:
             roofline["traffic"] = int(tr["fetch_bytes"] + tr["write_bytes"])
             roofline["traffic_source"] = os.path.relpath(tpath, ROOT)
+    # the bound it does sit on: VALU issue, the counted wave instructions of one launch over
+    # this run's kernel time against the chip's rate (SIMDs x clock / 4 cycles per wave64
+    # VALU instruction)
+    vpath = os.path.join(ROOT, "profiles", f"knn_{args.workload}_valu.json")
+    if os.path.exists(vpath):
+        with open(vpath) as f:
+            va = json.load(f)
+        if va.get("n") == n and va.get("bucket") == args.bucket:
+            rate = va["valu_wave_instructions_per_launch"] / (knn_ms * 1e-3)
+            peak = va["simds"] * va["clock_ghz"] * 1e9 / va["cycles_per_wave64_valu_instruction"]
+            roofline["valu_issue"] = {"wave_instructions": va["valu_wave_instructions_per_launch"],
+                                      "achieved_per_s": float(f"{rate:.4g}"),
+                                      "peak_per_s": float(f"{peak:.4g}"),
+                                      "frac": round(rate / peak, 3),
+                                      "source": os.path.relpath(vpath, ROOT)}
 
     if not args.no_grouping:
         extra["grouping"] = grouping_stage(gid, int(last.ngroups), max(3, args.steps),
